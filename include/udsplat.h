@@ -1,0 +1,140 @@
+/*
+ * udsplat.h -- C ABI of the MI355X (gfx950) splatting hot path.
+ *
+ * Every entry point takes raw device pointers, plain sizes and the HIP stream
+ * to launch on, and returns a status code (UDE_OK = 0, negative on error).
+ * Outputs are caller-allocated device buffers; scratch memory is a
+ * caller-provided workspace whose size is given by the matching
+ * *_workspace_bytes query.  No entry point allocates device memory or
+ * synchronises the host.
+ *
+ * Tensor layouts are those of the reference op surface:
+ *   slot tensors (indices, weights)  (bn, W, H, k), pixel (x=i, y=j) at j + H (i + W b)
+ *   points / normals                 (bn, N, 3) fp32 camera space
+ *   std-devs                         (sdBatch, N, 2), sdBatch in {1, bn}
+ * Index value -1 marks an empty slot.
+ */
+#ifndef UDSPLAT_H
+#define UDSPLAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* a HIP stream (hipStream_t) */
+typedef struct ihipStream_t* ude_stream_t;
+
+#define UDE_OK 0
+#define UDE_ERR_ARG (-1)
+#define UDE_ERR_LAUNCH (-2)
+#define UDE_ERR_WORKSPACE (-3)
+
+/* ---- sampling ---------------------------------------------------------- */
+
+/* Workspace for ude_sample (one view's tree, reused across views). */
+size_t ude_sample_workspace_bytes(int N, int W, int H, int k);
+
+/* sampling.sample -- replaces cudaSample / batchSample
+ * (FMMKdTreeSampling/sampling.cpp:48-94, samplingCuda.cu:197-321).
+ * out: int32 (bn, W, H, k).  seed keys the Philox4x32-10 uniforms (the
+ * reference seeds cuRAND with time(NULL)).  *errFlag (device int, caller
+ * zeroed) receives a non-zero bit if a descent exceeded 63 levels. */
+int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, double precision,
+               const float* pts, const float* nrm, const float* sd, int sdBatch,
+               uint64_t seed, int32_t* out, int* errFlag, void* ws, size_t wsBytes,
+               ude_stream_t stream);
+
+/* sampling.covariance -- replaces cudaCovariance (sampling.cpp:96-134,
+ * samplingCuda.cu:323-396).  out: (bn, N, 2, 2). */
+int ude_covariance(int bn, int N, int W, int H, double f, double smoothing, const float* pts,
+                   const float* nrm, const float* sd, int sdBatch, float* out, ude_stream_t stream);
+
+/* ---- splatting --------------------------------------------------------- */
+
+size_t ude_splat_workspace_bytes(int bn, int N);
+
+/* splatting.forward -- replaces cudaSplattingForward (splatting.cpp:43-82,
+ * splattingCuda.cu:18-101).  out: (bn, W, H, k) = min(w, 1), 0 for empty slots. */
+int ude_splat_forward(int bn, int N, int W, int H, int k, double f, double smoothing,
+                      const int32_t* idx, const float* pts, const float* nrm, const float* sd, int sdBatch,
+                      float* out, void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* splatting.backward -- replaces cudaSplattingBackward (splatting.cpp:84-129,
+ * splattingCuda.cu:114-234).  gPts, gNrm: (bn, N, 3), fully written.
+ * gSd: (sdBatch, N, 2); fully written if sdBatch == bn > 1, accumulated into
+ * (caller zeroed) if sdBatch == 1. */
+int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoothing,
+                       const float* gradIn, const int32_t* idx, const float* pts, const float* nrm,
+                       const float* sd, int sdBatch, float* gPts, float* gNrm, float* gSd,
+                       void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* ---- filtering (z-sort + alpha compositing) ----------------------------- */
+
+/* filtering.forward -- replaces cudaAlphaBlendForward (zSorting.cpp:24-32,
+ * zSortingCuda.cu:64-123).  pts/nrm: (pb, N, 3), pb in {1, bn}.  k <= 1024. */
+int ude_filter_forward(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
+                       const float* pts, const float* nrm, int pb, float* out, ude_stream_t stream);
+
+/* filtering.backward -- replaces cudaAlphaBlendBackward (zSorting.cpp:34-42,
+ * zSortingCuda.cu:180-243).  out: gradient w.r.t. the weights (bn, W, H, k). */
+int ude_filter_backward(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
+                        const float* gin, const float* pts, const float* nrm, int pb, float* out,
+                        ude_stream_t stream);
+
+/* ---- transform ---------------------------------------------------------- */
+
+/* transform.{affine,linear}Forward -- replaces cudaTransformForward
+ * (transform.cpp:34-60, transformCuda.cu:39-72).  M: (bn, 4, 4); v: (vb, N, 3). */
+int ude_transform_forward(int affine, int bn, int vb, int N, const float* M, const float* v, float* out,
+                          ude_stream_t stream);
+
+/* transform.{affine,linear}Backward -- replaces cudaTransformBackward
+ * (transform.cpp:62-85, transformCuda.cu:119-156).  gM (bn,4,4) and, when
+ * vb == 1 < bn, gV (1,N,3) are accumulated into (caller zeroed). */
+int ude_transform_backward(int affine, int bn, int vb, int N, const float* g, const float* M, const float* v,
+                           float* gM, float* gV, ude_stream_t stream);
+
+/* ---- interpolation ------------------------------------------------------ */
+
+/* interpolation.forward -- replaces cudaInterpolationForward
+ * (interpolation.cpp:31-58, interpolationCuda.cu:5-81).  attr: (ab, N, C). */
+int ude_interp_forward(int bn, int W, int H, int k, int ab, int N, int C, const int32_t* idx, const float* wts,
+                       const float* attr, float* out, ude_stream_t stream);
+
+/* interpolation.backward -- replaces cudaInterpolationBackward
+ * (interpolation.cpp:60-96, interpolationCuda.cu:83-167).  gW fully written,
+ * gA accumulated into (caller zeroed). */
+int ude_interp_backward(int bn, int W, int H, int k, int ab, int N, int C, const float* g, const int32_t* idx,
+                        const float* wts, const float* attr, float* gW, float* gA, ude_stream_t stream);
+
+/* ---- shading ------------------------------------------------------------ */
+
+/* shading.forwardSh -- replaces cudaShadingShForward (shading.cpp:141-168,
+ * shadingCuda.cu:177-242).  coef: (cb, 3, 9); normals/diffuse (bn, W, H, 3). */
+int ude_sh_forward(int bn, int W, int H, int cb, const float* coef, const float* nrm, const float* diff,
+                   float* out, ude_stream_t stream);
+
+/* shading.backwardSh -- replaces cudaShadingShBackward (shading.cpp:170-200,
+ * shadingCuda.cu:244-330).  gCoef accumulated into (caller zeroed). */
+int ude_sh_backward(int bn, int W, int H, int cb, const float* g, const float* coef, const float* nrm,
+                    const float* diff, float* gCoef, float* gN, float* gD, ude_stream_t stream);
+
+/* shading.forward (Lambert) -- replaces cudaShadingForward (shading.cpp:51-88,
+ * shadingCuda.cu:12-81).  dirs (bn, L, 3); ints (ib, L, 3). */
+int ude_lambert_forward(int bn, int W, int H, int L, int ib, const float* dirs, const float* ints,
+                        const float* nrm, const float* diff, float* out, ude_stream_t stream);
+
+/* shading.backward (Lambert) -- replaces cudaShadingBackward (shading.cpp:90-139,
+ * shadingCuda.cu:83-175).  gDir, gInt accumulated into (caller zeroed). */
+int ude_lambert_backward(int bn, int W, int H, int L, int ib, const float* g, const float* dirs,
+                         const float* ints, const float* nrm, const float* diff, float* gDir, float* gInt,
+                         float* gN, float* gD, ude_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UDSPLAT_H */

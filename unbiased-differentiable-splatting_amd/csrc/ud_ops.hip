@@ -1,0 +1,883 @@
+// ud_ops.hip -- per-slot operators of the splatting op surface (gfx950):
+//   splatting.forward / backward   (EWA weights and their adjoint)
+//   sampling.covariance
+//   filtering.forward / backward   (z-sort + alpha compositing)
+//   transform.{affine,linear}{Forward,Backward}
+//   interpolation.forward / backward
+//   shading.{forward,backward,forwardSh,backwardSh}
+//
+// Layouts follow the reference op surface: slot tensors are x-major
+// (bn, W, H, k) with pixel (i = x, j = y) at j + H (i + W b); point tensors
+// are (bn, N, 3) fp32, std-devs (1 | bn, N, 2).
+#include "ud_device.h"
+#include "../../include/udsplat.h"
+
+using namespace ud;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// EWA geometry in FP64.  A = inverse Jacobian of the local plane frame
+// (ewaSplatting.cuh:8-44), evaluated for T = double or T = D6 (forward-mode
+// duals over point (3) and normal (3)).
+// ---------------------------------------------------------------------------
+struct D6 {
+    double v, d[6];
+};
+__device__ inline D6 mk(double v) { D6 r; r.v = v; for (int i = 0; i < 6; i++) r.d[i] = 0.0; return r; }
+__device__ inline D6 operator+(D6 a, D6 b) { D6 r; r.v = a.v + b.v; for (int i = 0; i < 6; i++) r.d[i] = a.d[i] + b.d[i]; return r; }
+__device__ inline D6 operator-(D6 a, D6 b) { D6 r; r.v = a.v - b.v; for (int i = 0; i < 6; i++) r.d[i] = a.d[i] - b.d[i]; return r; }
+__device__ inline D6 operator-(D6 a) { D6 r; r.v = -a.v; for (int i = 0; i < 6; i++) r.d[i] = -a.d[i]; return r; }
+__device__ inline D6 operator*(D6 a, D6 b) { D6 r; r.v = a.v * b.v; for (int i = 0; i < 6; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+__device__ inline D6 operator*(D6 a, double s) { D6 r; r.v = a.v * s; for (int i = 0; i < 6; i++) r.d[i] = a.d[i] * s; return r; }
+__device__ inline D6 operator+(D6 a, double s) { a.v += s; return a; }
+__device__ inline D6 operator/(D6 a, D6 b)
+{
+    D6 r; r.v = a.v / b.v;
+    double ib = 1.0 / b.v;
+    for (int i = 0; i < 6; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
+    return r;
+}
+__device__ inline D6 dsqrt(D6 a) { D6 r; r.v = sqrt(a.v); double h = 0.5 / r.v; for (int i = 0; i < 6; i++) r.d[i] = a.d[i] * h; return r; }
+__device__ inline double dsqrt(double a) { return sqrt(a); }
+__device__ inline double val(double a) { return a; }
+__device__ inline double val(const D6& a) { return a.v; }
+__device__ inline D6 operator/(D6 a, double s) { return a * (1.0 / s); }
+template <class T> __device__ inline T mkT(double v);
+template <> __device__ inline double mkT<double>(double v) { return v; }
+template <> __device__ inline D6 mkT<D6>(double v) { return mk(v); }
+
+template <class T>
+__device__ inline T dot3T(const T* a, const T* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// screen point s and A = [[A00, A01], [0, A11]] for camera-space point p, normal n
+template <class T>
+__device__ inline void ewa_geom(const T* p, const T* n, double W, double H, double f,
+                                T& sx, T& sy, T& A00, T& A01, T& A11)
+{
+    sx = (-(p[0] * f)) / p[2] + W * 0.5;
+    sy = (p[1] * f) / p[2] + H * 0.5;
+    const double i2f = 1.0 / (2.0 * f);
+    T x0[3], x1[3];
+    // backproject(s + e_x), backproject(s + e_y): (W - 2 y_x) / 2f, -(H - 2 y_y) / 2f, 1
+    T y0x = sx + 1.0, y1y = sy + 1.0;
+    x0[0] = (mkT<T>(W) - y0x * 2.0) * i2f;
+    x0[1] = -((mkT<T>(H) - sy * 2.0) * i2f);
+    x0[2] = mkT<T>(1.0);
+    x1[0] = (mkT<T>(W) - sx * 2.0) * i2f;
+    x1[1] = -((mkT<T>(H) - y1y * 2.0) * i2f);
+    x1[2] = mkT<T>(1.0);
+    T a = dot3T(p, n);
+    T xt0 = a / dot3T(x0, n), xt1 = a / dot3T(x1, n);
+    T t0[3], t1[3];
+    for (int c = 0; c < 3; c++) { t0[c] = x0[c] * xt0 - p[c]; t1[c] = x1[c] * xt1 - p[c]; }
+    T nt0 = dsqrt(dot3T(t0, t0));
+    T u0[3], u1[3];
+    for (int c = 0; c < 3; c++) u0[c] = t0[c] / nt0;
+    u1[0] = u0[1] * n[2] - u0[2] * n[1];
+    u1[1] = u0[2] * n[0] - u0[0] * n[2];
+    u1[2] = u0[0] * n[1] - u0[1] * n[0];
+    A00 = nt0;
+    A01 = dot3T(t1, u0);
+    A11 = dot3T(t1, u1);
+}
+
+
+struct SplatRec {
+    float sxi, syi, sxf, syf;   // screen point split into integer part and fraction
+    float vi00, vi01, vi11;     // inverse screen covariance
+    float norm;                 // 1 / (2 pi sqrt(det V) |det A|)
+};
+
+// Per (view, splat) FP64 precompute of the screen Gaussian (ewaSplatting.cuh:24-67).
+__global__ void k_splat_prepare(int bn, int N, int W, int H, double f, double smoothing,
+                                const float* __restrict__ pts, const float* __restrict__ nrm,
+                                const float* __restrict__ sd, int sdBatch, SplatRec* __restrict__ rec)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)bn * N) return;
+    long b = t / N, id = t % N;
+    double p[3] = { pts[3 * t], pts[3 * t + 1], pts[3 * t + 2] };
+    double n[3] = { nrm[3 * t], nrm[3 * t + 1], nrm[3 * t + 2] };
+    long so = 2 * ((sdBatch > 1 ? b * N : 0) + id);
+    double s0 = sd[so], s1 = sd[so + 1];
+    double sx, sy, A00, A01, A11;
+    ewa_geom<double>(p, n, W, H, f, sx, sy, A00, A01, A11);
+    double detA = A00 * A11;
+    double J00 = A11 / detA, J01 = -A01 / detA, J11 = A00 / detA;
+    double V00 = J00 * J00 * s0 * s0 + J01 * J01 * s1 * s1 + smoothing;
+    double V01 = J01 * J11 * s1 * s1;
+    double V11 = J11 * J11 * s1 * s1 + smoothing;
+    double detV = V00 * V11 - V01 * V01;
+    SplatRec r;
+    double fx = floor(sx), fy = floor(sy);
+    r.sxi = (float)fx; r.syi = (float)fy;
+    r.sxf = (float)(sx - fx); r.syf = (float)(sy - fy);
+    r.vi00 = (float)(V11 / detV);
+    r.vi01 = (float)(-V01 / detV);
+    r.vi11 = (float)(V00 / detV);
+    r.norm = (float)(1.0 / (2.0 * UD_PI * sqrt(detV) * fabs(detA)));
+    rec[t] = r;
+}
+
+__device__ __forceinline__ float ewa_eval(const SplatRec& r, float px, float py, float& dx, float& dy)
+{
+    dx = (px - r.sxi) - r.sxf;
+    dy = (py - r.syi) - r.syf;
+    float e = r.vi00 * dx * dx + 2.0f * r.vi01 * dx * dy + r.vi11 * dy * dy;
+    return r.norm * __expf(-0.5f * e);
+}
+
+// splatting.forward: one thread per slot (splattingCuda.cu:18-66)
+__global__ void k_splat_forward(long nslots, int N, int W, int H, int k,
+                                const int32_t* __restrict__ idx, const SplatRec* __restrict__ rec,
+                                float* __restrict__ out)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nslots) return;
+    int id = idx[t];
+    if (id < 0) { out[t] = 0.0f; return; }
+    long pix = t / k;
+    long b = pix / ((long)W * H);
+    int r = (int)(pix - b * (long)W * H);
+    int i = r / H, j = r - (r / H) * H;
+    SplatRec s = rec[b * N + id];
+    float dx, dy;
+    float w = ewa_eval(s, (float)i, (float)j, dx, dy);
+    out[t] = fminf(w, 1.0f);
+}
+
+// splatting.backward, slot pass: accumulate per-splat moments
+//   m0 = sum g w, (m1x, m1y) = sum g w d, (m2xx, m2xy, m2yy) = sum g w d d^T
+// for slots whose unclamped weight is <= 1 (splattingCuda.cu:173).
+__global__ void k_splat_moments(long nslots, int N, int W, int H, int k,
+                                const int32_t* __restrict__ idx, const float* __restrict__ gin,
+                                const SplatRec* __restrict__ rec, float* __restrict__ mom)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nslots) return;
+    int id = idx[t];
+    if (id < 0) return;
+    long pix = t / k;
+    long b = pix / ((long)W * H);
+    int r = (int)(pix - b * (long)W * H);
+    int i = r / H, j = r - (r / H) * H;
+    long si = b * N + id;
+    SplatRec s = rec[si];
+    float dx, dy;
+    float w = ewa_eval(s, (float)i, (float)j, dx, dy);
+    if (!(w <= 1.0f)) return;
+    float gw = gin[t] * w;
+    if (gw == 0.0f) return;
+    float* m = mom + 8 * si;
+    atomicAdd(m + 0, gw);
+    atomicAdd(m + 1, gw * dx);
+    atomicAdd(m + 2, gw * dy);
+    atomicAdd(m + 3, gw * dx * dx);
+    atomicAdd(m + 4, gw * dx * dy);
+    atomicAdd(m + 5, gw * dy * dy);
+}
+
+// splatting.backward, splat pass: chain the moments through V(A, sigma),
+// A(p, n, s(p)) and s(p) in FP64 with forward-mode derivatives of A.
+__global__ void k_splat_finalize(int bn, int N, int W, int H, double f, double smoothing,
+                                 const float* __restrict__ pts, const float* __restrict__ nrm,
+                                 const float* __restrict__ sd, int sdBatch, const float* __restrict__ mom,
+                                 float* __restrict__ gPts, float* __restrict__ gNrm, float* __restrict__ gSd)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)bn * N) return;
+    long b = t / N, id = t % N;
+    long so = 2 * ((sdBatch > 1 ? b * N : 0) + id);
+    const float* m = mom + 8 * t;
+    double m0 = m[0], m1x = m[1], m1y = m[2], m2xx = m[3], m2xy = m[4], m2yy = m[5];
+    if (m0 == 0.0 && m1x == 0.0 && m1y == 0.0 && m2xx == 0.0 && m2xy == 0.0 && m2yy == 0.0) {
+        for (int c = 0; c < 3; c++) { gPts[3 * t + c] = 0.0f; gNrm[3 * t + c] = 0.0f; }
+        if (sdBatch > 1) { gSd[so] = 0.0f; gSd[so + 1] = 0.0f; }
+        return;
+    }
+    D6 p[3], n[3];
+    for (int c = 0; c < 3; c++) {
+        p[c] = mk(pts[3 * t + c]); p[c].d[c] = 1.0;
+        n[c] = mk(nrm[3 * t + c]); n[c].d[3 + c] = 1.0;
+    }
+    double s0 = sd[so], s1 = sd[so + 1];
+    D6 sx, sy, A00, A01, A11;
+    ewa_geom<D6>(p, n, W, H, f, sx, sy, A00, A01, A11);
+    double a00 = A00.v, a01 = A01.v, a11 = A11.v;
+    double detA = a00 * a11;
+    // J = A^-1 (upper triangular)
+    double J00 = a11 / detA, J01 = -a01 / detA, J10 = 0.0, J11 = a00 / detA;
+    double D0 = s0 * s0, D1 = s1 * s1;
+    double V00 = J00 * J00 * D0 + J01 * J01 * D1 + smoothing;
+    double V01 = J00 * J10 * D0 + J01 * J11 * D1;
+    double V11 = J10 * J10 * D0 + J11 * J11 * D1 + smoothing;
+    double detV = V00 * V11 - V01 * V01;
+    double I00 = V11 / detV, I01 = -V01 / detV, I11 = V00 / detV;
+    // G_s = Vinv m1
+    double Gsx = I00 * m1x + I01 * m1y, Gsy = I01 * m1x + I11 * m1y;
+    // G_V = 1/2 (Vinv M2 Vinv - m0 Vinv)
+    double P00 = I00 * m2xx + I01 * m2xy, P01 = I00 * m2xy + I01 * m2yy;
+    double P10 = I01 * m2xx + I11 * m2xy, P11 = I01 * m2xy + I11 * m2yy;
+    double G00 = 0.5 * ((P00 * I00 + P01 * I01) - m0 * I00);
+    double G01 = 0.5 * ((P00 * I01 + P01 * I11) - m0 * I01);
+    double G11 = 0.5 * ((P10 * I01 + P11 * I11) - m0 * I11);
+    double G10 = G01;
+    // dL/dsigma_k = 2 sigma_k J[:,k]^T G_V J[:,k]
+    double gs0 = 2.0 * s0 * (J00 * (G00 * J00 + G01 * J10) + J10 * (G10 * J00 + G11 * J10));
+    double gs1 = 2.0 * s1 * (J01 * (G00 * J01 + G01 * J11) + J11 * (G10 * J01 + G11 * J11));
+    // G_A = -2 J^T G_V J D J^T - m0 J^T
+    double JD00 = J00 * D0, JD01 = J01 * D1, JD10 = J10 * D0, JD11 = J11 * D1;
+    double C00 = JD00 * J00 + JD01 * J01, C01 = JD00 * J10 + JD01 * J11;   // C = J D J^T
+    double C10 = JD10 * J00 + JD11 * J01, C11 = JD10 * J10 + JD11 * J11;
+    double E00 = G00 * C00 + G01 * C10, E01 = G00 * C01 + G01 * C11;       // E = G_V C
+    double E10 = G10 * C00 + G11 * C10, E11 = G10 * C01 + G11 * C11;
+    // J^T E : (J^T)_{ij} = J_{ji}
+    double F00 = J00 * E00 + J10 * E10, F01 = J00 * E01 + J10 * E11;
+    double F10 = J01 * E00 + J11 * E10, F11 = J01 * E01 + J11 * E11;
+    double GA00 = -2.0 * F00 - m0 * J00;
+    double GA01 = -2.0 * F01 - m0 * J10;
+    double GA11 = -2.0 * F11 - m0 * J11;
+    (void)F10;
+    // dL/dp_c = sum GA : dA/dp_c + G_s . ds/dp_c ; dL/dn_c = sum GA : dA/dn_c
+    for (int c = 0; c < 3; c++) {
+        double gp = GA00 * A00.d[c] + GA01 * A01.d[c] + GA11 * A11.d[c] + Gsx * sx.d[c] + Gsy * sy.d[c];
+        double gn = GA00 * A00.d[3 + c] + GA01 * A01.d[3 + c] + GA11 * A11.d[3 + c];
+        gPts[3 * t + c] = (float)gp;
+        gNrm[3 * t + c] = (float)gn;
+    }
+    if (sdBatch > 1) { gSd[so] = (float)gs0; gSd[so + 1] = (float)gs1; }
+    else { atomicAdd(gSd + so, (float)gs0); atomicAdd(gSd + so + 1, (float)gs1); }
+}
+
+// sampling.covariance (samplingCuda.cu:323-361), FP64 internally
+__global__ void k_covariance(int bn, int N, int W, int H, double f, double smoothing,
+                             const float* __restrict__ pts, const float* __restrict__ nrm,
+                             const float* __restrict__ sd, int sdBatch, float* __restrict__ out)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)bn * N) return;
+    long b = t / N, id = t % N;
+    double p[3] = { pts[3 * t], pts[3 * t + 1], pts[3 * t + 2] };
+    double n[3] = { nrm[3 * t], nrm[3 * t + 1], nrm[3 * t + 2] };
+    long so = 2 * ((sdBatch > 1 ? b * N : 0) + id);
+    double s0 = sd[so], s1 = sd[so + 1];
+    double sx, sy, A00, A01, A11;
+    ewa_geom<double>(p, n, W, H, f, sx, sy, A00, A01, A11);
+    double detA = A00 * A11;
+    double J00 = A11 / detA, J01 = -A01 / detA, J11 = A00 / detA;
+    double V00 = J00 * J00 * s0 * s0 + J01 * J01 * s1 * s1 + smoothing;
+    double V01 = J01 * J11 * s1 * s1;
+    double V11 = J11 * J11 * s1 * s1 + smoothing;
+    out[4 * t] = (float)V00; out[4 * t + 1] = (float)V01;
+    out[4 * t + 2] = (float)V01; out[4 * t + 3] = (float)V11;
+}
+
+// ---------------------------------------------------------------------------
+// Compositing.  Depth of slot s along the pixel ray (alphacompositing.cuh:6-36),
+// FP32 in a fixed operation order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float ray_depth(int i, int j, float Wf, float Hf, float ff, const float* p, const float* n)
+{
+    float bx = (Wf - 2.0f * (float)i) / (2.0f * ff);
+    float by = -((Hf - 2.0f * (float)j) / (2.0f * ff));
+    float bz = 1.0f;
+    float nn = bx * bx;
+    nn = nn + by * by;
+    nn = nn + bz * bz;
+    float inv = 1.0f / sqrtf(nn);
+    float dir[3] = { -(bx * inv), -(by * inv), -(bz * inv) };
+    float num = dot3(p, n);
+    float den = dot3(dir, n);
+    float d = num / den;
+    return (d > 0.0f) ? d : 10000.0f;
+}
+
+// One wavefront per pixel (k <= 64): lane s owns slot s.  The stable
+// insertion argsort of the reference becomes a rank per lane:
+//   rank_s = #{d_j < d_s} + #{j < s : d_j == d_s}.
+template <bool BACKWARD>
+__global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, int H, int k, float ff,
+                                                     const int32_t* __restrict__ idx, const float* __restrict__ wts,
+                                                     const float* __restrict__ gin,
+                                                     const float* __restrict__ pts, const float* __restrict__ nrm, int pb,
+                                                     float* __restrict__ out)
+{
+    __shared__ float sbuf[4][64];
+    const int wv = threadIdx.x >> 6, s = threadIdx.x & 63;
+    long pix = (long)blockIdx.x * 4 + wv;
+    if (pix >= npix) return;   // whole wave exits together
+    long b = pix / ((long)W * H);
+    int r = (int)(pix - b * (long)W * H);
+    int i = r / H, j = r - (r / H) * H;
+    long base = pix * k;
+    const bool act = s < k;
+    int id = act ? idx[base + s] : -1;
+    float w = act ? wts[base + s] : 0.0f;
+    float d = 10000.0f;
+    if (id >= 0) {
+        long po = 3 * ((pb > 1 ? b * N : 0) + id);
+        float p[3] = { pts[po], pts[po + 1], pts[po + 2] };
+        float n[3] = { nrm[po], nrm[po + 1], nrm[po + 2] };
+        d = ray_depth(i, j, (float)W, (float)H, ff, p, n);
+    }
+    int rank = 0;
+    for (int q = 0; q < k; q++) {
+        float dq = __shfl(d, q, 64);
+        rank += (dq < d) || (dq == d && q < s);
+    }
+    // exclusive prefix product of (1 - w) in rank order
+    float* sb = sbuf[wv];
+    sb[s] = 1.0f;
+    __builtin_amdgcn_wave_barrier();
+    if (act) sb[rank] = 1.0f - w;
+    __builtin_amdgcn_wave_barrier();
+    float v = sb[s];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        float t = __shfl_up(v, o, 64);
+        if (s >= o) v *= t;
+    }
+    float excl = __shfl_up(v, 1, 64);
+    if (s == 0) excl = 1.0f;
+    float rem = __shfl(excl, rank, 64);
+    float blended = rem * w;
+    if (!BACKWARD) {
+        if (act) out[base + s] = blended;
+        return;
+    }
+    // backward (alphacompositingGrad.cuh:7-26):
+    //   g_s = gb_s / w_s [|w_s| > eps] - (sum_{rank > rank_s} gb) / (1 - w_s) [|1 - w_s| > eps]
+    float gb = act ? gin[base + s] * blended : 0.0f;
+    __builtin_amdgcn_wave_barrier();
+    sb[s] = 0.0f;
+    __builtin_amdgcn_wave_barrier();
+    if (act) sb[rank] = gb;
+    __builtin_amdgcn_wave_barrier();
+    // suffix sums: inclusive scan over reversed lanes
+    float x = sb[63 - s];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        float t = __shfl_up(x, o, 64);
+        if (s >= o) x += t;
+    }
+    // x at lane l = sum_{q >= 63 - l} sb[q]; suffix strictly after rank r is lane 63 - (r + 1)
+    int src = 62 - rank;
+    float after = __shfl(x, src < 0 ? 0 : src, 64);
+    if (rank == 63) after = 0.0f;
+    const float eps = 1e-4f;
+    float g = 0.0f;
+    if (fabsf(w) > eps) g += gb / w;
+    float den = -(1.0f - w);
+    if (fabsf(den) > eps) g += after / den;
+    if (act) out[base + s] = g;
+}
+
+// General k (64 < k <= 1024): one workgroup per pixel, ranks from LDS, serial scans.
+template <bool BACKWARD>
+__global__ __launch_bounds__(1024) void k_filter_block(long npix, int N, int W, int H, int k, float ff,
+                                                       const int32_t* __restrict__ idx, const float* __restrict__ wts,
+                                                       const float* __restrict__ gin,
+                                                       const float* __restrict__ pts, const float* __restrict__ nrm, int pb,
+                                                       float* __restrict__ out)
+{
+    __shared__ float sd_[1024], sw[1024], sx[1024];
+    __shared__ int srank[1024];
+    long pix = blockIdx.x;
+    long b = pix / ((long)W * H);
+    int r = (int)(pix - b * (long)W * H);
+    int i = r / H, j = r - (r / H) * H;
+    long base = pix * k;
+    int s = threadIdx.x;
+    bool act = s < k;
+    int id = act ? idx[base + s] : -1;
+    float w = act ? wts[base + s] : 0.0f;
+    float d = 10000.0f;
+    if (id >= 0) {
+        long po = 3 * ((pb > 1 ? b * N : 0) + id);
+        float p[3] = { pts[po], pts[po + 1], pts[po + 2] };
+        float n[3] = { nrm[po], nrm[po + 1], nrm[po + 2] };
+        d = ray_depth(i, j, (float)W, (float)H, ff, p, n);
+    }
+    if (act) { sd_[s] = d; sw[s] = w; }
+    __syncthreads();
+    int rank = 0;
+    if (act) {
+        for (int q = 0; q < k; q++) { float dq = sd_[q]; rank += (dq < d) || (dq == d && q < s); }
+        srank[rank] = s;
+    }
+    __syncthreads();
+    if (s == 0) {   // sx[rank] = exclusive product of (1 - w) in rank order
+        float rem = 1.0f;
+        for (int q = 0; q < k; q++) { sx[q] = rem; rem *= (1.0f - sw[srank[q]]); }
+    }
+    __syncthreads();
+    float blended = act ? sx[rank] * w : 0.0f;
+    if (!BACKWARD) { if (act) out[base + s] = blended; return; }
+    float gb = act ? gin[base + s] * blended : 0.0f;
+    __syncthreads();
+    if (act) sd_[rank] = gb;
+    __syncthreads();
+    if (s == 0) {   // sx[rank] = sum of gb strictly after rank
+        float acc = 0.0f;
+        for (int q = k - 1; q >= 0; q--) { sx[q] = acc; acc += sd_[q]; }
+    }
+    __syncthreads();
+    if (!act) return;
+    const float eps = 1e-4f;
+    float g = 0.0f;
+    if (fabsf(w) > eps) g += gb / w;
+    float den = -(1.0f - w);
+    if (fabsf(den) > eps) g += sx[rank] / den;
+    out[base + s] = g;
+}
+
+// ---------------------------------------------------------------------------
+// Transform (transformCuda.cu:11-117).  affine = 1 adds the translation column.
+// ---------------------------------------------------------------------------
+__global__ void k_transform_forward(int affine, int bn, int vb, int N, const float* __restrict__ M,
+                                    const float* __restrict__ v, float* __restrict__ out)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)bn * N) return;
+    long b = t / N, x = t % N;
+    const float* m = M + 16 * b;
+    long vo = 3 * ((vb > 1 ? b * N : 0) + x);
+    float a0 = v[vo], a1 = v[vo + 1], a2 = v[vo + 2];
+    for (int r = 0; r < 3; r++) {
+        float s = m[4 * r] * a0 + m[4 * r + 1] * a1 + m[4 * r + 2] * a2;
+        if (affine) s += m[4 * r + 3];
+        out[3 * t + r] = s;
+    }
+}
+
+// gradient: dM[r][c] = sum_n g_r v_c (+ g_r in column 3), dv = M33^T g.
+// The 12 matrix partial sums are reduced per workgroup, then one atomic each.
+__global__ __launch_bounds__(256) void k_transform_backward(int affine, int bn, int vb, int N, const float* __restrict__ g,
+                                                            const float* __restrict__ M, const float* __restrict__ v,
+                                                            float* __restrict__ gM, float* __restrict__ gV)
+{
+    __shared__ float red[4][12];
+    int b = blockIdx.y;
+    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float acc[12];
+    for (int q = 0; q < 12; q++) acc[q] = 0.0f;
+    if (x < N) {
+        const float* m = M + 16 * b;
+        long vo = 3 * ((vb > 1 ? (long)b * N : 0) + x);
+        long go = 3 * ((long)b * N + x);
+        float g0 = g[go], g1 = g[go + 1], g2 = g[go + 2];
+        float a[3] = { v[vo], v[vo + 1], v[vo + 2] };
+        float gg[3] = { g0, g1, g2 };
+        for (int r = 0; r < 3; r++) {
+            for (int c = 0; c < 3; c++) acc[4 * r + c] = gg[r] * a[c];
+            acc[4 * r + 3] = affine ? gg[r] : 0.0f;
+        }
+        float gv[3];
+        for (int c = 0; c < 3; c++) gv[c] = m[c] * g0 + m[4 + c] * g1 + m[8 + c] * g2;
+        if (vb > 1 || bn == 1) { for (int c = 0; c < 3; c++) gV[vo + c] = gv[c]; }
+        else { for (int c = 0; c < 3; c++) atomicAdd(gV + vo + c, gv[c]); }
+    }
+    for (int q = 0; q < 12; q++) acc[q] = wave_sum(acc[q]);
+    int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) for (int q = 0; q < 12; q++) red[wv][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        float s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        atomicAdd(gM + 16 * b + threadIdx.x, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation (interpolationCuda.cu:5-129).
+// ---------------------------------------------------------------------------
+__global__ void k_interp_forward(long npix, int W, int H, int k, int ab, int N, int C,
+                                 const int32_t* __restrict__ idx, const float* __restrict__ wts,
+                                 const float* __restrict__ attr, float* __restrict__ out)
+{
+    long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    long b = pix / ((long)W * H);
+    const long abase = ab > 1 ? b * N : 0;
+    if (C == 3) {
+        float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+        for (int s = 0; s < k; s++) {
+            int id = idx[pix * k + s];
+            if (id < 0) continue;
+            float w = wts[pix * k + s];
+            const float* a = attr + 3 * (abase + id);
+            o0 += w * a[0]; o1 += w * a[1]; o2 += w * a[2];
+        }
+        out[3 * pix] = o0; out[3 * pix + 1] = o1; out[3 * pix + 2] = o2;
+        return;
+    }
+    for (int c = 0; c < C; c++) out[(long)C * pix + c] = 0.0f;
+    for (int s = 0; s < k; s++) {
+        int id = idx[pix * k + s];
+        if (id < 0) continue;
+        float w = wts[pix * k + s];
+        const float* a = attr + (long)C * (abase + id);
+        for (int c = 0; c < C; c++) out[(long)C * pix + c] += w * a[c];
+    }
+}
+
+__global__ void k_interp_backward(long nslots, int W, int H, int k, int ab, int N, int C,
+                                  const float* __restrict__ g, const int32_t* __restrict__ idx,
+                                  const float* __restrict__ wts, const float* __restrict__ attr,
+                                  float* __restrict__ gW, float* __restrict__ gA)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nslots) return;
+    int id = idx[t];
+    if (id < 0) { gW[t] = 0.0f; return; }
+    long pix = t / k;
+    long b = pix / ((long)W * H);
+    long ao = (long)C * ((ab > 1 ? b * N : 0) + id);
+    float w = wts[t];
+    float acc = 0.0f;
+    for (int c = 0; c < C; c++) {
+        float gc = g[(long)C * pix + c];
+        acc += gc * attr[ao + c];
+        atomicAdd(gA + ao + c, gc * w);
+    }
+    gW[t] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Shading (shadingCuda.cu, diffuseShading*.cuh).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void sh_matrix(const float* c, float M[16])
+{
+    const float c1 = 0.429043f, c2 = 0.511664f, c3 = 0.743125f, c4 = 0.886227f, c5 = 0.247708f;
+    M[0] = c1 * c[8];  M[1] = c1 * c[4];  M[2] = c1 * c[7];  M[3] = c2 * c[3];
+    M[4] = c1 * c[4];  M[5] = -c1 * c[8]; M[6] = c1 * c[5];  M[7] = c2 * c[1];
+    M[8] = c1 * c[7];  M[9] = c1 * c[5];  M[10] = c3 * c[6]; M[11] = c2 * c[2];
+    M[12] = c2 * c[3]; M[13] = c2 * c[1]; M[14] = c2 * c[2]; M[15] = c4 * c[0] - c5 * c[6];
+}
+
+__device__ __forceinline__ float sh_n4(const float* n, float n4[4])
+{
+    float nn = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (nn > 1e-5f) { float inv = 1.0f / nn; n4[0] = n[0] * inv; n4[1] = n[1] * inv; n4[2] = n[2] * inv; n4[3] = 1.0f; }
+    else { n4[0] = n4[1] = n4[2] = 0.0f; n4[3] = 1.0f; }
+    return nn;
+}
+
+__global__ void k_sh_forward(long npix, int W, int H, int cb, const float* __restrict__ coef,
+                             const float* __restrict__ nrm, const float* __restrict__ diff, float* __restrict__ out)
+{
+    long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    long b = pix / ((long)W * H);
+    float n4[4];
+    sh_n4(nrm + 3 * pix, n4);
+    for (int ch = 0; ch < 3; ch++) {
+        float M[16];
+        sh_matrix(coef + 9 * (cb > 1 ? ch + 3 * b : ch), M);
+        float res = 0.0f;
+        for (int rr = 0; rr < 4; rr++) {
+            float y = n4[0] * M[rr] + n4[1] * M[4 + rr] + n4[2] * M[8 + rr] + n4[3] * M[12 + rr];
+            res += y * n4[rr];
+        }
+        out[3 * pix + ch] = diff[3 * pix + ch] * res / (float)UD_PI;
+    }
+}
+
+// backwardSh keeps the reference's per-component normal gradient
+// (shadingCuda.cu:283 pairs colour channel i with normal component i).
+__global__ __launch_bounds__(256) void k_sh_backward(long npix, int W, int H, int cb, const float* __restrict__ g,
+                                                     const float* __restrict__ coef, const float* __restrict__ nrm,
+                                                     const float* __restrict__ diff, float* __restrict__ gCoef,
+                                                     float* __restrict__ gN, float* __restrict__ gD)
+{
+    __shared__ float red[4][27];
+    const float c1 = 0.429043f, c2 = 0.511664f, c3 = 0.743125f, c4 = 0.886227f, c5 = 0.247708f;
+    long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // one workgroup never spans two views when W*H % 256 == 0; otherwise fall back to per-lane atomics
+    const long WH = (long)W * H;
+    long b0 = ((long)blockIdx.x * blockDim.x) / WH;
+    long bl = ((long)blockIdx.x * blockDim.x + blockDim.x - 1) / WH;
+    bool uniformView = (b0 == bl) || cb == 1;
+    float acc[27];
+    for (int q = 0; q < 27; q++) acc[q] = 0.0f;
+    long b = pix < npix ? pix / WH : b0;
+    if (pix < npix) {
+        const float* n = nrm + 3 * pix;
+        float n4[4];
+        float nn = sh_n4(n, n4);
+        float Gn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (nn > 1e-5f) {
+            float inv = 1.0f / nn, inv3 = inv * inv * inv;
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) Gn[3 * r + c] = (r == c ? inv : 0.0f) - n[r] * n[c] * inv3;
+        }
+        for (int ch = 0; ch < 3; ch++) {
+            float M[16];
+            sh_matrix(coef + 9 * (cb > 1 ? ch + 3 * b : ch), M);
+            float res = 0.0f, gN4[4];
+            for (int rr = 0; rr < 4; rr++) {
+                float y = n4[0] * M[rr] + n4[1] * M[4 + rr] + n4[2] * M[8 + rr] + n4[3] * M[12 + rr];
+                res += y * n4[rr];
+            }
+            for (int rr = 0; rr < 4; rr++) {
+                gN4[rr] = 0.0f;
+                for (int c = 0; c < 4; c++) gN4[rr] += (M[4 * rr + c] + M[4 * c + rr]) * n4[c];
+            }
+            float gnv = Gn[3 * ch] * gN4[0] + Gn[3 * ch + 1] * gN4[1] + Gn[3 * ch + 2] * gN4[2];
+            float gi = g[3 * pix + ch], di = diff[3 * pix + ch];
+            gD[3 * pix + ch] = gi * res / (float)UD_PI;
+            gN[3 * pix + ch] = gi * di * gnv / (float)UD_PI;
+            float sc = gi * di / (float)UD_PI;
+            float gM[16];
+            for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) gM[4 * r + c] = n4[r] * n4[c];
+            float gc[9];
+            gc[0] = c4 * gM[15];
+            gc[1] = c2 * (gM[7] + gM[13]);
+            gc[2] = c2 * (gM[14] + gM[11]);
+            gc[3] = c2 * (gM[12] + gM[3]);
+            gc[4] = c1 * (gM[4] + gM[1]);
+            gc[5] = c1 * (gM[9] + gM[6]);
+            gc[6] = c3 * gM[10] - c5 * gM[15];
+            gc[7] = c1 * (gM[8] + gM[2]);
+            gc[8] = c1 * (gM[0] - gM[5]);
+            for (int c = 0; c < 9; c++) acc[9 * ch + c] = sc * gc[c];
+        }
+    }
+    if (!uniformView) {
+        if (pix < npix)
+            for (int ch = 0; ch < 3; ch++)
+                for (int c = 0; c < 9; c++) atomicAdd(gCoef + 9 * (cb > 1 ? ch + 3 * b : ch) + c, acc[9 * ch + c]);
+        return;
+    }
+    for (int q = 0; q < 27; q++) acc[q] = wave_sum(acc[q]);
+    int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) for (int q = 0; q < 27; q++) red[wv][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < 27) {
+        int q = threadIdx.x, ch = q / 9, c = q % 9;
+        float s = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+        atomicAdd(gCoef + 9 * (cb > 1 ? ch + 3 * b0 : ch) + c, s);
+    }
+}
+
+__global__ void k_lambert_forward(long npix, int W, int H, int L, int ib, const float* __restrict__ dirs,
+                                  const float* __restrict__ ints, const float* __restrict__ nrm,
+                                  const float* __restrict__ diff, float* __restrict__ out)
+{
+    long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    long b = pix / ((long)W * H);
+    const float* n = nrm + 3 * pix;
+    const float* d = diff + 3 * pix;
+    float acc[3] = { 0.0f, 0.0f, 0.0f };
+    for (int l = 0; l < L; l++) {
+        const float* dir = dirs + 3 * (b * L + l);
+        const float* I = ints + 3 * ((ib > 1 ? b * L : 0) + l);
+        float c = dir[0] * n[0] + dir[1] * n[1] + dir[2] * n[2];
+        c = fmaxf(0.0f, c);
+        for (int ch = 0; ch < 3; ch++) acc[ch] += d[ch] / (float)UD_PI * I[ch] * c;
+    }
+    for (int ch = 0; ch < 3; ch++) out[3 * pix + ch] = acc[ch];
+}
+
+__global__ void k_lambert_backward(long npix, int W, int H, int L, int ib, const float* __restrict__ g,
+                                   const float* __restrict__ dirs, const float* __restrict__ ints,
+                                   const float* __restrict__ nrm, const float* __restrict__ diff,
+                                   float* __restrict__ gDir, float* __restrict__ gInt,
+                                   float* __restrict__ gN, float* __restrict__ gD)
+{
+    long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    long b = pix / ((long)W * H);
+    const float* n = nrm + 3 * pix;
+    const float* d = diff + 3 * pix;
+    const float* gi = g + 3 * pix;
+    float gn[3] = { 0, 0, 0 }, gd[3] = { 0, 0, 0 };
+    for (int l = 0; l < L; l++) {
+        const float* dir = dirs + 3 * (b * L + l);
+        long io = 3 * ((ib > 1 ? b * L : 0) + l);
+        const float* I = ints + io;
+        float c = dir[0] * n[0] + dir[1] * n[1] + dir[2] * n[2];
+        if (!(c > 0.0f)) continue;
+        float dco[3] = { d[0] * I[0], d[1] * I[1], d[2] * I[2] };
+        float gdc = gi[0] * dco[0] + gi[1] * dco[1] + gi[2] * dco[2];
+        float sc = c / (float)UD_PI;
+        for (int ch = 0; ch < 3; ch++) {
+            atomicAdd(gDir + 3 * (b * L + l) + ch, gdc * n[ch] / (float)UD_PI);
+            atomicAdd(gInt + io + ch, gi[ch] * d[ch] * sc);
+            gn[ch] += gdc * dir[ch] / (float)UD_PI;
+            gd[ch] += gi[ch] * I[ch] * sc;
+        }
+    }
+    for (int ch = 0; ch < 3; ch++) { gN[3 * pix + ch] = gn[ch]; gD[3 * pix + ch] = gd[ch]; }
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, long n, float v)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+inline unsigned grid1(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+inline int launch_status()
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+} // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+size_t ude_splat_workspace_bytes(int bn, int N)
+{
+    return (size_t)bn * N * (sizeof(SplatRec) + 8 * sizeof(float)) + 256;
+}
+
+int ude_splat_forward(int bn, int N, int W, int H, int k, double f, double smoothing,
+                      const int32_t* idx, const float* pts, const float* nrm, const float* sd, int sdBatch,
+                      float* out, void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
+    long nslots = (long)bn * W * H * k;
+    if (N == 0) {
+        k_fill_f32<<<grid1(nslots, 256), 256, 0, st>>>(out, nslots, 0.0f);
+        return launch_status();
+    }
+    if (wsBytes < ude_splat_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
+    SplatRec* rec = (SplatRec*)ws;
+    k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+    k_splat_forward<<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, rec, out);
+    return launch_status();
+}
+
+int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoothing,
+                       const float* gradIn, const int32_t* idx, const float* pts, const float* nrm,
+                       const float* sd, int sdBatch, float* gPts, float* gNrm, float* gSd,
+                       void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    if (wsBytes < ude_splat_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
+    long nslots = (long)bn * W * H * k;
+    SplatRec* rec = (SplatRec*)ws;
+    float* mom = (float*)(rec + (long)bn * N);
+    k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+    k_fill_f32<<<grid1(8L * bn * N, 256), 256, 0, st>>>(mom, 8L * bn * N, 0.0f);
+    k_splat_moments<<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, gradIn, rec, mom);
+    k_splat_finalize<<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
+                                                              mom, gPts, gNrm, gSd);
+    return launch_status();
+}
+
+int ude_covariance(int bn, int N, int W, int H, double f, double smoothing, const float* pts,
+                   const float* nrm, const float* sd, int sdBatch, float* out, hipStream_t st)
+{
+    if (bn <= 0 || N < 0) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    k_covariance<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, out);
+    return launch_status();
+}
+
+int ude_filter_forward(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
+                       const float* pts, const float* nrm, int pb, float* out, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || k > 1024) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    if (k <= 64)
+        k_filter_wave<false><<<grid1(npix, 4), 256, 0, st>>>(npix, N, W, H, k, (float)f, idx, wts, nullptr, pts, nrm, pb, out);
+    else
+        k_filter_block<false><<<(unsigned)npix, ((k + 63) / 64) * 64, 0, st>>>(npix, N, W, H, k, (float)f, idx, wts, nullptr, pts, nrm, pb, out);
+    return launch_status();
+}
+
+int ude_filter_backward(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
+                        const float* gin, const float* pts, const float* nrm, int pb, float* out, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || k > 1024) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    if (k <= 64)
+        k_filter_wave<true><<<grid1(npix, 4), 256, 0, st>>>(npix, N, W, H, k, (float)f, idx, wts, gin, pts, nrm, pb, out);
+    else
+        k_filter_block<true><<<(unsigned)npix, ((k + 63) / 64) * 64, 0, st>>>(npix, N, W, H, k, (float)f, idx, wts, gin, pts, nrm, pb, out);
+    return launch_status();
+}
+
+int ude_transform_forward(int affine, int bn, int vb, int N, const float* M, const float* v, float* out, hipStream_t st)
+{
+    if (bn <= 0 || N < 0 || (vb != 1 && vb != bn)) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    k_transform_forward<<<grid1((long)bn * N, 256), 256, 0, st>>>(affine, bn, vb, N, M, v, out);
+    return launch_status();
+}
+
+int ude_transform_backward(int affine, int bn, int vb, int N, const float* g, const float* M, const float* v,
+                           float* gM, float* gV, hipStream_t st)
+{
+    if (bn <= 0 || N < 0 || (vb != 1 && vb != bn)) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    dim3 grid(grid1(N, 256), bn);
+    k_transform_backward<<<grid, 256, 0, st>>>(affine, bn, vb, N, g, M, v, gM, gV);
+    return launch_status();
+}
+
+int ude_interp_forward(int bn, int W, int H, int k, int ab, int N, int C, const int32_t* idx, const float* wts,
+                       const float* attr, float* out, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_interp_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, k, ab, N, C, idx, wts, attr, out);
+    return launch_status();
+}
+
+int ude_interp_backward(int bn, int W, int H, int k, int ab, int N, int C, const float* g, const int32_t* idx,
+                        const float* wts, const float* attr, float* gW, float* gA, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0) return UDE_ERR_ARG;
+    long nslots = (long)bn * W * H * k;
+    k_interp_backward<<<grid1(nslots, 256), 256, 0, st>>>(nslots, W, H, k, ab, N, C, g, idx, wts, attr, gW, gA);
+    return launch_status();
+}
+
+int ude_sh_forward(int bn, int W, int H, int cb, const float* coef, const float* nrm, const float* diff,
+                   float* out, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_sh_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, cb, coef, nrm, diff, out);
+    return launch_status();
+}
+
+int ude_sh_backward(int bn, int W, int H, int cb, const float* g, const float* coef, const float* nrm,
+                    const float* diff, float* gCoef, float* gN, float* gD, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_sh_backward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, cb, g, coef, nrm, diff, gCoef, gN, gD);
+    return launch_status();
+}
+
+int ude_lambert_forward(int bn, int W, int H, int L, int ib, const float* dirs, const float* ints,
+                        const float* nrm, const float* diff, float* out, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || L < 0) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_lambert_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, L, ib, dirs, ints, nrm, diff, out);
+    return launch_status();
+}
+
+int ude_lambert_backward(int bn, int W, int H, int L, int ib, const float* g, const float* dirs, const float* ints,
+                         const float* nrm, const float* diff, float* gDir, float* gInt, float* gN, float* gD,
+                         hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || L < 0) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_lambert_backward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, L, ib, g, dirs, ints, nrm, diff, gDir, gInt, gN, gD);
+    return launch_status();
+}
+
+} // extern "C"

@@ -1,0 +1,934 @@
+// ud_sample.hip -- sampling.sample on gfx950: LBVH build, Hermite FGT and the
+// per-pixel sequential-Poisson sampler (reference: samplingCuda.cu:197-290,
+// buildDynamicRadixTree.cuh, evaluateDynamicRadixTree.cuh,
+// sequentialPoissonDynamicTree.cuh).
+//
+// Every stage runs on the device with sizes kept in device memory, so one view
+// costs no host synchronisation:
+//   1 k_bandwidth     per-splat isotropic h_k -> fixed-point sum (2^-40 units)
+//   2 k_view_params   h, query radius, screen bounds
+//   3 k_cull_*        order-preserving compaction of visible splats + Morton
+//   4 k_sort_*        4-pass LSD radix sort of 30-bit Morton keys (stable,
+//                     wavefront ballot ranking)
+//   5 k_hierarchy     Karras 2012 split on 64-bit keys (morton << 32 | pos)
+//   6 k_bottom_up     atomic fan-in: boxes, counts, centres, Hermite degree
+//   7 k_coef_*        coefficient slots (deg+1)^2, device-wide exclusive scan
+//   8 k_moments       Hermite moments per node over its contiguous leaf range,
+//                     accumulated in FP64 in leaf order
+//   9 k_sample        per pixel: FGT-guided capacity descent, sequential
+//                     Poisson keys u/p from Philox4x32-10, top-k in LDS
+// All FP32 arithmetic on the tree and in the sampler follows a fixed operation
+// order (no contraction) so that index sets are reproducible bit for bit.
+#include "ud_device.h"
+#include "../../include/udsplat.h"
+
+using namespace ud;
+
+namespace {
+
+constexpr int kMaxDegree = 16;
+constexpr int kMaxDepth = 64;        // descent trace bound (reference STACKSIZE)
+constexpr int kCullTile = 1024;      // elements per workgroup in the cull pass
+constexpr int kSortTile = 2048;      // elements per workgroup in a sort pass
+constexpr int kScanTile = 1024;      // elements per workgroup in a scan pass
+
+struct ViewState {
+    unsigned long long hsum;
+    unsigned int hcnt;
+    int nc;
+    float h, radius, lo, hix, hiy, s2h, prec;
+    int err;
+    int ncoef;
+    int pad[3];
+};
+
+struct NodeGeo {      // traversal record
+    float lox, loy, hix, hiy;
+    float cx, cy, r2;
+    int deg;
+};
+struct NodeLnk {
+    int left, right, parent, count;
+};
+struct NodeAux {
+    int off, first, last, pad;
+};
+
+struct TreeWs {
+    ViewState* vs;
+    uint32_t *keysA, *keysB;
+    int32_t *idsA, *idsB;
+    int *cullCounts, *sortHist, *scanTmp, *scanBlocks;
+    NodeGeo* geo;
+    NodeLnk* lnk;
+    NodeAux* aux;
+    int* flags;
+    float* coef;
+    long coefCap;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// carve the workspace; returns total bytes (ws may be null to only measure)
+size_t carve(void* ws, int N, TreeWs* t)
+{
+    long n = N > 0 ? N : 1;
+    long nn = 2 * n;
+    long nCull = (n + kCullTile - 1) / kCullTile;
+    long nSort = (n + kSortTile - 1) / kSortTile;
+    long nScan = (nn + kScanTile - 1) / kScanTile;
+    long coefCap = (long)(kMaxDegree + 1) * (kMaxDegree + 1) * (n - 1 > 0 ? n - 1 : 1) + n;
+    size_t off = 0;
+    char* base = (char*)ws;
+    auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return base ? (void*)(base + o) : nullptr; };
+    TreeWs w;
+    w.vs = (ViewState*)take(sizeof(ViewState));
+    w.keysA = (uint32_t*)take(4 * n);
+    w.keysB = (uint32_t*)take(4 * n);
+    w.idsA = (int32_t*)take(4 * n);
+    w.idsB = (int32_t*)take(4 * n);
+    w.cullCounts = (int*)take(4 * (nCull + 1));
+    w.sortHist = (int*)take(4 * 256 * (nSort + 1));
+    w.scanTmp = (int*)take(4 * nn);
+    w.scanBlocks = (int*)take(4 * (nScan + 1) > 4 * (256 * (nSort + 1)) ? 4 * (nScan + 1) : 4 * 256 * (nSort + 1));
+    w.geo = (NodeGeo*)take(sizeof(NodeGeo) * nn);
+    w.lnk = (NodeLnk*)take(sizeof(NodeLnk) * nn);
+    w.aux = (NodeAux*)take(sizeof(NodeAux) * nn);
+    w.flags = (int*)take(4 * n);
+    w.coef = (float*)take(4 * coefCap);
+    w.coefCap = coefCap;
+    if (t) *t = w;
+    return off;
+}
+
+// ---------------------------------------------------------------------------
+// Per-splat bandwidth (samplingCuda.cu:106-163) in FP32, fixed operation order.
+// ---------------------------------------------------------------------------
+__device__ float iso_h(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+{
+    float sx, sy;
+    screen_f32(Wf, Hf, ff, p, sx, sy);
+    float y0x = sx + 1.0f, y0y = sy, y1x = sx, y1y = sy + 1.0f;
+    float tf = 2.0f * ff;
+    float x0[3] = { (Wf - 2.0f * y0x) / tf, -((Hf - 2.0f * y0y) / tf), 1.0f };
+    float x1[3] = { (Wf - 2.0f * y1x) / tf, -((Hf - 2.0f * y1y) / tf), 1.0f };
+    float a = dot3(p, n);
+    float xt0 = a / dot3(x0, n), xt1 = a / dot3(x1, n);
+    float t0[3], t1[3];
+    for (int c = 0; c < 3; c++) { t0[c] = x0[c] * xt0 - p[c]; t1[c] = x1[c] * xt1 - p[c]; }
+    float nt0 = sqrtf(dot3(t0, t0));
+    float inv0 = 1.0f / nt0;
+    float u0[3] = { t0[0] * inv0, t0[1] * inv0, t0[2] * inv0 };
+    float u1[3] = { u0[1] * n[2] - u0[2] * n[1], u0[2] * n[0] - u0[0] * n[2], u0[0] * n[1] - u0[1] * n[0] };
+    float A00 = nt0, A01 = dot3(t1, u0), A10 = 0.0f, A11 = dot3(t1, u1);
+    float detA = A00 * A11 - A01 * A10;
+    float idA = 1.0f / detA;
+    float J00 = A11 * idA, J01 = -A01 * idA, J10 = -A10 * idA, J11 = A00 * idA;
+    float d0 = sd[0] * sd[0], d1 = sd[1] * sd[1];
+    float M00 = J00 * d0 + J01 * 0.0f, M01 = J00 * 0.0f + J01 * d1;
+    float M10 = J10 * d0 + J11 * 0.0f, M11 = J10 * 0.0f + J11 * d1;
+    float V00 = M00 * J00 + M01 * J01, V01 = M00 * J10 + M01 * J11;
+    float V10 = M10 * J00 + M11 * J01, V11 = M10 * J10 + M11 * J11;
+    V00 = V00 + smoothing; V01 = V01 + 0.0f; V10 = V10 + 0.0f; V11 = V11 + smoothing;
+    float dV = V00 * V11 - V01 * V10;
+    float idV = 1.0f / dV;
+    float I00 = V11 * idV, I01 = -V01 * idV, I10 = -V10 * idV, I11 = V00 * idV;
+    float tr = I00 + I11;
+    float det = I00 * I11 - I01 * I10;
+    float disc = 0.25f * (tr * tr) - det;
+    float sq = sqrtf(disc);
+    float l1 = 0.5f * tr + sq, l2 = 0.5f * tr - sq;
+    float lk = fminf(l1, l2);
+    return fminf(1.0f, 1.0f / sqrtf(lk));
+}
+
+__global__ void k_zero_state(ViewState* vs)
+{
+    if (threadIdx.x == 0) {
+        vs->hsum = 0; vs->hcnt = 0; vs->nc = 0; vs->err = 0; vs->ncoef = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bandwidth(int N, float Wf, float Hf, float ff, float smoothing,
+                                                   const float* __restrict__ pts, const float* __restrict__ nrm,
+                                                   const float* __restrict__ sd, ViewState* vs)
+{
+    __shared__ unsigned long long ssum[4];
+    __shared__ unsigned int scnt[4];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long q = 0;
+    unsigned int c = 0;
+    if (t < N) {
+        float hk = iso_h(Wf, Hf, ff, pts + 3L * t, nrm + 3L * t, sd + 2L * t, smoothing);
+        if (isfinite(hk)) { q = (unsigned long long)((double)hk * 1099511627776.0); c = 1; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        q += __shfl_xor(q, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ssum[wv] = q; scnt[wv] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = ssum[0] + ssum[1] + ssum[2] + ssum[3];
+        unsigned int cc = scnt[0] + scnt[1] + scnt[2] + scnt[3];
+        if (cc) { atomicAdd(&vs->hsum, s); atomicAdd(&vs->hcnt, cc); }
+    }
+}
+
+__global__ void k_view_params(ViewState* vs, float Wf, float Hf, double prec)
+{
+    if (threadIdx.x != 0) return;
+    double hd = ((double)vs->hsum * (1.0 / 1099511627776.0)) / (double)vs->hcnt;
+    float h = (float)hd;
+    float normalization = (float)(2.0 * UD_PI * (double)sqrtf(h));
+    float radius = (float)(sqrt(2.0) * (double)h * sqrt(-log(prec) - log((double)normalization)));
+    vs->h = h;
+    vs->radius = radius;
+    vs->lo = -radius;
+    vs->hix = Wf + radius;
+    vs->hiy = Hf + radius;
+    vs->s2h = sqrtf((2.0f * h) * h);
+    vs->prec = (float)prec;
+    vs->nc = 0;
+}
+
+__device__ __forceinline__ bool visible(const ViewState& v, float Wf, float Hf, float ff, const float* p, float& sx, float& sy)
+{
+    screen_f32(Wf, Hf, ff, p, sx, sy);
+    bool vis = (p[2] <= 0.0f);
+    vis &= (v.lo <= sx); vis &= (v.hix >= sx);
+    vis &= (v.lo <= sy); vis &= (v.hiy >= sy);
+    return vis;
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton2(float ux, float uy)
+{
+    float cx = fminf(fmaxf(ux * 1024.0f, 0.0f), 1023.0f);
+    float cy = fminf(fmaxf(uy * 1024.0f, 0.0f), 1023.0f);
+    return 2u * expand_bits((uint32_t)cx) + expand_bits((uint32_t)cy);
+}
+
+// exclusive scan of one int per thread over a 256-thread workgroup
+__device__ __forceinline__ int block_excl_scan(int v, int* sh /*[4]*/, int& total)
+{
+    int incl = wave_incl_scan(v);
+    int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) sh[wv] = incl;
+    __syncthreads();
+    int off = 0;
+    for (int q = 0; q < wv; q++) off += sh[q];
+    total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return off + incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_cull_count(int N, float Wf, float Hf, float ff,
+                                                    const float* __restrict__ pts, const ViewState* __restrict__ vs,
+                                                    int* __restrict__ counts)
+{
+    __shared__ int sh[4];
+    ViewState v = *vs;
+    int base = blockIdx.x * kCullTile;
+    int c = 0;
+    for (int r = 0; r < kCullTile / 256; r++) {
+        int e = base + r * 256 + threadIdx.x;
+        float sx, sy;
+        if (e < N && visible(v, Wf, Hf, ff, pts + 3L * e, sx, sy)) c++;
+    }
+    int total;
+    block_excl_scan(c, sh, total);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// single-workgroup exclusive scan of n ints in place; total to *totalOut
+__global__ __launch_bounds__(1024) void k_scan_small(int* data, int n, int* totalOut)
+{
+    __shared__ int sh[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        int e = base + threadIdx.x;
+        int v = e < n ? data[e] : 0;
+        int incl = wave_incl_scan(v);
+        int wv = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 63) sh[wv] = incl;
+        __syncthreads();
+        int off = 0;
+        for (int q = 0; q < wv; q++) off += sh[q];
+        int tot = 0;
+        for (int q = 0; q < 16; q++) tot += sh[q];
+        int c = carry;
+        if (e < n) data[e] = c + off + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *totalOut = carry;
+}
+
+__global__ __launch_bounds__(256) void k_cull_scatter(int N, float Wf, float Hf, float ff,
+                                                      const float* __restrict__ pts, const ViewState* __restrict__ vs,
+                                                      const int* __restrict__ offs, uint32_t* __restrict__ keys,
+                                                      int32_t* __restrict__ ids)
+{
+    __shared__ int sh[4];
+    ViewState v = *vs;
+    int base = blockIdx.x * kCullTile;
+    int run = offs[blockIdx.x];
+    for (int r = 0; r < kCullTile / 256; r++) {
+        int e = base + r * 256 + threadIdx.x;
+        float sx = 0.0f, sy = 0.0f;
+        bool vis = e < N && visible(v, Wf, Hf, ff, pts + 3L * e, sx, sy);
+        int total;
+        int pos = run + block_excl_scan(vis ? 1 : 0, sh, total);
+        if (vis) {
+            float ux = (sx - v.lo) / (v.hix - v.lo);
+            float uy = (sy - v.lo) / (v.hiy - v.lo);
+            keys[pos] = morton2(ux, uy);
+            ids[pos] = e;
+        }
+        run += total;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits.  Histogram -> digit-major scan -> stable scatter.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sort_hist(const ViewState* __restrict__ vs, const uint32_t* __restrict__ keys,
+                                                   int shift, int nblk, int* __restrict__ hist)
+{
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int n = vs->nc;
+    int base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortTile / 256; r++) {
+        int e = base + r * 256 + threadIdx.x;
+        if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(const ViewState* __restrict__ vs, const uint32_t* __restrict__ kin,
+                                                      const int32_t* __restrict__ iin, int shift, int nblk,
+                                                      const int* __restrict__ hist, uint32_t* __restrict__ kout,
+                                                      int32_t* __restrict__ iout)
+{
+    __shared__ int run[256];
+    __shared__ int wc[4][256];
+    int n = vs->nc;
+    int base = blockIdx.x * kSortTile;
+    run[threadIdx.x] = hist[threadIdx.x * nblk + blockIdx.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kSortTile / 256; r++) {
+        for (int q = 0; q < 4; q++) wc[q][threadIdx.x] = 0;
+        __syncthreads();
+        int e = base + r * 256 + threadIdx.x;
+        bool act = e < n;
+        uint32_t key = act ? kin[e] : 0u;
+        int32_t id = act ? iin[e] : 0;
+        uint32_t dg = (key >> shift) & 255u;
+        // lanes of this wave with the same digit
+        unsigned long long m = __ballot(act);
+        for (int b = 0; b < 8; b++) {
+            unsigned long long bb = __ballot((dg >> b) & 1u);
+            m &= ((dg >> b) & 1u) ? bb : ~bb;
+        }
+        int rk = __popcll(m & lt);
+        if (act && rk == 0) wc[wv][dg] = __popcll(m);
+        __syncthreads();
+        if (act) {
+            int off = run[dg];
+            for (int q = 0; q < wv; q++) off += wc[q][dg];
+            int pos = off + rk;
+            kout[pos] = key;
+            iout[pos] = id;
+        }
+        __syncthreads();
+        run[threadIdx.x] += wc[0][threadIdx.x] + wc[1][threadIdx.x] + wc[2][threadIdx.x] + wc[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Karras hierarchy on 64-bit unique keys (buildRadixTree.cuh:76-167).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t key64(const uint32_t* k, int i) { return ((uint64_t)k[i] << 32) | (uint32_t)i; }
+
+__device__ __forceinline__ int delta(const uint32_t* k, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    return __clzll(key64(k, i) ^ key64(k, j));
+}
+
+__device__ float trunc_r2(float h, float prec, int cnt)
+{
+    float v = -2.0f * (h * h);
+    v = v * plogf(prec / (float)cnt);
+    float R = sqrtf(v);
+    return R * R;
+}
+
+__global__ __launch_bounds__(256) void k_hierarchy(float Wf, float Hf, float ff, const float* __restrict__ pts,
+                                                   const ViewState* __restrict__ vs, const uint32_t* __restrict__ keys,
+                                                   const int32_t* __restrict__ ids, NodeGeo* __restrict__ geo,
+                                                   NodeLnk* __restrict__ lnk, NodeAux* __restrict__ aux)
+{
+    int n = vs->nc;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int L0 = n - 1;
+    float h = vs->h, prec = vs->prec;
+    // leaf t
+    {
+        int nd = L0 + t;
+        float sx, sy;
+        screen_f32(Wf, Hf, ff, pts + 3L * ids[t], sx, sy);
+        NodeGeo g;
+        g.lox = sx; g.loy = sy; g.hix = sx; g.hiy = sy; g.cx = sx; g.cy = sy;
+        g.r2 = trunc_r2(h, prec, 1);
+        g.deg = 0;
+        geo[nd] = g;
+        lnk[nd].left = -1; lnk[nd].right = -1; lnk[nd].count = 1;
+        NodeAux a; a.off = 0; a.first = t; a.last = t; a.pad = 0;
+        aux[nd] = a;
+    }
+    if (t == 0) lnk[0].parent = -1;
+    if (t >= n - 1) return;
+    // internal node t
+    int i0, j0;
+    if (t == 0) { i0 = 0; j0 = n - 1; }
+    else {
+        int dl = delta(keys, n, t, t - 1), dr = delta(keys, n, t, t + 1);
+        int d = dr > dl ? 1 : -1;
+        int dmin = dl < dr ? dl : dr;
+        int lmax = 2;
+        while (delta(keys, n, t, t + d * lmax) > dmin) lmax <<= 1;
+        int l = 0;
+        for (int s = lmax >> 1; s > 0; s >>= 1)
+            if (delta(keys, n, t, t + (l + s) * d) > dmin) l += s;
+        int jdx = t + l * d;
+        i0 = d < 0 ? jdx : t;
+        j0 = d < 0 ? t : jdx;
+    }
+    uint64_t fc = key64(keys, i0), lc = key64(keys, j0);
+    int sp;
+    if (fc == lc) sp = (i0 + j0) >> 1;
+    else {
+        int dn = __clzll(fc ^ lc);
+        sp = i0;
+        int stride = j0 - i0;
+        do {
+            stride = (stride + 1) >> 1;
+            int mid = sp + stride;
+            if (mid < j0 && __clzll(fc ^ key64(keys, mid)) > dn) sp = mid;
+        } while (stride > 1);
+    }
+    int lc_ = sp, rc_ = sp + 1;
+    if (i0 == sp) lc_ += L0;
+    if (j0 == sp + 1) rc_ += L0;
+    lnk[t].left = lc_; lnk[t].right = rc_;
+    lnk[lc_].parent = t; lnk[rc_].parent = t;
+    aux[t].first = i0; aux[t].last = j0;
+}
+
+// hermiteError (hermitefmm.cuh:184-195), integer powers by repeated products
+__device__ float hermite_error(float r, float nr, int p)
+{
+    if (r >= 1.0f) return r;
+    float om = 1.0f - r;
+    float a = nr / (om * om);
+    float rp = 1.0f;
+    for (int i = 0; i < p; i++) rp = rp * r;
+    float fac = 1.0f;
+    for (int i = 1; i <= p; i++) fac = fac * (float)i;
+    float b = rp / sqrtf(fac);
+    float c = b * b;
+    float d = 2.0f * (1.0f - rp) * b;
+    return a * (c + d);
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int* p, const ViewState* __restrict__ vs, int mode, int v)
+{
+    // mode 0: n = nc - 1 (fan-in flags)
+    int n = vs->nc - 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+// bottom-up pass: the second thread to reach a node merges its children
+// (buildDynamicRadixTree.cuh:126-223, with the children's own centres)
+__global__ __launch_bounds__(256) void k_bottom_up(const ViewState* __restrict__ vs, NodeGeo* geo, NodeLnk* lnk, int* flags)
+{
+    int n = vs->nc;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || n < 2) return;
+    float h = vs->h, prec = vs->prec;
+    int nd = (n - 1) + t;
+    int p = lnk[nd].parent;
+    while (p >= 0) {
+        __threadfence();
+        int old = atomicAdd(&flags[p], 1);
+        if (old == 0) return;
+        __threadfence();
+        NodeLnk L = lnk[p];
+        NodeGeo gl = geo[L.left], gr = geo[L.right];
+        int cl = __atomic_load_n(&lnk[L.left].count, __ATOMIC_RELAXED);
+        int cr = __atomic_load_n(&lnk[L.right].count, __ATOMIC_RELAXED);
+        NodeGeo g;
+        g.lox = fminf(gl.lox, gr.lox); g.loy = fminf(gl.loy, gr.loy);
+        g.hix = fmaxf(gl.hix, gr.hix); g.hiy = fmaxf(gl.hiy, gr.hiy);
+        int cn = cl + cr;
+        float num = (float)cl * gl.cx;
+        num = num + (float)cr * gr.cx;
+        g.cx = num / (float)cn;
+        num = (float)cl * gl.cy;
+        num = num + (float)cr * gr.cy;
+        g.cy = num / (float)cn;
+        float mn = 0.0f;
+        mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
+        mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
+        float rr = mn / h;
+        int deg;
+        if (gl.deg < 0 && gr.deg < 0) deg = -1;
+        else {
+            int d = gl.deg > gr.deg ? gl.deg : gr.deg;
+            while (d <= kMaxDegree && hermite_error(rr, (float)cn, d) > prec && rr < 1.0f) d++;
+            deg = (rr < 1.0f && d <= kMaxDegree) ? d : -1;
+        }
+        g.deg = deg;
+        g.r2 = trunc_r2(h, prec, cn);
+        geo[p] = g;
+        lnk[p].count = cn;
+        nd = p;
+        p = L.parent;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_coef_sizes(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo, int* sizes)
+{
+    int nn = 2 * vs->nc - 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn) return;
+    int d = geo[t].deg;
+    sizes[t] = d >= 0 ? (d + 1) * (d + 1) : 0;
+}
+
+// device-wide exclusive scan, phase 1: per-tile scan + tile totals
+__global__ __launch_bounds__(256) void k_scan_tiles(const ViewState* __restrict__ vs, int* data, int* tileSums)
+{
+    __shared__ int sh[4];
+    int nn = 2 * vs->nc - 1;
+    int base = blockIdx.x * kScanTile;
+    if (base >= nn) { if (threadIdx.x == 0) tileSums[blockIdx.x] = 0; return; }
+    int v[4], s = 0;
+    for (int q = 0; q < 4; q++) {
+        int e = base + 4 * threadIdx.x + q;
+        v[q] = e < nn ? data[e] : 0;
+        s += v[q];
+    }
+    int total;
+    int ex = block_excl_scan(s, sh, total);
+    for (int q = 0; q < 4; q++) {
+        int e = base + 4 * threadIdx.x + q;
+        if (e < nn) data[e] = ex;
+        ex += v[q];
+    }
+    if (threadIdx.x == 0) tileSums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const ViewState* __restrict__ vs, const int* data,
+                                                    const int* tileOffs, NodeAux* aux)
+{
+    int nn = 2 * vs->nc - 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn) return;
+    aux[t].off = data[t] + tileOffs[t / kScanTile];
+}
+
+// Hermite moments A[a][b] = sum_leaves tx^a ty^b / (a! b!), t = (x - c) / sqrt(2 h^2)
+// (hermitefmm.cuh:119-138), one thread per node, FP64 accumulation in leaf order.
+__global__ __launch_bounds__(64) void k_moments(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
+                                                const NodeAux* __restrict__ aux, float* __restrict__ coef, long coefCap)
+{
+    int n = vs->nc;
+    int nn = 2 * n - 1;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn) return;
+    if (vs->ncoef > coefCap) return;
+    NodeGeo g = geo[t];
+    int p = g.deg;
+    if (p < 0) return;
+    NodeAux a = aux[t];
+    if (p == 0 && t >= n - 1) { coef[a.off] = 1.0f; return; }
+    const int L0 = n - 1;
+    float s2h = vs->s2h;
+    double fact[kMaxDegree + 1];
+    fact[0] = 1.0;
+    for (int q = 1; q <= kMaxDegree; q++) fact[q] = fact[q - 1] * q;
+    const int P1 = p + 1;
+    for (int ax = 0; ax <= p; ax++) {
+        double acc[kMaxDegree + 1];
+        for (int b = 0; b <= kMaxDegree; b++) acc[b] = 0.0;
+        for (int pos = a.first; pos <= a.last; pos++) {
+            NodeGeo lf = geo[L0 + pos];
+            float tx = (lf.cx - g.cx) / s2h;
+            float ty = (lf.cy - g.cy) / s2h;
+            // reproduce px = tx^ax / ax! and py = ty^b / b! exactly as the oracle:
+            double px = 1.0;
+            for (int q = 0; q < ax; q++) px = px * (double)tx;
+            double mx = px / fact[ax];
+            double py = 1.0;
+            for (int b = 0; b <= p; b++) {
+                acc[b] += mx * (py / fact[b]);
+                py = py * (double)ty;
+            }
+        }
+        for (int b = 0; b <= p; b++) coef[a.off + ax * P1 + b] = (float)acc[b];
+    }
+}
+
+__global__ void k_check_coef(ViewState* vs, long coefCap)
+{
+    if (threadIdx.x == 0 && (long)vs->ncoef > coefCap) vs->err |= 2;
+}
+
+// ---------------------------------------------------------------------------
+// FGT evaluation (evaluateDynamicRadixTree.cuh:128-242, hermitefmm.cuh:35-47, 213-237)
+// ---------------------------------------------------------------------------
+struct TreeView {
+    const NodeGeo* geo;
+    const NodeLnk* lnk;
+    const NodeAux* aux;
+    const float* coef;
+    float s2h;
+};
+
+__device__ __forceinline__ float eval_node(const TreeView& T, const NodeGeo& g, int nd, float qx, float qy)
+{
+    int p = g.deg;
+    float tx = (qx - g.cx) / T.s2h;
+    float ty = (qy - g.cy) / T.s2h;
+    float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+    const float* c = T.coef + T.aux[nd].off;
+    if (p == 0) {
+        float w = 0.0f;
+        w = w + (c[0] * ex) * ey;
+        return w;
+    }
+    // Hermite functions h_n(t) = 2t h_{n-1} - 2(n-1) h_{n-2}, h_0 = e^{-t^2}
+    float hy[kMaxDegree + 1];
+    hy[0] = ey;
+    hy[1] = (2.0f * ty) * ey;
+    for (int n = 2; n <= p; n++) hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
+    float w = 0.0f;
+    float hprev = 0.0f, ha = ex;
+    for (int a = 0; a <= p; a++) {
+        if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
+        else if (a >= 2) {
+            float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+            hprev = ha; ha = hn;
+        }
+        const float* row = c + a * (p + 1);
+        for (int b = 0; b <= p; b++) w = w + (row[b] * ha) * hy[b];
+    }
+    return w;
+}
+
+__device__ __forceinline__ bool truncated(const NodeGeo& g, float qx, float qy)
+{
+    float cx = fminf(fmaxf(qx, g.lox), g.hix);
+    float cy = fminf(fmaxf(qy, g.loy), g.hiy);
+    float dx = qx - cx, dy = qy - cy;
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    return !(d2 < g.r2);
+}
+
+__device__ __noinline__ float tree_weight(const TreeView& T, int entry, float qx, float qy)
+{
+    NodeGeo g = T.geo[entry];
+    if (g.deg >= 0) return eval_node(T, g, entry, qx, qy);
+    int rootparent = T.lnk[entry].parent;
+    int nd = entry;
+    float w = 0.0f;
+    while (true) {
+        while (g.deg < 0 && !truncated(g, qx, qy)) { nd = T.lnk[nd].left; g = T.geo[nd]; }
+        if (g.deg >= 0) w = w + eval_node(T, g, nd, qx, qy);
+        int pr = T.lnk[nd].parent;
+        while (pr != rootparent && nd == T.lnk[pr].right) { nd = pr; pr = T.lnk[pr].parent; }
+        if (pr != rootparent) { nd = T.lnk[pr].right; g = T.geo[nd]; }
+        else break;   // climbed back to the entry node
+    }
+    return w;
+}
+
+// nextLeaf (sequentialPoissonDynamicTree.cuh:16-53)
+__device__ __forceinline__ int next_leaf(const TreeView& T, int L0, int sp, int idx)
+{
+    if (idx >= L0) {
+        while (true) {
+            int pr = T.lnk[idx].parent;
+            if (pr == sp) break;
+            if (idx == T.lnk[pr].left) { idx = T.lnk[pr].right; break; }
+            idx = pr;
+        }
+    }
+    while (idx < L0) idx = T.lnk[idx].left;
+    return idx;
+}
+
+__device__ __forceinline__ float key_of(unsigned long long v) { return __uint_as_float((uint32_t)(v >> 32)); }
+
+// Sampler: one lane per pixel, 8x8 pixel tile per wavefront.  Per pixel the
+// capacity dictionary and the children's FGT weights are kept in a trie of
+// the visited descent paths (private memory), so a launch re-uses every weight
+// an earlier launch computed on the same path prefix.  The top-k list
+// (ascending keys, ties keep insertion order) lives in LDS, slot-major.
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint32_t seedLo, uint32_t seedHi,
+                                               uint32_t view, const ViewState* __restrict__ vs, TreeView T0,
+                                               int32_t* __restrict__ out, int* __restrict__ errOut)
+{
+    TreeView T = T0;
+    T.s2h = vs->s2h;
+    constexpr int LMAX = (KMAX <= 16) ? 4 : (KMAX <= 36) ? 6 : (KMAX <= 64) ? 8 : (KMAX <= 144) ? 12 : 16;
+    constexpr int TMAX = LMAX * (kMaxDepth - 1) + 1;
+    extern __shared__ unsigned long long list[];
+    const int lane = threadIdx.x;
+    const int tilesY = (H + 7) >> 3, tilesX = (W + 7) >> 3;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int ti = tile / tilesY, tj = tile - (tile / tilesY) * tilesY;
+    (void)tilesX;
+    const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
+    const bool valid = (i < W) && (j < H);
+    const unsigned long long empty = ((unsigned long long)0x7f800000u << 32) | 0xffffffffull;
+    for (int s = 0; s < k; s++) list[s * 64 + lane] = empty;
+    const int n = vs->nc;
+    const float prec = vs->prec;
+    if (valid && n > 0 && vs->err == 0) {
+        const int L0 = n - 1;
+        const float qx = (float)i, qy = (float)j;
+        const uint32_t pixel = (uint32_t)(j + H * i);
+        float total = fmaxf(0.0f, tree_weight(T, 0, qx, qy));
+        if (total >= prec) {
+            int tNode[TMAX];
+            float tCap[TMAX], tWl[TMAX], tWr[TMAX];
+            int tKids[TMAX];            // (left entry + 1) | (right entry + 1) << 16
+            int pe[kMaxDepth];
+            float pp[kMaxDepth];
+            int nt = 1;
+            tNode[0] = 0; tCap[0] = 1.0f; tWl[0] = -1.0f; tWr[0] = 0.0f; tKids[0] = 0;
+            bool bad = false;
+            for (int la = 0; la < L && !bad; la++) {
+                int e = 0, nd = 0, depth = 0;
+                while (T.lnk[nd].count > k) {
+                    NodeLnk lk = T.lnk[nd];
+                    if (tWl[e] < 0.0f) {
+                        tWl[e] = fmaxf(0.0f, tree_weight(T, lk.left, qx, qy));
+                        tWr[e] = fmaxf(0.0f, tree_weight(T, lk.right, qx, qy));
+                    }
+                    float wl = tWl[e], wr = tWr[e];
+                    float sum = wl + wr;
+                    wl = wl / sum; wr = wr / sum;
+                    int kids = tKids[e];
+                    int el = (kids & 0xffff) - 1, er = (kids >> 16) - 1;
+                    float cl = el >= 0 ? tCap[el] : 1.0f;
+                    float cr = er >= 0 ? tCap[er] : 1.0f;
+                    float sl = wl * cl, sr = wr * cr;
+                    bool goRight = sl < sr;
+                    int ce = goRight ? er : el;
+                    int cnode = goRight ? lk.right : lk.left;
+                    if (ce < 0) {
+                        if (nt >= TMAX || nt >= 0xffff) { bad = true; break; }
+                        ce = nt++;
+                        tNode[ce] = cnode; tCap[ce] = 1.0f; tWl[ce] = -1.0f; tWr[ce] = 0.0f; tKids[ce] = 0;
+                        tKids[e] = goRight ? ((kids & 0xffff) | ((ce + 1) << 16)) : ((kids & ~0xffff) | (ce + 1));
+                    }
+                    depth++;
+                    if (depth >= kMaxDepth) { bad = true; break; }
+                    pe[depth] = ce;
+                    pp[depth] = goRight ? wr : wl;
+                    e = ce;
+                    nd = cnode;
+                }
+                if (bad) break;
+                float prob = 1.0f;
+                for (int d = depth; d >= 1; d--) {
+                    int ce = pe[d];
+                    tCap[ce] = fmaxf(0.0f, tCap[ce] - prob);
+                    prob = prob * pp[d];
+                }
+                // sequential Poisson keys over the leaves of the reached subtree
+                const int sp = T.lnk[nd].parent;
+                const int cnt = T.lnk[nd].count;
+                int lf = nd;
+                u32x4 rnd = { 0, 0, 0, 0 };
+                for (int kk = 0; kk < cnt; kk++) {
+                    lf = next_leaf(T, L0, sp, lf);
+                    NodeGeo lg = T.geo[lf];
+                    float p = fmaxf(0.0f, eval_node(T, lg, lf, qx, qy));
+                    if ((kk & 3) == 0) {
+                        u32x4 c = { (uint32_t)(kk >> 2), (uint32_t)la, pixel, view };
+                        rnd = philox4x32_10(c, seedLo, seedHi);
+                    }
+                    uint32_t x = ((kk & 3) == 0) ? rnd.x : ((kk & 3) == 1) ? rnd.y : ((kk & 3) == 2) ? rnd.z : rnd.w;
+                    float key = u01(x) / p;
+                    unsigned long long last = list[(k - 1) * 64 + lane];
+                    if (!(key < key_of(last))) continue;
+                    int id = T.aux[lf].pad;   // sorted position's splat id (stored in pad)
+                    unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) | (uint32_t)id;
+                    int pos = k - 1;
+                    while (pos > 0) {
+                        unsigned long long prev = list[(pos - 1) * 64 + lane];
+                        if (key < key_of(prev)) { list[pos * 64 + lane] = prev; pos--; }
+                        else break;
+                    }
+                    list[pos * 64 + lane] = ent;
+                }
+            }
+            if (bad) atomicOr(errOut, 1);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (valid) {
+        int32_t* o = out + ((long)j + (long)H * i) * k;
+        for (int s = 0; s < k; s++) o[s] = (int32_t)(uint32_t)(list[s * 64 + lane] & 0xffffffffull);
+    }
+}
+
+// leaf splat ids into NodeAux.pad (so the sampler reads one record per leaf)
+__global__ __launch_bounds__(256) void k_leaf_ids(const ViewState* __restrict__ vs, const int32_t* __restrict__ ids, NodeAux* aux)
+{
+    int n = vs->nc;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) aux[n - 1 + t].pad = ids[t];
+}
+
+__global__ void k_fill_idx(int32_t* p, long n)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = -1;
+}
+
+__global__ void k_copy_err(const ViewState* vs, int* err)
+{
+    if (threadIdx.x == 0 && vs->err) atomicOr(err, vs->err);
+}
+
+inline unsigned grid1(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int build_tree(int N, int W, int H, double f, double smoothing, double prec, const float* pts, const float* nrm,
+               const float* sd, const TreeWs& w, hipStream_t st)
+{
+    float Wf = (float)W, Hf = (float)H, ff = (float)f;
+    long nCull = (N + kCullTile - 1) / kCullTile;
+    long nSort = (N + kSortTile - 1) / kSortTile;
+    long nScan = (2L * N + kScanTile - 1) / kScanTile;
+    k_zero_state<<<1, 64, 0, st>>>(w.vs);
+    k_bandwidth<<<grid1(N, 256), 256, 0, st>>>(N, Wf, Hf, ff, (float)smoothing, pts, nrm, sd, w.vs);
+    k_view_params<<<1, 64, 0, st>>>(w.vs, Wf, Hf, prec);
+    k_cull_count<<<(unsigned)nCull, 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts);
+    k_scan_small<<<1, 1024, 0, st>>>(w.cullCounts, (int)nCull, &w.vs->nc);
+    k_cull_scatter<<<(unsigned)nCull, 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts, w.keysA, w.idsA);
+    uint32_t* kin = w.keysA; uint32_t* kout = w.keysB;
+    int32_t* iin = w.idsA; int32_t* iout = w.idsB;
+    for (int shift = 0; shift < 32; shift += 8) {
+        k_sort_hist<<<(unsigned)nSort, 256, 0, st>>>(w.vs, kin, shift, (int)nSort, w.sortHist);
+        k_scan_small<<<1, 1024, 0, st>>>(w.sortHist, (int)(256 * nSort), w.scanBlocks);
+        k_sort_scatter<<<(unsigned)nSort, 256, 0, st>>>(w.vs, kin, iin, shift, (int)nSort, w.sortHist, kout, iout);
+        uint32_t* tk = kin; kin = kout; kout = tk;
+        int32_t* ti = iin; iin = iout; iout = ti;
+    }
+    // after 4 passes the sorted data is back in keysA / idsA
+    k_hierarchy<<<grid1(N, 256), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux);
+    k_leaf_ids<<<grid1(N, 256), 256, 0, st>>>(w.vs, iin, w.aux);
+    k_fill_i32<<<grid1(N, 256), 256, 0, st>>>(w.flags, w.vs, 0, 0);
+    k_bottom_up<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.flags);
+    k_coef_sizes<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.geo, w.scanTmp);
+    k_scan_tiles<<<(unsigned)nScan, 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks);
+    k_scan_small<<<1, 1024, 0, st>>>(w.scanBlocks, (int)nScan, &w.vs->ncoef);
+    k_scan_apply<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks, w.aux);
+    k_check_coef<<<1, 64, 0, st>>>(w.vs, w.coefCap);
+    k_moments<<<grid1(2L * N, 64), 64, 0, st>>>(w.vs, w.geo, w.aux, w.coef, w.coefCap);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+int num_launches(int k)
+{
+    int L = 0;
+    while ((double)L < sqrt((double)k)) L++;
+    return L;
+}
+
+template <int KMAX>
+void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32_t view, const TreeWs& w,
+                   int32_t* out, int* err, hipStream_t st)
+{
+    TreeView T;
+    T.geo = w.geo; T.lnk = w.lnk; T.aux = w.aux; T.coef = w.coef;
+    T.s2h = 0.0f;   // the kernel reads s2h from the device-side view state
+    unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
+    size_t lds = (size_t)k * 64 * sizeof(unsigned long long);
+    hipLaunchKernelGGL((k_sample<KMAX>), dim3(tiles), dim3(64), lds, st, W, H, k, num_launches(k), seedLo, seedHi,
+                       view, (const ViewState*)w.vs, T, out, err);
+}
+
+} // namespace
+
+extern "C" {
+
+size_t ude_sample_workspace_bytes(int N, int W, int H, int k)
+{
+    (void)W; (void)H; (void)k;
+    return carve(nullptr, N, nullptr) + 256;
+}
+
+int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, double prec,
+               const float* pts, const float* nrm, const float* sd, int sdBatch,
+               uint64_t seed, int32_t* out, int* errFlag, void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
+    if (k > 256) return UDE_ERR_ARG;
+    long perView = (long)W * H * k;
+    if (N == 0) {
+        k_fill_idx<<<grid1(perView * bn, 256), 256, 0, st>>>(out, perView * bn);
+        return UDE_OK;
+    }
+    if (wsBytes < ude_sample_workspace_bytes(N, W, H, k)) return UDE_ERR_WORKSPACE;
+    TreeWs w;
+    carve(ws, N, &w);
+    uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
+    for (int b = 0; b < bn; b++) {
+        const float* pb = pts + 3L * N * b;
+        const float* nb = nrm + 3L * N * b;
+        const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
+        int rc = build_tree(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
+        if (rc != UDE_OK) return rc;
+        int32_t* ob = out + perView * b;
+        if (k <= 16) launch_sample<16>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
+        else if (k <= 32) launch_sample<32>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
+        else if (k <= 64) launch_sample<64>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
+        else if (k <= 128) launch_sample<128>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
+        else launch_sample<256>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
+        k_copy_err<<<1, 64, 0, st>>>(w.vs, errFlag);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+} // extern "C"

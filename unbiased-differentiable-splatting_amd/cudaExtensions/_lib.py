@@ -1,0 +1,102 @@
+"""Loader of the gfx950 splatting library (libudsplat.so, C ABI in include/udsplat.h).
+
+Every op of this package runs on the HIP library; there is no CPU or PyTorch
+fallback.  A missing library, a non-GPU tensor or a failing kernel launch
+raises.
+"""
+import ctypes
+import os
+
+import torch
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libudsplat.so")
+
+_lib = None
+
+UDE_OK = 0
+_ERRORS = {-1: "invalid argument", -2: "kernel launch failed", -3: "workspace too small"}
+
+c_int = ctypes.c_int
+c_dbl = ctypes.c_double
+c_ptr = ctypes.c_void_p
+c_size = ctypes.c_size_t
+
+_SIGS = {
+    "ude_sample_workspace_bytes": (c_size, [c_int] * 4),
+    "ude_sample": (c_int, [c_int] * 5 + [c_dbl] * 3 + [c_ptr] * 3 + [c_int, ctypes.c_uint64, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
+    "ude_covariance": (c_int, [c_int] * 4 + [c_dbl] * 2 + [c_ptr] * 3 + [c_int, c_ptr, c_ptr]),
+    "ude_splat_workspace_bytes": (c_size, [c_int] * 2),
+    "ude_splat_forward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 4 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
+    "ude_splat_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
+    "ude_filter_forward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
+    "ude_filter_backward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
+    "ude_transform_forward": (c_int, [c_int] * 4 + [c_ptr] * 4),
+    "ude_transform_backward": (c_int, [c_int] * 4 + [c_ptr] * 6),
+    "ude_interp_forward": (c_int, [c_int] * 7 + [c_ptr] * 5),
+    "ude_interp_backward": (c_int, [c_int] * 7 + [c_ptr] * 7),
+    "ude_sh_forward": (c_int, [c_int] * 4 + [c_ptr] * 5),
+    "ude_sh_backward": (c_int, [c_int] * 4 + [c_ptr] * 8),
+    "ude_lambert_forward": (c_int, [c_int] * 5 + [c_ptr] * 6),
+    "ude_lambert_backward": (c_int, [c_int] * 5 + [c_ptr] * 10),
+}
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libudsplat.so not found at %s; build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `make -C unbiased-differentiable-splatting_amd/csrc`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != UDE_OK:
+        raise RuntimeError("%s failed: %s (%d)" % (what, _ERRORS.get(rc, "error"), rc))
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
+
+
+def stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_gpu(*tensors, dtype=torch.float32):
+    dev = None
+    for t in tensors:
+        if not isinstance(t, torch.Tensor):
+            raise TypeError("expected a torch.Tensor")
+        if not t.is_cuda:
+            raise RuntimeError("splatting ops run on the GPU only: got a tensor on %s" % t.device)
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise RuntimeError("all tensors must be on the same device")
+    return dev
+
+
+def f32(t, name):
+    if t.dtype != torch.float32:
+        raise TypeError("%s must be float32 (got %s)" % (name, t.dtype))
+    return t.contiguous()
+
+
+def i32(t, name):
+    if t.dtype != torch.int32:
+        raise TypeError("%s must be int32 (got %s)" % (name, t.dtype))
+    return t.contiguous()
+
+
+def workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)

@@ -1,0 +1,72 @@
+"""sampling -- FGT-guided per-pixel sequential Poisson sampling.
+
+Mirrors FMMKdTreeSampling/sampling.cpp:48-183 (``sample``, ``covariance``).
+The reference seeds cuRAND with time(NULL); here the uniforms come from
+Philox4x32-10 keyed by (seed, view, pixel, launch, leaf ordinal).  ``seed``
+is an optional keyword; without it a per-process counter advances on every
+call (``manual_seed`` resets it), so consecutive frames draw fresh samples.
+"""
+import torch
+
+from . import _lib
+
+_seed_state = {"next": 0}
+last_error = None   # device int32 flag of the last sample() call (non-zero: descent too deep)
+
+
+def manual_seed(seed):
+    _seed_state["next"] = int(seed)
+
+
+def _next_seed():
+    s = _seed_state["next"]
+    _seed_state["next"] = s + 1
+    return s
+
+
+def sample(width, height, focallength, points, normals, colors, stdDevs, numSamples, smoothing,
+           precision=0.001, seed=None):
+    """Returns int32 indices (bn, width, height, numSamples); -1 marks an empty slot."""
+    global last_error
+    dev = _lib.require_gpu(points, normals, stdDevs)
+    points = _lib.f32(points, "points")
+    normals = _lib.f32(normals, "normals")
+    stdDevs = _lib.f32(stdDevs, "stdDevs")
+    bn, N, _ = points.shape
+    W, H, k = int(width), int(height), int(numSamples)
+    if k > 256:
+        raise ValueError("numSamples > 256 is not supported")
+    out = torch.empty((bn, W, H, k), dtype=torch.int32, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    L = _lib.lib()
+    wsb = L.ude_sample_workspace_bytes(N, W, H, k)
+    ws = _lib.workspace(wsb, dev)
+    s = _next_seed() if seed is None else int(seed)
+    rc = L.ude_sample(bn, N, W, H, k, float(focallength), float(smoothing), float(precision),
+                      _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs), stdDevs.shape[0],
+                      s & 0xFFFFFFFFFFFFFFFF, _lib.ptr(out), _lib.ptr(err), _lib.ptr(ws), ws.numel(),
+                      _lib.stream(dev))
+    _lib.check(rc, "sampling.sample")
+    last_error = err
+    return out
+
+
+def covariance(width, height, focallength, points, normals, stdDevs, smoothing):
+    """Per-splat screen-space covariance (bn, N, 2, 2) (samplingCuda.cu:323-361)."""
+    dev = _lib.require_gpu(points, normals, stdDevs)
+    points = _lib.f32(points, "points")
+    normals = _lib.f32(normals, "normals")
+    stdDevs = _lib.f32(stdDevs, "stdDevs")
+    bn, N, _ = points.shape
+    out = torch.zeros((bn, N, 2, 2), dtype=torch.float32, device=dev)
+    rc = _lib.lib().ude_covariance(bn, N, int(width), int(height), float(focallength), float(smoothing),
+                                   _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs), stdDevs.shape[0],
+                                   _lib.ptr(out), _lib.stream(dev))
+    _lib.check(rc, "sampling.covariance")
+    return out
+
+
+def check_errors():
+    """Synchronising check of the last sample() call's device error flag."""
+    if last_error is not None and int(last_error.item()) != 0:
+        raise RuntimeError("sampling.sample: descent exceeded the 63-level trace bound")

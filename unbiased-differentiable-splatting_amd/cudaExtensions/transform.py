@@ -1,0 +1,51 @@
+"""transform -- per-point 4x4 affine / linear transforms and their adjoints.
+
+Mirrors FMMKdTreeSampling/transform.cpp:34-94.
+"""
+import torch
+
+from . import _lib
+
+
+def _forward(affine, transformation, vector):
+    dev = _lib.require_gpu(transformation, vector)
+    M = _lib.f32(transformation, "transformation")
+    v = _lib.f32(vector, "vector")
+    bn = M.shape[0]
+    vb, N, _ = v.shape
+    out = torch.empty((bn, N, 3), dtype=torch.float32, device=dev)
+    rc = _lib.lib().ude_transform_forward(1 if affine else 0, bn, vb, N, _lib.ptr(M), _lib.ptr(v), _lib.ptr(out),
+                                          _lib.stream(dev))
+    _lib.check(rc, "transform.forward")
+    return out
+
+
+def _backward(affine, gradIn, transformation, vector):
+    dev = _lib.require_gpu(gradIn, transformation, vector)
+    g = _lib.f32(gradIn, "gradIn")
+    M = _lib.f32(transformation, "transformation")
+    v = _lib.f32(vector, "vector")
+    bn = M.shape[0]
+    vb, N, _ = v.shape
+    gM = torch.zeros(M.shape, dtype=torch.float32, device=dev)
+    gV = torch.zeros(v.shape, dtype=torch.float32, device=dev)
+    rc = _lib.lib().ude_transform_backward(1 if affine else 0, bn, vb, N, _lib.ptr(g), _lib.ptr(M), _lib.ptr(v),
+                                           _lib.ptr(gM), _lib.ptr(gV), _lib.stream(dev))
+    _lib.check(rc, "transform.backward")
+    return [gM, gV]
+
+
+def affineForward(transformation, vector):
+    return _forward(True, transformation, vector)
+
+
+def affineBackward(gradIn, transformation, vector):
+    return _backward(True, gradIn, transformation, vector)
+
+
+def linearForward(transformation, vector):
+    return _forward(False, transformation, vector)
+
+
+def linearBackward(gradIn, transformation, vector):
+    return _backward(False, gradIn, transformation, vector)
