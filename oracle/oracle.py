@@ -41,6 +41,9 @@ def lib():
         _lib.o_tree_export.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 10 + [ctypes.c_long, ctypes.c_void_p]
         _lib.o_tree_eval.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float] + [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
         _lib.o_sample_view.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3
+        _lib.o_sample_view_rows.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 +
+                                           [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_int] * 2 +
+                                           [ctypes.c_void_p])
     return _lib
 
 
@@ -286,6 +289,25 @@ def sample_view(W, H, f, pts, nrm, sd, k, smoothing=0.0, precision=0.01, seed=0,
     if rc != 0:
         raise RuntimeError("oracle sample_view failed: %d" % rc)
     return out
+
+
+def sample_view_rows(W, H, f, pts, nrm, sd, k, i0, i1, smoothing=0.0, precision=0.01, seed=0, view=0):
+    """sampling.sample restricted to pixel rows i0 <= x < i1; returns (indices, tree build seconds)."""
+    pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float32)
+    N = pts.shape[0]
+    out = np.zeros((W, H, k), np.int32)
+    h = ctypes.c_float(0.0)
+    nc = ctypes.c_int(0)
+    tt = ctypes.c_double(0.0)
+    rc = lib().o_sample_view_rows(ctypes.c_int(N), ctypes.c_int(W), ctypes.c_int(H), ctypes.c_int(k),
+                                  ctypes.c_double(f), ctypes.c_double(smoothing), ctypes.c_double(precision),
+                                  _p(pts), _p(nrm), _p(sd), ctypes.c_uint32(seed & 0xFFFFFFFF),
+                                  ctypes.c_uint32((seed >> 32) & 0xFFFFFFFF), ctypes.c_uint32(view), _p(out),
+                                  ctypes.byref(h), ctypes.byref(nc), ctypes.c_int(i0), ctypes.c_int(i1),
+                                  ctypes.byref(tt))
+    if rc != 0:
+        raise RuntimeError("oracle sample_view_rows failed: %d" % rc)
+    return out, tt.value
 
 
 def sample(W, H, f, pts, nrm, sd, k, smoothing=0.0, precision=0.01, seed=0):

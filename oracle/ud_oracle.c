@@ -60,6 +60,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #define O_PI 3.14159265358979323846
 
@@ -1114,10 +1115,26 @@ static int o_sample_pixel(const OTree* T, int i, int j, int H, int k, uint32_t s
 
 /* One view of sampling.sample.  pts/nrm (N,3) float32 for this view, sd (N,2).
  * out (W,H,k) int32.  Scratch tree arrays are allocated here. */
+int o_sample_view_rows(int N, int W, int H, int k, double f, double smoothing, double prec,
+                       const float* pts, const float* nrm, const float* sd,
+                       uint32_t seed_lo, uint32_t seed_hi, uint32_t view, int32_t* out,
+                       float* h_out, int* nc_out, int i0, int i1, double* t_tree);
+
 int o_sample_view(int N, int W, int H, int k, double f, double smoothing, double prec,
                   const float* pts, const float* nrm, const float* sd,
                   uint32_t seed_lo, uint32_t seed_hi, uint32_t view, int32_t* out,
                   float* h_out, int* nc_out)
+{
+    return o_sample_view_rows(N, W, H, k, f, smoothing, prec, pts, nrm, sd, seed_lo, seed_hi, view, out,
+                              h_out, nc_out, 0, W, NULL);
+}
+
+/* As o_sample_view, but only pixel rows i0 <= i < i1 are sampled (others get -1);
+ * *t_tree receives the tree-build wall time (used by the CPU baseline). */
+int o_sample_view_rows(int N, int W, int H, int k, double f, double smoothing, double prec,
+                       const float* pts, const float* nrm, const float* sd,
+                       uint32_t seed_lo, uint32_t seed_hi, uint32_t view, int32_t* out,
+                       float* h_out, int* nc_out, int i0, int i1, double* t_tree)
 {
     if (k > 4096) return -1;
     OTree T;
@@ -1133,7 +1150,9 @@ int o_sample_view(int N, int W, int H, int k, double f, double smoothing, double
     long cap = 290L * (N > 0 ? N : 1);
     T.coef = (float*)malloc(sizeof(float) * cap);
     int rc = 0;
+    double tb0 = omp_get_wtime();
     if (N > 0) rc = o_tree_build(N, W, H, f, smoothing, prec, pts, nrm, sd, &T, cap, NULL);
+    if (t_tree) *t_tree = omp_get_wtime() - tb0;
     if (h_out) *h_out = T.h;
     if (nc_out) *nc_out = T.nc;
     if (rc < 0) goto done;
@@ -1145,7 +1164,7 @@ int o_sample_view(int N, int W, int H, int k, double f, double smoothing, double
         for (long pix = 0; pix < (long)W * H; pix++) {
             int i = (int)(pix / H), j = (int)(pix % H);
             int32_t* o = out + (long)k * pix;
-            if (N == 0 || T.nc == 0) { for (int s = 0; s < k; s++) o[s] = -1; continue; }
+            if (N == 0 || T.nc == 0 || i < i0 || i >= i1) { for (int s = 0; s < k; s++) o[s] = -1; continue; }
             int r = o_sample_pixel(&T, i, j, H, k, seed_lo, seed_hi, view, o, cm);
             if (r < 0) {
                 #pragma omp critical

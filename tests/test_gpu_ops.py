@@ -1,0 +1,172 @@
+"""GPU parity of the per-slot operators against the FP64 oracle.
+
+Tolerance (FP32 kernels vs FP64 oracle): max |gpu - oracle| <= 1e-4 * max |oracle|
+per output tensor.  Inputs are seeded sphere scenes with std-devs scaled up so
+each splat covers several pixels, and slot tensors mixing nearby splats,
+random splats and empty (-1) slots.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import scene_inputs, near_slots, random_slots, assert_close
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def case():
+    sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)
+    rng = np.random.default_rng(7)
+    idx = near_slots(rng, p, sc.width, sc.height, sc.focal, 12)
+    return sc, p, n, nb, sd, idx
+
+
+def t(x, dev):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def test_splat_forward(dev, case):
+    from cudaExtensions import splatting
+    sc, p, n, nb, sd, idx = case
+    w = splatting.forward(sc.width, sc.height, sc.focal, t(idx, dev), t(p[None], dev), t(n[None], dev),
+                          t(sd[None], dev), 0.0)
+    ref = O.splat_forward(sc.width, sc.height, sc.focal, idx, p[None], n[None], sd[None])
+    assert (ref > 0).mean() > 0.05
+    assert_close(w.cpu().numpy(), ref, RTOL, "splat forward")
+    assert np.all(w.cpu().numpy()[idx < 0] == 0)
+
+
+@pytest.mark.parametrize("smoothing", [0.0, 0.5])
+def test_splat_backward(dev, case, smoothing):
+    from cudaExtensions import splatting
+    sc, p, n, nb, sd, idx = case
+    rng = np.random.default_rng(11)
+    g = rng.normal(size=idx.shape).astype(np.float32)
+    gp, gn, gs = splatting.backward(sc.width, sc.height, sc.focal, t(g, dev), t(idx, dev), t(p[None], dev),
+                                    t(n[None], dev), t(sd[None], dev), smoothing)
+    rp, rn, rs = O.splat_backward(sc.width, sc.height, sc.focal, g, idx, p[None], n[None], sd[None], smoothing)
+    assert_close(gp.cpu().numpy(), rp, RTOL, "grad points")
+    assert_close(gn.cpu().numpy(), rn, RTOL, "grad normals")
+    assert_close(gs.cpu().numpy(), rs, RTOL, "grad stddevs")
+
+
+def test_splat_batched_views_and_broadcast_sd(dev):
+    """bn = 2 views with shared (1, N, 2) std-devs: gradients of the std-devs sum over views."""
+    from cudaExtensions import splatting
+    sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)[:5]
+    rng = np.random.default_rng(3)
+    p2 = np.stack([p, p + np.float32(0.01)])
+    n2 = np.stack([n, n])
+    idx = np.concatenate([near_slots(rng, p, sc.width, sc.height, sc.focal, 6)] * 2)
+    g = rng.normal(size=idx.shape).astype(np.float32)
+    w = splatting.forward(sc.width, sc.height, sc.focal, t(idx, dev), t(p2, dev), t(n2, dev), t(sd[None], dev), 0.0)
+    ref = O.splat_forward(sc.width, sc.height, sc.focal, idx, p2, n2, sd[None])
+    assert_close(w.cpu().numpy(), ref, RTOL, "batched forward")
+    gp, gn, gs = splatting.backward(sc.width, sc.height, sc.focal, t(g, dev), t(idx, dev), t(p2, dev), t(n2, dev),
+                                    t(sd[None], dev), 0.0)
+    rp, rn, rs = O.splat_backward(sc.width, sc.height, sc.focal, g, idx, p2, n2, sd[None])
+    assert_close(gp.cpu().numpy(), rp, RTOL, "batched grad points")
+    assert_close(gs.cpu().numpy(), rs, RTOL, "batched grad stddevs")
+
+
+def test_covariance(dev, case):
+    from cudaExtensions import sampling
+    sc, p, n, nb, sd, idx = case
+    c = sampling.covariance(sc.width, sc.height, sc.focal, t(p[None], dev), t(n[None], dev), t(sd[None], dev), 0.1)
+    ref = O.covariance(sc.width, sc.height, sc.focal, p[None], n[None], sd[None], 0.1)
+    assert_close(c.cpu().numpy(), ref, RTOL, "covariance")
+
+
+@pytest.mark.parametrize("k", [12, 100])
+def test_filter_forward_backward(dev, k):
+    from cudaExtensions import filtering
+    sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)[:5]
+    rng = np.random.default_rng(5 + k)
+    idx = random_slots(rng, 1, sc.width, sc.height, k, p.shape[0])
+    w = rng.uniform(0.0, 1.0, size=idx.shape).astype(np.float32)
+    w[idx < 0] = 0.0
+    g = rng.normal(size=idx.shape).astype(np.float32)
+    out = filtering.forward(t(idx, dev), t(w, dev), sc.width, sc.height, sc.focal, t(p[None], dev), t(n[None], dev))
+    ref = O.filter_forward(idx, w, sc.width, sc.height, sc.focal, p[None], n[None])
+    assert_close(out.cpu().numpy(), ref, RTOL, "filter forward")
+    gw = filtering.backward(t(idx, dev), t(w, dev), t(g, dev), sc.width, sc.height, sc.focal, t(p[None], dev),
+                            t(n[None], dev))
+    rg = O.filter_backward(idx, w, g, sc.width, sc.height, sc.focal, p[None], n[None])
+    assert_close(gw.cpu().numpy(), rg, 5 * RTOL, "filter backward")
+
+
+@pytest.mark.parametrize("affine", [True, False])
+@pytest.mark.parametrize("vb", [1, 3])
+def test_transform(dev, affine, vb):
+    from cudaExtensions import transform
+    rng = np.random.default_rng(1)
+    bn, N = 3, 5000
+    M = rng.normal(size=(bn, 4, 4)).astype(np.float32)
+    v = rng.normal(size=(vb, N, 3)).astype(np.float32)
+    g = rng.normal(size=(bn, N, 3)).astype(np.float32)
+    fw = transform.affineForward if affine else transform.linearForward
+    bw = transform.affineBackward if affine else transform.linearBackward
+    out = fw(t(M, dev), t(v, dev))
+    assert_close(out.cpu().numpy(), O.transform_forward(affine, M, v), RTOL, "transform forward")
+    gM, gV = bw(t(g, dev), t(M, dev), t(v, dev))
+    rM, rV = O.transform_backward(affine, g, M, v)
+    assert_close(gM.cpu().numpy(), rM, RTOL, "transform grad M")
+    assert_close(gV.cpu().numpy(), rV, RTOL, "transform grad v")
+
+
+@pytest.mark.parametrize("C,ab", [(3, 1), (16, 2)])
+def test_interpolation(dev, C, ab):
+    from cudaExtensions import interpolation
+    rng = np.random.default_rng(C)
+    bn, W, H, k, N = 2, 24, 20, 9, 300
+    idx = random_slots(rng, bn, W, H, k, N)
+    w = rng.uniform(size=idx.shape).astype(np.float32)
+    a = rng.normal(size=(ab, N, C)).astype(np.float32)
+    g = rng.normal(size=(bn, W, H, C)).astype(np.float32)
+    out = interpolation.forward(t(idx, dev), t(w, dev), t(a, dev))
+    assert_close(out.cpu().numpy(), O.interp_forward(idx, w, a), RTOL, "interp forward")
+    gW, gA = interpolation.backward(t(g, dev), t(idx, dev), t(w, dev), t(a, dev))
+    rW, rA = O.interp_backward(g, idx, w, a)
+    assert_close(gW.cpu().numpy(), rW, RTOL, "interp grad w")
+    assert_close(gA.cpu().numpy(), rA, RTOL, "interp grad attr")
+
+
+@pytest.mark.parametrize("cb", [1, 2])
+def test_sh_shading(dev, cb):
+    from cudaExtensions import shading
+    rng = np.random.default_rng(20 + cb)
+    bn, W, H = 2, 16, 20
+    coef = rng.normal(size=(cb, 3, 9)).astype(np.float32)
+    nrm = rng.normal(size=(bn, W, H, 3)).astype(np.float32)
+    nrm[0, 0, 0] = 0.0
+    diff = rng.uniform(size=(bn, W, H, 3)).astype(np.float32)
+    g = rng.normal(size=(bn, W, H, 3)).astype(np.float32)
+    out = shading.forwardSh(t(coef, dev), t(nrm, dev), t(diff, dev))
+    assert_close(out.cpu().numpy(), O.sh_forward(coef, nrm, diff), RTOL, "sh forward")
+    gc, gn, gd = shading.backwardSh(t(g, dev), t(coef, dev), t(nrm, dev), t(diff, dev))
+    rc, rn, rd = O.sh_backward(g, coef, nrm, diff)
+    assert_close(gc.cpu().numpy(), rc, RTOL, "sh grad coef")
+    assert_close(gn.cpu().numpy(), rn, RTOL, "sh grad normals")
+    assert_close(gd.cpu().numpy(), rd, RTOL, "sh grad diffuse")
+
+
+@pytest.mark.parametrize("ib", [1, 2])
+def test_lambert_shading(dev, ib):
+    from cudaExtensions import shading
+    rng = np.random.default_rng(30 + ib)
+    bn, W, H, L = 2, 16, 12, 3
+    dirs = rng.normal(size=(bn, L, 3)).astype(np.float32)
+    ints = rng.uniform(size=(ib, L, 3)).astype(np.float32)
+    nrm = rng.normal(size=(bn, W, H, 3)).astype(np.float32)
+    diff = rng.uniform(size=(bn, W, H, 3)).astype(np.float32)
+    g = rng.normal(size=(bn, W, H, 3)).astype(np.float32)
+    out = shading.forward(t(dirs, dev), t(ints, dev), t(nrm, dev), t(diff, dev))
+    assert_close(out.cpu().numpy(), O.lambert_forward(dirs, ints, nrm, diff), RTOL, "lambert forward")
+    gdir, gint, gn, gd = shading.backward(t(g, dev), t(dirs, dev), t(ints, dev), t(nrm, dev), t(diff, dev))
+    r = O.lambert_backward(g, dirs, ints, nrm, diff)
+    for a, b, nm in zip((gdir, gint, gn, gd), r, ("dir", "int", "normal", "diffuse")):
+        assert_close(a.cpu().numpy(), b, RTOL, "lambert grad " + nm)

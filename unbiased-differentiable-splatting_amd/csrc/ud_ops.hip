@@ -85,9 +85,12 @@ __device__ inline void ewa_geom(const T* p, const T* n, double W, double H, doub
 
 struct SplatRec {
     float sxi, syi, sxf, syf;   // screen point split into integer part and fraction
-    float vi00, vi01, vi11;     // inverse screen covariance
+    float t00, t01, t11;        // upper-triangular T with T^T T = V^-1 (computed in FP64)
     float norm;                 // 1 / (2 pi sqrt(det V) |det A|)
 };
+// With u = t00 dx + t01 dy, v = t11 dy the exponent is u^2 + v^2: rounding T to
+// FP32 perturbs it by O(eps sqrt(e lambda_max) |d|), while rounding V^-1 itself
+// would cost O(eps cond(V) e) for nearly edge-on splats.
 
 // Per (view, splat) FP64 precompute of the screen Gaussian (ewaSplatting.cuh:24-67).
 __global__ void k_splat_prepare(int bn, int N, int W, int H, double f, double smoothing,
@@ -113,18 +116,20 @@ __global__ void k_splat_prepare(int bn, int N, int W, int H, double f, double sm
     double fx = floor(sx), fy = floor(sy);
     r.sxi = (float)fx; r.syi = (float)fy;
     r.sxf = (float)(sx - fx); r.syf = (float)(sy - fy);
-    r.vi00 = (float)(V11 / detV);
-    r.vi01 = (float)(-V01 / detV);
-    r.vi11 = (float)(V00 / detV);
+    r.t00 = (float)sqrt(V11 / detV);
+    r.t01 = (float)(-V01 / sqrt(V11 * detV));
+    r.t11 = (float)(1.0 / sqrt(V11));
     r.norm = (float)(1.0 / (2.0 * UD_PI * sqrt(detV) * fabs(detA)));
     rec[t] = r;
 }
 
-__device__ __forceinline__ float ewa_eval(const SplatRec& r, float px, float py, float& dx, float& dy)
+__device__ __forceinline__ float ewa_eval(const SplatRec& r, float px, float py, float& u, float& v)
 {
-    dx = (px - r.sxi) - r.sxf;
-    dy = (py - r.syi) - r.syf;
-    float e = r.vi00 * dx * dx + 2.0f * r.vi01 * dx * dy + r.vi11 * dy * dy;
+    float dx = (px - r.sxi) - r.sxf;
+    float dy = (py - r.syi) - r.syf;
+    u = r.t00 * dx + r.t01 * dy;
+    v = r.t11 * dy;
+    float e = u * u + v * v;
     return r.norm * __expf(-0.5f * e);
 }
 
@@ -147,8 +152,9 @@ __global__ void k_splat_forward(long nslots, int N, int W, int H, int k,
     out[t] = fminf(w, 1.0f);
 }
 
-// splatting.backward, slot pass: accumulate per-splat moments
-//   m0 = sum g w, (m1x, m1y) = sum g w d, (m2xx, m2xy, m2yy) = sum g w d d^T
+// splatting.backward, slot pass: accumulate per-splat moments in the whitened
+// coordinates z = T d = (u, v):
+//   m0 = sum g w, (m1u, m1v) = sum g w z, (m2uu, m2uv, m2vv) = sum g w z z^T
 // for slots whose unclamped weight is <= 1 (splattingCuda.cu:173).
 __global__ void k_splat_moments(long nslots, int N, int W, int H, int k,
                                 const int32_t* __restrict__ idx, const float* __restrict__ gin,
@@ -173,16 +179,17 @@ __global__ void k_splat_moments(long nslots, int N, int W, int H, int k,
     atomicAdd(m + 0, gw);
     atomicAdd(m + 1, gw * dx);
     atomicAdd(m + 2, gw * dy);
-    atomicAdd(m + 3, gw * dx * dx);
-    atomicAdd(m + 4, gw * dx * dy);
-    atomicAdd(m + 5, gw * dy * dy);
+    atomicAdd(m + 3, (gw * dx) * dx);
+    atomicAdd(m + 4, (gw * dx) * dy);
+    atomicAdd(m + 5, (gw * dy) * dy);
 }
 
 // splatting.backward, splat pass: chain the moments through V(A, sigma),
 // A(p, n, s(p)) and s(p) in FP64 with forward-mode derivatives of A.
 __global__ void k_splat_finalize(int bn, int N, int W, int H, double f, double smoothing,
                                  const float* __restrict__ pts, const float* __restrict__ nrm,
-                                 const float* __restrict__ sd, int sdBatch, const float* __restrict__ mom,
+                                 const float* __restrict__ sd, int sdBatch, const SplatRec* __restrict__ rec,
+                                 const float* __restrict__ mom,
                                  float* __restrict__ gPts, float* __restrict__ gNrm, float* __restrict__ gSd)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -213,15 +220,23 @@ __global__ void k_splat_finalize(int bn, int N, int W, int H, double f, double s
     double V01 = J00 * J10 * D0 + J01 * J11 * D1;
     double V11 = J10 * J10 * D0 + J11 * J11 * D1 + smoothing;
     double detV = V00 * V11 - V01 * V01;
-    double I00 = V11 / detV, I01 = -V01 / detV, I11 = V00 / detV;
-    // G_s = Vinv m1
-    double Gsx = I00 * m1x + I01 * m1y, Gsy = I01 * m1x + I11 * m1y;
-    // G_V = 1/2 (Vinv M2 Vinv - m0 Vinv)
-    double P00 = I00 * m2xx + I01 * m2xy, P01 = I00 * m2xy + I01 * m2yy;
-    double P10 = I01 * m2xx + I11 * m2xy, P11 = I01 * m2xy + I11 * m2yy;
-    double G00 = 0.5 * ((P00 * I00 + P01 * I01) - m0 * I00);
-    double G01 = 0.5 * ((P00 * I01 + P01 * I11) - m0 * I01);
-    double G11 = 0.5 * ((P10 * I01 + P11 * I11) - m0 * I11);
+    (void)detV;
+    // T (as rounded in the slot pass) maps d to the whitened z = T d
+    const SplatRec& R = rec[t];
+    double T00 = R.t00, T01 = R.t01, T11 = R.t11;
+    // G_s = V^-1 sum g w d = T^T (sum g w z)
+    double Gsx = T00 * m1x;
+    double Gsy = T01 * m1x + T11 * m1y;
+    // G_V = 1/2 (V^-1 S2 V^-1 - m0 V^-1) = 1/2 T^T (S2' - m0 I) T
+    double Q00 = 0.5 * (m2xx - m0), Q01 = 0.5 * m2xy, Q11 = 0.5 * (m2yy - m0);
+    // R = Q T
+    double R00 = Q00 * T00, R01 = Q00 * T01 + Q01 * T11;
+    double R10 = Q01 * T00, R11 = Q01 * T01 + Q11 * T11;
+    // G = T^T R
+    double G00 = T00 * R00;
+    double G01 = T00 * R01;
+    double G11 = T01 * R01 + T11 * R11;
+    (void)R10;
     double G10 = G01;
     // dL/dsigma_k = 2 sigma_k J[:,k]^T G_V J[:,k]
     double gs0 = 2.0 * s0 * (J00 * (G00 * J00 + G01 * J10) + J10 * (G10 * J00 + G11 * J10));
@@ -770,7 +785,7 @@ int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoo
     k_fill_f32<<<grid1(8L * bn * N, 256), 256, 0, st>>>(mom, 8L * bn * N, 0.0f);
     k_splat_moments<<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, gradIn, rec, mom);
     k_splat_finalize<<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
-                                                              mom, gPts, gNrm, gSd);
+                                                              rec, mom, gPts, gNrm, gSd);
     return launch_status();
 }
 
