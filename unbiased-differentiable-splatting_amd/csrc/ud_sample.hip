@@ -693,11 +693,22 @@ __device__ __forceinline__ int next_leaf(const TreeView& T, int L0, int sp, int 
 
 __device__ __forceinline__ float key_of(unsigned long long v) { return __uint_as_float((uint32_t)(v >> 32)); }
 
-// Sampler: one lane per pixel, 8x8 pixel tile per wavefront.  Per pixel the
-// capacity dictionary and the children's FGT weights are kept in a trie of
-// the visited descent paths (private memory), so a launch re-uses every weight
-// an earlier launch computed on the same path prefix.  The top-k list
-// (ascending keys, ties keep insertion order) lives in LDS, slot-major.
+// Sampler: one lane per pixel, 8x8 pixel tile per wavefront.
+//
+// Every FGT weight the descents need is a sequential sum over a subset of the
+// frontier of the root's walk (nodes with degree >= 0, or truncated ones that
+// contribute 0): the walk from any node n stops exactly where the root's walk
+// stops inside n's subtree, because the stop rule depends only on the node and
+// the pixel.  So one depth-first pass over the root's walk, adding each
+// frontier value to the accumulators of its ancestors in traversal order,
+// produces W(n) for every node on it bit for bit as the reference's
+// per-call walks (evaluateDynamicRadixTree.cuh:128-174) would.  The pass
+// records, for each walked node with more than k sources, the weights of its
+// two children in a trie of descent states; the ceil(sqrt(k)) descents then
+// read weights and capacities from the trie and evaluate the few remaining
+// weights (below clusters that were not walked) on demand.  The top-k list of
+// (key, splat) pairs (ascending keys, ties keep insertion order) lives in LDS,
+// slot-major.
 template <int KMAX>
 __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint32_t seedLo, uint32_t seedHi,
                                                uint32_t view, const ViewState* __restrict__ vs, TreeView T0,
@@ -706,13 +717,13 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
     TreeView T = T0;
     T.s2h = vs->s2h;
     constexpr int LMAX = (KMAX <= 16) ? 4 : (KMAX <= 36) ? 6 : (KMAX <= 64) ? 8 : (KMAX <= 144) ? 12 : 16;
-    constexpr int TMAX = LMAX * (kMaxDepth - 1) + 1;
+    constexpr int DFSMAX = 192;                       // trie entries the walk may create
+    constexpr int TMAX = DFSMAX + LMAX * (kMaxDepth - 1);
     extern __shared__ unsigned long long list[];
     const int lane = threadIdx.x;
-    const int tilesY = (H + 7) >> 3, tilesX = (W + 7) >> 3;
+    const int tilesY = (H + 7) >> 3;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int ti = tile / tilesY, tj = tile - (tile / tilesY) * tilesY;
-    (void)tilesX;
     const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
     const bool valid = (i < W) && (j < H);
     const unsigned long long empty = ((unsigned long long)0x7f800000u << 32) | 0xffffffffull;
@@ -723,25 +734,82 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
         const int L0 = n - 1;
         const float qx = (float)i, qy = (float)j;
         const uint32_t pixel = (uint32_t)(j + H * i);
-        float total = fmaxf(0.0f, tree_weight(T, 0, qx, qy));
-        if (total >= prec) {
-            int tNode[TMAX];
-            float tCap[TMAX], tWl[TMAX], tWr[TMAX];
-            int tKids[TMAX];            // (left entry + 1) | (right entry + 1) << 16
+        float tCap[TMAX], tWl[TMAX], tWr[TMAX];
+        int tKids[TMAX];               // (left entry + 1) | (right entry + 1) << 16
+        int nt = 1;
+        const float unset = __builtin_nanf("");   // weight not computed yet
+        tCap[0] = 1.0f; tWl[0] = unset; tWr[0] = unset; tKids[0] = 0;   // entry 0: the root
+        bool bad = false;
+        float total = 0.0f;
+        // ---- one depth-first pass over the root's FGT walk ----
+        {
+            float pacc[kMaxDepth];
+            int pent[kMaxDepth], pright[kMaxDepth], pcount[kMaxDepth];
+            uint64_t right = 0;        // bit d: node at depth d is its parent's right child
+            int d = 0, nd = 0;
+            while (true) {
+                NodeGeo g = T.geo[nd];
+                float wv = 0.0f;
+                bool walk = false;
+                if (g.deg >= 0) {
+                    wv = eval_node(T, g, nd, qx, qy);
+                    // ancestors whose weight a descent can compare: depth 0 and every
+                    // node whose parent has more than k sources
+                    for (int a = 0; a < d; a++) {
+                        if (a > 0 && pcount[a - 1] <= k) break;
+                        pacc[a] = pacc[a] + wv;
+                    }
+                } else if (!truncated(g, qx, qy)) {
+                    walk = true;
+                }
+                if (walk) {
+                    if (d >= kMaxDepth - 1) { bad = true; break; }
+                    NodeLnk lk = T.lnk[nd];
+                    pacc[d] = 0.0f;
+                    pright[d] = lk.right;
+                    pcount[d] = lk.count;
+                    int e = -1;
+                    if (d == 0) e = 0;
+                    else if (lk.count > k && pent[d - 1] >= 0 && nt < DFSMAX) {
+                        e = nt++;
+                        tCap[e] = 1.0f; tWl[e] = unset; tWr[e] = unset; tKids[e] = 0;
+                        int pe = pent[d - 1];
+                        if ((right >> d) & 1ull) tKids[pe] = (tKids[pe] & 0xffff) | ((e + 1) << 16);
+                        else tKids[pe] = (tKids[pe] & ~0xffff) | (e + 1);
+                    }
+                    pent[d] = e;
+                    d++;
+                    nd = lk.left;
+                    right &= ~(1ull << d);
+                    continue;
+                }
+                // node at depth d is complete with weight wv; report and advance
+                bool done = false;
+                while (true) {
+                    if (d == 0) { total = wv; done = true; break; }
+                    int pe = pent[d - 1];
+                    bool isRight = (right >> d) & 1ull;
+                    if (pe >= 0) { if (isRight) tWr[pe] = wv; else tWl[pe] = wv; }
+                    if (!isRight) { right |= (1ull << d); nd = pright[d - 1]; break; }
+                    d--;
+                    wv = pacc[d];
+                }
+                if (done) break;
+            }
+        }
+        total = fmaxf(0.0f, total);
+        if (!bad && total >= prec) {
             int pe[kMaxDepth];
             float pp[kMaxDepth];
-            int nt = 1;
-            tNode[0] = 0; tCap[0] = 1.0f; tWl[0] = -1.0f; tWr[0] = 0.0f; tKids[0] = 0;
-            bool bad = false;
             for (int la = 0; la < L && !bad; la++) {
                 int e = 0, nd = 0, depth = 0;
                 while (T.lnk[nd].count > k) {
                     NodeLnk lk = T.lnk[nd];
-                    if (tWl[e] < 0.0f) {
-                        tWl[e] = fmaxf(0.0f, tree_weight(T, lk.left, qx, qy));
-                        tWr[e] = fmaxf(0.0f, tree_weight(T, lk.right, qx, qy));
+                    if (tWl[e] != tWl[e]) {
+                        tWl[e] = tree_weight(T, lk.left, qx, qy);
+                        tWr[e] = tree_weight(T, lk.right, qx, qy);
                     }
-                    float wl = tWl[e], wr = tWr[e];
+                    float wl = fmaxf(0.0f, tWl[e]), wr = fmaxf(0.0f, tWr[e]);
                     float sum = wl + wr;
                     wl = wl / sum; wr = wr / sum;
                     int kids = tKids[e];
@@ -753,9 +821,9 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                     int ce = goRight ? er : el;
                     int cnode = goRight ? lk.right : lk.left;
                     if (ce < 0) {
-                        if (nt >= TMAX || nt >= 0xffff) { bad = true; break; }
+                        if (nt >= TMAX) { bad = true; break; }
                         ce = nt++;
-                        tNode[ce] = cnode; tCap[ce] = 1.0f; tWl[ce] = -1.0f; tWr[ce] = 0.0f; tKids[ce] = 0;
+                        tCap[ce] = 1.0f; tWl[ce] = unset; tWr[ce] = unset; tKids[ce] = 0;
                         tKids[e] = goRight ? ((kids & 0xffff) | ((ce + 1) << 16)) : ((kids & ~0xffff) | (ce + 1));
                     }
                     depth++;
@@ -767,10 +835,10 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                 }
                 if (bad) break;
                 float prob = 1.0f;
-                for (int d = depth; d >= 1; d--) {
-                    int ce = pe[d];
+                for (int dd = depth; dd >= 1; dd--) {
+                    int ce = pe[dd];
                     tCap[ce] = fmaxf(0.0f, tCap[ce] - prob);
-                    prob = prob * pp[d];
+                    prob = prob * pp[dd];
                 }
                 // sequential Poisson keys over the leaves of the reached subtree
                 const int sp = T.lnk[nd].parent;
@@ -789,7 +857,7 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                     float key = u01(x) / p;
                     unsigned long long last = list[(k - 1) * 64 + lane];
                     if (!(key < key_of(last))) continue;
-                    int id = T.aux[lf].pad;   // sorted position's splat id (stored in pad)
+                    int id = T.aux[lf].pad;   // splat id of the leaf
                     unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) | (uint32_t)id;
                     int pos = k - 1;
                     while (pos > 0) {
@@ -800,8 +868,8 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                     list[pos * 64 + lane] = ent;
                 }
             }
-            if (bad) atomicOr(errOut, 1);
         }
+        if (bad) atomicOr(errOut, 1);
     }
     __builtin_amdgcn_wave_barrier();
     if (valid) {
