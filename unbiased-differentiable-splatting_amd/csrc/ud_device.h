@@ -9,6 +9,10 @@
 
 #define UD_PI 3.14159265358979323846
 
+// pointer into device (global) memory: lets the compiler emit global_load
+// instead of flat_load for data reached through structs and non-inlined calls
+#define UD_GLOBAL __attribute__((address_space(1)))
+
 namespace ud {
 
 constexpr int kWave = 64;

@@ -39,7 +39,9 @@ struct ViewState {
     float h, radius, lo, hix, hiy, s2h, prec;
     int err;
     int ncoef;
-    int pad[3];
+    int nseeds;       // block-spanning fan-in starts (bottom-up, second level)
+    int nmom;         // internal nodes with a Hermite expansion
+    int pad;
 };
 
 struct NodeGeo {      // traversal record
@@ -63,14 +65,20 @@ struct TreeWs {
     NodeLnk* lnk;
     NodeAux* aux;
     int* flags;
+    int* seeds;
+    int* momList;
+    float* trie;
+    float2* leafXY;       // sorted position -> screen position of the leaf
     float* coef;
     long coefCap;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+int trie_entries(int k);
+
 // carve the workspace; returns total bytes (ws may be null to only measure)
-size_t carve(void* ws, int N, TreeWs* t)
+size_t carve(void* ws, int N, int W, int H, int k, TreeWs* t)
 {
     long n = N > 0 ? N : 1;
     long nn = 2 * n;
@@ -95,6 +103,11 @@ size_t carve(void* ws, int N, TreeWs* t)
     w.lnk = (NodeLnk*)take(sizeof(NodeLnk) * nn);
     w.aux = (NodeAux*)take(sizeof(NodeAux) * nn);
     w.flags = (int*)take(4 * n);
+    w.seeds = (int*)take(4 * n);
+    w.momList = (int*)take(4 * n);
+    long lanes = (long)((W + 7) / 8) * ((H + 7) / 8) * 64;
+    w.trie = (float*)take(16L * trie_entries(k) * lanes);
+    w.leafXY = (float2*)take(8 * n);
     w.coef = (float*)take(4 * coefCap);
     w.coefCap = coefCap;
     if (t) *t = w;
@@ -145,7 +158,7 @@ __device__ float iso_h(float Wf, float Hf, float ff, const float* p, const float
 __global__ void k_zero_state(ViewState* vs)
 {
     if (threadIdx.x == 0) {
-        vs->hsum = 0; vs->hcnt = 0; vs->nc = 0; vs->err = 0; vs->ncoef = 0;
+        vs->hsum = 0; vs->hcnt = 0; vs->nc = 0; vs->err = 0; vs->ncoef = 0; vs->nseeds = 0; vs->nmom = 0;
     }
 }
 
@@ -321,16 +334,49 @@ __global__ __launch_bounds__(256) void k_sort_hist(const ViewState* __restrict__
     hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
+// exclusive scan of hist[d][0..nblk) for each digit d (one workgroup per digit);
+// digit totals to tot[d]
+__global__ __launch_bounds__(256) void k_sort_scan_digits(int* hist, int nblk, int* tot)
+{
+    __shared__ int sh[4];
+    __shared__ int carry;
+    int* h = hist + (long)blockIdx.x * nblk;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nblk; base += 256) {
+        int e = base + threadIdx.x;
+        int v = e < nblk ? h[e] : 0;
+        int total;
+        int ex = block_excl_scan(v, sh, total);
+        int c = carry;
+        if (e < nblk) h[e] = c + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+// digit bases: exclusive scan of the 256 digit totals (in place)
+__global__ __launch_bounds__(256) void k_sort_scan_top(int* tot)
+{
+    __shared__ int sh[4];
+    int total;
+    int v = tot[threadIdx.x];
+    int ex = block_excl_scan(v, sh, total);
+    tot[threadIdx.x] = ex;
+}
+
 __global__ __launch_bounds__(256) void k_sort_scatter(const ViewState* __restrict__ vs, const uint32_t* __restrict__ kin,
                                                       const int32_t* __restrict__ iin, int shift, int nblk,
-                                                      const int* __restrict__ hist, uint32_t* __restrict__ kout,
-                                                      int32_t* __restrict__ iout)
+                                                      const int* __restrict__ hist, const int* __restrict__ digitBase,
+                                                      uint32_t* __restrict__ kout, int32_t* __restrict__ iout)
 {
     __shared__ int run[256];
     __shared__ int wc[4][256];
     int n = vs->nc;
     int base = blockIdx.x * kSortTile;
-    run[threadIdx.x] = hist[threadIdx.x * nblk + blockIdx.x];
+    run[threadIdx.x] = hist[threadIdx.x * nblk + blockIdx.x] + digitBase[threadIdx.x];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int r = 0; r < kSortTile / 256; r++) {
@@ -385,7 +431,8 @@ __device__ float trunc_r2(float h, float prec, int cnt)
 __global__ __launch_bounds__(256) void k_hierarchy(float Wf, float Hf, float ff, const float* __restrict__ pts,
                                                    const ViewState* __restrict__ vs, const uint32_t* __restrict__ keys,
                                                    const int32_t* __restrict__ ids, NodeGeo* __restrict__ geo,
-                                                   NodeLnk* __restrict__ lnk, NodeAux* __restrict__ aux)
+                                                   NodeLnk* __restrict__ lnk, NodeAux* __restrict__ aux,
+                                                   float2* __restrict__ leafXY)
 {
     int n = vs->nc;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -402,6 +449,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(float Wf, float Hf, float ff,
         g.r2 = trunc_r2(h, prec, 1);
         g.deg = 0;
         geo[nd] = g;
+        leafXY[t] = make_float2(sx, sy);
         lnk[nd].left = -1; lnk[nd].right = -1; lnk[nd].count = 1;
         NodeAux a; a.off = 0; a.first = t; a.last = t; a.pad = 0;
         aux[nd] = a;
@@ -469,62 +517,103 @@ __global__ __launch_bounds__(256) void k_fill_i32(int* p, const ViewState* __res
     if (t < n) p[t] = v;
 }
 
-// bottom-up pass: the second thread to reach a node merges its children
-// (buildDynamicRadixTree.cuh:126-223, with the children's own centres)
-__global__ __launch_bounds__(256) void k_bottom_up(const ViewState* __restrict__ vs, NodeGeo* geo, NodeLnk* lnk, int* flags)
+// Merge the two (complete) children of internal node p
+// (buildDynamicRadixTree.cuh:126-223, with the children's own centres).
+__device__ __forceinline__ void merge_node(int p, float h, float prec, NodeGeo* geo, NodeLnk* lnk)
+{
+    NodeLnk L = lnk[p];
+    NodeGeo gl = geo[L.left], gr = geo[L.right];
+    int cl = lnk[L.left].count, cr = lnk[L.right].count;
+    NodeGeo g;
+    g.lox = fminf(gl.lox, gr.lox); g.loy = fminf(gl.loy, gr.loy);
+    g.hix = fmaxf(gl.hix, gr.hix); g.hiy = fmaxf(gl.hiy, gr.hiy);
+    int cn = cl + cr;
+    float num = (float)cl * gl.cx;
+    num = num + (float)cr * gr.cx;
+    g.cx = num / (float)cn;
+    num = (float)cl * gl.cy;
+    num = num + (float)cr * gr.cy;
+    g.cy = num / (float)cn;
+    float mn = 0.0f;
+    mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
+    mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
+    float rr = mn / h;
+    int deg;
+    if (gl.deg < 0 && gr.deg < 0) deg = -1;
+    else {
+        int d = gl.deg > gr.deg ? gl.deg : gr.deg;
+        while (d <= kMaxDegree && hermite_error(rr, (float)cn, d) > prec && rr < 1.0f) d++;
+        deg = (rr < 1.0f && d <= kMaxDegree) ? d : -1;
+    }
+    g.deg = deg;
+    g.r2 = trunc_r2(h, prec, cn);
+    geo[p] = g;
+    lnk[p].count = cn;
+}
+
+// Bottom-up pass, level 1: one workgroup per block of 1024 sorted leaves.  The
+// second thread to reach a node whose leaf range lies inside the block merges
+// it; both arrivals come from the same workgroup, so workgroup-scope fences
+// order the exchange.  A thread that reaches a node spanning the block's
+// boundary hands its (complete) node to level 2.
+constexpr int kFanBlock = 1024;
+__global__ __launch_bounds__(kFanBlock) void k_bottom_up_local(ViewState* vs, NodeGeo* geo, NodeLnk* lnk,
+                                                               const NodeAux* __restrict__ aux, int* flags, int* seeds)
 {
     int n = vs->nc;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n || n < 2) return;
+    int b0 = blockIdx.x * kFanBlock;
+    int pos = b0 + threadIdx.x;
+    if (pos >= n || n < 2) return;
     float h = vs->h, prec = vs->prec;
-    int nd = (n - 1) + t;
+    int nd = (n - 1) + pos;
+    int p = lnk[nd].parent;
+    while (p >= 0) {
+        NodeAux a = aux[p];
+        if (a.first < b0 || a.last >= b0 + kFanBlock) {
+            int q = atomicAdd(&vs->nseeds, 1);
+            seeds[q] = nd;
+            return;
+        }
+        __threadfence_block();
+        int old = atomicAdd(&flags[p], 1);
+        if (old == 0) return;
+        __threadfence_block();
+        merge_node(p, h, prec, geo, lnk);
+        nd = p;
+        p = lnk[p].parent;
+    }
+}
+
+// Bottom-up pass, level 2: the few block-spanning nodes, device-scope fan-in.
+__global__ __launch_bounds__(256) void k_bottom_up_global(const ViewState* __restrict__ vs, NodeGeo* geo, NodeLnk* lnk,
+                                                          int* flags, const int* __restrict__ seeds)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= vs->nseeds) return;
+    float h = vs->h, prec = vs->prec;
+    int nd = seeds[t];
     int p = lnk[nd].parent;
     while (p >= 0) {
         __threadfence();
         int old = atomicAdd(&flags[p], 1);
         if (old == 0) return;
         __threadfence();
-        NodeLnk L = lnk[p];
-        NodeGeo gl = geo[L.left], gr = geo[L.right];
-        int cl = __atomic_load_n(&lnk[L.left].count, __ATOMIC_RELAXED);
-        int cr = __atomic_load_n(&lnk[L.right].count, __ATOMIC_RELAXED);
-        NodeGeo g;
-        g.lox = fminf(gl.lox, gr.lox); g.loy = fminf(gl.loy, gr.loy);
-        g.hix = fmaxf(gl.hix, gr.hix); g.hiy = fmaxf(gl.hiy, gr.hiy);
-        int cn = cl + cr;
-        float num = (float)cl * gl.cx;
-        num = num + (float)cr * gr.cx;
-        g.cx = num / (float)cn;
-        num = (float)cl * gl.cy;
-        num = num + (float)cr * gr.cy;
-        g.cy = num / (float)cn;
-        float mn = 0.0f;
-        mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
-        mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
-        float rr = mn / h;
-        int deg;
-        if (gl.deg < 0 && gr.deg < 0) deg = -1;
-        else {
-            int d = gl.deg > gr.deg ? gl.deg : gr.deg;
-            while (d <= kMaxDegree && hermite_error(rr, (float)cn, d) > prec && rr < 1.0f) d++;
-            deg = (rr < 1.0f && d <= kMaxDegree) ? d : -1;
-        }
-        g.deg = deg;
-        g.r2 = trunc_r2(h, prec, cn);
-        geo[p] = g;
-        lnk[p].count = cn;
+        merge_node(p, h, prec, geo, lnk);
         nd = p;
-        p = L.parent;
+        p = lnk[p].parent;
     }
 }
 
-__global__ __launch_bounds__(256) void k_coef_sizes(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo, int* sizes)
+__global__ __launch_bounds__(256) void k_coef_sizes(ViewState* vs, const NodeGeo* __restrict__ geo, int* sizes,
+                                                    int* momList)
 {
-    int nn = 2 * vs->nc - 1;
+    int n = vs->nc;
+    int nn = 2 * n - 1;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nn) return;
     int d = geo[t].deg;
     sizes[t] = d >= 0 ? (d + 1) * (d + 1) : 0;
+    if (d >= 0 && t < n - 1) momList[atomicAdd(&vs->nmom, 1)] = t;
 }
 
 // device-wide exclusive scan, phase 1: per-tile scan + tile totals
@@ -560,34 +649,39 @@ __global__ __launch_bounds__(256) void k_scan_apply(const ViewState* __restrict_
 }
 
 // Hermite moments A[a][b] = sum_leaves tx^a ty^b / (a! b!), t = (x - c) / sqrt(2 h^2)
-// (hermitefmm.cuh:119-138), one thread per node, FP64 accumulation in leaf order.
-__global__ __launch_bounds__(64) void k_moments(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
-                                                const NodeAux* __restrict__ aux, float* __restrict__ coef, long coefCap)
+// (hermitefmm.cuh:119-138).  Leaves carry the single coefficient 1.  Internal
+// nodes with an expansion are listed by k_coef_sizes; one work item computes
+// row a of one node, accumulating in FP64 in leaf order (grid-stride loop).
+__global__ __launch_bounds__(256) void k_moments(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
+                                                 const NodeAux* __restrict__ aux, const int* __restrict__ momList,
+                                                 float* __restrict__ coef, long coefCap)
 {
-    int n = vs->nc;
-    int nn = 2 * n - 1;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nn) return;
+    const int n = vs->nc;
     if (vs->ncoef > coefCap) return;
-    NodeGeo g = geo[t];
-    int p = g.deg;
-    if (p < 0) return;
-    NodeAux a = aux[t];
-    if (p == 0 && t >= n - 1) { coef[a.off] = 1.0f; return; }
     const int L0 = n - 1;
-    float s2h = vs->s2h;
+    const long stride = (long)gridDim.x * blockDim.x;
+    // leaves
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+        coef[aux[L0 + t].off] = 1.0f;
+    const float s2h = vs->s2h;
     double fact[kMaxDegree + 1];
     fact[0] = 1.0;
     for (int q = 1; q <= kMaxDegree; q++) fact[q] = fact[q - 1] * q;
-    const int P1 = p + 1;
-    for (int ax = 0; ax <= p; ax++) {
+    const long items = (long)vs->nmom * (kMaxDegree + 1);
+    for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < items; w += stride) {
+        int nd = momList[w / (kMaxDegree + 1)];
+        int ax = (int)(w % (kMaxDegree + 1));
+        NodeGeo g = geo[nd];
+        int p = g.deg;
+        if (ax > p) continue;
+        NodeAux a = aux[nd];
+        const int P1 = p + 1;
         double acc[kMaxDegree + 1];
         for (int b = 0; b <= kMaxDegree; b++) acc[b] = 0.0;
         for (int pos = a.first; pos <= a.last; pos++) {
             NodeGeo lf = geo[L0 + pos];
             float tx = (lf.cx - g.cx) / s2h;
             float ty = (lf.cy - g.cy) / s2h;
-            // reproduce px = tx^ax / ax! and py = ty^b / b! exactly as the oracle:
             double px = 1.0;
             for (int q = 0; q < ax; q++) px = px * (double)tx;
             double mx = px / fact[ax];
@@ -609,13 +703,34 @@ __global__ void k_check_coef(ViewState* vs, long coefCap)
 // ---------------------------------------------------------------------------
 // FGT evaluation (evaluateDynamicRadixTree.cuh:128-242, hermitefmm.cuh:35-47, 213-237)
 // ---------------------------------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef int i4v __attribute__((ext_vector_type(4)));
+
 struct TreeView {
-    const NodeGeo* geo;
-    const NodeLnk* lnk;
-    const NodeAux* aux;
-    const float* coef;
+    const NodeGeo UD_GLOBAL* geo;
+    const NodeLnk UD_GLOBAL* lnk;
+    const NodeAux UD_GLOBAL* aux;
+    const float UD_GLOBAL* coef;
     float s2h;
 };
+
+__device__ __forceinline__ NodeGeo ld_geo(const TreeView& T, int i)
+{
+    const f4v UD_GLOBAL* q = (const f4v UD_GLOBAL*)(T.geo + i);
+    f4v a = q[0], b = q[1];
+    NodeGeo g;
+    g.lox = a.x; g.loy = a.y; g.hix = a.z; g.hiy = a.w;
+    g.cx = b.x; g.cy = b.y; g.r2 = b.z; g.deg = __float_as_int(b.w);
+    return g;
+}
+
+__device__ __forceinline__ NodeLnk ld_lnk(const TreeView& T, int i)
+{
+    i4v a = *(const i4v UD_GLOBAL*)(T.lnk + i);
+    NodeLnk l;
+    l.left = a.x; l.right = a.y; l.parent = a.z; l.count = a.w;
+    return l;
+}
 
 __device__ __forceinline__ float eval_node(const TreeView& T, const NodeGeo& g, int nd, float qx, float qy)
 {
@@ -623,17 +738,20 @@ __device__ __forceinline__ float eval_node(const TreeView& T, const NodeGeo& g, 
     float tx = (qx - g.cx) / T.s2h;
     float ty = (qy - g.cy) / T.s2h;
     float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
-    const float* c = T.coef + T.aux[nd].off;
+    const float UD_GLOBAL* c = T.coef + T.aux[nd].off;
     if (p == 0) {
         float w = 0.0f;
         w = w + (c[0] * ex) * ey;
         return w;
     }
     // Hermite functions h_n(t) = 2t h_{n-1} - 2(n-1) h_{n-2}, h_0 = e^{-t^2}
+    // (loops unrolled to the maximum degree and predicated, so hy stays in registers)
     float hy[kMaxDegree + 1];
     hy[0] = ey;
     hy[1] = (2.0f * ty) * ey;
-    for (int n = 2; n <= p; n++) hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
+#pragma unroll
+    for (int n = 2; n <= kMaxDegree; n++)
+        hy[n] = (n <= p) ? (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2] : 0.0f;
     float w = 0.0f;
     float hprev = 0.0f, ha = ex;
     for (int a = 0; a <= p; a++) {
@@ -642,8 +760,10 @@ __device__ __forceinline__ float eval_node(const TreeView& T, const NodeGeo& g, 
             float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
             hprev = ha; ha = hn;
         }
-        const float* row = c + a * (p + 1);
-        for (int b = 0; b <= p; b++) w = w + (row[b] * ha) * hy[b];
+        const float UD_GLOBAL* row = c + a * (p + 1);
+#pragma unroll
+        for (int b = 0; b <= kMaxDegree; b++)
+            if (b <= p) w = w + (row[b] * ha) * hy[b];
     }
     return w;
 }
@@ -660,17 +780,17 @@ __device__ __forceinline__ bool truncated(const NodeGeo& g, float qx, float qy)
 
 __device__ __noinline__ float tree_weight(const TreeView& T, int entry, float qx, float qy)
 {
-    NodeGeo g = T.geo[entry];
+    NodeGeo g = ld_geo(T, entry);
     if (g.deg >= 0) return eval_node(T, g, entry, qx, qy);
     int rootparent = T.lnk[entry].parent;
     int nd = entry;
     float w = 0.0f;
     while (true) {
-        while (g.deg < 0 && !truncated(g, qx, qy)) { nd = T.lnk[nd].left; g = T.geo[nd]; }
+        while (g.deg < 0 && !truncated(g, qx, qy)) { nd = T.lnk[nd].left; g = ld_geo(T, nd); }
         if (g.deg >= 0) w = w + eval_node(T, g, nd, qx, qy);
         int pr = T.lnk[nd].parent;
         while (pr != rootparent && nd == T.lnk[pr].right) { nd = pr; pr = T.lnk[pr].parent; }
-        if (pr != rootparent) { nd = T.lnk[pr].right; g = T.geo[nd]; }
+        if (pr != rootparent) { nd = T.lnk[pr].right; g = ld_geo(T, nd); }
         else break;   // climbed back to the entry node
     }
     return w;
@@ -709,18 +829,40 @@ __device__ __forceinline__ float key_of(unsigned long long v) { return __uint_as
 // weights (below clusters that were not walked) on demand.  The top-k list of
 // (key, splat) pairs (ascending keys, ties keep insertion order) lives in LDS,
 // slot-major.
+// Trie of descent states, one slot array per lane in device memory, fields
+// interleaved by lane ([entry][field][lane]) so a wavefront touching the same
+// entry index issues one coalesced access.
+struct TrieRef {
+    float UD_GLOBAL* base;
+    long stride;                    // lanes in the grid
+    __device__ float get(int e, int f) const { return base[(4L * e + f) * stride]; }
+    __device__ void set(int e, int f, float v) const { base[(4L * e + f) * stride] = v; }
+    __device__ int kids(int e) const { return __float_as_int(get(e, 3)); }
+    __device__ void setKids(int e, int v) const { set(e, 3, __int_as_float(v)); }
+    __device__ void init(int e, float wUnset) const { set(e, 0, 1.0f); set(e, 1, wUnset); set(e, 2, wUnset); setKids(e, 0); }
+};
+
 template <int KMAX>
 __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint32_t seedLo, uint32_t seedHi,
-                                               uint32_t view, const ViewState* __restrict__ vs, TreeView T0,
+                                               uint32_t view, const ViewState* __restrict__ vs,
+                                               const NodeGeo* geo, const NodeLnk* lnk, const NodeAux* aux,
+                                               const float* coef, float* trieWs, int tcap,
+                                               const float2* __restrict__ leafXY, const int32_t* __restrict__ leafId,
                                                int32_t* __restrict__ out, int* __restrict__ errOut)
 {
-    TreeView T = T0;
+    TreeView T;
+    T.geo = (const NodeGeo UD_GLOBAL*)geo;
+    T.lnk = (const NodeLnk UD_GLOBAL*)lnk;
+    T.aux = (const NodeAux UD_GLOBAL*)aux;
+    T.coef = (const float UD_GLOBAL*)coef;
     T.s2h = vs->s2h;
-    constexpr int LMAX = (KMAX <= 16) ? 4 : (KMAX <= 36) ? 6 : (KMAX <= 64) ? 8 : (KMAX <= 144) ? 12 : 16;
-    constexpr int DFSMAX = 192;                       // trie entries the walk may create
-    constexpr int TMAX = DFSMAX + LMAX * (kMaxDepth - 1);
+    const int TMAX = tcap;
+    const int DFSMAX = tcap - L * 32;                 // entries the walk may create
     extern __shared__ unsigned long long list[];
     const int lane = threadIdx.x;
+    TrieRef tr;
+    tr.stride = (long)gridDim.x * 64;
+    tr.base = (float UD_GLOBAL*)trieWs + (long)blockIdx.x * 64 + lane;
     const int tilesY = (H + 7) >> 3;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int ti = tile / tilesY, tj = tile - (tile / tilesY) * tilesY;
@@ -730,73 +872,101 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
     for (int s = 0; s < k; s++) list[s * 64 + lane] = empty;
     const int n = vs->nc;
     const float prec = vs->prec;
-    if (valid && n > 0 && vs->err == 0) {
-        const int L0 = n - 1;
-        const float qx = (float)i, qy = (float)j;
-        const uint32_t pixel = (uint32_t)(j + H * i);
-        float tCap[TMAX], tWl[TMAX], tWr[TMAX];
-        int tKids[TMAX];               // (left entry + 1) | (right entry + 1) << 16
-        int nt = 1;
-        const float unset = __builtin_nanf("");   // weight not computed yet
-        tCap[0] = 1.0f; tWl[0] = unset; tWr[0] = unset; tKids[0] = 0;   // entry 0: the root
-        bool bad = false;
-        float total = 0.0f;
-        // ---- one depth-first pass over the root's FGT walk ----
-        {
-            float pacc[kMaxDepth];
-            int pent[kMaxDepth], pright[kMaxDepth], pcount[kMaxDepth];
-            uint64_t right = 0;        // bit d: node at depth d is its parent's right child
-            int d = 0, nd = 0;
-            while (true) {
-                NodeGeo g = T.geo[nd];
-                float wv = 0.0f;
-                bool walk = false;
-                if (g.deg >= 0) {
+    const bool runTree = n > 0 && vs->err == 0;          // wave-uniform
+    const int L0 = n - 1;
+    const float qx = (float)i, qy = (float)j;
+    const uint32_t pixel = (uint32_t)(j + H * i);
+    const float unset = __builtin_nanf("");              // weight not computed yet
+    // trie fields: cap, wl, wr, kids = (left entry + 1) | (right entry + 1) << 16
+    int nt = 1;
+    bool bad = false;
+    float total = 0.0f;
+    if (runTree) {
+        tr.init(0, unset);                                // entry 0: the root
+        // ---- one depth-first pass over the union of the 64 lanes' FGT walks ----
+        // The wavefront visits nodes in the reference's left-to-right order; a lane
+        // is active inside a node exactly when its own walk enters that node, so
+        // each lane sees its own frontier in its own order.  Node records and
+        // Hermite coefficients are then read at wave-uniform addresses.
+        __shared__ int s_right[kMaxDepth], s_count[kMaxDepth];
+        __shared__ unsigned long long s_walk[kMaxDepth];
+        float pacc[kMaxDepth];
+        int pent[kMaxDepth];
+        uint64_t rightBits = 0;      // bit d: node at depth d is its parent's right child
+        int d = 0, nd = 0;
+        bool act = valid;
+        bool overflow = false;
+        auto report = [&](int dd, float w) {
+            if (dd == 0) { total = w; return; }
+            int pe = pent[dd - 1];
+            if (pe >= 0) tr.set(pe, ((rightBits >> dd) & 1ull) ? 2 : 1, w);
+        };
+        while (true) {
+            nd = __builtin_amdgcn_readfirstlane(nd);
+            NodeGeo g = ld_geo(T, nd);
+            float wv = 0.0f;
+            bool walk = false;
+            if (g.deg >= 0) {
+                if (act) {
                     wv = eval_node(T, g, nd, qx, qy);
                     // ancestors whose weight a descent can compare: depth 0 and every
                     // node whose parent has more than k sources
                     for (int a = 0; a < d; a++) {
-                        if (a > 0 && pcount[a - 1] <= k) break;
+                        if (a > 0 && s_count[a - 1] <= k) break;
                         pacc[a] = pacc[a] + wv;
                     }
-                } else if (!truncated(g, qx, qy)) {
-                    walk = true;
                 }
+            } else {
+                walk = act && !truncated(g, qx, qy);
+            }
+            unsigned long long wm = __ballot(walk);
+            if (wm != 0ull) {
+                if (d >= kMaxDepth - 1) { overflow = true; break; }
+                NodeLnk lk = ld_lnk(T, nd);
+                s_right[d] = lk.right;
+                s_count[d] = lk.count;
+                s_walk[d] = wm;
+                if (act && !walk) report(d, 0.0f);       // truncated for this lane
                 if (walk) {
-                    if (d >= kMaxDepth - 1) { bad = true; break; }
-                    NodeLnk lk = T.lnk[nd];
                     pacc[d] = 0.0f;
-                    pright[d] = lk.right;
-                    pcount[d] = lk.count;
                     int e = -1;
                     if (d == 0) e = 0;
                     else if (lk.count > k && pent[d - 1] >= 0 && nt < DFSMAX) {
                         e = nt++;
-                        tCap[e] = 1.0f; tWl[e] = unset; tWr[e] = unset; tKids[e] = 0;
+                        tr.init(e, unset);
                         int pe = pent[d - 1];
-                        if ((right >> d) & 1ull) tKids[pe] = (tKids[pe] & 0xffff) | ((e + 1) << 16);
-                        else tKids[pe] = (tKids[pe] & ~0xffff) | (e + 1);
+                        int pk = tr.kids(pe);
+                        if ((rightBits >> d) & 1ull) tr.setKids(pe, (pk & 0xffff) | ((e + 1) << 16));
+                        else tr.setKids(pe, (pk & ~0xffff) | (e + 1));
                     }
                     pent[d] = e;
-                    d++;
-                    nd = lk.left;
-                    right &= ~(1ull << d);
-                    continue;
                 }
-                // node at depth d is complete with weight wv; report and advance
-                bool done = false;
-                while (true) {
-                    if (d == 0) { total = wv; done = true; break; }
-                    int pe = pent[d - 1];
-                    bool isRight = (right >> d) & 1ull;
-                    if (pe >= 0) { if (isRight) tWr[pe] = wv; else tWl[pe] = wv; }
-                    if (!isRight) { right |= (1ull << d); nd = pright[d - 1]; break; }
-                    d--;
-                    wv = pacc[d];
-                }
-                if (done) break;
+                d++;
+                nd = lk.left;
+                rightBits &= ~(1ull << d);
+                act = walk;
+                continue;
             }
+            // the node is complete for every active lane; report and advance
+            if (act) report(d, wv);
+            bool done = false;
+            while (true) {
+                if (d == 0) { done = true; break; }
+                if (!((rightBits >> d) & 1ull)) {
+                    rightBits |= (1ull << d);
+                    nd = s_right[d - 1];
+                    act = (s_walk[d - 1] >> lane) & 1ull;
+                    break;
+                }
+                d--;
+                act = (s_walk[d] >> lane) & 1ull;
+                if (act) report(d, pacc[d]);
+            }
+            if (done) break;
         }
+        if (overflow) bad = true;
+    }
+    if (valid && runTree) {
         total = fmaxf(0.0f, total);
         if (!bad && total >= prec) {
             int pe[kMaxDepth];
@@ -804,18 +974,21 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
             for (int la = 0; la < L && !bad; la++) {
                 int e = 0, nd = 0, depth = 0;
                 while (T.lnk[nd].count > k) {
-                    NodeLnk lk = T.lnk[nd];
-                    if (tWl[e] != tWl[e]) {
-                        tWl[e] = tree_weight(T, lk.left, qx, qy);
-                        tWr[e] = tree_weight(T, lk.right, qx, qy);
+                    NodeLnk lk = ld_lnk(T, nd);
+                    float wl = tr.get(e, 1), wr = tr.get(e, 2);
+                    if (wl != wl) {
+                        wl = tree_weight(T, lk.left, qx, qy);
+                        wr = tree_weight(T, lk.right, qx, qy);
+                        tr.set(e, 1, wl);
+                        tr.set(e, 2, wr);
                     }
-                    float wl = fmaxf(0.0f, tWl[e]), wr = fmaxf(0.0f, tWr[e]);
+                    wl = fmaxf(0.0f, wl); wr = fmaxf(0.0f, wr);
                     float sum = wl + wr;
                     wl = wl / sum; wr = wr / sum;
-                    int kids = tKids[e];
+                    int kids = tr.kids(e);
                     int el = (kids & 0xffff) - 1, er = (kids >> 16) - 1;
-                    float cl = el >= 0 ? tCap[el] : 1.0f;
-                    float cr = er >= 0 ? tCap[er] : 1.0f;
+                    float cl = el >= 0 ? tr.get(el, 0) : 1.0f;
+                    float cr = er >= 0 ? tr.get(er, 0) : 1.0f;
                     float sl = wl * cl, sr = wr * cr;
                     bool goRight = sl < sr;
                     int ce = goRight ? er : el;
@@ -823,8 +996,8 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                     if (ce < 0) {
                         if (nt >= TMAX) { bad = true; break; }
                         ce = nt++;
-                        tCap[ce] = 1.0f; tWl[ce] = unset; tWr[ce] = unset; tKids[ce] = 0;
-                        tKids[e] = goRight ? ((kids & 0xffff) | ((ce + 1) << 16)) : ((kids & ~0xffff) | (ce + 1));
+                        tr.init(ce, unset);
+                        tr.setKids(e, goRight ? ((kids & 0xffff) | ((ce + 1) << 16)) : ((kids & ~0xffff) | (ce + 1)));
                     }
                     depth++;
                     if (depth >= kMaxDepth) { bad = true; break; }
@@ -837,35 +1010,50 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                 float prob = 1.0f;
                 for (int dd = depth; dd >= 1; dd--) {
                     int ce = pe[dd];
-                    tCap[ce] = fmaxf(0.0f, tCap[ce] - prob);
+                    tr.set(ce, 0, fmaxf(0.0f, tr.get(ce, 0) - prob));
                     prob = prob * pp[dd];
                 }
-                // sequential Poisson keys over the leaves of the reached subtree
-                const int sp = T.lnk[nd].parent;
+                // sequential Poisson keys over the leaves of the reached subtree: in
+                // a Karras tree they are the contiguous sorted range
+                // [first, first + count), in the left-to-right order nextLeaf visits
+                // (sequentialPoissonDynamicTree.cuh:16-53), and every leaf carries the
+                // single Hermite coefficient 1, so E(leaf) = e^{-tx^2} e^{-ty^2}.
                 const int cnt = T.lnk[nd].count;
-                int lf = nd;
-                u32x4 rnd = { 0, 0, 0, 0 };
-                for (int kk = 0; kk < cnt; kk++) {
-                    lf = next_leaf(T, L0, sp, lf);
-                    NodeGeo lg = T.geo[lf];
-                    float p = fmaxf(0.0f, eval_node(T, lg, lf, qx, qy));
-                    if ((kk & 3) == 0) {
-                        u32x4 c = { (uint32_t)(kk >> 2), (uint32_t)la, pixel, view };
-                        rnd = philox4x32_10(c, seedLo, seedHi);
+                const int first = (nd >= L0) ? nd - L0 : T.aux[nd].first;
+                for (int kk0 = 0; kk0 < cnt; kk0 += 4) {
+                    u32x4 c4 = { (uint32_t)(kk0 >> 2), (uint32_t)la, pixel, view };
+                    u32x4 rnd = philox4x32_10(c4, seedLo, seedHi);
+                    float2 xy[4];
+                    int ids4[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        int pos = first + kk0 + q;
+                        bool ok = kk0 + q < cnt;
+                        xy[q] = ok ? leafXY[pos] : make_float2(0.0f, 0.0f);
+                        ids4[q] = ok ? leafId[pos] : -1;
                     }
-                    uint32_t x = ((kk & 3) == 0) ? rnd.x : ((kk & 3) == 1) ? rnd.y : ((kk & 3) == 2) ? rnd.z : rnd.w;
-                    float key = u01(x) / p;
-                    unsigned long long last = list[(k - 1) * 64 + lane];
-                    if (!(key < key_of(last))) continue;
-                    int id = T.aux[lf].pad;   // splat id of the leaf
-                    unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) | (uint32_t)id;
-                    int pos = k - 1;
-                    while (pos > 0) {
-                        unsigned long long prev = list[(pos - 1) * 64 + lane];
-                        if (key < key_of(prev)) { list[pos * 64 + lane] = prev; pos--; }
-                        else break;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (kk0 + q >= cnt) break;
+                        float tx = (qx - xy[q].x) / T.s2h;
+                        float ty = (qy - xy[q].y) / T.s2h;
+                        float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+                        float wgt = 0.0f;
+                        wgt = wgt + (1.0f * ex) * ey;
+                        float p = fmaxf(0.0f, wgt);
+                        uint32_t x = (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w;
+                        float key = u01(x) / p;
+                        unsigned long long last = list[(k - 1) * 64 + lane];
+                        if (!(key < key_of(last))) continue;
+                        unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) | (uint32_t)ids4[q];
+                        int pos = k - 1;
+                        while (pos > 0) {
+                            unsigned long long prev = list[(pos - 1) * 64 + lane];
+                            if (key < key_of(prev)) { list[pos * 64 + lane] = prev; pos--; }
+                            else break;
+                        }
+                        list[pos * 64 + lane] = ent;
                     }
-                    list[pos * 64 + lane] = ent;
                 }
             }
         }
@@ -916,22 +1104,25 @@ int build_tree(int N, int W, int H, double f, double smoothing, double prec, con
     int32_t* iin = w.idsA; int32_t* iout = w.idsB;
     for (int shift = 0; shift < 32; shift += 8) {
         k_sort_hist<<<(unsigned)nSort, 256, 0, st>>>(w.vs, kin, shift, (int)nSort, w.sortHist);
-        k_scan_small<<<1, 1024, 0, st>>>(w.sortHist, (int)(256 * nSort), w.scanBlocks);
-        k_sort_scatter<<<(unsigned)nSort, 256, 0, st>>>(w.vs, kin, iin, shift, (int)nSort, w.sortHist, kout, iout);
+        k_sort_scan_digits<<<256, 256, 0, st>>>(w.sortHist, (int)nSort, w.scanBlocks);
+        k_sort_scan_top<<<1, 256, 0, st>>>(w.scanBlocks);
+        k_sort_scatter<<<(unsigned)nSort, 256, 0, st>>>(w.vs, kin, iin, shift, (int)nSort, w.sortHist, w.scanBlocks,
+                                                        kout, iout);
         uint32_t* tk = kin; kin = kout; kout = tk;
         int32_t* ti = iin; iin = iout; iout = ti;
     }
     // after 4 passes the sorted data is back in keysA / idsA
-    k_hierarchy<<<grid1(N, 256), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux);
+    k_hierarchy<<<grid1(N, 256), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux, w.leafXY);
     k_leaf_ids<<<grid1(N, 256), 256, 0, st>>>(w.vs, iin, w.aux);
     k_fill_i32<<<grid1(N, 256), 256, 0, st>>>(w.flags, w.vs, 0, 0);
-    k_bottom_up<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.flags);
-    k_coef_sizes<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.geo, w.scanTmp);
+    k_bottom_up_local<<<grid1(N, kFanBlock), kFanBlock, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.flags, w.seeds);
+    k_bottom_up_global<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.flags, w.seeds);
+    k_coef_sizes<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.geo, w.scanTmp, w.momList);
     k_scan_tiles<<<(unsigned)nScan, 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks);
     k_scan_small<<<1, 1024, 0, st>>>(w.scanBlocks, (int)nScan, &w.vs->ncoef);
     k_scan_apply<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks, w.aux);
     k_check_coef<<<1, 64, 0, st>>>(w.vs, w.coefCap);
-    k_moments<<<grid1(2L * N, 64), 64, 0, st>>>(w.vs, w.geo, w.aux, w.coef, w.coefCap);
+    k_moments<<<2048, 256, 0, st>>>(w.vs, w.geo, w.aux, w.momList, w.coef, w.coefCap);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
 }
@@ -943,17 +1134,19 @@ int num_launches(int k)
     return L;
 }
 
+// trie entries per lane: room for the walk plus 32 new entries per descent
+int trie_entries(int k) { return 192 + 32 * num_launches(k); }
+
 template <int KMAX>
 void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32_t view, const TreeWs& w,
                    int32_t* out, int* err, hipStream_t st)
 {
-    TreeView T;
-    T.geo = w.geo; T.lnk = w.lnk; T.aux = w.aux; T.coef = w.coef;
-    T.s2h = 0.0f;   // the kernel reads s2h from the device-side view state
     unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
     size_t lds = (size_t)k * 64 * sizeof(unsigned long long);
     hipLaunchKernelGGL((k_sample<KMAX>), dim3(tiles), dim3(64), lds, st, W, H, k, num_launches(k), seedLo, seedHi,
-                       view, (const ViewState*)w.vs, T, out, err);
+                       view, (const ViewState*)w.vs, (const NodeGeo*)w.geo, (const NodeLnk*)w.lnk,
+                       (const NodeAux*)w.aux, (const float*)w.coef, w.trie, trie_entries(k),
+                       (const float2*)w.leafXY, (const int32_t*)w.idsA, out, err);
 }
 
 } // namespace
@@ -962,8 +1155,7 @@ extern "C" {
 
 size_t ude_sample_workspace_bytes(int N, int W, int H, int k)
 {
-    (void)W; (void)H; (void)k;
-    return carve(nullptr, N, nullptr) + 256;
+    return carve(nullptr, N, W, H, k, nullptr) + 256;
 }
 
 int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, double prec,
@@ -979,7 +1171,7 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
     }
     if (wsBytes < ude_sample_workspace_bytes(N, W, H, k)) return UDE_ERR_WORKSPACE;
     TreeWs w;
-    carve(ws, N, &w);
+    carve(ws, N, W, H, k, &w);
     uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
     for (int b = 0; b < bn; b++) {
         const float* pb = pts + 3L * N * b;
