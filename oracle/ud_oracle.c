@@ -622,7 +622,7 @@ void o_lambert_backward(int bn, int W, int H, int L, int ib, const double* g, co
 typedef struct {
     int nc;                    /* culled splat count */
     float h, radius, lo, hix, hiy;
-    float prec, s2h;
+    float prec, s2h, is2h;     /* is2h = 1 / sqrt(2 h^2), the Hermite argument scale */
     /* per sorted position */
     uint32_t* morton;          /* nc */
     int32_t* ids;              /* nc, splat id of sorted position */
@@ -815,6 +815,7 @@ int o_tree_build(int N, int W, int H, double f, double smoothing_d, double prec_
     T->lo = -radius; T->hix = Wf + radius; T->hiy = Hf + radius;
     T->prec = (float)prec_d;
     T->s2h = sqrtf((2.0f * h) * h);
+    T->is2h = 1.0f / T->s2h;
     /* cull (order-preserving) + Morton */
     int nc = 0;
     for (int i = 0; i < N; i++) {
@@ -927,10 +928,14 @@ int o_tree_build(int N, int W, int H, double f, double smoothing_d, double prec_
     }
     T->ncoef = tot;
     if (tot > coefCap) return -2;
-    /* moments: per node over its leaf range, FP64 accumulation in leaf order */
-    double fact[17];
-    fact[0] = 1.0;
-    for (int a = 1; a <= 16; a++) fact[a] = fact[a - 1] * a;
+    /* moments: per node over its leaf range, FP64 accumulation in leaf order,
+     * monomials scaled by correctly rounded reciprocal factorials */
+    double ifact[17];
+    {
+        double fa = 1.0;
+        ifact[0] = 1.0;
+        for (int a = 1; a <= 16; a++) { fa = fa * a; ifact[a] = 1.0 / fa; }
+    }
     for (int nd = 0; nd < nn; nd++) {
         int p = T->deg[nd];
         if (p < 0) continue;
@@ -939,11 +944,11 @@ int o_tree_build(int N, int W, int H, double f, double smoothing_d, double prec_
         for (int q = 0; q < P1 * P1; q++) acc[q] = 0.0;
         for (int pos = T->first[nd]; pos <= T->last[nd]; pos++) {
             int lf = L0 + pos;
-            float tx = (T->cen[2 * lf] - T->cen[2 * nd]) / T->s2h;
-            float ty = (T->cen[2 * lf + 1] - T->cen[2 * nd + 1]) / T->s2h;
+            float tx = (T->cen[2 * lf] - T->cen[2 * nd]) * T->is2h;
+            float ty = (T->cen[2 * lf + 1] - T->cen[2 * nd + 1]) * T->is2h;
             double mx[17], my[17], px = 1.0, py = 1.0;
             for (int a = 0; a <= p; a++) {
-                mx[a] = px / fact[a]; my[a] = py / fact[a];
+                mx[a] = px * ifact[a]; my[a] = py * ifact[a];
                 px = px * (double)tx; py = py * (double)ty;
             }
             for (int a = 0; a <= p; a++)
@@ -958,8 +963,8 @@ int o_tree_build(int N, int W, int H, double f, double smoothing_d, double prec_
 static float o_eval_node(const OTree* T, int nd, float qx, float qy)
 {
     int p = T->deg[nd];
-    float tx = (qx - T->cen[2 * nd]) / T->s2h;
-    float ty = (qy - T->cen[2 * nd + 1]) / T->s2h;
+    float tx = (qx - T->cen[2 * nd]) * T->is2h;
+    float ty = (qy - T->cen[2 * nd + 1]) * T->is2h;
     float hx[17], hy[17];
     float ex = oexpf(-(tx * tx)), ey = oexpf(-(ty * ty));
     hx[0] = ex; hy[0] = ey;
@@ -1208,7 +1213,7 @@ int o_tree_eval(int nc, float h, float prec, float s2h, const uint32_t* morton, 
     OTree T;
     memset(&T, 0, sizeof(T));
     long nn = 2L * (N > 0 ? N : 1);
-    T.nc = nc; T.h = h; T.prec = prec; T.s2h = s2h;
+    T.nc = nc; T.h = h; T.prec = prec; T.s2h = s2h; T.is2h = 1.0f / s2h;
     T.morton = (uint32_t*)morton; T.ids = (int32_t*)ids;
     T.left = ibuf; T.right = ibuf + nn; T.parent = ibuf + 2 * nn; T.count = ibuf + 3 * nn;
     T.deg = ibuf + 4 * nn; T.off = ibuf + 5 * nn; T.first = ibuf + 6 * nn; T.last = ibuf + 7 * nn;

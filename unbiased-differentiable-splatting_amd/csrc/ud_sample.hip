@@ -36,7 +36,7 @@ struct ViewState {
     unsigned long long hsum;
     unsigned int hcnt;
     int nc;
-    float h, radius, lo, hix, hiy, s2h, prec;
+    float h, radius, lo, hix, hiy, s2h, prec, is2h;
     int err;
     int ncoef;
     int nseeds;       // block-spanning fan-in starts (bottom-up, second level)
@@ -202,6 +202,7 @@ __global__ void k_view_params(ViewState* vs, float Wf, float Hf, double prec)
     vs->hix = Wf + radius;
     vs->hiy = Hf + radius;
     vs->s2h = sqrtf((2.0f * h) * h);
+    vs->is2h = 1.0f / vs->s2h;
     vs->prec = (float)prec;
     vs->nc = 0;
 }
@@ -610,10 +611,17 @@ __global__ __launch_bounds__(256) void k_coef_sizes(ViewState* vs, const NodeGeo
     int n = vs->nc;
     int nn = 2 * n - 1;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nn) return;
-    int d = geo[t].deg;
-    sizes[t] = d >= 0 ? (d + 1) * (d + 1) : 0;
-    if (d >= 0 && t < n - 1) momList[atomicAdd(&vs->nmom, 1)] = t;
+    int d = t < nn ? geo[t].deg : -1;
+    if (t < nn) sizes[t] = d >= 0 ? (d + 1) * (d + 1) : 0;
+    bool want = d >= 0 && t < n - 1;
+    unsigned long long m = __ballot(want);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&vs->nmom, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (want) momList[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
 }
 
 // device-wide exclusive scan, phase 1: per-tile scan + tile totals
@@ -650,8 +658,16 @@ __global__ __launch_bounds__(256) void k_scan_apply(const ViewState* __restrict_
 
 // Hermite moments A[a][b] = sum_leaves tx^a ty^b / (a! b!), t = (x - c) / sqrt(2 h^2)
 // (hermitefmm.cuh:119-138).  Leaves carry the single coefficient 1.  Internal
-// nodes with an expansion are listed by k_coef_sizes; one work item computes
-// row a of one node, accumulating in FP64 in leaf order (grid-stride loop).
+// nodes with an expansion are listed by k_coef_sizes; thread (node, row a)
+// computes row a of one node, accumulating in FP64 in leaf order.  A workgroup
+// of 255 threads covers 15 nodes x 17 rows.
+constexpr int kMomNodes = 15;
+
+// 1 / a! as correctly rounded doubles (IEEE division, evaluated at compile time)
+constexpr double ifact_c(int a) { double f = 1.0; for (int q = 2; q <= a; q++) f = f * q; return 1.0 / f; }
+__constant__ double kInvFact[kMaxDegree + 1] = {
+    ifact_c(0), ifact_c(1), ifact_c(2), ifact_c(3), ifact_c(4), ifact_c(5), ifact_c(6), ifact_c(7), ifact_c(8),
+    ifact_c(9), ifact_c(10), ifact_c(11), ifact_c(12), ifact_c(13), ifact_c(14), ifact_c(15), ifact_c(16) };
 __global__ __launch_bounds__(256) void k_moments(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
                                                  const NodeAux* __restrict__ aux, const int* __restrict__ momList,
                                                  float* __restrict__ coef, long coefCap)
@@ -659,40 +675,46 @@ __global__ __launch_bounds__(256) void k_moments(const ViewState* __restrict__ v
     const int n = vs->nc;
     if (vs->ncoef > coefCap) return;
     const int L0 = n - 1;
-    const long stride = (long)gridDim.x * blockDim.x;
-    // leaves
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
-        coef[aux[L0 + t].off] = 1.0f;
-    const float s2h = vs->s2h;
-    double fact[kMaxDegree + 1];
-    fact[0] = 1.0;
-    for (int q = 1; q <= kMaxDegree; q++) fact[q] = fact[q - 1] * q;
-    const long items = (long)vs->nmom * (kMaxDegree + 1);
-    for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < items; w += stride) {
-        int nd = momList[w / (kMaxDegree + 1)];
-        int ax = (int)(w % (kMaxDegree + 1));
-        NodeGeo g = geo[nd];
-        int p = g.deg;
-        if (ax > p) continue;
-        NodeAux a = aux[nd];
-        const int P1 = p + 1;
-        double acc[kMaxDegree + 1];
-        for (int b = 0; b <= kMaxDegree; b++) acc[b] = 0.0;
-        for (int pos = a.first; pos <= a.last; pos++) {
-            NodeGeo lf = geo[L0 + pos];
-            float tx = (lf.cx - g.cx) / s2h;
-            float ty = (lf.cy - g.cy) / s2h;
-            double px = 1.0;
-            for (int q = 0; q < ax; q++) px = px * (double)tx;
-            double mx = px / fact[ax];
-            double py = 1.0;
-            for (int b = 0; b <= p; b++) {
-                acc[b] += mx * (py / fact[b]);
+    const int t = threadIdx.x;
+    // leaves: 256 per workgroup
+    {
+        long lf = (long)blockIdx.x * 256 + t;
+        if (lf < n) coef[aux[L0 + lf].off] = 1.0f;
+    }
+    if (t >= kMomNodes * (kMaxDegree + 1)) return;
+    const long slot = (long)blockIdx.x * kMomNodes + t / (kMaxDegree + 1);
+    const int ax = t % (kMaxDegree + 1);
+    if (slot >= vs->nmom) return;
+    const int nd = momList[slot];
+    NodeGeo g = geo[nd];
+    const int p = g.deg;
+    if (ax > p) return;
+    const float is2h = vs->is2h;
+    NodeAux a = aux[nd];
+    const int P1 = p + 1;
+    const double ifx = kInvFact[ax];
+    double acc[kMaxDegree + 1];
+#pragma unroll
+    for (int bb = 0; bb <= kMaxDegree; bb++) acc[bb] = 0.0;
+    for (int pos = a.first; pos <= a.last; pos++) {
+        NodeGeo lf = geo[L0 + pos];
+        float tx = (lf.cx - g.cx) * is2h;
+        float ty = (lf.cy - g.cy) * is2h;
+        double px = 1.0;
+        for (int q = 0; q < ax; q++) px = px * (double)tx;
+        double mx = px * ifx;
+        double py = 1.0;
+#pragma unroll
+        for (int bb = 0; bb <= kMaxDegree; bb++) {
+            if (bb <= p) {
+                acc[bb] += mx * (py * ifact_c(bb));
                 py = py * (double)ty;
             }
         }
-        for (int b = 0; b <= p; b++) coef[a.off + ax * P1 + b] = (float)acc[b];
     }
+#pragma unroll
+    for (int bb = 0; bb <= kMaxDegree; bb++)
+        if (bb <= p) coef[a.off + ax * P1 + bb] = (float)acc[bb];
 }
 
 __global__ void k_check_coef(ViewState* vs, long coefCap)
@@ -711,7 +733,7 @@ struct TreeView {
     const NodeLnk UD_GLOBAL* lnk;
     const NodeAux UD_GLOBAL* aux;
     const float UD_GLOBAL* coef;
-    float s2h;
+    float s2h, is2h;
 };
 
 __device__ __forceinline__ NodeGeo ld_geo(const TreeView& T, int i)
@@ -735,8 +757,8 @@ __device__ __forceinline__ NodeLnk ld_lnk(const TreeView& T, int i)
 __device__ __forceinline__ float eval_node(const TreeView& T, const NodeGeo& g, int nd, float qx, float qy)
 {
     int p = g.deg;
-    float tx = (qx - g.cx) / T.s2h;
-    float ty = (qy - g.cy) / T.s2h;
+    float tx = (qx - g.cx) * T.is2h;
+    float ty = (qy - g.cy) * T.is2h;
     float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
     const float UD_GLOBAL* c = T.coef + T.aux[nd].off;
     if (p == 0) {
@@ -765,6 +787,50 @@ __device__ __forceinline__ float eval_node(const TreeView& T, const NodeGeo& g, 
         for (int b = 0; b <= kMaxDegree; b++)
             if (b <= p) w = w + (row[b] * ha) * hy[b];
     }
+    return w;
+}
+
+// Hermite evaluation with the coefficient block staged in LDS: called by all 64
+// lanes of a wavefront for the same node (uniform p and block), so the block
+// is fetched with one coalesced load per 64 coefficients and then read with
+// LDS broadcasts.  Lanes with act == false only help with the load.
+__device__ __forceinline__ float eval_node_staged(const TreeView& T, const NodeGeo& g, int nd, float qx, float qy,
+                                                  bool act, float* sc, int lane)
+{
+    const int p = g.deg;
+    const int np = (p + 1) * (p + 1);
+    const float UD_GLOBAL* c = T.coef + T.aux[nd].off;
+    for (int q = lane; q < np; q += 64) sc[q] = c[q];
+    __builtin_amdgcn_wave_barrier();
+    float w = 0.0f;
+    if (act) {
+        float tx = (qx - g.cx) * T.is2h;
+        float ty = (qy - g.cy) * T.is2h;
+        float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+        if (p == 0) {
+            w = w + (sc[0] * ex) * ey;
+        } else {
+            float hy[kMaxDegree + 1];
+            hy[0] = ey;
+            hy[1] = (2.0f * ty) * ey;
+#pragma unroll
+            for (int n = 2; n <= kMaxDegree; n++)
+                hy[n] = (n <= p) ? (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2] : 0.0f;
+            float hprev = 0.0f, ha = ex;
+            for (int a = 0; a <= p; a++) {
+                if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
+                else if (a >= 2) {
+                    float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+                    hprev = ha; ha = hn;
+                }
+                const float* row = sc + a * (p + 1);
+#pragma unroll
+                for (int b = 0; b <= kMaxDegree; b++)
+                    if (b <= p) w = w + (row[b] * ha) * hy[b];
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
     return w;
 }
 
@@ -856,6 +922,7 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
     T.aux = (const NodeAux UD_GLOBAL*)aux;
     T.coef = (const float UD_GLOBAL*)coef;
     T.s2h = vs->s2h;
+    T.is2h = vs->is2h;
     const int TMAX = tcap;
     const int DFSMAX = tcap - L * 32;                 // entries the walk may create
     extern __shared__ unsigned long long list[];
@@ -890,6 +957,7 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
         // Hermite coefficients are then read at wave-uniform addresses.
         __shared__ int s_right[kMaxDepth], s_count[kMaxDepth];
         __shared__ unsigned long long s_walk[kMaxDepth];
+        __shared__ float s_coef[(kMaxDegree + 1) * (kMaxDegree + 1)];
         float pacc[kMaxDepth];
         int pent[kMaxDepth];
         uint64_t rightBits = 0;      // bit d: node at depth d is its parent's right child
@@ -904,11 +972,12 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
         while (true) {
             nd = __builtin_amdgcn_readfirstlane(nd);
             NodeGeo g = ld_geo(T, nd);
+            NodeLnk lk = ld_lnk(T, nd);        // issued together with the geometry
             float wv = 0.0f;
             bool walk = false;
             if (g.deg >= 0) {
+                wv = eval_node_staged(T, g, nd, qx, qy, act, s_coef, lane);
                 if (act) {
-                    wv = eval_node(T, g, nd, qx, qy);
                     // ancestors whose weight a descent can compare: depth 0 and every
                     // node whose parent has more than k sources
                     for (int a = 0; a < d; a++) {
@@ -922,7 +991,6 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
             unsigned long long wm = __ballot(walk);
             if (wm != 0ull) {
                 if (d >= kMaxDepth - 1) { overflow = true; break; }
-                NodeLnk lk = ld_lnk(T, nd);
                 s_right[d] = lk.right;
                 s_count[d] = lk.count;
                 s_walk[d] = wm;
@@ -1035,8 +1103,8 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
                         if (kk0 + q >= cnt) break;
-                        float tx = (qx - xy[q].x) / T.s2h;
-                        float ty = (qy - xy[q].y) / T.s2h;
+                        float tx = (qx - xy[q].x) * T.is2h;
+                        float ty = (qy - xy[q].y) * T.is2h;
                         float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
                         float wgt = 0.0f;
                         wgt = wgt + (1.0f * ex) * ey;
@@ -1122,7 +1190,8 @@ int build_tree(int N, int W, int H, double f, double smoothing, double prec, con
     k_scan_small<<<1, 1024, 0, st>>>(w.scanBlocks, (int)nScan, &w.vs->ncoef);
     k_scan_apply<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks, w.aux);
     k_check_coef<<<1, 64, 0, st>>>(w.vs, w.coefCap);
-    k_moments<<<2048, 256, 0, st>>>(w.vs, w.geo, w.aux, w.momList, w.coef, w.coefCap);
+    // enough workgroups for max(N leaves / 256, N nodes / 15)
+    k_moments<<<grid1(N, kMomNodes), 256, 0, st>>>(w.vs, w.geo, w.aux, w.momList, w.coef, w.coefCap);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
 }
