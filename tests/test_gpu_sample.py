@@ -105,3 +105,15 @@ def test_sample_properties_large(dev):
     assert np.all(p[np.unique(v), 2] <= 0)
     c = run_gpu(dev, sc, p, nb, sd, k, seed=22)
     assert not np.array_equal(a, c)
+
+
+def test_sample_golden_fixture(dev):
+    """Sampled indices equal the committed oracle fixture (256 splats, 64x64, k = 8)."""
+    import os
+    from cudaExtensions import sampling
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden_cfg2.npz"))
+    W, H, k, seed = (int(v) for v in g["dims"])
+    out = sampling.sample(W, H, float(g["focal"]), t(g["pts"][None], dev), t(g["nrm"][None], dev), None,
+                          t(g["sd"][None], dev), k, 0.0, float(g["precision"]), seed=seed)
+    sampling.check_errors()
+    assert np.array_equal(out[0].cpu().numpy(), g["idx"])
