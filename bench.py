@@ -24,6 +24,9 @@ for _p in (PKG, ROOT):
 
 METRIC = "fwd+bwd frames/s at 1M splats 512² (learned vs plain rates, equal grad variance)"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the sampler kernel at the 1M-splat 512x512 k=40 frame, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (calibrated against a copy kernel); None until measured
+TRAFFIC_BYTES = None
 
 
 def build_problem(dev, n_override=None):
@@ -111,10 +114,10 @@ def algorithmic_bytes(N, P, k):
     }
 
 
-def cpu_baseline(sc, rows=64, k=40, threads=None):
-    """Oracle (C, OpenMP) on a bounded sample of the same frame: full tree build,
-    then sampling + EWA/compositing/interpolation fwd+bwd on `rows` pixel rows;
-    the per-row part is scaled to the full image."""
+def cpu_baseline(sc, rows=None, k=40, threads=None):
+    """Oracle (C, OpenMP) on the same frame: full tree build, then sampling +
+    EWA/compositing/interpolation fwd+bwd on `rows` pixel rows (default: all;
+    fewer rows are scaled to the full image)."""
     import numpy as np
     from oracle import oracle as O
     threads = threads or min(16, os.cpu_count() or 1)
@@ -122,6 +125,7 @@ def cpu_baseline(sc, rows=64, k=40, threads=None):
     p, n = sc.camera_space()
     nb = np.where(n[:, 2:3] > 0, -n, n).astype(np.float32)
     W, H, f = sc.width, sc.height, sc.focal
+    rows = rows or W
     i0 = W // 2 - rows // 2
     t0 = time.time()
     idx, t_tree = O.sample_view_rows(W, H, f, p, nb, sc.stddevs, k, i0, i0 + rows, 0.0, 0.01, seed=3)
@@ -141,8 +145,8 @@ def cpu_baseline(sc, rows=64, k=40, threads=None):
     frame = t_tree + (W / float(rows)) * (t_sample + t_ops)
     return {"value": 1.0 / frame, "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": "oracle tree build over all %d splats (%.2fs) + sampling and EWA/compositing/"
-                      "interpolation fwd+bwd on %d of %d pixel rows (%.2fs), per-row part scaled x%d"
-                      % (sc.n, t_tree, rows, W, t_sample + t_ops, W // rows)}
+                      "interpolation fwd+bwd on %d of %d pixel rows (%.2fs), per-row part scaled x%g"
+                      % (sc.n, t_tree, rows, W, t_sample + t_ops, W / float(rows))}
 
 
 def main():
@@ -152,6 +156,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--splats", type=int, default=None, help="override the splat count (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of known size (1 GiB read + 1 GiB written per launch), "
+                         "to scale HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -159,6 +166,15 @@ def main():
         sys.exit(2)
     import torch
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 3}))
+        return
     print("bench: building workload", file=sys.stderr, flush=True)
     sc, r, params, cam, target = build_problem(dev, args.splats)
     P, Nn, S, A = params
@@ -178,6 +194,8 @@ def main():
         step()
     torch.cuda.synchronize()
     timer.active = True
+    from cudaExtensions import sampling
+    sampling.profile_sampler_begin()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.time()
@@ -188,14 +206,17 @@ def main():
     torch.cuda.synchronize()
     wall = time.time() - t0
     timer.active = False
+    ks_ms, ks_n = sampling.profile_sampler_end()
     dev_ms = e0.elapsed_time(e1)
     ms_per_step = max(dev_ms, wall * 1e3) / args.steps
     fps = 1000.0 / ms_per_step
     ops = timer.summary(args.steps)
     Np, Pp, k = sc.n, sc.width * sc.height, 40
     nbytes = algorithmic_bytes(Np, Pp, k)
-    dom = max(ops, key=ops.get)
-    achieved = nbytes[dom] / (ops[dom] * 1e-3) / 1e9
+    # dominant kernel: the sampler kernel (one launch per view), timed with HIP events
+    # around each launch on its own stream; its algorithmic bytes are the sample op's
+    ks_avg = ks_ms / max(ks_n, 1)
+    achieved = nbytes["sample"] / (ks_avg * 1e-3) / 1e9
     frame_bytes = 68 * Pp * k + 332 * Np + 24 * Pp
     line = {
         "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
@@ -204,8 +225,9 @@ def main():
         "config": {"workload": "1M surfels on 64 ellipsoids, one 512x512 view, k=40, precision 0.01, plain rates",
                    "splats": Np, "width": sc.width, "height": sc.height, "samples_per_pixel": k,
                    "parallelism": "none (1 GPU)"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": "k_sample", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES,
+                     "kernel_ms": ks_avg, "kernel_launches": ks_n, "algorithmic_bytes": nbytes["sample"],
                      "frame_frac": frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "ops_ms": {k_: round(v, 4) for k_, v in sorted(ops.items())},
     }

@@ -47,6 +47,14 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
                uint64_t seed, int32_t* out, int* errFlag, void* ws, size_t wsBytes,
                ude_stream_t stream);
 
+/* Profiling aid (no counterpart in the reference): between begin and end,
+ * every launch of the sampler kernel (the per-pixel FGT descent + sequential
+ * Poisson pass, one per view) is bracketed by HIP events on its stream.  end
+ * waits for the recorded events and returns the summed kernel time and the
+ * number of launches.  Host-thread-unsafe; meant for benchmarks. */
+int ude_profile_sampler_begin(void);
+int ude_profile_sampler_end(double* totalMs, int* launches);
+
 /* sampling.covariance -- replaces cudaCovariance (sampling.cpp:96-134,
  * samplingCuda.cu:323-396).  out: (bn, N, 2, 2). */
 int ude_covariance(int bn, int N, int W, int H, double f, double smoothing, const float* pts,

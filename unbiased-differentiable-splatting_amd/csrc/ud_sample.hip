@@ -22,6 +22,9 @@
 #include "ud_device.h"
 #include "../../include/udsplat.h"
 
+#include <utility>
+#include <vector>
+
 using namespace ud;
 
 namespace {
@@ -892,12 +895,28 @@ __device__ __forceinline__ float key_of(unsigned long long v) { return __uint_as
 // records, for each walked node with more than k sources, the weights of its
 // two children in a trie of descent states; the ceil(sqrt(k)) descents then
 // read weights and capacities from the trie and evaluate the few remaining
-// weights (below clusters that were not walked) on demand.  The top-k list of
-// (key, splat) pairs (ascending keys, ties keep insertion order) lives in LDS,
-// slot-major.
+// weights (below clusters that were not walked) on demand.  The reference
+// keeps the k smallest keys in a sorted insertion list (ties keep insertion
+// order, sequentialPoissonRadixTree.cuh:15-41); the same set and order are the
+// k smallest (key, draw number) words, kept here in a per-lane max-heap in LDS
+// (slot-major) and heap-sorted once at the end: O(log k) LDS steps per
+// accepted key instead of a lane-divergent O(k) shift.
 // Trie of descent states, one slot array per lane in device memory, fields
 // interleaved by lane ([entry][field][lane]) so a wavefront touching the same
 // entry index issues one coalesced access.
+// per-depth arrays of the current DFS path (private memory; every index is
+// wave-uniform)
+struct DepthF {
+    float a[kMaxDepth];
+    __device__ float get(int i) const { return a[i]; }
+    __device__ void set(int i, float v) { a[i] = v; }
+};
+struct DepthI {
+    int a[kMaxDepth];
+    __device__ int get(int i) const { return a[i]; }
+    __device__ void set(int i, int v) { a[i] = v; }
+};
+
 struct TrieRef {
     float UD_GLOBAL* base;
     long stride;                    // lanes in the grid
@@ -907,6 +926,26 @@ struct TrieRef {
     __device__ void setKids(int e, int v) const { set(e, 3, __int_as_float(v)); }
     __device__ void init(int e, float wUnset) const { set(e, 0, 1.0f); set(e, 1, wUnset); set(e, 2, wUnset); setKids(e, 0); }
 };
+
+// Replace the top of a per-lane max-heap of n words (LDS, slot-major
+// [slot][lane]) by x and restore the heap order.
+__device__ __forceinline__ void heap_replace_top(unsigned long long* list, int lane, int n, unsigned long long x)
+{
+    int i = 0;
+    while (true) {
+        int c = 2 * i + 1;
+        if (c >= n) break;
+        unsigned long long vc = list[c * 64 + lane];
+        if (c + 1 < n) {
+            unsigned long long vr = list[(c + 1) * 64 + lane];
+            if (vr > vc) { vc = vr; c = c + 1; }
+        }
+        if (vc <= x) break;
+        list[i * 64 + lane] = vc;
+        i = c;
+    }
+    list[i * 64 + lane] = x;
+}
 
 template <int KMAX>
 __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint32_t seedLo, uint32_t seedHi,
@@ -935,8 +974,11 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
     const int ti = tile / tilesY, tj = tile - (tile / tilesY) * tilesY;
     const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
     const bool valid = (i < W) && (j < H);
+    // per-lane bounded max-heap of the k smallest (key, draw number) words; the
+    // padding word (key +inf, draw 0xffffffff) is larger than any real entry
     const unsigned long long empty = ((unsigned long long)0x7f800000u << 32) | 0xffffffffull;
     for (int s = 0; s < k; s++) list[s * 64 + lane] = empty;
+    int firstPos[16];                                   // per launch: first leaf position drawn
     const int n = vs->nc;
     const float prec = vs->prec;
     const bool runTree = n > 0 && vs->err == 0;          // wave-uniform
@@ -958,15 +1000,17 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
         __shared__ int s_right[kMaxDepth], s_count[kMaxDepth];
         __shared__ unsigned long long s_walk[kMaxDepth];
         __shared__ float s_coef[(kMaxDegree + 1) * (kMaxDegree + 1)];
-        float pacc[kMaxDepth];
-        int pent[kMaxDepth];
+        // per-depth accumulators and trie entries of the current path
+        DepthF pacc;
+        DepthI pent;
+        int big = 0;                 // leading path nodes with more than k sources
         uint64_t rightBits = 0;      // bit d: node at depth d is its parent's right child
         int d = 0, nd = 0;
         bool act = valid;
         bool overflow = false;
         auto report = [&](int dd, float w) {
             if (dd == 0) { total = w; return; }
-            int pe = pent[dd - 1];
+            int pe = pent.get(dd - 1);
             if (pe >= 0) tr.set(pe, ((rightBits >> dd) & 1ull) ? 2 : 1, w);
         };
         while (true) {
@@ -979,11 +1023,10 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                 wv = eval_node_staged(T, g, nd, qx, qy, act, s_coef, lane);
                 if (act) {
                     // ancestors whose weight a descent can compare: depth 0 and every
-                    // node whose parent has more than k sources
-                    for (int a = 0; a < d; a++) {
-                        if (a > 0 && s_count[a - 1] <= k) break;
-                        pacc[a] = pacc[a] + wv;
-                    }
+                    // node whose parent has more than k sources (a prefix of the path,
+                    // since source counts shrink with depth)
+                    const int amax = d < big + 1 ? d : big + 1;
+                    for (int a = 0; a < amax; a++) pacc.set(a, pacc.get(a) + wv);
                 }
             } else {
                 walk = act && !truncated(g, qx, qy);
@@ -996,19 +1039,20 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                 s_walk[d] = wm;
                 if (act && !walk) report(d, 0.0f);       // truncated for this lane
                 if (walk) {
-                    pacc[d] = 0.0f;
+                    pacc.set(d, 0.0f);
                     int e = -1;
                     if (d == 0) e = 0;
-                    else if (lk.count > k && pent[d - 1] >= 0 && nt < DFSMAX) {
+                    else if (lk.count > k && pent.get(d - 1) >= 0 && nt < DFSMAX) {
                         e = nt++;
                         tr.init(e, unset);
-                        int pe = pent[d - 1];
+                        int pe = pent.get(d - 1);
                         int pk = tr.kids(pe);
                         if ((rightBits >> d) & 1ull) tr.setKids(pe, (pk & 0xffff) | ((e + 1) << 16));
                         else tr.setKids(pe, (pk & ~0xffff) | (e + 1));
                     }
-                    pent[d] = e;
+                    pent.set(d, e);
                 }
+                if (big == d && __builtin_amdgcn_readfirstlane(lk.count) > k) big = d + 1;
                 d++;
                 nd = lk.left;
                 rightBits &= ~(1ull << d);
@@ -1027,8 +1071,9 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                     break;
                 }
                 d--;
+                if (big > d) big = d;
                 act = (s_walk[d] >> lane) & 1ull;
-                if (act) report(d, pacc[d]);
+                if (act) report(d, pacc.get(d));
             }
             if (done) break;
         }
@@ -1088,17 +1133,16 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                 // single Hermite coefficient 1, so E(leaf) = e^{-tx^2} e^{-ty^2}.
                 const int cnt = T.lnk[nd].count;
                 const int first = (nd >= L0) ? nd - L0 : T.aux[nd].first;
+                firstPos[la] = first;
                 for (int kk0 = 0; kk0 < cnt; kk0 += 4) {
                     u32x4 c4 = { (uint32_t)(kk0 >> 2), (uint32_t)la, pixel, view };
                     u32x4 rnd = philox4x32_10(c4, seedLo, seedHi);
                     float2 xy[4];
-                    int ids4[4];
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
                         int pos = first + kk0 + q;
                         bool ok = kk0 + q < cnt;
                         xy[q] = ok ? leafXY[pos] : make_float2(0.0f, 0.0f);
-                        ids4[q] = ok ? leafId[pos] : -1;
                     }
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
@@ -1111,16 +1155,12 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                         float p = fmaxf(0.0f, wgt);
                         uint32_t x = (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w;
                         float key = u01(x) / p;
-                        unsigned long long last = list[(k - 1) * 64 + lane];
-                        if (!(key < key_of(last))) continue;
-                        unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) | (uint32_t)ids4[q];
-                        int pos = k - 1;
-                        while (pos > 0) {
-                            unsigned long long prev = list[(pos - 1) * 64 + lane];
-                            if (key < key_of(prev)) { list[pos * 64 + lane] = prev; pos--; }
-                            else break;
-                        }
-                        list[pos * 64 + lane] = ent;
+                        if (!(key < __builtin_huge_valf())) continue;
+                        // (key, draw number) in one ordered word: keys are >= 0, so
+                        // their bit patterns order like the floats
+                        unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) |
+                                                 (uint32_t)((la << 16) | (kk0 + q));
+                        if (ent < list[lane]) heap_replace_top(list, lane, k, ent);
                     }
                 }
             }
@@ -1129,8 +1169,17 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
     }
     __builtin_amdgcn_wave_barrier();
     if (valid) {
+        // heap -> ascending order (heapsort), then draw numbers -> splat ids
+        for (int end = k - 1; end > 0; end--) {
+            unsigned long long x = list[end * 64 + lane];
+            list[end * 64 + lane] = list[lane];
+            heap_replace_top(list, lane, end, x);
+        }
         int32_t* o = out + ((long)j + (long)H * i) * k;
-        for (int s = 0; s < k; s++) o[s] = (int32_t)(uint32_t)(list[s * 64 + lane] & 0xffffffffull);
+        for (int s = 0; s < k; s++) {
+            uint32_t dn = (uint32_t)(list[s * 64 + lane] & 0xffffffffull);
+            o[s] = dn == 0xffffffffu ? -1 : leafId[firstPos[dn >> 16] + (int)(dn & 0xffffu)];
+        }
     }
 }
 
@@ -1206,16 +1255,45 @@ int num_launches(int k)
 // trie entries per lane: room for the walk plus 32 new entries per descent
 int trie_entries(int k) { return 192 + 32 * num_launches(k); }
 
+// Optional timing of the sampler kernel: while enabled, every k_sample launch
+// is bracketed by a pair of HIP events recorded on its stream (single host
+// thread; events are pooled and reused across profiling windows).
+struct SamplerTiming {
+    bool on = false;
+    int used = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+};
+SamplerTiming g_timing;
+
+bool timing_pair(hipEvent_t* a, hipEvent_t* b)
+{
+    if (!g_timing.on) return false;
+    if (g_timing.used == (int)g_timing.pool.size()) {
+        hipEvent_t x, y;
+        if (hipEventCreate(&x) != hipSuccess) return false;
+        if (hipEventCreate(&y) != hipSuccess) { (void)hipEventDestroy(x); return false; }
+        g_timing.pool.push_back({x, y});
+    }
+    *a = g_timing.pool[g_timing.used].first;
+    *b = g_timing.pool[g_timing.used].second;
+    g_timing.used++;
+    return true;
+}
+
 template <int KMAX>
 void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32_t view, const TreeWs& w,
                    int32_t* out, int* err, hipStream_t st)
 {
     unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
     size_t lds = (size_t)k * 64 * sizeof(unsigned long long);
+    hipEvent_t e0, e1;
+    bool timed = timing_pair(&e0, &e1);
+    if (timed) (void)hipEventRecord(e0, st);
     hipLaunchKernelGGL((k_sample<KMAX>), dim3(tiles), dim3(64), lds, st, W, H, k, num_launches(k), seedLo, seedHi,
                        view, (const ViewState*)w.vs, (const NodeGeo*)w.geo, (const NodeLnk*)w.lnk,
                        (const NodeAux*)w.aux, (const float*)w.coef, w.trie, trie_entries(k),
                        (const float2*)w.leafXY, (const int32_t*)w.idsA, out, err);
+    if (timed) (void)hipEventRecord(e1, st);
 }
 
 } // namespace
@@ -1258,6 +1336,31 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+int ude_profile_sampler_begin(void)
+{
+    g_timing.on = true;
+    g_timing.used = 0;
+    return UDE_OK;
+}
+
+int ude_profile_sampler_end(double* totalMs, int* launches)
+{
+    double tot = 0.0;
+    int n = g_timing.used;
+    g_timing.on = false;
+    for (int q = 0; q < n; q++) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(g_timing.pool[q].second) != hipSuccess) return UDE_ERR_LAUNCH;
+        if (hipEventElapsedTime(&ms, g_timing.pool[q].first, g_timing.pool[q].second) != hipSuccess)
+            return UDE_ERR_LAUNCH;
+        tot += ms;
+    }
+    g_timing.used = 0;
+    if (totalMs) *totalMs = tot;
+    if (launches) *launches = n;
+    return UDE_OK;
 }
 
 } // extern "C"

@@ -70,3 +70,17 @@ def check_errors():
     """Synchronising check of the last sample() call's device error flag."""
     if last_error is not None and int(last_error.item()) != 0:
         raise RuntimeError("sampling.sample: descent exceeded the 63-level trace bound")
+
+
+def profile_sampler_begin():
+    """Start bracketing every sampler-kernel launch with HIP events (benchmarks)."""
+    _lib.check(_lib.lib().ude_profile_sampler_begin(), "profile_sampler_begin")
+
+
+def profile_sampler_end():
+    """Stop; returns (summed sampler-kernel milliseconds, number of launches)."""
+    import ctypes
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int(0)
+    _lib.check(_lib.lib().ude_profile_sampler_end(ctypes.byref(ms), ctypes.byref(n)), "profile_sampler_end")
+    return ms.value, n.value
