@@ -24,9 +24,10 @@ for _p in (PKG, ROOT):
 
 METRIC = "fwd+bwd frames/s at 1M splats 512² (learned vs plain rates, equal grad variance)"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the sampler kernel at the 1M-splat 512x512 k=40 frame, from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (calibrated against a copy kernel); None until measured
-TRAFFIC_BYTES = None
+# HBM bytes per launch of the sampler kernel at the 1M-splat 512x512 k=40 frame: rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE on k_sample, each scaled by the same counter on a 1 GiB
+# device copy (bench.py --calibrate-copy); measured offline, not by this script
+TRAFFIC_BYTES = 3.374e9
 
 
 def build_problem(dev, n_override=None):
