@@ -47,6 +47,18 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
                uint64_t seed, int32_t* out, int* errFlag, void* ws, size_t wsBytes,
                ude_stream_t stream);
 
+/* Workspace for ude_fmm_weights (one view's tree, reused across views). */
+size_t ude_fmm_weights_workspace_bytes(int N, int W, int H);
+
+/* sampling.approximateFMMWeights -- replaces cudaApproxWeights / batchApproxWeights
+ * (FMMKdTreeSampling/sampling.cpp:136-175, samplingCuda.cu:398-493): the
+ * fixed-order (degree 6) Hermite estimate of sum_i exp(-|q - s_i|^2 / 2h^2) /
+ * (2 pi sqrt(h)) at every pixel q.  out: float (bn, W, H), pixel (i, j) at
+ * j + H i (the reference writes j + W i, identical for W == H). */
+int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, double precision,
+                    const float* pts, const float* nrm, const float* sd, int sdBatch,
+                    float* out, void* ws, size_t wsBytes, ude_stream_t stream);
+
 /* Profiling aid (no counterpart in the reference): between begin and end,
  * every launch of the sampler kernel (the per-pixel FGT descent + sequential
  * Poisson pass, one per view) is bracketed by HIP events on its stream.  end

@@ -41,6 +41,7 @@ def lib():
         _lib.o_tree_export.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 10 + [ctypes.c_long, ctypes.c_void_p]
         _lib.o_tree_eval.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float] + [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
         _lib.o_sample_view.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3
+        _lib.o_fmm_weights_view.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 4
         _lib.o_sample_view_rows.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 +
                                            [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_int] * 2 +
                                            [ctypes.c_void_p])
@@ -317,4 +318,25 @@ def sample(W, H, f, pts, nrm, sd, k, smoothing=0.0, precision=0.01, seed=0):
     for b in range(bn):
         sdb = sd[b] if sd.shape[0] > 1 else sd[0]
         outs.append(sample_view(W, H, f, pts[b], nrm[b], sdb, k, smoothing, precision, seed, b))
+    return np.stack(outs)
+
+
+def fmm_weights_view(W, H, f, pts, nrm, sd, smoothing=0.0, precision=0.01):
+    """sampling.approximateFMMWeights for one view: float32 (W, H)."""
+    pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float32)
+    out = np.zeros((W, H), np.float32)
+    rc = lib().o_fmm_weights_view(ctypes.c_int(pts.shape[0]), ctypes.c_int(W), ctypes.c_int(H), ctypes.c_double(f),
+                                  ctypes.c_double(smoothing), ctypes.c_double(precision), _p(pts), _p(nrm), _p(sd),
+                                  _p(out))
+    if rc != 0:
+        raise RuntimeError("oracle fmm_weights_view failed: %d" % rc)
+    return out
+
+
+def fmm_weights(W, H, f, pts, nrm, sd, smoothing=0.0, precision=0.01):
+    """sampling.approximateFMMWeights for a batch: (bn, W, H)."""
+    outs = []
+    for b in range(pts.shape[0]):
+        sdb = sd[b] if sd.shape[0] > 1 else sd[0]
+        outs.append(fmm_weights_view(W, H, f, pts[b], nrm[b], sdb, smoothing, precision))
     return np.stack(outs)

@@ -1183,6 +1183,140 @@ done:
     return rc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* approximateFMMWeights: fixed-order (p = 6) Hermite FGT density image        */
+/* (sampling.cpp:136-175, samplingCuda.cu:398-493; tree buildRadixTree.cuh:   */
+/* 449-606, 722-807; evaluation transformRadixTree.cuh:15-69, 152-222).        */
+/* Same h / cull / Morton / Karras tree as sample; every internal node gets   */
+/* the bound hermiteError(r, n, 6); nodes with bound <= precision carry the   */
+/* degree-6 moments of their leaves times the leaf coefficient               */
+/* c = 1 / (2 pi sqrt(h)); leaves carry c alone.  Per pixel an explicit LIFO  */
+/* stack from the root evaluates a popped node if it is a leaf or accepted,   */
+/* else pushes its left then right child when the child's box meets the disc */
+/* of the global query radius around the pixel (so right subtrees are summed */
+/* first).  Output x-major (j + H i); the reference writes j + W i, which is */
+/* the same image for W == H and out of bounds otherwise.                    */
+/* ------------------------------------------------------------------------ */
+#define O_FIXED_P 6
+
+static int o_box_meets_disc(const OTree* T, int nd, float qx, float qy, float rad)
+{
+    const float* b = T->box + 4 * nd;
+    float cx = fminf(fmaxf(qx, b[0]), b[2]);
+    float cy = fminf(fmaxf(qy, b[1]), b[3]);
+    float dx = qx - cx, dy = qy - cy;
+    float d2 = 0.0f;
+    d2 = d2 + dx * dx;
+    d2 = d2 + dy * dy;
+    return d2 < rad * rad;
+}
+
+int o_fmm_weights_view(int N, int W, int H, double f, double smoothing, double prec,
+                       const float* pts, const float* nrm, const float* sd, float* out)
+{
+    for (long q = 0; q < (long)W * H; q++) out[q] = 0.0f;
+    if (N <= 0) return 0;
+    OTree T;
+    memset(&T, 0, sizeof(T));
+    long nn = 2L * N;
+    T.morton = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    T.ids = (int32_t*)malloc(sizeof(int32_t) * N);
+    int32_t* ib = (int32_t*)malloc(sizeof(int32_t) * nn * 8);
+    T.left = ib; T.right = ib + nn; T.parent = ib + 2 * nn; T.count = ib + 3 * nn;
+    T.deg = ib + 4 * nn; T.off = ib + 5 * nn; T.first = ib + 6 * nn; T.last = ib + 7 * nn;
+    float* fb = (float*)malloc(sizeof(float) * nn * 7);
+    T.box = fb; T.cen = fb + 4 * nn; T.r2 = fb + 6 * nn;
+    long cap = 290L * N;
+    T.coef = (float*)malloc(sizeof(float) * cap);
+    int rc = o_tree_build(N, W, H, f, smoothing, prec, pts, nrm, sd, &T, cap, NULL);
+    int nc = T.nc;
+    float* c6 = NULL;
+    char* acc6 = NULL;
+    if (rc < 0 || nc == 0) goto done;
+    {
+        const int P1 = O_FIXED_P + 1;
+        const float h = T.h, fprec = T.prec;
+        const float c = (float)(1.0 / (2.0 * O_PI * (double)sqrtf(h)));
+        const int L0 = nc - 1;
+        c6 = (float*)malloc(sizeof(float) * (size_t)P1 * P1 * (nc > 1 ? nc - 1 : 1));
+        acc6 = (char*)calloc(nc > 1 ? nc - 1 : 1, 1);
+        double ifact[P1];
+        double fa = 1.0;
+        ifact[0] = 1.0;
+        for (int a = 1; a < P1; a++) { fa = fa * a; ifact[a] = 1.0 / fa; }
+        for (int nd = 0; nd < L0; nd++) {
+            const float* b = T.box + 4 * nd;
+            float mn = 0.0f;
+            for (int d = 0; d < 2; d++) {
+                float cc = T.cen[2 * nd + d];
+                mn = fmaxf(mn, fmaxf(cc - b[d], b[2 + d] - cc));
+            }
+            float rr = mn / h;
+            acc6[nd] = o_hermite_error(rr, (float)T.count[nd], O_FIXED_P) <= fprec;
+            if (!acc6[nd]) continue;
+            double acc[P1 * P1];
+            for (int q = 0; q < P1 * P1; q++) acc[q] = 0.0;
+            for (int pos = T.first[nd]; pos <= T.last[nd]; pos++) {
+                int lf = L0 + pos;
+                float tx = (T.cen[2 * lf] - T.cen[2 * nd]) * T.is2h;
+                float ty = (T.cen[2 * lf + 1] - T.cen[2 * nd + 1]) * T.is2h;
+                double mx[P1], my[P1], px = 1.0, py = 1.0;
+                for (int a = 0; a < P1; a++) {
+                    mx[a] = px * ifact[a]; my[a] = py * ifact[a];
+                    px = px * (double)tx; py = py * (double)ty;
+                }
+                for (int a = 0; a < P1; a++)
+                    for (int bb = 0; bb < P1; bb++) acc[a * P1 + bb] += mx[a] * my[bb];
+            }
+            for (int q = 0; q < P1 * P1; q++) c6[(size_t)nd * P1 * P1 + q] = (float)((double)c * acc[q]);
+        }
+        const float rad = T.radius;
+        #pragma omp parallel for schedule(dynamic, 16)
+        for (long pix = 0; pix < (long)W * H; pix++) {
+            int i = (int)(pix / H), j = (int)(pix % H);
+            float qx = (float)i, qy = (float)j;
+            int stack[256];
+            int sp = 0;
+            stack[sp++] = 0;
+            float w = 0.0f;
+            while (sp > 0) {
+                int nd = stack[--sp];
+                int leaf = nd >= L0;
+                if (leaf || acc6[nd]) {
+                    float tx = (qx - T.cen[2 * nd]) * T.is2h;
+                    float ty = (qy - T.cen[2 * nd + 1]) * T.is2h;
+                    float ex = oexpf(-(tx * tx)), ey = oexpf(-(ty * ty));
+                    float v = 0.0f;
+                    if (leaf) {
+                        v = v + (c * ex) * ey;
+                    } else {
+                        float hx[P1], hy[P1];
+                        hx[0] = ex; hy[0] = ey;
+                        hx[1] = (2.0f * tx) * ex; hy[1] = (2.0f * ty) * ey;
+                        for (int n = 2; n < P1; n++) {
+                            hx[n] = (2.0f * tx) * hx[n - 1] - (2.0f * (float)(n - 1)) * hx[n - 2];
+                            hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
+                        }
+                        const float* cc = c6 + (size_t)nd * P1 * P1;
+                        for (int a = 0; a < P1; a++)
+                            for (int bb = 0; bb < P1; bb++) v = v + (cc[a * P1 + bb] * hx[a]) * hy[bb];
+                    }
+                    w = w + v;
+                    continue;
+                }
+                if (sp + 2 > 256) { w = NAN; break; }
+                if (o_box_meets_disc(&T, T.left[nd], qx, qy, rad)) stack[sp++] = T.left[nd];
+                if (o_box_meets_disc(&T, T.right[nd], qx, qy, rad)) stack[sp++] = T.right[nd];
+            }
+            out[j + (long)H * i] = w;
+        }
+    }
+done:
+    free(c6); free(acc6);
+    free(T.morton); free(T.ids); free(ib); free(fb); free(T.coef);
+    return rc < 0 ? rc : 0;
+}
+
 /* Tree export for stage-by-stage checks.  Caller provides arrays sized for N
  * sorted entries / 2N-1 nodes / coefCap coefficients; returns nc or < 0. */
 int o_tree_export(int N, int W, int H, double f, double smoothing, double prec,

@@ -33,6 +33,8 @@ constexpr int kMaxDegree = 16;
 constexpr int kMaxDepth = 64;        // descent trace bound (reference STACKSIZE)
 constexpr int kCullTile = 1024;      // elements per workgroup in the cull pass
 constexpr int kSortTile = 2048;      // elements per workgroup in a sort pass
+constexpr int kFixedDegree = 6;      // approximateFMMWeights expansion order (hermitefmm.cuh:11)
+constexpr int kFixedCoefs = (kFixedDegree + 1) * (kFixedDegree + 1);
 constexpr int kScanTile = 1024;      // elements per workgroup in a scan pass
 
 struct ViewState {
@@ -88,7 +90,11 @@ size_t carve(void* ws, int N, int W, int H, int k, TreeWs* t)
     long nCull = (n + kCullTile - 1) / kCullTile;
     long nSort = (n + kSortTile - 1) / kSortTile;
     long nScan = (nn + kScanTile - 1) / kScanTile;
-    long coefCap = (long)(kMaxDegree + 1) * (kMaxDegree + 1) * (n - 1 > 0 ? n - 1 : 1) + n;
+    // k > 0: sampler (dynamic degrees, descent trie); k <= 0: fixed-order density
+    // (degree-6 moments of accepted internal nodes only, no trie)
+    const bool fixedOrder = k <= 0;
+    long coefCap = fixedOrder ? (long)kFixedCoefs * (n - 1 > 0 ? n - 1 : 1)
+                              : (long)(kMaxDegree + 1) * (kMaxDegree + 1) * (n - 1 > 0 ? n - 1 : 1) + n;
     size_t off = 0;
     char* base = (char*)ws;
     auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return base ? (void*)(base + o) : nullptr; };
@@ -109,7 +115,7 @@ size_t carve(void* ws, int N, int W, int H, int k, TreeWs* t)
     w.seeds = (int*)take(4 * n);
     w.momList = (int*)take(4 * n);
     long lanes = (long)((W + 7) / 8) * ((H + 7) / 8) * 64;
-    w.trie = (float*)take(16L * trie_entries(k) * lanes);
+    w.trie = fixedOrder ? nullptr : (float*)take(16L * trie_entries(k) * lanes);
     w.leafXY = (float2*)take(8 * n);
     w.coef = (float*)take(4 * coefCap);
     w.coefCap = coefCap;
@@ -1191,6 +1197,12 @@ __global__ __launch_bounds__(256) void k_leaf_ids(const ViewState* __restrict__ 
     if (t < n) aux[n - 1 + t].pad = ids[t];
 }
 
+__global__ void k_fill_f32(float* p, long n, float v)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
 __global__ void k_fill_idx(int32_t* p, long n)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1202,15 +1214,201 @@ __global__ void k_copy_err(const ViewState* vs, int* err)
     if (threadIdx.x == 0 && vs->err) atomicOr(err, vs->err);
 }
 
+// ---------------------------------------------------------------------------
+// approximateFMMWeights: fixed-order (degree 6) Hermite density image
+// (sampling.cpp:136-175, samplingCuda.cu:398-493, buildRadixTree.cuh:449-606,
+// transformRadixTree.cuh:15-69, 152-222).
+// ---------------------------------------------------------------------------
+
+// leaf coefficient 1 / (2 pi sqrt(h)) (buildRadixTree.cuh:580)
+__device__ __forceinline__ float fixed_leaf_coef(float h)
+{
+    return (float)(1.0 / (2.0 * UD_PI * (double)sqrtf(h)));
+}
+
+// Internal nodes whose degree-6 truncation bound meets the precision carry an
+// expansion: flag them in NodeAux.pad and give them a 49-float block.
+__global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, const NodeGeo* __restrict__ geo,
+                                                      const NodeLnk* __restrict__ lnk, NodeAux* aux, int* list)
+{
+    const int n = vs->nc;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool acc = false;
+    if (t < n - 1) {
+        NodeGeo g = geo[t];
+        float mn = 0.0f;
+        mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
+        mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
+        float rr = mn / vs->h;
+        acc = hermite_error(rr, (float)lnk[t].count, kFixedDegree) <= vs->prec;
+    }
+    unsigned long long m = __ballot(acc);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (m != 0ull) {
+        int leader = __ffsll((long long)m) - 1;
+        if (lane == leader) base = atomicAdd(&vs->nmom, __popcll(m));
+        base = __shfl(base, leader, 64);
+    }
+    if (t < n - 1) {
+        int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+        aux[t].pad = acc ? 1 : 0;
+        aux[t].off = acc ? slot * kFixedCoefs : 0;
+        if (acc) list[slot] = t;
+    }
+}
+
+// Degree-6 moments, scaled by the leaf coefficient: thread (node, row a) sums
+// row a over the node's leaf range in FP64 (leaf order).  36 nodes x 7 rows
+// per 256-thread workgroup.
+constexpr int kFixedNodesPerBlock = 36;
+__global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
+                                                       const NodeAux* __restrict__ aux, const int* __restrict__ list,
+                                                       float* __restrict__ coef)
+{
+    const int t = threadIdx.x;
+    if (t >= kFixedNodesPerBlock * (kFixedDegree + 1)) return;
+    const long slot = (long)blockIdx.x * kFixedNodesPerBlock + t / (kFixedDegree + 1);
+    const int ax = t % (kFixedDegree + 1);
+    if (slot >= vs->nmom) return;
+    const int nd = list[slot];
+    const int L0 = vs->nc - 1;
+    const float is2h = vs->is2h;
+    const double c = (double)fixed_leaf_coef(vs->h);
+    NodeGeo g = geo[nd];
+    NodeAux a = aux[nd];
+    const double ifx = kInvFact[ax];
+    double acc[kFixedDegree + 1];
+#pragma unroll
+    for (int bb = 0; bb <= kFixedDegree; bb++) acc[bb] = 0.0;
+    for (int pos = a.first; pos <= a.last; pos++) {
+        NodeGeo lf = geo[L0 + pos];
+        float tx = (lf.cx - g.cx) * is2h;
+        float ty = (lf.cy - g.cy) * is2h;
+        double px = 1.0;
+        for (int q = 0; q < ax; q++) px = px * (double)tx;
+        double mx = px * ifx;
+        double py = 1.0;
+#pragma unroll
+        for (int bb = 0; bb <= kFixedDegree; bb++) {
+            acc[bb] += mx * (py * ifact_c(bb));
+            py = py * (double)ty;
+        }
+    }
+#pragma unroll
+    for (int bb = 0; bb <= kFixedDegree; bb++)
+        coef[a.off + ax * (kFixedDegree + 1) + bb] = (float)(c * acc[bb]);
+}
+
+__device__ __forceinline__ bool box_meets_disc(const NodeGeo& g, float qx, float qy, float rad)
+{
+    float cx = fminf(fmaxf(qx, g.lox), g.hix);
+    float cy = fminf(fmaxf(qy, g.loy), g.hiy);
+    float dx = qx - cx, dy = qy - cy;
+    float d2 = 0.0f;
+    d2 = d2 + dx * dx;
+    d2 = d2 + dy * dy;
+    return d2 < rad * rad;
+}
+
+// One lane per pixel, 8x8 pixel tile per wavefront, 4 tiles per workgroup.
+// The reference's explicit stack pushes the left then the right child of an
+// unaccepted node when the child's box meets the query disc, so it sums the
+// right subtree first; the same order is walked here without a stack, via
+// parent links.
+__global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState* __restrict__ vs,
+                                                  const NodeGeo* geo, const NodeLnk* lnk, const NodeAux* aux,
+                                                  const float* coef, float* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int tilesY = (H + 7) >> 3;
+    const int ntiles = ((W + 7) >> 3) * tilesY;
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= ntiles) return;
+    const int ti = wv / tilesY, tj = wv - (wv / tilesY) * tilesY;
+    const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
+    if (i >= W || j >= H) return;
+    const int n = vs->nc;
+    float w = 0.0f;
+    if (n > 0) {
+        TreeView T;
+        T.geo = (const NodeGeo UD_GLOBAL*)geo;
+        T.lnk = (const NodeLnk UD_GLOBAL*)lnk;
+        T.aux = (const NodeAux UD_GLOBAL*)aux;
+        T.coef = (const float UD_GLOBAL*)coef;
+        T.s2h = vs->s2h;
+        T.is2h = vs->is2h;
+        const int L0 = n - 1;
+        const float c = fixed_leaf_coef(vs->h);
+        const float rad = vs->radius;
+        const float qx = (float)i, qy = (float)j;
+        int nd = 0;
+        while (true) {
+            NodeGeo g = ld_geo(T, nd);
+            bool climb = true;
+            if (nd >= L0 || T.aux[nd].pad) {
+                float tx = (qx - g.cx) * T.is2h;
+                float ty = (qy - g.cy) * T.is2h;
+                float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+                float v = 0.0f;
+                if (nd >= L0) {
+                    v = v + (c * ex) * ey;
+                } else {
+                    float hy[kFixedDegree + 1];
+                    hy[0] = ey;
+                    hy[1] = (2.0f * ty) * ey;
+#pragma unroll
+                    for (int q = 2; q <= kFixedDegree; q++)
+                        hy[q] = (2.0f * ty) * hy[q - 1] - (2.0f * (float)(q - 1)) * hy[q - 2];
+                    const float UD_GLOBAL* cc = T.coef + T.aux[nd].off;
+                    float hprev = 0.0f, ha = ex;
+#pragma unroll
+                    for (int a = 0; a <= kFixedDegree; a++) {
+                        if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
+                        else if (a >= 2) {
+                            float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+                            hprev = ha; ha = hn;
+                        }
+#pragma unroll
+                        for (int b = 0; b <= kFixedDegree; b++)
+                            v = v + (cc[a * (kFixedDegree + 1) + b] * ha) * hy[b];
+                    }
+                }
+                w = w + v;
+            } else {
+                NodeLnk lk = ld_lnk(T, nd);
+                if (box_meets_disc(ld_geo(T, lk.right), qx, qy, rad)) { nd = lk.right; climb = false; }
+                else if (box_meets_disc(ld_geo(T, lk.left), qx, qy, rad)) { nd = lk.left; climb = false; }
+            }
+            if (!climb) continue;
+            // climb to the next unvisited left sibling (the right one was taken first)
+            bool done = true;
+            while (nd != 0) {
+                int p = T.lnk[nd].parent;
+                NodeLnk pl = ld_lnk(T, p);
+                if (nd == pl.right && box_meets_disc(ld_geo(T, pl.left), qx, qy, rad)) {
+                    nd = pl.left;
+                    done = false;
+                    break;
+                }
+                nd = p;
+            }
+            if (done) break;
+        }
+    }
+    out[(long)j + (long)H * i] = w;
+}
+
 inline unsigned grid1(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-int build_tree(int N, int W, int H, double f, double smoothing, double prec, const float* pts, const float* nrm,
-               const float* sd, const TreeWs& w, hipStream_t st)
+// Tree structure of one view: bandwidth, cull, sort, hierarchy, bottom-up
+// boxes / centres / counts / dynamic degrees.
+int build_structure(int N, int W, int H, double f, double smoothing, double prec, const float* pts, const float* nrm,
+                    const float* sd, const TreeWs& w, hipStream_t st)
 {
     float Wf = (float)W, Hf = (float)H, ff = (float)f;
     long nCull = (N + kCullTile - 1) / kCullTile;
     long nSort = (N + kSortTile - 1) / kSortTile;
-    long nScan = (2L * N + kScanTile - 1) / kScanTile;
     k_zero_state<<<1, 64, 0, st>>>(w.vs);
     k_bandwidth<<<grid1(N, 256), 256, 0, st>>>(N, Wf, Hf, ff, (float)smoothing, pts, nrm, sd, w.vs);
     k_view_params<<<1, 64, 0, st>>>(w.vs, Wf, Hf, prec);
@@ -1234,6 +1432,14 @@ int build_tree(int N, int W, int H, double f, double smoothing, double prec, con
     k_fill_i32<<<grid1(N, 256), 256, 0, st>>>(w.flags, w.vs, 0, 0);
     k_bottom_up_local<<<grid1(N, kFanBlock), kFanBlock, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.flags, w.seeds);
     k_bottom_up_global<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.flags, w.seeds);
+    hipError_t e0 = hipGetLastError();
+    return e0 == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+// Dynamic-degree Hermite moments of the sampler's tree.
+int build_moments(int N, const TreeWs& w, hipStream_t st)
+{
+    long nScan = (2L * N + kScanTile - 1) / kScanTile;
     k_coef_sizes<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.geo, w.scanTmp, w.momList);
     k_scan_tiles<<<(unsigned)nScan, 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks);
     k_scan_small<<<1, 1024, 0, st>>>(w.scanBlocks, (int)nScan, &w.vs->ncoef);
@@ -1324,7 +1530,8 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
         const float* pb = pts + 3L * N * b;
         const float* nb = nrm + 3L * N * b;
         const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
-        int rc = build_tree(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
+        int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
+        if (rc == UDE_OK) rc = build_moments(N, w, st);
         if (rc != UDE_OK) return rc;
         int32_t* ob = out + perView * b;
         if (k <= 16) launch_sample<16>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
@@ -1333,6 +1540,40 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
         else if (k <= 128) launch_sample<128>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
         else launch_sample<256>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
         k_copy_err<<<1, 64, 0, st>>>(w.vs, errFlag);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+size_t ude_fmm_weights_workspace_bytes(int N, int W, int H)
+{
+    return carve(nullptr, N, W, H, 0, nullptr) + 256;
+}
+
+int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, double prec,
+                    const float* pts, const float* nrm, const float* sd, int sdBatch,
+                    float* out, void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || N < 0) return UDE_ERR_ARG;
+    long perView = (long)W * H;
+    if (N == 0) {
+        k_fill_f32<<<grid1(perView * bn, 256), 256, 0, st>>>(out, perView * bn, 0.0f);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+    }
+    if (wsBytes < ude_fmm_weights_workspace_bytes(N, W, H)) return UDE_ERR_WORKSPACE;
+    TreeWs w;
+    carve(ws, N, W, H, 0, &w);
+    unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
+    for (int b = 0; b < bn; b++) {
+        const float* pb = pts + 3L * N * b;
+        const float* nb = nrm + 3L * N * b;
+        const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
+        int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
+        if (rc != UDE_OK) return rc;
+        k_fixed_select<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.momList);
+        k_fixed_moments<<<grid1(N, kFixedNodesPerBlock), 256, 0, st>>>(w.vs, w.geo, w.aux, w.momList, w.coef);
+        k_fmm_eval<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, w.vs, w.geo, w.lnk, w.aux, w.coef, out + perView * b);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;

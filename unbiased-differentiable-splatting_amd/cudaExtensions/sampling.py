@@ -66,6 +66,26 @@ def covariance(width, height, focallength, points, normals, stdDevs, smoothing):
     return out
 
 
+def approximateFMMWeights(width, height, focallength, points, normals, colors, stdDevs, smoothing, precision):
+    """Fixed-order (degree 6) Hermite FGT density image (bn, width, height):
+    sum_i exp(-|q - s_i|^2 / 2h^2) / (2 pi sqrt(h)) at every pixel q
+    (sampling.cpp:136-175).  ``colors`` is accepted and unused, as in the reference."""
+    dev = _lib.require_gpu(points, normals, stdDevs)
+    points = _lib.f32(points, "points")
+    normals = _lib.f32(normals, "normals")
+    stdDevs = _lib.f32(stdDevs, "stdDevs")
+    bn, N, _ = points.shape
+    W, H = int(width), int(height)
+    out = torch.empty((bn, W, H), dtype=torch.float32, device=dev)
+    L = _lib.lib()
+    ws = _lib.workspace(L.ude_fmm_weights_workspace_bytes(N, W, H), dev)
+    rc = L.ude_fmm_weights(bn, N, W, H, float(focallength), float(smoothing), float(precision),
+                           _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs), stdDevs.shape[0],
+                           _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream(dev))
+    _lib.check(rc, "sampling.approximateFMMWeights")
+    return out
+
+
 def check_errors():
     """Synchronising check of the last sample() call's device error flag."""
     if last_error is not None and int(last_error.item()) != 0:
