@@ -59,6 +59,27 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
                     const float* pts, const float* nrm, const float* sd, int sdBatch,
                     float* out, void* ws, size_t wsBytes, ude_stream_t stream);
 
+/* Workspace for ude_sample_poisson. */
+size_t ude_poisson_workspace_bytes(int N, int W, int H);
+
+/* Poisson-rate sampling with inclusion probabilities (no counterpart in the
+ * reference, whose plain-Poisson kernel poissonRadixTree.cuh:70-200 is dead
+ * code with unknown inclusion probabilities).  At pixel q every culled splat i
+ * is included independently with the defensive-mixture probability
+ *   pi_i = min(1, r/2 (g0_i(q) / max(D0(q), c0) + g1_i(q) / max(D1(q), c1))),
+ * r = rate * tileScale, g_b = c_b exp(-|q - s_i|^2 / 2 h_b^2), c_b = 1 / (2 pi
+ * sqrt(h_b)), h_0 the tree's bandwidth, h_1 = max(h_0, 99.9th percentile of
+ * the visible splats' sqrt(lambda_max(V_i))), D_b the fixed-order density of h_b
+ * (nothing where D0 < precision).  tileScale (optional, (bn, ceil(W/16),
+ * ceil(H/16))) multiplies the rate per 16x16 tile.  Outputs (bn, W, H, kcap):
+ * splat ids (-1 padded) and 1 / pi_i (0 padded), so sum_s w_s a_s / pi_s is a
+ * Horvitz-Thompson estimate of sum_i w_i a_i.  A pixel with more than kcap
+ * inclusions sets bit 4 of *errFlag. */
+int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double smoothing, double precision,
+                       double rate, const float* tileScale, const float* pts, const float* nrm, const float* sd,
+                       int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, int* errFlag, void* ws,
+                       size_t wsBytes, ude_stream_t stream);
+
 /* Profiling aid (no counterpart in the reference): between begin and end,
  * every launch of the sampler kernel (the per-pixel FGT descent + sequential
  * Poisson pass, one per view) is bracketed by HIP events on its stream.  end

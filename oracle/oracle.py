@@ -42,6 +42,10 @@ def lib():
         _lib.o_tree_eval.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float] + [ctypes.c_void_p] * 5 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
         _lib.o_sample_view.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3
         _lib.o_fmm_weights_view.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 4
+        _lib.o_poisson_view.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 4 + [ctypes.c_void_p] * 4 +
+                                       [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3)
+        _lib.o_poisson_wide_h.restype = ctypes.c_float
+        _lib.o_poisson_wide_h.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3
         _lib.o_sample_view_rows.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 +
                                            [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_int] * 2 +
                                            [ctypes.c_void_p])
@@ -340,3 +344,44 @@ def fmm_weights(W, H, f, pts, nrm, sd, smoothing=0.0, precision=0.01):
         sdb = sd[b] if sd.shape[0] > 1 else sd[0]
         outs.append(fmm_weights_view(W, H, f, pts[b], nrm[b], sdb, smoothing, precision))
     return np.stack(outs)
+
+
+def poisson_view(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.01, seed=0, view=0, tile_scale=None):
+    """Poisson-rate sampling of one view: (ids (W,H,kcap) int32, 1/pi (W,H,kcap) float32,
+    densities (D0, D1) (2,W,H) float32, overflow flag).  tile_scale: optional
+    (ceil(W/16), ceil(H/16)) rate multipliers."""
+    pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float32)
+    ts = None
+    if tile_scale is not None:
+        ts = _c(tile_scale, np.float32)
+        assert ts.shape == ((W + 15) // 16, (H + 15) // 16)
+    ids = np.zeros((W, H, kcap), np.int32)
+    ipi = np.zeros((W, H, kcap), np.float32)
+    dens = np.zeros((2, W, H), np.float32)
+    rc = lib().o_poisson_view(ctypes.c_int(pts.shape[0]), ctypes.c_int(W), ctypes.c_int(H), ctypes.c_int(kcap),
+                              ctypes.c_double(f), ctypes.c_double(smoothing), ctypes.c_double(precision),
+                              ctypes.c_double(rate), _p(ts) if ts is not None else None, _p(pts), _p(nrm), _p(sd),
+                              ctypes.c_uint32(seed & 0xFFFFFFFF), ctypes.c_uint32((seed >> 32) & 0xFFFFFFFF),
+                              ctypes.c_uint32(view), _p(ids), _p(ipi), _p(dens))
+    if rc < 0:
+        raise RuntimeError("oracle poisson_view failed: %d" % rc)
+    return ids, ipi, dens, rc
+
+
+def poisson(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.01, seed=0, tile_scale=None):
+    """Poisson-rate sampling for a batch: (ids (bn,W,H,kcap), 1/pi (bn,W,H,kcap))."""
+    ids, ipi = [], []
+    for b in range(pts.shape[0]):
+        sdb = sd[b] if sd.shape[0] > 1 else sd[0]
+        a, p_, _, _ = poisson_view(W, H, f, pts[b], nrm[b], sdb, kcap, rate, smoothing, precision, seed, b,
+                                   None if tile_scale is None else tile_scale[b])
+        ids.append(a); ipi.append(p_)
+    return np.stack(ids), np.stack(ipi)
+
+
+def poisson_wide_h(W, H, f, pts, nrm, sd, smoothing=0.0, precision=0.01):
+    """The wide proposal bandwidth of Poisson-rate sampling for one view."""
+    pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float32)
+    return lib().o_poisson_wide_h(ctypes.c_int(pts.shape[0]), ctypes.c_int(W), ctypes.c_int(H), ctypes.c_double(f),
+                                  ctypes.c_double(smoothing), ctypes.c_double(precision), _p(pts), _p(nrm), _p(sd))
+

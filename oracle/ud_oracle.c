@@ -653,8 +653,10 @@ static float o_dot3f(const float* a, const float* b)
     return x;
 }
 
-/* per-splat isotropic bandwidth h_k, samplingCuda.cu:106-163, FP32 */
-float o_iso_h(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+/* screen-space covariance V (row-major) of one splat in FP32, the operation
+ * order of samplingCuda.cu:106-150 */
+static void o_cov_f32(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd,
+                      float smoothing, float V[4])
 {
     float sx, sy;
     o_screen_f32(Wf, Hf, ff, p, &sx, &sy);
@@ -685,7 +687,16 @@ float o_iso_h(float Wf, float Hf, float ff, const float* p, const float* n, cons
     /* V = M1 * J^T + s I */
     float V00 = M00 * J00 + M01 * J01, V01 = M00 * J10 + M01 * J11;
     float V10 = M10 * J00 + M11 * J01, V11 = M10 * J10 + M11 * J11;
-    V00 = V00 + smoothing; V01 = V01 + 0.0f; V10 = V10 + 0.0f; V11 = V11 + smoothing;
+    V[0] = V00 + smoothing; V[1] = V01 + 0.0f; V[2] = V10 + 0.0f; V[3] = V11 + smoothing;
+}
+
+/* per-splat isotropic bandwidth h_k = 1 / sqrt(lambda_min(V^-1)) before the cap,
+ * samplingCuda.cu:106-163, FP32 */
+float o_iso_hraw(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+{
+    float V[4];
+    o_cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V);
+    float V00 = V[0], V01 = V[1], V10 = V[2], V11 = V[3];
     float dV = V00 * V11 - V01 * V10;
     float idV = 1.0f / dV;
     float I00 = V11 * idV, I01 = -V01 * idV, I10 = -V10 * idV, I11 = V00 * idV;
@@ -695,7 +706,27 @@ float o_iso_h(float Wf, float Hf, float ff, const float* p, const float* n, cons
     float sq = sqrtf(disc);
     float l1 = 0.5f * tr + sq, l2 = 0.5f * tr - sq;
     float lk = fminf(l1, l2);
-    return fminf(1.0f, 1.0f / sqrtf(lk));
+    return 1.0f / sqrtf(lk);
+}
+
+/* splat bandwidth for the Poisson-rate proposal: sqrt(lambda_max(V)) from V
+ * itself (well conditioned, unlike the smallest eigenvalue of V^-1 for
+ * splats seen at grazing angles); NaN for degenerate geometry */
+float o_prop_h(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+{
+    float V[4];
+    o_cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V);
+    float tr = V[0] + V[3];
+    float det = V[0] * V[3] - V[1] * V[2];
+    float disc = 0.25f * (tr * tr) - det;
+    float lmax = 0.5f * tr + sqrtf(fmaxf(disc, 0.0f));
+    return sqrtf(lmax);
+}
+
+/* samplingCuda.cu:160: the isotropic bandwidth is capped at one pixel */
+float o_iso_h(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+{
+    return fminf(1.0f, o_iso_hraw(Wf, Hf, ff, p, n, sd, smoothing));
 }
 
 static uint32_t o_expand_bits(uint32_t v)
@@ -802,8 +833,9 @@ int o_tree_build(int N, int W, int H, double f, double smoothing_d, double prec_
     /* bandwidth: order-independent fixed-point mean of finite h_k */
     uint64_t hsum = 0, hcnt = 0;
     for (int i = 0; i < N; i++) {
-        float hk = o_iso_h(Wf, Hf, ff, pts + 3 * i, nrm + 3 * i, sd + 2 * i, smoothing);
-        if (hk_out) hk_out[i] = hk;
+        float hraw = o_iso_hraw(Wf, Hf, ff, pts + 3 * i, nrm + 3 * i, sd + 2 * i, smoothing);
+        float hk = fminf(1.0f, hraw);
+        if (hk_out) hk_out[i] = hraw;
         if (isfinite(hk)) { hsum += (uint64_t)((double)hk * 1099511627776.0); hcnt++; }
     }
     double hd = ((double)hsum * (1.0 / 1099511627776.0)) / (double)hcnt;
@@ -1211,110 +1243,361 @@ static int o_box_meets_disc(const OTree* T, int nd, float qx, float qy, float ra
     return d2 < rad * rad;
 }
 
+/* Fixed-order tree of one view: the sampler's tree structure plus, for up to
+ * two bandwidths, degree-6 moments (times the leaf coefficient c = 1 / (2 pi
+ * sqrt(h))) at the internal nodes whose truncation bound meets the precision.
+ * Bandwidth 0 is the tree's h; bandwidth 1 (Poisson-rate sampling only) is the
+ * wide proposal bandwidth hw = max(h, 99.9th percentile of the visible splats'
+ * h_i = sqrt(lambda_max(V_i)), rounded up to 1/16 px, at most 16) (degenerate
+ * splats skipped). */
+typedef struct {
+    float h, is2h, c, radius;
+    float* c6;
+    char* acc6;
+} OBand;
+
+typedef struct {
+    OTree T;
+    int32_t* ib;
+    float* fb;
+    OBand band[2];
+    int nband;
+    int ok;
+} OFixed;
+
+static void o_fixed_free(OFixed* F)
+{
+    for (int b = 0; b < 2; b++) { free(F->band[b].c6); free(F->band[b].acc6); }
+    free(F->T.morton); free(F->T.ids); free(F->ib); free(F->fb); free(F->T.coef);
+}
+
+/* query radius of a bandwidth, samplingCuda.cu:251-252 */
+static float o_band_radius(float h, double prec)
+{
+    float normalization = (float)(2.0 * O_PI * (double)sqrtf(h));
+    return (float)(sqrt(2.0) * (double)h * sqrt(-log(prec) - log((double)normalization)));
+}
+
+static void o_band_build(OFixed* F, OBand* B, float h, double prec)
+{
+    const OTree* T = &F->T;
+    const int nc = T->nc, L0 = nc - 1;
+    const int P1 = O_FIXED_P + 1;
+    B->h = h;
+    B->is2h = 1.0f / sqrtf((2.0f * h) * h);
+    B->c = (float)(1.0 / (2.0 * O_PI * (double)sqrtf(h)));
+    B->radius = o_band_radius(h, prec);
+    B->c6 = (float*)malloc(sizeof(float) * (size_t)P1 * P1 * (nc > 1 ? nc - 1 : 1));
+    B->acc6 = (char*)calloc(nc > 1 ? nc - 1 : 1, 1);
+    double ifact[O_FIXED_P + 1];
+    double fa = 1.0;
+    ifact[0] = 1.0;
+    for (int a = 1; a < P1; a++) { fa = fa * a; ifact[a] = 1.0 / fa; }
+    for (int nd = 0; nd < L0; nd++) {
+        const float* b = T->box + 4 * nd;
+        float mn = 0.0f;
+        for (int d = 0; d < 2; d++) {
+            float cc = T->cen[2 * nd + d];
+            mn = fmaxf(mn, fmaxf(cc - b[d], b[2 + d] - cc));
+        }
+        float rr = mn / h;
+        B->acc6[nd] = o_hermite_error(rr, (float)T->count[nd], O_FIXED_P) <= T->prec;
+        if (!B->acc6[nd]) continue;
+        double acc[(O_FIXED_P + 1) * (O_FIXED_P + 1)];
+        for (int q = 0; q < P1 * P1; q++) acc[q] = 0.0;
+        for (int pos = T->first[nd]; pos <= T->last[nd]; pos++) {
+            int lf = L0 + pos;
+            float tx = (T->cen[2 * lf] - T->cen[2 * nd]) * B->is2h;
+            float ty = (T->cen[2 * lf + 1] - T->cen[2 * nd + 1]) * B->is2h;
+            double mx[O_FIXED_P + 1], my[O_FIXED_P + 1], px = 1.0, py = 1.0;
+            for (int a = 0; a < P1; a++) {
+                mx[a] = px * ifact[a]; my[a] = py * ifact[a];
+                px = px * (double)tx; py = py * (double)ty;
+            }
+            for (int a = 0; a < P1; a++)
+                for (int bb = 0; bb < P1; bb++) acc[a * P1 + bb] += mx[a] * my[bb];
+        }
+        for (int q = 0; q < P1 * P1; q++) B->c6[(size_t)nd * P1 * P1 + q] = (float)((double)B->c * acc[q]);
+    }
+}
+
+#define O_PROPOSAL_HMAX 16.0f
+#define O_HBINS 256              /* bandwidth histogram: 1/16 px bins over [0, 16) */
+
+static int o_fixed_build(int N, int W, int H, double f, double smoothing, double prec,
+                         const float* pts, const float* nrm, const float* sd, int nband, OFixed* F)
+{
+    memset(F, 0, sizeof(*F));
+    OTree* T = &F->T;
+    long nn = 2L * (N > 0 ? N : 1);
+    T->morton = (uint32_t*)malloc(sizeof(uint32_t) * (N > 0 ? N : 1));
+    T->ids = (int32_t*)malloc(sizeof(int32_t) * (N > 0 ? N : 1));
+    F->ib = (int32_t*)malloc(sizeof(int32_t) * nn * 8);
+    int32_t* ib = F->ib;
+    T->left = ib; T->right = ib + nn; T->parent = ib + 2 * nn; T->count = ib + 3 * nn;
+    T->deg = ib + 4 * nn; T->off = ib + 5 * nn; T->first = ib + 6 * nn; T->last = ib + 7 * nn;
+    F->fb = (float*)malloc(sizeof(float) * nn * 7);
+    T->box = F->fb; T->cen = F->fb + 4 * nn; T->r2 = F->fb + 6 * nn;
+    long cap = 290L * (N > 0 ? N : 1);
+    T->coef = (float*)malloc(sizeof(float) * cap);
+    if (N <= 0) return 0;
+    int rc = o_tree_build(N, W, H, f, smoothing, prec, pts, nrm, sd, T, cap, NULL);
+    if (rc < 0) return rc;
+    if (T->nc == 0) return 0;
+    F->nband = nband;
+    o_band_build(F, &F->band[0], T->h, prec);
+    if (nband > 1) {
+        /* 99.9th percentile of the visible splats' bandwidths, on a 1/16 px grid */
+        long hist[O_HBINS];
+        memset(hist, 0, sizeof(hist));
+        for (int pos = 0; pos < T->nc; pos++) {
+            int id = T->ids[pos];
+            float hi = o_prop_h((float)W, (float)H, (float)f, pts + 3L * id, nrm + 3L * id, sd + 2L * id,
+                                (float)smoothing);
+            if (!(hi >= 0.0f)) continue;                               /* degenerate: NaN */
+            int bin = hi < O_PROPOSAL_HMAX ? (int)(hi * 16.0f) : O_HBINS - 1;
+            hist[bin]++;
+        }
+        long total = 0;
+        for (int b = 0; b < O_HBINS; b++) total += hist[b];
+        long target = total - total / 1000, cum = 0;
+        int qb = 0;
+        for (int b = 0; b < O_HBINS; b++) { cum += hist[b]; if (cum >= target) { qb = b; break; } }
+        float hw = fmaxf(T->h, (float)(qb + 1) * 0.0625f);
+        o_band_build(F, &F->band[1], hw, prec);
+    }
+    F->ok = 1;
+    return 0;
+}
+
+/* c exp(-|q - y|^2 / 2h^2) of one point, in evaluation order */
+static float o_band_point(const OBand* B, float qx, float qy, float x, float y)
+{
+    float tx = (qx - x) * B->is2h;
+    float ty = (qy - y) * B->is2h;
+    float ex = oexpf(-(tx * tx)), ey = oexpf(-(ty * ty));
+    float v = 0.0f;
+    v = v + (B->c * ex) * ey;
+    return v;
+}
+
+/* density c sum_i exp(-|q - s_i|^2 / 2h^2) at one pixel for bandwidth b
+ * (reference stack order) */
+static float o_fixed_eval(const OFixed* F, int b, float qx, float qy)
+{
+    const OTree* T = &F->T;
+    const OBand* B = &F->band[b];
+    if (!F->ok) return 0.0f;
+    const int P1 = O_FIXED_P + 1;
+    const int L0 = T->nc - 1;
+    int stack[256];
+    int sp = 0;
+    stack[sp++] = 0;
+    float w = 0.0f;
+    while (sp > 0) {
+        int nd = stack[--sp];
+        if (nd >= L0) { w = w + o_band_point(B, qx, qy, T->cen[2 * nd], T->cen[2 * nd + 1]); continue; }
+        if (B->acc6[nd]) {
+            float tx = (qx - T->cen[2 * nd]) * B->is2h;
+            float ty = (qy - T->cen[2 * nd + 1]) * B->is2h;
+            float ex = oexpf(-(tx * tx)), ey = oexpf(-(ty * ty));
+            float hx[O_FIXED_P + 1], hy[O_FIXED_P + 1];
+            hx[0] = ex; hy[0] = ey;
+            hx[1] = (2.0f * tx) * ex; hy[1] = (2.0f * ty) * ey;
+            for (int n = 2; n < P1; n++) {
+                hx[n] = (2.0f * tx) * hx[n - 1] - (2.0f * (float)(n - 1)) * hx[n - 2];
+                hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
+            }
+            const float* cc = B->c6 + (size_t)nd * P1 * P1;
+            float v = 0.0f;
+            for (int a = 0; a < P1; a++)
+                for (int bb = 0; bb < P1; bb++) v = v + (cc[a * P1 + bb] * hx[a]) * hy[bb];
+            w = w + v;
+            continue;
+        }
+        if (sp + 2 > 256) return NAN;
+        if (o_box_meets_disc(T, T->left[nd], qx, qy, B->radius)) stack[sp++] = T->left[nd];
+        if (o_box_meets_disc(T, T->right[nd], qx, qy, B->radius)) stack[sp++] = T->right[nd];
+    }
+    return w;
+}
+
 int o_fmm_weights_view(int N, int W, int H, double f, double smoothing, double prec,
                        const float* pts, const float* nrm, const float* sd, float* out)
 {
     for (long q = 0; q < (long)W * H; q++) out[q] = 0.0f;
     if (N <= 0) return 0;
-    OTree T;
-    memset(&T, 0, sizeof(T));
-    long nn = 2L * N;
-    T.morton = (uint32_t*)malloc(sizeof(uint32_t) * N);
-    T.ids = (int32_t*)malloc(sizeof(int32_t) * N);
-    int32_t* ib = (int32_t*)malloc(sizeof(int32_t) * nn * 8);
-    T.left = ib; T.right = ib + nn; T.parent = ib + 2 * nn; T.count = ib + 3 * nn;
-    T.deg = ib + 4 * nn; T.off = ib + 5 * nn; T.first = ib + 6 * nn; T.last = ib + 7 * nn;
-    float* fb = (float*)malloc(sizeof(float) * nn * 7);
-    T.box = fb; T.cen = fb + 4 * nn; T.r2 = fb + 6 * nn;
-    long cap = 290L * N;
-    T.coef = (float*)malloc(sizeof(float) * cap);
-    int rc = o_tree_build(N, W, H, f, smoothing, prec, pts, nrm, sd, &T, cap, NULL);
-    int nc = T.nc;
-    float* c6 = NULL;
-    char* acc6 = NULL;
-    if (rc < 0 || nc == 0) goto done;
-    {
-        const int P1 = O_FIXED_P + 1;
-        const float h = T.h, fprec = T.prec;
-        const float c = (float)(1.0 / (2.0 * O_PI * (double)sqrtf(h)));
-        const int L0 = nc - 1;
-        c6 = (float*)malloc(sizeof(float) * (size_t)P1 * P1 * (nc > 1 ? nc - 1 : 1));
-        acc6 = (char*)calloc(nc > 1 ? nc - 1 : 1, 1);
-        double ifact[P1];
-        double fa = 1.0;
-        ifact[0] = 1.0;
-        for (int a = 1; a < P1; a++) { fa = fa * a; ifact[a] = 1.0 / fa; }
-        for (int nd = 0; nd < L0; nd++) {
-            const float* b = T.box + 4 * nd;
-            float mn = 0.0f;
-            for (int d = 0; d < 2; d++) {
-                float cc = T.cen[2 * nd + d];
-                mn = fmaxf(mn, fmaxf(cc - b[d], b[2 + d] - cc));
-            }
-            float rr = mn / h;
-            acc6[nd] = o_hermite_error(rr, (float)T.count[nd], O_FIXED_P) <= fprec;
-            if (!acc6[nd]) continue;
-            double acc[P1 * P1];
-            for (int q = 0; q < P1 * P1; q++) acc[q] = 0.0;
-            for (int pos = T.first[nd]; pos <= T.last[nd]; pos++) {
-                int lf = L0 + pos;
-                float tx = (T.cen[2 * lf] - T.cen[2 * nd]) * T.is2h;
-                float ty = (T.cen[2 * lf + 1] - T.cen[2 * nd + 1]) * T.is2h;
-                double mx[P1], my[P1], px = 1.0, py = 1.0;
-                for (int a = 0; a < P1; a++) {
-                    mx[a] = px * ifact[a]; my[a] = py * ifact[a];
-                    px = px * (double)tx; py = py * (double)ty;
-                }
-                for (int a = 0; a < P1; a++)
-                    for (int bb = 0; bb < P1; bb++) acc[a * P1 + bb] += mx[a] * my[bb];
-            }
-            for (int q = 0; q < P1 * P1; q++) c6[(size_t)nd * P1 * P1 + q] = (float)((double)c * acc[q]);
-        }
-        const float rad = T.radius;
+    OFixed F;
+    int rc = o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, 1, &F);
+    if (rc == 0 && F.ok) {
         #pragma omp parallel for schedule(dynamic, 16)
         for (long pix = 0; pix < (long)W * H; pix++) {
             int i = (int)(pix / H), j = (int)(pix % H);
-            float qx = (float)i, qy = (float)j;
-            int stack[256];
-            int sp = 0;
-            stack[sp++] = 0;
-            float w = 0.0f;
-            while (sp > 0) {
-                int nd = stack[--sp];
-                int leaf = nd >= L0;
-                if (leaf || acc6[nd]) {
-                    float tx = (qx - T.cen[2 * nd]) * T.is2h;
-                    float ty = (qy - T.cen[2 * nd + 1]) * T.is2h;
-                    float ex = oexpf(-(tx * tx)), ey = oexpf(-(ty * ty));
-                    float v = 0.0f;
-                    if (leaf) {
-                        v = v + (c * ex) * ey;
-                    } else {
-                        float hx[P1], hy[P1];
-                        hx[0] = ex; hy[0] = ey;
-                        hx[1] = (2.0f * tx) * ex; hy[1] = (2.0f * ty) * ey;
-                        for (int n = 2; n < P1; n++) {
-                            hx[n] = (2.0f * tx) * hx[n - 1] - (2.0f * (float)(n - 1)) * hx[n - 2];
-                            hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
-                        }
-                        const float* cc = c6 + (size_t)nd * P1 * P1;
-                        for (int a = 0; a < P1; a++)
-                            for (int bb = 0; bb < P1; bb++) v = v + (cc[a * P1 + bb] * hx[a]) * hy[bb];
-                    }
-                    w = w + v;
-                    continue;
-                }
-                if (sp + 2 > 256) { w = NAN; break; }
-                if (o_box_meets_disc(&T, T.left[nd], qx, qy, rad)) stack[sp++] = T.left[nd];
-                if (o_box_meets_disc(&T, T.right[nd], qx, qy, rad)) stack[sp++] = T.right[nd];
-            }
-            out[j + (long)H * i] = w;
+            out[j + (long)H * i] = o_fixed_eval(&F, 0, (float)i, (float)j);
         }
     }
-done:
-    free(c6); free(acc6);
-    free(T.morton); free(T.ids); free(ib); free(fb); free(T.coef);
+    o_fixed_free(&F);
     return rc < 0 ? rc : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Poisson-rate sampling with inclusion probabilities (no reference kernel;  */
+/* the plain-Poisson idea of poissonRadixTree.cuh:70-200 made exact).        */
+/* Per pixel q, every culled splat i is included independently with the      */
+/* defensive-mixture probability (Hesterberg 1995)                           */
+/*   pi_i(q) = min(1, lambda [ a g0_i(q) / N0(q) + (1 - a) g1_i(q) / N1(q) ]) */
+/* g_b = c_b exp(-|q - s_i|^2 / 2 h_b^2) for the tree bandwidth h_0 = h and  */
+/* the wide bandwidth h_1 = hw >= 99.9% of the visible splats' bandwidths,  */
+/* N_b = max(D_b, c_b) with D_b the fixed-order density of band b, a = 1/2;  */
+/* so sum_i pi_i stays near lambda and, for splats not wider than hw, EWA    */
+/* weight / pi stays bounded.  No samples where D0 < precision.              */
+/* Realised by a left-to-right tree walk: a node whose bound U (the same     */
+/* expression at the point of its box closest to q) is zero is skipped; a    */
+/* leaf, or a node with count U <= 1, is thinned: candidates at rate U by    */
+/* geometric skips over its contiguous leaf range, each kept with            */
+/* probability pi_i / U.  Uniforms: Philox4x32-10 at counter (draw >> 2,     */
+/* 0x80000000, pixel, view), word draw & 3, per-pixel draw counter.  Output: */
+/* ids and 1 / pi_i in discovery order, -1 / 0 padded.                       */
+/* ------------------------------------------------------------------------ */
+/* log(1 - u) for u in [0, 1]: series for small u, log otherwise (FP32, fixed order) */
+static float o_log1m(float u)
+{
+    if (u >= 1.0f) return -INFINITY;
+    if (u < 0.0625f) {
+        /* -(u + u^2/2 + ... + u^8/8) */
+        float p = 1.0f / 8.0f;
+        p = p * u + 1.0f / 7.0f;
+        p = p * u + 1.0f / 6.0f;
+        p = p * u + 1.0f / 5.0f;
+        p = p * u + 1.0f / 4.0f;
+        p = p * u + 1.0f / 3.0f;
+        p = p * u + 0.5f;
+        p = p * u + 1.0f;
+        return -(p * u);
+    }
+    return ologf(1.0f - u);
+}
+
+static float o_puniform(uint32_t seed_lo, uint32_t seed_hi, uint32_t view, uint32_t pixel, uint32_t draw)
+{
+    uint32_t ctr[4] = { draw >> 2, 0x80000000u, pixel, view };
+    uint32_t key[2] = { seed_lo, seed_hi };
+    uint32_t o[4];
+    o_philox4x32_10(ctr, key, o);
+    return (float)((o[draw & 3u] >> 8) + 1u) * 5.9604644775390625e-08f;
+}
+
+/* mixture proposal at point (x, y): s0 g0 + s1 g1 */
+static float o_mix(const OFixed* F, float s0, float s1, float qx, float qy, float x, float y)
+{
+    float v0 = o_band_point(&F->band[0], qx, qy, x, y);
+    float v1 = o_band_point(&F->band[1], qx, qy, x, y);
+    float m = s0 * v0;
+    m = m + s1 * v1;
+    return m;
+}
+
+static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float lambda,
+                           uint32_t seed_lo, uint32_t seed_hi, uint32_t view, int32_t* ids, float* ipi)
+{
+    const OTree* T = &F->T;
+    for (int s = 0; s < kcap; s++) { ids[s] = -1; ipi[s] = 0.0f; }
+    if (!F->ok) return 0;
+    float qx = (float)i, qy = (float)j;
+    float D0 = o_fixed_eval(F, 0, qx, qy);
+    if (!(D0 >= T->prec)) return 0;
+    float D1 = o_fixed_eval(F, 1, qx, qy);
+    /* normalisers floored at one splat's peak c_b: where the pixel sits in a gap,
+     * D_b (truncated at the query radius) misses the nearby splats it normalises */
+    const float half = 0.5f * lambda;
+    const float s0 = half / fmaxf(D0, F->band[0].c);
+    const float s1 = half / fmaxf(D1, F->band[1].c);
+    const int L0 = T->nc - 1;
+    const uint32_t pixel = (uint32_t)(j + H * i);
+    uint32_t draw = 0;
+    int ns = 0, over = 0;
+    int nd = 0;
+    while (1) {
+        const float* b = T->box + 4 * nd;
+        float cx = fminf(fmaxf(qx, b[0]), b[2]);
+        float cy = fminf(fmaxf(qy, b[1]), b[3]);
+        float U = fminf(1.0f, o_mix(F, s0, s1, qx, qy, cx, cy));
+        int descend = 0;
+        if (U > 0.0f) {
+            int leaf = nd >= L0;
+            int cnt = T->count[nd];
+            if (!leaf && (float)cnt * U > 1.0f) descend = 1;
+            else {
+                int first = leaf ? nd - L0 : T->first[nd];
+                int last = leaf ? nd - L0 : T->last[nd];
+                float l1 = o_log1m(U);
+                int pos = first - 1;
+                while (1) {
+                    float u = o_puniform(seed_lo, seed_hi, view, pixel, draw++);
+                    float r = ologf(u) / l1;             /* failures before the next candidate */
+                    if (!(r < (float)(last - pos))) break;
+                    pos += (int)r + 1;
+                    int lf = L0 + pos;
+                    float pi = fminf(1.0f, o_mix(F, s0, s1, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]));
+                    float a = o_puniform(seed_lo, seed_hi, view, pixel, draw++);
+                    if (a * U <= pi) {          /* P = pi / U (exactly 1 when pi = U) */
+                        if (ns < kcap) { ids[ns] = T->ids[pos]; ipi[ns] = 1.0f / pi; ns++; }
+                        else over = 1;
+                    }
+                }
+            }
+        }
+        if (descend) { nd = T->left[nd]; continue; }
+        /* next node in left-to-right order */
+        int done = 1;
+        while (nd != 0) {
+            int p = T->parent[nd];
+            if (nd == T->left[p]) { nd = T->right[p]; done = 0; break; }
+            nd = p;
+        }
+        if (done) break;
+    }
+    return over;
+}
+
+/* tscale (optional): per 16x16 pixel tile rate multiplier, (ceil(W/16), ceil(H/16));
+ * dens (optional, 2 W H): D0 then D1 per pixel */
+int o_poisson_view(int N, int W, int H, int kcap, double f, double smoothing, double prec, double lambda,
+                   const float* tscale, const float* pts, const float* nrm, const float* sd, uint32_t seed_lo,
+                   uint32_t seed_hi, uint32_t view, int32_t* ids, float* ipi, float* dens)
+{
+    const int tsH = (H + 15) / 16;
+    OFixed F;
+    int rc = (N > 0) ? o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, 2, &F) : 0;
+    if (N <= 0) memset(&F, 0, sizeof(F));
+    int over = 0;
+    if (rc == 0) {
+        #pragma omp parallel for schedule(dynamic, 16) reduction(|:over)
+        for (long pix = 0; pix < (long)W * H; pix++) {
+            int i = (int)(pix / H), j = (int)(pix % H);
+            float lam = (float)lambda;
+            if (tscale) lam = lam * tscale[(long)(i >> 4) * tsH + (j >> 4)];
+            over |= o_poisson_pixel(&F, i, j, H, kcap, lam, seed_lo, seed_hi, view,
+                                    ids + (long)kcap * pix, ipi + (long)kcap * pix);
+            if (dens) {
+                dens[pix] = F.ok ? o_fixed_eval(&F, 0, (float)i, (float)j) : 0.0f;
+                dens[(long)W * H + pix] = F.ok ? o_fixed_eval(&F, 1, (float)i, (float)j) : 0.0f;
+            }
+        }
+    }
+    if (N > 0) o_fixed_free(&F);
+    return rc < 0 ? rc : over;
+}
+
+/* wide proposal bandwidth of one view (for tests) */
+float o_poisson_wide_h(int N, int W, int H, double f, double smoothing, double prec,
+                       const float* pts, const float* nrm, const float* sd)
+{
+    OFixed F;
+    float hw = 0.0f;
+    if (N > 0 && o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, 2, &F) == 0 && F.ok) hw = F.band[1].h;
+    if (N > 0) o_fixed_free(&F);
+    return hw;
 }
 
 /* Tree export for stage-by-stage checks.  Caller provides arrays sized for N

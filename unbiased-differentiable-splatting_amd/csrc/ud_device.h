@@ -74,6 +74,25 @@ __device__ __forceinline__ float plogf(float x)
     return fmaf(ef, ln2hi, fmaf(ef, ln2lo, lm));
 }
 
+// log(1 - u) for u in [0, 1]: the log1p series -(u + u^2/2 + ... + u^8/8) for
+// u < 1/16 (truncation error < u^9 / 9 relative), plogf(1 - u) otherwise.
+__device__ __forceinline__ float plog1mf(float u)
+{
+    if (u >= 1.0f) return -__builtin_huge_valf();
+    if (u < 0.0625f) {
+        float p = 1.0f / 8.0f;
+        p = p * u + 1.0f / 7.0f;
+        p = p * u + 1.0f / 6.0f;
+        p = p * u + 1.0f / 5.0f;
+        p = p * u + 1.0f / 4.0f;
+        p = p * u + 1.0f / 3.0f;
+        p = p * u + 0.5f;
+        p = p * u + 1.0f;
+        return -(p * u);
+    }
+    return plogf(1.0f - u);
+}
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 counter-based generator (Salmon et al., SC'11).
 // ---------------------------------------------------------------------------

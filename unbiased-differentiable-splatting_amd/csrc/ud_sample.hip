@@ -37,6 +37,12 @@ constexpr int kFixedDegree = 6;      // approximateFMMWeights expansion order (h
 constexpr int kFixedCoefs = (kFixedDegree + 1) * (kFixedDegree + 1);
 constexpr int kScanTile = 1024;      // elements per workgroup in a scan pass
 
+// Gaussian bandwidth of a fixed-order density: h, 1 / sqrt(2 h^2), the leaf
+// coefficient 1 / (2 pi sqrt(h)) and the query radius
+struct Band {
+    float h, is2h, c, radius;
+};
+
 struct ViewState {
     unsigned long long hsum;
     unsigned int hcnt;
@@ -47,6 +53,9 @@ struct ViewState {
     int nseeds;       // block-spanning fan-in starts (bottom-up, second level)
     int nmom;         // internal nodes with a Hermite expansion
     int pad;
+    Band band[2];     // fixed-order densities: tree bandwidth, wide proposal bandwidth
+    int nfix[2];      // accepted internal nodes per band
+    int hhist[256];   // visible splats' proposal bandwidths, 1/16 px bins over [0, 16)
 };
 
 struct NodeGeo {      // traversal record
@@ -76,25 +85,33 @@ struct TreeWs {
     float2* leafXY;       // sorted position -> screen position of the leaf
     float* coef;
     long coefCap;
+    // fixed-order densities (approximateFMMWeights, Poisson-rate sampling)
+    int* fixOff[2];       // per internal node and band: coefficient offset, -1 = no expansion
+    int* fixList[2];      // per band: accepted internal nodes
+    float* fixCoef[2];    // per band: 49 coefficients per accepted node
+    float* hraw;          // per splat id: proposal bandwidth sqrt(lambda_max(V))
+    float* dens[2];       // per pixel and band: density D_b(q)
 };
+
+enum WsMode { kModeSample = 0, kModeFixed = 1, kModePoisson = 2 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 int trie_entries(int k);
 
 // carve the workspace; returns total bytes (ws may be null to only measure)
-size_t carve(void* ws, int N, int W, int H, int k, TreeWs* t)
+// mode kModeSample: dynamic degrees + descent trie; kModeFixed: degree-6 moments of
+// the accepted internal nodes only; kModePoisson: fixed tree + proposal bandwidths
+// + density image
+size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
 {
     long n = N > 0 ? N : 1;
     long nn = 2 * n;
     long nCull = (n + kCullTile - 1) / kCullTile;
     long nSort = (n + kSortTile - 1) / kSortTile;
     long nScan = (nn + kScanTile - 1) / kScanTile;
-    // k > 0: sampler (dynamic degrees, descent trie); k <= 0: fixed-order density
-    // (degree-6 moments of accepted internal nodes only, no trie)
-    const bool fixedOrder = k <= 0;
-    long coefCap = fixedOrder ? (long)kFixedCoefs * (n - 1 > 0 ? n - 1 : 1)
-                              : (long)(kMaxDegree + 1) * (kMaxDegree + 1) * (n - 1 > 0 ? n - 1 : 1) + n;
+    const bool fixedOrder = mode != kModeSample;
+    long coefCap = (long)(kMaxDegree + 1) * (kMaxDegree + 1) * (n - 1 > 0 ? n - 1 : 1) + n;
     size_t off = 0;
     char* base = (char*)ws;
     auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return base ? (void*)(base + o) : nullptr; };
@@ -117,16 +134,30 @@ size_t carve(void* ws, int N, int W, int H, int k, TreeWs* t)
     long lanes = (long)((W + 7) / 8) * ((H + 7) / 8) * 64;
     w.trie = fixedOrder ? nullptr : (float*)take(16L * trie_entries(k) * lanes);
     w.leafXY = (float2*)take(8 * n);
-    w.coef = (float*)take(4 * coefCap);
-    w.coefCap = coefCap;
+    w.coef = fixedOrder ? nullptr : (float*)take(4 * coefCap);
+    w.coefCap = fixedOrder ? 0 : coefCap;
+    w.hraw = nullptr;
+    for (int b = 0; b < 2; b++) { w.fixOff[b] = w.fixList[b] = nullptr; w.fixCoef[b] = w.dens[b] = nullptr; }
+    const int nb = mode == kModePoisson ? 2 : mode == kModeFixed ? 1 : 0;
+    for (int b = 0; b < nb; b++) {
+        w.fixOff[b] = (int*)take(4 * n);
+        w.fixList[b] = (int*)take(4 * n);
+        w.fixCoef[b] = (float*)take(4L * kFixedCoefs * (n - 1 > 0 ? n - 1 : 1));
+    }
+    if (mode == kModePoisson) {
+        w.hraw = (float*)take(4 * n);
+        w.dens[0] = (float*)take(4L * W * H);
+        w.dens[1] = (float*)take(4L * W * H);
+    }
     if (t) *t = w;
     return off;
 }
 
 // ---------------------------------------------------------------------------
-// Per-splat bandwidth (samplingCuda.cu:106-163) in FP32, fixed operation order.
-// ---------------------------------------------------------------------------
-__device__ float iso_h(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+// Screen-space covariance V (row-major) of one splat in FP32, the operation
+// order of samplingCuda.cu:106-150.
+__device__ void cov_f32(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd,
+                        float smoothing, float V[4])
 {
     float sx, sy;
     screen_f32(Wf, Hf, ff, p, sx, sy);
@@ -151,7 +182,16 @@ __device__ float iso_h(float Wf, float Hf, float ff, const float* p, const float
     float M10 = J10 * d0 + J11 * 0.0f, M11 = J10 * 0.0f + J11 * d1;
     float V00 = M00 * J00 + M01 * J01, V01 = M00 * J10 + M01 * J11;
     float V10 = M10 * J00 + M11 * J01, V11 = M10 * J10 + M11 * J11;
-    V00 = V00 + smoothing; V01 = V01 + 0.0f; V10 = V10 + 0.0f; V11 = V11 + smoothing;
+    V[0] = V00 + smoothing; V[1] = V01 + 0.0f; V[2] = V10 + 0.0f; V[3] = V11 + smoothing;
+}
+
+// Per-splat bandwidth before the one-pixel cap, 1 / sqrt(lambda_min(V^-1))
+// (samplingCuda.cu:106-163) in FP32, fixed operation order.
+__device__ float iso_hraw(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+{
+    float V[4];
+    cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V);
+    float V00 = V[0], V01 = V[1], V10 = V[2], V11 = V[3];
     float dV = V00 * V11 - V01 * V10;
     float idV = 1.0f / dV;
     float I00 = V11 * idV, I01 = -V01 * idV, I10 = -V10 * idV, I11 = V00 * idV;
@@ -161,19 +201,41 @@ __device__ float iso_h(float Wf, float Hf, float ff, const float* p, const float
     float sq = sqrtf(disc);
     float l1 = 0.5f * tr + sq, l2 = 0.5f * tr - sq;
     float lk = fminf(l1, l2);
-    return fminf(1.0f, 1.0f / sqrtf(lk));
+    return 1.0f / sqrtf(lk);
+}
+
+// Splat bandwidth for the Poisson-rate proposal: sqrt(lambda_max(V)) from V
+// itself (well conditioned, unlike the smallest eigenvalue of V^-1 for splats
+// seen at grazing angles); NaN for degenerate geometry.
+__device__ float prop_h(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+{
+    float V[4];
+    cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V);
+    float tr = V[0] + V[3];
+    float det = V[0] * V[3] - V[1] * V[2];
+    float disc = 0.25f * (tr * tr) - det;
+    float lmax = 0.5f * tr + sqrtf(fmaxf(disc, 0.0f));
+    return sqrtf(lmax);
+}
+
+// isotropic bandwidth capped at one pixel (samplingCuda.cu:160)
+__device__ float iso_h(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
+{
+    return fminf(1.0f, iso_hraw(Wf, Hf, ff, p, n, sd, smoothing));
 }
 
 __global__ void k_zero_state(ViewState* vs)
 {
     if (threadIdx.x == 0) {
         vs->hsum = 0; vs->hcnt = 0; vs->nc = 0; vs->err = 0; vs->ncoef = 0; vs->nseeds = 0; vs->nmom = 0;
+        vs->nfix[0] = 0; vs->nfix[1] = 0;
     }
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) vs->hhist[b] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_bandwidth(int N, float Wf, float Hf, float ff, float smoothing,
                                                    const float* __restrict__ pts, const float* __restrict__ nrm,
-                                                   const float* __restrict__ sd, ViewState* vs)
+                                                   const float* __restrict__ sd, ViewState* vs, float* hraw)
 {
     __shared__ unsigned long long ssum[4];
     __shared__ unsigned int scnt[4];
@@ -182,6 +244,7 @@ __global__ __launch_bounds__(256) void k_bandwidth(int N, float Wf, float Hf, fl
     unsigned int c = 0;
     if (t < N) {
         float hk = iso_h(Wf, Hf, ff, pts + 3L * t, nrm + 3L * t, sd + 2L * t, smoothing);
+        if (hraw) hraw[t] = prop_h(Wf, Hf, ff, pts + 3L * t, nrm + 3L * t, sd + 2L * t, smoothing);
         if (isfinite(hk)) { q = (unsigned long long)((double)hk * 1099511627776.0); c = 1; }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -198,13 +261,29 @@ __global__ __launch_bounds__(256) void k_bandwidth(int N, float Wf, float Hf, fl
     }
 }
 
+// query radius r with exp(-r^2 / 2h^2) / (2 pi sqrt(h)) = precision (samplingCuda.cu:251-252)
+__device__ float band_radius(float h, double prec)
+{
+    float normalization = (float)(2.0 * UD_PI * (double)sqrtf(h));
+    return (float)(sqrt(2.0) * (double)h * sqrt(-log(prec) - log((double)normalization)));
+}
+
+__device__ Band make_band(float h, double prec)
+{
+    Band b;
+    b.h = h;
+    b.is2h = 1.0f / sqrtf((2.0f * h) * h);
+    b.c = (float)(1.0 / (2.0 * UD_PI * (double)sqrtf(h)));    // buildRadixTree.cuh:580
+    b.radius = band_radius(h, prec);
+    return b;
+}
+
 __global__ void k_view_params(ViewState* vs, float Wf, float Hf, double prec)
 {
     if (threadIdx.x != 0) return;
     double hd = ((double)vs->hsum * (1.0 / 1099511627776.0)) / (double)vs->hcnt;
     float h = (float)hd;
-    float normalization = (float)(2.0 * UD_PI * (double)sqrtf(h));
-    float radius = (float)(sqrt(2.0) * (double)h * sqrt(-log(prec) - log((double)normalization)));
+    float radius = band_radius(h, prec);
     vs->h = h;
     vs->radius = radius;
     vs->lo = -radius;
@@ -214,6 +293,7 @@ __global__ void k_view_params(ViewState* vs, float Wf, float Hf, double prec)
     vs->is2h = 1.0f / vs->s2h;
     vs->prec = (float)prec;
     vs->nc = 0;
+    vs->band[0] = make_band(h, prec);
 }
 
 __device__ __forceinline__ bool visible(const ViewState& v, float Wf, float Hf, float ff, const float* p, float& sx, float& sy)
@@ -1215,21 +1295,17 @@ __global__ void k_copy_err(const ViewState* vs, int* err)
 }
 
 // ---------------------------------------------------------------------------
-// approximateFMMWeights: fixed-order (degree 6) Hermite density image
-// (sampling.cpp:136-175, samplingCuda.cu:398-493, buildRadixTree.cuh:449-606,
-// transformRadixTree.cuh:15-69, 152-222).
+// Fixed-order (degree 6) Hermite densities D_b(q) = c_b sum_i exp(-|q - s_i|^2 /
+// 2 h_b^2) on the shared tree: approximateFMMWeights (band 0, the tree's h;
+// sampling.cpp:136-175, samplingCuda.cu:398-493, buildRadixTree.cuh:449-606,
+// transformRadixTree.cuh:15-69, 152-222) and the normalisers of Poisson-rate
+// sampling (bands 0 and 1).
 // ---------------------------------------------------------------------------
 
-// leaf coefficient 1 / (2 pi sqrt(h)) (buildRadixTree.cuh:580)
-__device__ __forceinline__ float fixed_leaf_coef(float h)
-{
-    return (float)(1.0 / (2.0 * UD_PI * (double)sqrtf(h)));
-}
-
-// Internal nodes whose degree-6 truncation bound meets the precision carry an
-// expansion: flag them in NodeAux.pad and give them a 49-float block.
-__global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, const NodeGeo* __restrict__ geo,
-                                                      const NodeLnk* __restrict__ lnk, NodeAux* aux, int* list)
+// Internal nodes whose degree-6 truncation bound at band b meets the
+// precision carry an expansion: a 49-float block at fixOff[node].
+__global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int b, const NodeGeo* __restrict__ geo,
+                                                      const NodeLnk* __restrict__ lnk, int* fixOff, int* list)
 {
     const int n = vs->nc;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1239,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, const NodeG
         float mn = 0.0f;
         mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
         mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
-        float rr = mn / vs->h;
+        float rr = mn / vs->band[b].h;
         acc = hermite_error(rr, (float)lnk[t].count, kFixedDegree) <= vs->prec;
     }
     unsigned long long m = __ballot(acc);
@@ -1247,36 +1323,37 @@ __global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, const NodeG
     int base = 0;
     if (m != 0ull) {
         int leader = __ffsll((long long)m) - 1;
-        if (lane == leader) base = atomicAdd(&vs->nmom, __popcll(m));
+        if (lane == leader) base = atomicAdd(&vs->nfix[b], __popcll(m));
         base = __shfl(base, leader, 64);
     }
     if (t < n - 1) {
         int slot = base + __popcll(m & ((1ull << lane) - 1ull));
-        aux[t].pad = acc ? 1 : 0;
-        aux[t].off = acc ? slot * kFixedCoefs : 0;
+        fixOff[t] = acc ? slot * kFixedCoefs : -1;
         if (acc) list[slot] = t;
     }
 }
 
-// Degree-6 moments, scaled by the leaf coefficient: thread (node, row a) sums
-// row a over the node's leaf range in FP64 (leaf order).  36 nodes x 7 rows
-// per 256-thread workgroup.
+// Degree-6 moments at band b, scaled by the band's leaf coefficient: thread
+// (node, row a) sums row a over the node's leaf range in FP64 (leaf order).
+// 36 nodes x 7 rows per 256-thread workgroup.
 constexpr int kFixedNodesPerBlock = 36;
-__global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
-                                                       const NodeAux* __restrict__ aux, const int* __restrict__ list,
+__global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restrict__ vs, int b,
+                                                       const NodeGeo* __restrict__ geo, const NodeAux* __restrict__ aux,
+                                                       const int* __restrict__ list, const int* __restrict__ fixOff,
                                                        float* __restrict__ coef)
 {
     const int t = threadIdx.x;
     if (t >= kFixedNodesPerBlock * (kFixedDegree + 1)) return;
     const long slot = (long)blockIdx.x * kFixedNodesPerBlock + t / (kFixedDegree + 1);
     const int ax = t % (kFixedDegree + 1);
-    if (slot >= vs->nmom) return;
+    if (slot >= vs->nfix[b]) return;
     const int nd = list[slot];
     const int L0 = vs->nc - 1;
-    const float is2h = vs->is2h;
-    const double c = (double)fixed_leaf_coef(vs->h);
+    const float is2h = vs->band[b].is2h;
+    const double c = (double)vs->band[b].c;
     NodeGeo g = geo[nd];
     NodeAux a = aux[nd];
+    const int off = fixOff[nd];
     const double ifx = kInvFact[ax];
     double acc[kFixedDegree + 1];
 #pragma unroll
@@ -1297,7 +1374,7 @@ __global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restri
     }
 #pragma unroll
     for (int bb = 0; bb <= kFixedDegree; bb++)
-        coef[a.off + ax * (kFixedDegree + 1) + bb] = (float)(c * acc[bb]);
+        coef[off + ax * (kFixedDegree + 1) + bb] = (float)(c * acc[bb]);
 }
 
 __device__ __forceinline__ bool box_meets_disc(const NodeGeo& g, float qx, float qy, float rad)
@@ -1311,14 +1388,26 @@ __device__ __forceinline__ bool box_meets_disc(const NodeGeo& g, float qx, float
     return d2 < rad * rad;
 }
 
+// c exp(-tx^2) exp(-ty^2), t = (q - x) / sqrt(2 h^2), in leaf-evaluation order
+__device__ __forceinline__ float band_point(const Band& B, float qx, float qy, float x, float y)
+{
+    float tx = (qx - x) * B.is2h;
+    float ty = (qy - y) * B.is2h;
+    float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+    float v = 0.0f;
+    v = v + (B.c * ex) * ey;
+    return v;
+}
+
 // One lane per pixel, 8x8 pixel tile per wavefront, 4 tiles per workgroup.
 // The reference's explicit stack pushes the left then the right child of an
 // unaccepted node when the child's box meets the query disc, so it sums the
 // right subtree first; the same order is walked here without a stack, via
 // parent links.
-__global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState* __restrict__ vs,
-                                                  const NodeGeo* geo, const NodeLnk* lnk, const NodeAux* aux,
-                                                  const float* coef, float* __restrict__ out)
+__global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState* __restrict__ vs, int b,
+                                                  const NodeGeo* geo, const NodeLnk* lnk,
+                                                  const int* __restrict__ fixOff, const float* coef,
+                                                  float* __restrict__ out)
 {
     const int lane = threadIdx.x & 63;
     const int tilesY = (H + 7) >> 3;
@@ -1334,51 +1423,46 @@ __global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState*
         TreeView T;
         T.geo = (const NodeGeo UD_GLOBAL*)geo;
         T.lnk = (const NodeLnk UD_GLOBAL*)lnk;
-        T.aux = (const NodeAux UD_GLOBAL*)aux;
-        T.coef = (const float UD_GLOBAL*)coef;
-        T.s2h = vs->s2h;
-        T.is2h = vs->is2h;
+        const float UD_GLOBAL* cf = (const float UD_GLOBAL*)coef;
+        const Band B = vs->band[b];
         const int L0 = n - 1;
-        const float c = fixed_leaf_coef(vs->h);
-        const float rad = vs->radius;
         const float qx = (float)i, qy = (float)j;
         int nd = 0;
         while (true) {
             NodeGeo g = ld_geo(T, nd);
             bool climb = true;
-            if (nd >= L0 || T.aux[nd].pad) {
-                float tx = (qx - g.cx) * T.is2h;
-                float ty = (qy - g.cy) * T.is2h;
+            const int off = nd >= L0 ? -1 : fixOff[nd];
+            if (nd >= L0) {
+                w = w + band_point(B, qx, qy, g.cx, g.cy);
+            } else if (off >= 0) {
+                float tx = (qx - g.cx) * B.is2h;
+                float ty = (qy - g.cy) * B.is2h;
                 float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+                float hy[kFixedDegree + 1];
+                hy[0] = ey;
+                hy[1] = (2.0f * ty) * ey;
+#pragma unroll
+                for (int q = 2; q <= kFixedDegree; q++)
+                    hy[q] = (2.0f * ty) * hy[q - 1] - (2.0f * (float)(q - 1)) * hy[q - 2];
+                const float UD_GLOBAL* cc = cf + off;
                 float v = 0.0f;
-                if (nd >= L0) {
-                    v = v + (c * ex) * ey;
-                } else {
-                    float hy[kFixedDegree + 1];
-                    hy[0] = ey;
-                    hy[1] = (2.0f * ty) * ey;
+                float hprev = 0.0f, ha = ex;
 #pragma unroll
-                    for (int q = 2; q <= kFixedDegree; q++)
-                        hy[q] = (2.0f * ty) * hy[q - 1] - (2.0f * (float)(q - 1)) * hy[q - 2];
-                    const float UD_GLOBAL* cc = T.coef + T.aux[nd].off;
-                    float hprev = 0.0f, ha = ex;
-#pragma unroll
-                    for (int a = 0; a <= kFixedDegree; a++) {
-                        if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
-                        else if (a >= 2) {
-                            float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
-                            hprev = ha; ha = hn;
-                        }
-#pragma unroll
-                        for (int b = 0; b <= kFixedDegree; b++)
-                            v = v + (cc[a * (kFixedDegree + 1) + b] * ha) * hy[b];
+                for (int a = 0; a <= kFixedDegree; a++) {
+                    if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
+                    else if (a >= 2) {
+                        float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+                        hprev = ha; ha = hn;
                     }
+#pragma unroll
+                    for (int bb = 0; bb <= kFixedDegree; bb++)
+                        v = v + (cc[a * (kFixedDegree + 1) + bb] * ha) * hy[bb];
                 }
                 w = w + v;
             } else {
                 NodeLnk lk = ld_lnk(T, nd);
-                if (box_meets_disc(ld_geo(T, lk.right), qx, qy, rad)) { nd = lk.right; climb = false; }
-                else if (box_meets_disc(ld_geo(T, lk.left), qx, qy, rad)) { nd = lk.left; climb = false; }
+                if (box_meets_disc(ld_geo(T, lk.right), qx, qy, B.radius)) { nd = lk.right; climb = false; }
+                else if (box_meets_disc(ld_geo(T, lk.left), qx, qy, B.radius)) { nd = lk.left; climb = false; }
             }
             if (!climb) continue;
             // climb to the next unvisited left sibling (the right one was taken first)
@@ -1386,7 +1470,7 @@ __global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState*
             while (nd != 0) {
                 int p = T.lnk[nd].parent;
                 NodeLnk pl = ld_lnk(T, p);
-                if (nd == pl.right && box_meets_disc(ld_geo(T, pl.left), qx, qy, rad)) {
+                if (nd == pl.right && box_meets_disc(ld_geo(T, pl.left), qx, qy, B.radius)) {
                     nd = pl.left;
                     done = false;
                     break;
@@ -1397,6 +1481,172 @@ __global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState*
         }
     }
     out[(long)j + (long)H * i] = w;
+}
+
+// ---------------------------------------------------------------------------
+// Poisson-rate sampling with inclusion probabilities (no reference kernel; its
+// dead plain-Poisson kernel, poissonRadixTree.cuh:70-200, draws Bernoulli(k p)
+// only inside the reached subtree, which leaves the inclusion probabilities
+// unknown).  Every culled splat i is included at pixel q independently with the
+// defensive-mixture probability (Hesterberg 1995)
+//   pi_i = min(1, lambda/2 (g0_i(q) / N0(q) + g1_i(q) / N1(q))),
+// g_b = c_b exp(-|q - s_i|^2 / 2 h_b^2) for the tree bandwidth h_0 = h and the
+// wide bandwidth h_1 = max(h, 99.9th percentile of the visible splats' own
+// bandwidths sqrt(lambda_max(V_i))), and N_b = max(D_b, c_b) with D_b the
+// fixed-order density of band b: the expected count stays near lambda, and EWA
+// weight / pi stays bounded for splats not wider than h_1.  Pixels with
+// D_0 < precision draw nothing.
+// ---------------------------------------------------------------------------
+constexpr float kProposalHCap = 16.0f;
+
+// histogram of the visible splats' proposal bandwidths (1/16 px bins, the
+// last bin holds everything >= 16; degenerate splats are skipped)
+__global__ __launch_bounds__(256) void k_wide_hist(ViewState* vs, const int32_t* __restrict__ ids,
+                                                   const float* __restrict__ hraw)
+{
+    __shared__ int sh[256];
+    sh[threadIdx.x] = 0;
+    __syncthreads();
+    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos < vs->nc) {
+        float hi = hraw[ids[pos]];
+        if (hi >= 0.0f) atomicAdd(&sh[hi < kProposalHCap ? (int)(hi * 16.0f) : 255], 1);
+    }
+    __syncthreads();
+    if (sh[threadIdx.x]) atomicAdd(&vs->hhist[threadIdx.x], sh[threadIdx.x]);
+}
+
+// wide band: h_1 = max(h, 99.9th percentile of the histogram, rounded up to 1/16 px)
+__global__ void k_wide_band(ViewState* vs)
+{
+    if (threadIdx.x != 0) return;
+    long total = 0;
+    for (int b = 0; b < 256; b++) total += vs->hhist[b];
+    long target = total - total / 1000, cum = 0;
+    int qb = 0;
+    for (int b = 0; b < 256; b++) {
+        cum += vs->hhist[b];
+        if (cum >= target) { qb = b; break; }
+    }
+    const float hw = fmaxf(vs->h, (float)(qb + 1) * 0.0625f);
+    vs->band[1] = make_band(hw, (double)vs->prec);
+}
+
+// mixture proposal s0 g0 + s1 g1 at (x, y)
+__device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float s0, float s1, float qx, float qy,
+                                           float x, float y)
+{
+    float v0 = band_point(B0, qx, qy, x, y);
+    float v1 = band_point(B1, qx, qy, x, y);
+    float m = s0 * v0;
+    m = m + s1 * v1;
+    return m;
+}
+
+// One lane per pixel (8x8 pixel tile per wavefront, 4 per workgroup).  The walk
+// visits nodes left to right; a node's bound U (the mixture at the point of its
+// box closest to q) is >= pi_i of each of its leaves.  U = 0: skip the
+// subtree.  A leaf, or a node with count U <= 1, is thinned: candidate leaves at
+// rate U by geometric skips over its contiguous leaf range (skip = floor(log u /
+// log(1 - U))), each kept with probability pi_i / U, i.e. included with
+// probability pi_i, independently.  Uniforms are Philox4x32-10 words at counter
+// (draw >> 2, 0x80000000, pixel, view), word draw & 3, with a per-pixel draw
+// number.
+__global__ __launch_bounds__(256) void k_poisson(int W, int H, int kcap, float rate, const float* __restrict__ tscale,
+                                                 int tsH, uint32_t seedLo, uint32_t seedHi, uint32_t view,
+                                                 const ViewState* __restrict__ vs, const NodeGeo* geo,
+                                                 const NodeLnk* lnk, const NodeAux* aux,
+                                                 const int32_t* __restrict__ leafId, const float* __restrict__ dens0,
+                                                 const float* __restrict__ dens1, int32_t* __restrict__ outIds,
+                                                 float* __restrict__ outIpi, int* err)
+{
+    const int lane = threadIdx.x & 63;
+    const int tilesY = (H + 7) >> 3;
+    const int ntiles = ((W + 7) >> 3) * tilesY;
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= ntiles) return;
+    const int ti = wv / tilesY, tj = wv - (wv / tilesY) * tilesY;
+    const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
+    if (i >= W || j >= H) return;
+    const long pix = (long)j + (long)H * i;
+    int32_t* oid = outIds + pix * kcap;
+    float* oip = outIpi + pix * kcap;
+    int ns = 0;
+    bool over = false;
+    const int n = vs->nc;
+    const float D0 = n > 0 ? dens0[pix] : 0.0f;
+    if (n > 0 && D0 >= vs->prec) {
+        TreeView T;
+        T.geo = (const NodeGeo UD_GLOBAL*)geo;
+        T.lnk = (const NodeLnk UD_GLOBAL*)lnk;
+        T.aux = (const NodeAux UD_GLOBAL*)aux;
+        const int L0 = n - 1;
+        const Band B0 = vs->band[0], B1 = vs->band[1];
+        float lam = rate;
+        if (tscale) lam = lam * tscale[(long)(i >> 4) * tsH + (j >> 4)];
+        const float half = 0.5f * lam;
+        const float s0 = half / fmaxf(D0, B0.c);
+        const float s1 = half / fmaxf(dens1[pix], B1.c);
+        const float qx = (float)i, qy = (float)j;
+        const uint32_t pixel = (uint32_t)pix;
+        uint32_t draw = 0, blkId = 0xffffffffu;
+        u32x4 blk = { 0u, 0u, 0u, 0u };
+        auto uniform = [&]() {
+            uint32_t bi = draw >> 2;
+            if (bi != blkId) {
+                blkId = bi;
+                u32x4 ctr = { bi, 0x80000000u, pixel, view };
+                blk = philox4x32_10(ctr, seedLo, seedHi);
+            }
+            uint32_t q = draw & 3u;
+            uint32_t x = q == 0 ? blk.x : q == 1 ? blk.y : q == 2 ? blk.z : blk.w;
+            draw++;
+            return u01(x);
+        };
+        int nd = 0;
+        while (true) {
+            NodeGeo g = ld_geo(T, nd);
+            float cx = fminf(fmaxf(qx, g.lox), g.hix);
+            float cy = fminf(fmaxf(qy, g.loy), g.hiy);
+            float U = fminf(1.0f, mix_value(B0, B1, s0, s1, qx, qy, cx, cy));
+            bool descend = false;
+            if (U > 0.0f) {
+                const bool leaf = nd >= L0;
+                NodeLnk lk = ld_lnk(T, nd);
+                if (!leaf && (float)lk.count * U > 1.0f) descend = true;
+                else {
+                    int first = leaf ? nd - L0 : T.aux[nd].first;
+                    int last = leaf ? nd - L0 : T.aux[nd].last;
+                    const float l1 = plog1mf(U);
+                    int pos = first - 1;
+                    while (true) {
+                        float u = uniform();
+                        float r = plogf(u) / l1;          // failures before the next candidate
+                        if (!(r < (float)(last - pos))) break;
+                        pos += (int)r + 1;
+                        NodeGeo lg = ld_geo(T, L0 + pos);
+                        float pi = fminf(1.0f, mix_value(B0, B1, s0, s1, qx, qy, lg.cx, lg.cy));
+                        float a = uniform();
+                        if (a * U <= pi) {            // P = pi / U, exactly 1 when pi = U
+                            if (ns < kcap) { oid[ns] = leafId[pos]; oip[ns] = 1.0f / pi; ns++; }
+                            else over = true;
+                        }
+                    }
+                }
+                if (descend) { nd = lk.left; continue; }
+            }
+            bool done = true;
+            while (nd != 0) {
+                int p = T.lnk[nd].parent;
+                int pl = T.lnk[p].left;
+                if (nd == pl) { nd = T.lnk[p].right; done = false; break; }
+                nd = p;
+            }
+            if (done) break;
+        }
+    }
+    for (int s = ns; s < kcap; s++) { oid[s] = -1; oip[s] = 0.0f; }
+    if (over) atomicOr(err, 4);
 }
 
 inline unsigned grid1(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -1410,7 +1660,7 @@ int build_structure(int N, int W, int H, double f, double smoothing, double prec
     long nCull = (N + kCullTile - 1) / kCullTile;
     long nSort = (N + kSortTile - 1) / kSortTile;
     k_zero_state<<<1, 64, 0, st>>>(w.vs);
-    k_bandwidth<<<grid1(N, 256), 256, 0, st>>>(N, Wf, Hf, ff, (float)smoothing, pts, nrm, sd, w.vs);
+    k_bandwidth<<<grid1(N, 256), 256, 0, st>>>(N, Wf, Hf, ff, (float)smoothing, pts, nrm, sd, w.vs, w.hraw);
     k_view_params<<<1, 64, 0, st>>>(w.vs, Wf, Hf, prec);
     k_cull_count<<<(unsigned)nCull, 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts);
     k_scan_small<<<1, 1024, 0, st>>>(w.cullCounts, (int)nCull, &w.vs->nc);
@@ -1502,13 +1752,23 @@ void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32
     if (timed) (void)hipEventRecord(e1, st);
 }
 
+// fixed-order density of band b over one view into out (W x H)
+void fixed_density(int N, int W, int H, int b, const TreeWs& w, float* out, hipStream_t st)
+{
+    unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
+    k_fixed_select<<<grid1(N, 256), 256, 0, st>>>(w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixList[b]);
+    k_fixed_moments<<<grid1(N, kFixedNodesPerBlock), 256, 0, st>>>(w.vs, b, w.geo, w.aux, w.fixList[b], w.fixOff[b],
+                                                                    w.fixCoef[b]);
+    k_fmm_eval<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixCoef[b], out);
+}
+
 } // namespace
 
 extern "C" {
 
 size_t ude_sample_workspace_bytes(int N, int W, int H, int k)
 {
-    return carve(nullptr, N, W, H, k, nullptr) + 256;
+    return carve(nullptr, N, W, H, k, kModeSample, nullptr) + 256;
 }
 
 int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, double prec,
@@ -1524,7 +1784,7 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
     }
     if (wsBytes < ude_sample_workspace_bytes(N, W, H, k)) return UDE_ERR_WORKSPACE;
     TreeWs w;
-    carve(ws, N, W, H, k, &w);
+    carve(ws, N, W, H, k, kModeSample, &w);
     uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
     for (int b = 0; b < bn; b++) {
         const float* pb = pts + 3L * N * b;
@@ -1547,7 +1807,7 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
 
 size_t ude_fmm_weights_workspace_bytes(int N, int W, int H)
 {
-    return carve(nullptr, N, W, H, 0, nullptr) + 256;
+    return carve(nullptr, N, W, H, 0, kModeFixed, nullptr) + 256;
 }
 
 int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, double prec,
@@ -1563,17 +1823,58 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
     }
     if (wsBytes < ude_fmm_weights_workspace_bytes(N, W, H)) return UDE_ERR_WORKSPACE;
     TreeWs w;
-    carve(ws, N, W, H, 0, &w);
-    unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
+    carve(ws, N, W, H, 0, kModeFixed, &w);
     for (int b = 0; b < bn; b++) {
         const float* pb = pts + 3L * N * b;
         const float* nb = nrm + 3L * N * b;
         const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
         if (rc != UDE_OK) return rc;
-        k_fixed_select<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.momList);
-        k_fixed_moments<<<grid1(N, kFixedNodesPerBlock), 256, 0, st>>>(w.vs, w.geo, w.aux, w.momList, w.coef);
-        k_fmm_eval<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, w.vs, w.geo, w.lnk, w.aux, w.coef, out + perView * b);
+        fixed_density(N, W, H, 0, w, out + perView * b, st);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+size_t ude_poisson_workspace_bytes(int N, int W, int H)
+{
+    return carve(nullptr, N, W, H, 0, kModePoisson, nullptr) + 256;
+}
+
+int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double smoothing, double prec,
+                       double rate, const float* tileScale, const float* pts, const float* nrm, const float* sd,
+                       int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, int* errFlag, void* ws,
+                       size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || kcap <= 0 || N < 0 || !(rate >= 0.0)) return UDE_ERR_ARG;
+    long perView = (long)W * H;
+    if (N == 0) {
+        k_fill_idx<<<grid1(perView * kcap * bn, 256), 256, 0, st>>>(outIds, perView * kcap * bn);
+        k_fill_f32<<<grid1(perView * kcap * bn, 256), 256, 0, st>>>(outIpi, perView * kcap * bn, 0.0f);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+    }
+    if (wsBytes < ude_poisson_workspace_bytes(N, W, H)) return UDE_ERR_WORKSPACE;
+    TreeWs w;
+    carve(ws, N, W, H, 0, kModePoisson, &w);
+    unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
+    const int tsW = (W + 15) / 16, tsH = (H + 15) / 16;
+    uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
+    for (int b = 0; b < bn; b++) {
+        const float* pb = pts + 3L * N * b;
+        const float* nb = nrm + 3L * N * b;
+        const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
+        int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
+        if (rc != UDE_OK) return rc;
+        k_wide_hist<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.idsA, w.hraw);
+        k_wide_band<<<1, 64, 0, st>>>(w.vs);
+        fixed_density(N, W, H, 0, w, w.dens[0], st);
+        fixed_density(N, W, H, 1, w, w.dens[1], st);
+        const float* ts = tileScale ? tileScale + (long)tsW * tsH * b : nullptr;
+        k_poisson<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, kcap, (float)rate, ts, tsH, sLo, sHi, (uint32_t)b, w.vs,
+                                                   w.geo, w.lnk, w.aux, w.idsA, w.dens[0], w.dens[1],
+                                                   outIds + perView * kcap * b, outIpi + perView * kcap * b,
+                                                   errFlag);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;

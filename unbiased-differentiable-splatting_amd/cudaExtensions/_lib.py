@@ -27,6 +27,9 @@ _SIGS = {
     "ude_sample": (c_int, [c_int] * 5 + [c_dbl] * 3 + [c_ptr] * 3 + [c_int, ctypes.c_uint64, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "ude_fmm_weights_workspace_bytes": (c_size, [c_int] * 3),
     "ude_fmm_weights": (c_int, [c_int] * 4 + [c_dbl] * 3 + [c_ptr] * 3 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
+    "ude_poisson_workspace_bytes": (c_size, [c_int] * 3),
+    "ude_sample_poisson": (c_int, [c_int] * 5 + [c_dbl] * 4 + [c_ptr] * 4 + [c_int, ctypes.c_uint64] + [c_ptr] * 4 +
+                           [c_size, c_ptr]),
     "ude_profile_sampler_begin": (c_int, []),
     "ude_profile_sampler_end": (c_int, [c_ptr, c_ptr]),
     "ude_covariance": (c_int, [c_int] * 4 + [c_dbl] * 2 + [c_ptr] * 3 + [c_int, c_ptr, c_ptr]),

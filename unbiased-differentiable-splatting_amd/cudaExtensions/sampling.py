@@ -86,10 +86,61 @@ def approximateFMMWeights(width, height, focallength, points, normals, colors, s
     return out
 
 
+def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, numSamples, smoothing,
+                  precision=0.001, seed=None, rate=None, tileScale=None):
+    """Poisson-rate sampling with inclusion probabilities (no reference counterpart).
+
+    Every visible splat i is included at pixel q independently with the
+    defensive-mixture probability
+    pi_i = min(1, r/2 (g0_i / max(D0, c0) + g1_i / max(D1, c1))), r = rate times the
+    tile's multiplier, g_b = c_b exp(-|q - s_i|^2 / 2 h_b^2) for the tree bandwidth
+    (b = 0) and a wide bandwidth covering the visible splats (b = 1), D_b the
+    fixed-order density of bandwidth b; nothing where D0 < precision.  Returns
+    (indices int32 (bn, W, H, numSamples), -1 padded; invPi float32, 1 / pi_i per
+    slot, 0 padded), so that sum_s weight_s * value_s * invPi_s is an unbiased
+    estimate of the sum over all visible splats.  ``rate`` (expected samples per
+    pixel, default numSamples / 4) and the optional per-16x16-tile multipliers
+    ``tileScale`` (bn, ceil(W/16), ceil(H/16)) set the sampling budget; a pixel
+    with more than numSamples inclusions sets an error flag checked by
+    check_errors()."""
+    global last_error
+    dev = _lib.require_gpu(points, normals, stdDevs)
+    points = _lib.f32(points, "points")
+    normals = _lib.f32(normals, "normals")
+    stdDevs = _lib.f32(stdDevs, "stdDevs")
+    bn, N, _ = points.shape
+    W, H, k = int(width), int(height), int(numSamples)
+    rate = float(k) / 4.0 if rate is None else float(rate)
+    ts = None
+    if tileScale is not None:
+        _lib.require_gpu(tileScale)
+        ts = _lib.f32(tileScale, "tileScale")
+        if tuple(ts.shape) != (bn, (W + 15) // 16, (H + 15) // 16):
+            raise ValueError("tileScale must have shape (bn, ceil(W/16), ceil(H/16))")
+    ids = torch.empty((bn, W, H, k), dtype=torch.int32, device=dev)
+    ipi = torch.empty((bn, W, H, k), dtype=torch.float32, device=dev)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    L = _lib.lib()
+    ws = _lib.workspace(L.ude_poisson_workspace_bytes(N, W, H), dev)
+    s = _next_seed() if seed is None else int(seed)
+    rc = L.ude_sample_poisson(bn, N, W, H, k, float(focallength), float(smoothing), float(precision), rate,
+                              _lib.ptr(ts), _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs),
+                              stdDevs.shape[0], s & 0xFFFFFFFFFFFFFFFF, _lib.ptr(ids), _lib.ptr(ipi), _lib.ptr(err),
+                              _lib.ptr(ws), ws.numel(), _lib.stream(dev))
+    _lib.check(rc, "sampling.samplePoisson")
+    last_error = err
+    return ids, ipi
+
+
 def check_errors():
     """Synchronising check of the last sample() call's device error flag."""
-    if last_error is not None and int(last_error.item()) != 0:
-        raise RuntimeError("sampling.sample: descent exceeded the 63-level trace bound")
+    if last_error is not None:
+        e = int(last_error.item())
+        if e & 4:
+            raise RuntimeError("sampling.samplePoisson: a pixel drew more than numSamples splats (raise "
+                               "numSamples or lower the rate)")
+        if e != 0:
+            raise RuntimeError("sampling.sample: descent exceeded the 63-level trace bound")
 
 
 def profile_sampler_begin():
