@@ -80,6 +80,29 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
                        int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, int* errFlag, void* ws,
                        size_t wsBytes, ude_stream_t stream);
 
+/* ---- per-tile rate network (no counterpart in the reference) ----------- */
+
+/* A 32 -> 64 -> 64 -> 1 perceptron (ReLU) from 32 features of a 16x16 pixel
+ * tile to the log of its Poisson-rate multiplier; bfloat16 matrix-core
+ * products with FP32 accumulation.  Parameters: FP32 block of
+ * ude_mlp_param_count() floats (W1 [32][64], b1, W2 [64][64], b2, W3 [64], b3). */
+int ude_mlp_param_count(void);
+size_t ude_mlp_workspace_bytes(int T);
+
+/* y[t] = network(X[t]) for T rows of 32 features. */
+int ude_mlp_forward(int T, const float* X, const float* params, float* y, void* ws, size_t wsBytes,
+                    ude_stream_t stream);
+
+/* Weighted mean-square loss sum_t w_t (y_t - target_t)^2 / sum_t w_t (weight
+ * optional) and its gradient with respect to the parameter block (grad, fully
+ * written); *loss receives the loss. */
+int ude_mlp_loss_grad(int T, const float* X, const float* params, const float* target, const float* weight,
+                      float* grad, float* loss, void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* One Adam step (bias-corrected, step >= 1) on n FP32 parameters. */
+int ude_adam_step(int n, float* params, const float* grad, float* m, float* v, float lr, float beta1, float beta2,
+                  float eps, int step, ude_stream_t stream);
+
 /* Profiling aid (no counterpart in the reference): between begin and end,
  * every launch of the sampler kernel (the per-pixel FGT descent + sequential
  * Poisson pass, one per view) is bracketed by HIP events on its stream.  end

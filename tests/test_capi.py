@@ -50,6 +50,8 @@ def test_python_bindings_match_header():
                 assert ca is ctypes.c_void_p, (name, decl)
             elif decl.startswith("double"):
                 assert ca is ctypes.c_double, (name, decl)
+            elif decl.startswith("float"):
+                assert ca is ctypes.c_float, (name, decl)
             elif decl.startswith("uint64_t"):
                 assert ca is ctypes.c_uint64, (name, decl)
             elif decl.startswith("size_t"):
