@@ -73,12 +73,29 @@ size_t ude_poisson_workspace_bytes(int N, int W, int H);
  * (nothing where D0 < precision).  tileScale (optional, (bn, ceil(W/16),
  * ceil(H/16))) multiplies the rate per 16x16 tile.  Outputs (bn, W, H, kcap):
  * splat ids (-1 padded) and 1 / pi_i (0 padded), so sum_s w_s a_s / pi_s is a
- * Horvitz-Thompson estimate of sum_i w_i a_i.  A pixel with more than kcap
- * inclusions sets bit 4 of *errFlag. */
+ * Horvitz-Thompson estimate of sum_i w_i a_i.  densOut (optional, (bn, 2, W, H))
+ * receives D0 and D1.  A pixel with more than kcap inclusions sets bit 4 of
+ * *errFlag. */
 int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double smoothing, double precision,
                        double rate, const float* tileScale, const float* pts, const float* nrm, const float* sd,
-                       int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, int* errFlag, void* ws,
-                       size_t wsBytes, ude_stream_t stream);
+                       int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, float* densOut, int* errFlag,
+                       void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* Features of 16x16 pixel tiles for the rate network (no counterpart in the
+ * reference).  dens: (bn, 2, W, H) densities D0, D1 as returned by
+ * ude_sample_poisson (densOut); gimg (optional): (bn, W, H, 3) image gradient.
+ * out: (bn, ceil(W/16), ceil(H/16), 32): [0] log(1 + visible splat centres),
+ * [1] [2] mean / std of their depth, [3] [4] mean / max of their bandwidth,
+ * [5..8] mean / max / min / std of log(1 + D0) over the tile's pixels,
+ * [9..12] the same for D1, [13] fraction of pixels with D0 >= precision,
+ * [14..16] mean / max / std of |gimg|, [17] log(1 + sum |gimg|^2), [18..25]
+ * per quadrant mean log(1 + D0) and mean |gimg|, [26] [27] tile centre / (W, H),
+ * [28] log(1 + sum D0), [29] log(1 + sum |gimg| D0), [30] pixel coverage,
+ * [31] log(1 + count coverage). */
+size_t ude_tile_features_workspace_bytes(int W, int H);
+int ude_tile_features(int bn, int N, int W, int H, double f, double smoothing, double precision, const float* pts,
+                      const float* nrm, const float* sd, int sdBatch, const float* dens, const float* gimg,
+                      float* out, void* ws, size_t wsBytes, ude_stream_t stream);
 
 /* ---- per-tile rate network (no counterpart in the reference) ----------- */
 

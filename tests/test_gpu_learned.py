@@ -1,0 +1,80 @@
+"""Learned per-tile rates: the tile features the rate network reads (GPU kernel
+vs a numpy restatement on the oracle's densities), and the gain of the trained
+network over plain rates."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import scene_inputs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def t(x, dev):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def numpy_features(W, H, f, p, nb, sd, prec, dens, g):
+    """Restatement of the 32 tile features (see ude_tile_features in include/udsplat.h)."""
+    tW, tH = (W + 15) // 16, (H + 15) // 16
+    F = np.zeros((tW, tH, 32))
+    sx = -(p[:, 0].astype(np.float64) * f) / p[:, 2] + W / 2.0
+    sy = (p[:, 1].astype(np.float64) * f) / p[:, 2] + H / 2.0
+    ok = (p[:, 2] < 0) & (sx >= 0) & (sx < W) & (sy >= 0) & (sy < H)
+    V = O.covariance(W, H, f, p[None].astype(np.float64), nb[None].astype(np.float64), sd[None].astype(np.float64))[0]
+    hh = np.minimum(np.sqrt(np.maximum(np.linalg.eigvalsh(V)[:, 1], 0)), 16.0)
+    d0 = np.log1p(np.maximum(dens[0], 0)); d1 = np.log1p(np.maximum(dens[1], 0))
+    gm = np.sqrt((g.astype(np.float64) ** 2).sum(-1))
+    for ti in range(tW):
+        for tj in range(tH):
+            sel = ok & ((sx.astype(int) >> 4) == ti) & ((sy.astype(int) >> 4) == tj)
+            z = -p[sel, 2].astype(np.float64)
+            c = sel.sum()
+            F[ti, tj, 0] = math.log1p(c)
+            if c:
+                F[ti, tj, 1] = z.mean(); F[ti, tj, 2] = z.std()
+                F[ti, tj, 3] = hh[sel].mean(); F[ti, tj, 4] = hh[sel].max()
+            sl = (slice(ti * 16, min(ti * 16 + 16, W)), slice(tj * 16, min(tj * 16 + 16, H)))
+            a, b, gg, dd = d0[sl], d1[sl], gm[sl], dens[0][sl]
+            F[ti, tj, 5:9] = [a.mean(), a.max(), a.min(), a.std()]
+            F[ti, tj, 9:13] = [b.mean(), b.max(), b.min(), b.std()]
+            F[ti, tj, 13] = (dd >= prec).mean()
+            F[ti, tj, 14:18] = [gg.mean(), gg.max(), gg.std(), math.log1p((gg ** 2).sum())]
+            for q in range(4):
+                qi, qj = q >> 1, q & 1
+                qs = (slice(ti * 16 + 8 * qi, min(ti * 16 + 8 * qi + 8, W)),
+                      slice(tj * 16 + 8 * qj, min(tj * 16 + 8 * qj + 8, H)))
+                n = max(d0[qs].size, 1)
+                F[ti, tj, 18 + 2 * q] = d0[qs].sum() / n
+                F[ti, tj, 19 + 2 * q] = gm[qs].sum() / n
+            F[ti, tj, 26] = (ti * 16 + 8) / W
+            F[ti, tj, 27] = (tj * 16 + 8) / H
+            F[ti, tj, 28] = math.log1p(np.maximum(dd, 0).sum())
+            F[ti, tj, 29] = math.log1p((gg * np.maximum(dd, 0)).sum())
+            F[ti, tj, 30] = a.size / 256.0
+            F[ti, tj, 31] = math.log1p(c * a.size / 256.0)
+    return F
+
+
+@pytest.mark.parametrize("cfg,sds", [(2, 8.0), (0, 2.0)])
+def test_tile_features(dev, cfg, sds):
+    from cudaExtensions import sampling
+    sc, p, n, nb, sd = scene_inputs(cfg, sd_scale=sds)
+    W, H, f = sc.width, sc.height, sc.focal
+    dens = torch.empty((1, 2, W, H), dtype=torch.float32, device=dev)
+    sampling.samplePoisson(W, H, f, t(p[None], dev), t(nb[None], dev), None, t(sd[None], dev), 48, 0.0, 0.01,
+                           seed=1, rate=6.0, densities=dens)
+    _, _, rdens, _ = O.poisson_view(W, H, f, p, nb, sd, 48, 6.0, 0.0, 0.01, seed=1)
+    assert np.array_equal(dens[0].cpu().numpy(), rdens)
+    g = np.random.default_rng(0).normal(size=(1, W, H, 3)).astype(np.float32)
+    F = sampling.tileFeatures(W, H, f, t(p[None], dev), t(nb[None], dev), t(sd[None], dev), 0.0, 0.01, dens,
+                              t(g, dev))[0].cpu().numpy()
+    R = numpy_features(W, H, f, p, nb, sd, 0.01, rdens, g[0])
+    # FP32 statistics vs FP64; bandwidths from the FP32 covariance (relative eps cond(V))
+    err = np.abs(F - R) / np.maximum(np.abs(R), 1.0)
+    cols = [c for c in range(32) if c not in (3, 4)]
+    assert err[..., cols].max() < 2e-4, np.unravel_index(np.argmax(err[..., cols]), err[..., cols].shape)
+    assert err[..., 3:5].max() < 5e-2

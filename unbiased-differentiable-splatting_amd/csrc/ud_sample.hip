@@ -1649,6 +1649,146 @@ __global__ __launch_bounds__(256) void k_poisson(int W, int H, int kcap, float r
     if (over) atomicOr(err, 4);
 }
 
+// ---------------------------------------------------------------------------
+// Per-tile features for the rate network (no reference counterpart): 16x16
+// pixel tiles, 32 features each (layout in ude_tile_features).  Splat
+// statistics are accumulated in fixed point with integer atomics, pixel
+// statistics by fixed-order reductions, so the features are reproducible.
+// ---------------------------------------------------------------------------
+constexpr int kTileFeat = 32;
+constexpr double kFix = 1048576.0;          // 2^20 fixed-point unit
+
+struct TileAcc {
+    unsigned long long cnt, sz, szz, sh;
+    unsigned int hmax, pad;
+};
+
+__global__ __launch_bounds__(256) void k_tile_splats(int N, int W, int H, float ff, float smoothing,
+                                                     const float* __restrict__ pts, const float* __restrict__ nrm,
+                                                     const float* __restrict__ sd, TileAcc* acc)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const float* p = pts + 3L * t;
+    if (!(p[2] < 0.0f)) return;
+    float sx, sy;
+    screen_f32((float)W, (float)H, ff, p, sx, sy);
+    if (!(sx >= 0.0f && sx < (float)W && sy >= 0.0f && sy < (float)H)) return;
+    const int tH = (H + 15) / 16;
+    TileAcc* a = acc + (long)((int)sx >> 4) * tH + ((int)sy >> 4);
+    float z = -p[2];
+    float h = prop_h((float)W, (float)H, ff, p, nrm + 3L * t, sd + 2L * t, smoothing);
+    if (!(h >= 0.0f)) h = 0.0f;
+    h = fminf(h, kProposalHCap);
+    atomicAdd(&a->cnt, 1ull);
+    atomicAdd(&a->sz, (unsigned long long)((double)z * kFix));
+    atomicAdd(&a->szz, (unsigned long long)((double)z * (double)z * kFix));
+    atomicAdd(&a->sh, (unsigned long long)((double)h * kFix));
+    atomicMax(&a->hmax, __float_as_uint(h));
+}
+
+// block-wide sum / max / min over 256 threads, fixed order
+__device__ __forceinline__ float blk_sum(float v, float* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    float r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ float blk_max(float v, float* sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    float r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// one workgroup per tile, one thread per pixel
+__global__ __launch_bounds__(256) void k_tile_pixels(int W, int H, float prec, const float* __restrict__ dens,
+                                                     const float* __restrict__ gimg, const TileAcc* __restrict__ acc,
+                                                     float* __restrict__ out)
+{
+    __shared__ float sh[256];
+    const int tH = (H + 15) / 16;
+    const int tile = blockIdx.x;
+    const int ti = tile / tH, tj = tile - (tile / tH) * tH;
+    const int i = ti * 16 + (threadIdx.x >> 4), j = tj * 16 + (threadIdx.x & 15);
+    const bool in = i < W && j < H;
+    const long pix = (long)j + (long)H * i;
+    const float d0 = in ? log1pf(fmaxf(dens[pix], 0.0f)) : 0.0f;
+    const float d1 = in ? log1pf(fmaxf(dens[(long)W * H + pix], 0.0f)) : 0.0f;
+    float gm = 0.0f;
+    if (in && gimg) {
+        const float* g = gimg + 3 * pix;
+        float s2 = g[0] * g[0];
+        s2 = s2 + g[1] * g[1];
+        s2 = s2 + g[2] * g[2];
+        gm = sqrtf(s2);
+    }
+    const float np = blk_sum(in ? 1.0f : 0.0f, sh);
+    const float inv = 1.0f / fmaxf(np, 1.0f);
+    float f[kTileFeat];
+    // splat statistics
+    const TileAcc a = acc[tile];
+    const float cnt = (float)a.cnt;
+    const float ic = a.cnt ? 1.0f / cnt : 0.0f;
+    const float mz = (float)((double)a.sz / kFix) * ic;
+    const float mzz = (float)((double)a.szz / kFix) * ic;
+    f[0] = log1pf(cnt);
+    f[1] = mz;
+    f[2] = sqrtf(fmaxf(mzz - mz * mz, 0.0f));
+    f[3] = (float)((double)a.sh / kFix) * ic;
+    f[4] = __uint_as_float(a.hmax);
+    // densities (log1p), coverage, gradient magnitudes
+    const float m0 = blk_sum(d0, sh) * inv, m1 = blk_sum(d1, sh) * inv;
+    f[5] = m0;
+    f[6] = blk_max(in ? d0 : 0.0f, sh);
+    f[7] = -blk_max(in ? -d0 : -1e30f, sh);
+    f[8] = sqrtf(fmaxf(blk_sum(in ? (d0 - m0) * (d0 - m0) : 0.0f, sh) * inv, 0.0f));
+    f[9] = m1;
+    f[10] = blk_max(in ? d1 : 0.0f, sh);
+    f[11] = -blk_max(in ? -d1 : -1e30f, sh);
+    f[12] = sqrtf(fmaxf(blk_sum(in ? (d1 - m1) * (d1 - m1) : 0.0f, sh) * inv, 0.0f));
+    f[13] = blk_sum((in && dens[pix] >= prec) ? 1.0f : 0.0f, sh) * inv;
+    const float mg = blk_sum(gm, sh) * inv;
+    f[14] = mg;
+    f[15] = blk_max(gm, sh);
+    f[16] = sqrtf(fmaxf(blk_sum(in ? (gm - mg) * (gm - mg) : 0.0f, sh) * inv, 0.0f));
+    f[17] = log1pf(blk_sum(gm * gm, sh));
+    // quadrants: mean log density and mean gradient magnitude
+    const int quad = ((threadIdx.x >> 7) << 1) | ((threadIdx.x >> 3) & 1);
+    for (int q = 0; q < 4; q++) {
+        const bool mine = quad == q && in;
+        const float nq = fmaxf(blk_sum(mine ? 1.0f : 0.0f, sh), 1.0f);
+        f[18 + 2 * q] = blk_sum(mine ? d0 : 0.0f, sh) / nq;
+        f[19 + 2 * q] = blk_sum(mine ? gm : 0.0f, sh) / nq;
+    }
+    f[26] = ((float)ti * 16.0f + 8.0f) / (float)W;
+    f[27] = ((float)tj * 16.0f + 8.0f) / (float)H;
+    f[28] = log1pf(blk_sum(in ? fmaxf(dens[pix], 0.0f) : 0.0f, sh));
+    f[29] = log1pf(blk_sum(in ? gm * fmaxf(dens[pix], 0.0f) : 0.0f, sh));
+    f[30] = np * (1.0f / 256.0f);
+    f[31] = log1pf(cnt * np * (1.0f / 256.0f));
+    if (threadIdx.x < kTileFeat) out[(long)tile * kTileFeat + threadIdx.x] = f[threadIdx.x];
+}
+
+__global__ void k_zero_u64(unsigned long long* p, long n)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0ull;
+}
+
 inline unsigned grid1(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Tree structure of one view: bandwidth, cull, sort, hierarchy, bottom-up
@@ -1843,14 +1983,15 @@ size_t ude_poisson_workspace_bytes(int N, int W, int H)
 
 int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double smoothing, double prec,
                        double rate, const float* tileScale, const float* pts, const float* nrm, const float* sd,
-                       int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, int* errFlag, void* ws,
-                       size_t wsBytes, hipStream_t st)
+                       int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, float* densOut, int* errFlag,
+                       void* ws, size_t wsBytes, hipStream_t st)
 {
     if (bn <= 0 || W <= 0 || H <= 0 || kcap <= 0 || N < 0 || !(rate >= 0.0)) return UDE_ERR_ARG;
     long perView = (long)W * H;
     if (N == 0) {
         k_fill_idx<<<grid1(perView * kcap * bn, 256), 256, 0, st>>>(outIds, perView * kcap * bn);
         k_fill_f32<<<grid1(perView * kcap * bn, 256), 256, 0, st>>>(outIpi, perView * kcap * bn, 0.0f);
+        if (densOut) k_fill_f32<<<grid1(perView * 2 * bn, 256), 256, 0, st>>>(densOut, perView * 2 * bn, 0.0f);
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
     }
@@ -1868,13 +2009,42 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         if (rc != UDE_OK) return rc;
         k_wide_hist<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.idsA, w.hraw);
         k_wide_band<<<1, 64, 0, st>>>(w.vs);
-        fixed_density(N, W, H, 0, w, w.dens[0], st);
-        fixed_density(N, W, H, 1, w, w.dens[1], st);
+        float* d0 = densOut ? densOut + perView * 2 * b : w.dens[0];
+        float* d1 = densOut ? d0 + perView : w.dens[1];
+        fixed_density(N, W, H, 0, w, d0, st);
+        fixed_density(N, W, H, 1, w, d1, st);
         const float* ts = tileScale ? tileScale + (long)tsW * tsH * b : nullptr;
         k_poisson<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, kcap, (float)rate, ts, tsH, sLo, sHi, (uint32_t)b, w.vs,
-                                                   w.geo, w.lnk, w.aux, w.idsA, w.dens[0], w.dens[1],
+                                                   w.geo, w.lnk, w.aux, w.idsA, d0, d1,
                                                    outIds + perView * kcap * b, outIpi + perView * kcap * b,
                                                    errFlag);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+size_t ude_tile_features_workspace_bytes(int W, int H)
+{
+    return sizeof(TileAcc) * (size_t)((W + 15) / 16) * ((H + 15) / 16) + 256;
+}
+
+int ude_tile_features(int bn, int N, int W, int H, double f, double smoothing, double prec, const float* pts,
+                      const float* nrm, const float* sd, int sdBatch, const float* dens, const float* gimg,
+                      float* out, void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || N < 0 || !dens) return UDE_ERR_ARG;
+    if (wsBytes < ude_tile_features_workspace_bytes(W, H)) return UDE_ERR_WORKSPACE;
+    const int tiles = ((W + 15) / 16) * ((H + 15) / 16);
+    TileAcc* acc = (TileAcc*)ws;
+    const long perView = (long)W * H;
+    for (int b = 0; b < bn; b++) {
+        k_zero_u64<<<grid1((long)tiles * 5, 256), 256, 0, st>>>((unsigned long long*)acc, (long)tiles * 5);
+        if (N > 0)
+            k_tile_splats<<<grid1(N, 256), 256, 0, st>>>(N, W, H, (float)f, (float)smoothing, pts + 3L * N * b,
+                                                         nrm + 3L * N * b, sd + (sdBatch > 1 ? 2L * N * b : 0), acc);
+        k_tile_pixels<<<tiles, 256, 0, st>>>(W, H, (float)prec, dens + 2 * perView * b,
+                                             gimg ? gimg + 3 * perView * b : nullptr, acc,
+                                             out + (long)tiles * kTileFeat * b);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;

@@ -28,8 +28,10 @@ _SIGS = {
     "ude_fmm_weights_workspace_bytes": (c_size, [c_int] * 3),
     "ude_fmm_weights": (c_int, [c_int] * 4 + [c_dbl] * 3 + [c_ptr] * 3 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "ude_poisson_workspace_bytes": (c_size, [c_int] * 3),
-    "ude_sample_poisson": (c_int, [c_int] * 5 + [c_dbl] * 4 + [c_ptr] * 4 + [c_int, ctypes.c_uint64] + [c_ptr] * 4 +
+    "ude_sample_poisson": (c_int, [c_int] * 5 + [c_dbl] * 4 + [c_ptr] * 4 + [c_int, ctypes.c_uint64] + [c_ptr] * 5 +
                            [c_size, c_ptr]),
+    "ude_tile_features_workspace_bytes": (c_size, [c_int] * 2),
+    "ude_tile_features": (c_int, [c_int] * 4 + [c_dbl] * 3 + [c_ptr] * 3 + [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
     "ude_mlp_param_count": (c_int, []),
     "ude_mlp_workspace_bytes": (c_size, [c_int]),
     "ude_mlp_forward": (c_int, [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),

@@ -87,7 +87,7 @@ def approximateFMMWeights(width, height, focallength, points, normals, colors, s
 
 
 def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, numSamples, smoothing,
-                  precision=0.001, seed=None, rate=None, tileScale=None):
+                  precision=0.001, seed=None, rate=None, tileScale=None, densities=None):
     """Poisson-rate sampling with inclusion probabilities (no reference counterpart).
 
     Every visible splat i is included at pixel q independently with the
@@ -102,7 +102,8 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
     pixel, default numSamples / 4) and the optional per-16x16-tile multipliers
     ``tileScale`` (bn, ceil(W/16), ceil(H/16)) set the sampling budget; a pixel
     with more than numSamples inclusions sets an error flag checked by
-    check_errors()."""
+    check_errors().  ``densities`` (optional float32 (bn, 2, W, H)) receives D0 and
+    D1 (the input of tileFeatures)."""
     global last_error
     dev = _lib.require_gpu(points, normals, stdDevs)
     points = _lib.f32(points, "points")
@@ -123,13 +124,48 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
     L = _lib.lib()
     ws = _lib.workspace(L.ude_poisson_workspace_bytes(N, W, H), dev)
     s = _next_seed() if seed is None else int(seed)
+    if densities is not None:
+        _lib.require_gpu(densities)
+        if densities.dtype != torch.float32 or tuple(densities.shape) != (bn, 2, W, H) or \
+                not densities.is_contiguous():
+            raise ValueError("densities must be a contiguous float32 tensor of shape (bn, 2, W, H)")
     rc = L.ude_sample_poisson(bn, N, W, H, k, float(focallength), float(smoothing), float(precision), rate,
                               _lib.ptr(ts), _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs),
-                              stdDevs.shape[0], s & 0xFFFFFFFFFFFFFFFF, _lib.ptr(ids), _lib.ptr(ipi), _lib.ptr(err),
-                              _lib.ptr(ws), ws.numel(), _lib.stream(dev))
+                              stdDevs.shape[0], s & 0xFFFFFFFFFFFFFFFF, _lib.ptr(ids), _lib.ptr(ipi),
+                              _lib.ptr(densities), _lib.ptr(err), _lib.ptr(ws), ws.numel(), _lib.stream(dev))
     _lib.check(rc, "sampling.samplePoisson")
     last_error = err
     return ids, ipi
+
+
+def tileFeatures(width, height, focallength, points, normals, stdDevs, smoothing, precision, densities,
+                 imageGrad=None):
+    """32 features per 16x16 pixel tile (bn, ceil(W/16), ceil(H/16), 32) for the rate
+    network, from the splats, the densities returned by samplePoisson and
+    (optionally) the image gradient (bn, W, H, 3)."""
+    dev = _lib.require_gpu(points, normals, stdDevs, densities)
+    points = _lib.f32(points, "points")
+    normals = _lib.f32(normals, "normals")
+    stdDevs = _lib.f32(stdDevs, "stdDevs")
+    densities = _lib.f32(densities, "densities")
+    bn, N, _ = points.shape
+    W, H = int(width), int(height)
+    if tuple(densities.shape) != (bn, 2, W, H):
+        raise ValueError("densities must have shape (bn, 2, W, H)")
+    g = None
+    if imageGrad is not None:
+        _lib.require_gpu(imageGrad)
+        g = _lib.f32(imageGrad, "imageGrad")
+        if tuple(g.shape) != (bn, W, H, 3):
+            raise ValueError("imageGrad must have shape (bn, W, H, 3)")
+    out = torch.empty((bn, (W + 15) // 16, (H + 15) // 16, 32), dtype=torch.float32, device=dev)
+    L = _lib.lib()
+    ws = _lib.workspace(L.ude_tile_features_workspace_bytes(W, H), dev)
+    rc = L.ude_tile_features(bn, N, W, H, float(focallength), float(smoothing), float(precision), _lib.ptr(points),
+                             _lib.ptr(normals), _lib.ptr(stdDevs), stdDevs.shape[0], _lib.ptr(densities), _lib.ptr(g),
+                             _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream(dev))
+    _lib.check(rc, "sampling.tileFeatures")
+    return out
 
 
 def check_errors():
