@@ -66,11 +66,12 @@ size_t ude_poisson_workspace_bytes(int N, int W, int H);
  * reference, whose plain-Poisson kernel poissonRadixTree.cuh:70-200 is dead
  * code with unknown inclusion probabilities).  At pixel q every culled splat i
  * is included independently with the defensive-mixture probability
- *   pi_i = min(1, r/2 (g0_i(q) / max(D0(q), c0) + g1_i(q) / max(D1(q), c1))),
+ *   pi_i = min(1, r p_i),  p_i = (g0_i(q) / max(D0(q), c0) + g1_i(q) / max(D1(q), c1)) / 2,
  * r = rate * tileScale, g_b = c_b exp(-|q - s_i|^2 / 2 h_b^2), c_b = 1 / (2 pi
  * sqrt(h_b)), h_0 the tree's bandwidth, h_1 = max(h_0, 99.9th percentile of
  * the visible splats' sqrt(lambda_max(V_i))), D_b the fixed-order density of h_b
- * (nothing where D0 < precision).  tileScale (optional, (bn, ceil(W/16),
+ * (nothing where D0 < precision; pairs with p_i < 1e-4 are outside the support,
+ * which bounds 1 / pi by 1e4 / r).  tileScale (optional, (bn, ceil(W/16),
  * ceil(H/16))) multiplies the rate per 16x16 tile.  Outputs (bn, W, H, kcap):
  * splat ids (-1 padded) and 1 / pi_i (0 padded), so sum_s w_s a_s / pi_s is a
  * Horvitz-Thompson estimate of sum_i w_i a_i.  densOut (optional, (bn, 2, W, H))

@@ -1322,6 +1322,7 @@ static void o_band_build(OFixed* F, OBand* B, float h, double prec)
 }
 
 #define O_PROPOSAL_HMAX 16.0f
+#define O_PMIN 1e-4f                 /* support: pairs whose rate-free proposal p >= O_PMIN */
 #define O_HBINS 256              /* bandwidth histogram: 1/16 px bins over [0, 16) */
 
 static int o_fixed_build(int N, int W, int H, double f, double smoothing, double prec,
@@ -1445,12 +1446,15 @@ int o_fmm_weights_view(int N, int W, int H, double f, double smoothing, double p
 /* the plain-Poisson idea of poissonRadixTree.cuh:70-200 made exact).        */
 /* Per pixel q, every culled splat i is included independently with the      */
 /* defensive-mixture probability (Hesterberg 1995)                           */
-/*   pi_i(q) = min(1, lambda [ a g0_i(q) / N0(q) + (1 - a) g1_i(q) / N1(q) ]) */
+/*   pi_i(q) = min(1, lambda p_i(q)),                                         */
+/*   p_i(q) = a g0_i(q) / N0(q) + (1 - a) g1_i(q) / N1(q)                     */
 /* g_b = c_b exp(-|q - s_i|^2 / 2 h_b^2) for the tree bandwidth h_0 = h and  */
 /* the wide bandwidth h_1 = hw >= 99.9% of the visible splats' bandwidths,  */
 /* N_b = max(D_b, c_b) with D_b the fixed-order density of band b, a = 1/2;  */
 /* so sum_i pi_i stays near lambda and, for splats not wider than hw, EWA    */
-/* weight / pi stays bounded.  No samples where D0 < precision.              */
+/* weight / pi stays bounded.  No samples where D0 < precision.  Support:   */
+/* pairs with p >= 1e-4 (rate independent), so 1 / pi <= 1e4 / lambda; far   */
+/* tails of splats wider than h_1 are outside it.                            */
 /* Realised by a left-to-right tree walk: a node whose bound U (the same     */
 /* expression at the point of its box closest to q) is zero is skipped; a    */
 /* leaf, or a node with count U <= 1, is thinned: candidates at rate U by    */
@@ -1487,13 +1491,13 @@ static float o_puniform(uint32_t seed_lo, uint32_t seed_hi, uint32_t view, uint3
     return (float)((o[draw & 3u] >> 8) + 1u) * 5.9604644775390625e-08f;
 }
 
-/* mixture proposal at point (x, y): s0 g0 + s1 g1 */
-static float o_mix(const OFixed* F, float s0, float s1, float qx, float qy, float x, float y)
+/* mixture proposal at point (x, y): t0 g0 + t1 g1 */
+static float o_mix(const OFixed* F, float t0, float t1, float qx, float qy, float x, float y)
 {
     float v0 = o_band_point(&F->band[0], qx, qy, x, y);
     float v1 = o_band_point(&F->band[1], qx, qy, x, y);
-    float m = s0 * v0;
-    m = m + s1 * v1;
+    float m = t0 * v0;
+    m = m + t1 * v1;
     return m;
 }
 
@@ -1509,9 +1513,8 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
     float D1 = o_fixed_eval(F, 1, qx, qy);
     /* normalisers floored at one splat's peak c_b: where the pixel sits in a gap,
      * D_b (truncated at the query radius) misses the nearby splats it normalises */
-    const float half = 0.5f * lambda;
-    const float s0 = half / fmaxf(D0, F->band[0].c);
-    const float s1 = half / fmaxf(D1, F->band[1].c);
+    const float t0 = 0.5f / fmaxf(D0, F->band[0].c);
+    const float t1 = 0.5f / fmaxf(D1, F->band[1].c);
     const int L0 = T->nc - 1;
     const uint32_t pixel = (uint32_t)(j + H * i);
     uint32_t draw = 0;
@@ -1521,9 +1524,10 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
         const float* b = T->box + 4 * nd;
         float cx = fminf(fmaxf(qx, b[0]), b[2]);
         float cy = fminf(fmaxf(qy, b[1]), b[3]);
-        float U = fminf(1.0f, o_mix(F, s0, s1, qx, qy, cx, cy));
+        float pB = o_mix(F, t0, t1, qx, qy, cx, cy);     /* proposal bound over the node */
+        float U = fminf(1.0f, lambda * pB);
         int descend = 0;
-        if (U > 0.0f) {
+        if (pB >= O_PMIN && U > 0.0f) {
             int leaf = nd >= L0;
             int cnt = T->count[nd];
             if (!leaf && (float)cnt * U > 1.0f) descend = 1;
@@ -1538,7 +1542,9 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
                     if (!(r < (float)(last - pos))) break;
                     pos += (int)r + 1;
                     int lf = L0 + pos;
-                    float pi = fminf(1.0f, o_mix(F, s0, s1, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]));
+                    float pl = o_mix(F, t0, t1, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]);
+                    if (!(pl >= O_PMIN)) continue;       /* outside the support */
+                    float pi = fminf(1.0f, lambda * pl);
                     float a = o_puniform(seed_lo, seed_hi, view, pixel, draw++);
                     if (a * U <= pi) {          /* P = pi / U (exactly 1 when pi = U) */
                         if (ns < kcap) { ids[ns] = T->ids[pos]; ipi[ns] = 1.0f / pi; ns++; }

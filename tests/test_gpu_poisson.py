@@ -81,16 +81,11 @@ def test_poisson_gradients_unbiased(dev):
         return torch.cat([gP.reshape(-1), gN.reshape(-1), gS.reshape(-1), gA.reshape(-1)]).double()
 
     from cudaExtensions import sampling
-    # dense reference: every splat at every pixel that can draw samples (D0 >= precision);
-    # all 256 splats are visible, and pairs whose inclusion probability underflows in
-    # FP32 lie > 13 h_1 apart, where the EWA weight is 0 in FP32 as well
-    T = O.Tree(W, H, f, p, nb, sd, 0.0, 0.01)
-    assert T.nc == p.shape[0]
-    D0 = sampling.approximateFMMWeights(W, H, f, P, NB, None, SD, 0.0, 0.01)
-    mask = (D0 >= 0.01)[..., None]
-    ids_d = torch.where(mask, torch.arange(p.shape[0], device=dev, dtype=torch.int32).view(1, 1, 1, -1),
-                        torch.full((1,), -1, device=dev, dtype=torch.int32)).contiguous()
-    ipi_d = mask.float().expand(-1, -1, -1, p.shape[0]).contiguous()
+    # dense reference: every (pixel, splat) pair of the support (rate-free proposal
+    # p >= 1e-4), drawn with certainty by a rate so large that pi = min(1, rate p) = 1
+    ids_d, ipi_d = sampling.samplePoisson(W, H, f, P, NB, None, SD, 256, 0.0, 0.01, seed=0, rate=1e30)
+    sampling.check_errors()
+    assert bool(((ipi_d == 1.0) | ((ids_d < 0) & (ipi_d == 0.0))).all())
     dense = grads(ids_d, ipi_d)
     S = 4096
     s1 = torch.zeros_like(dense)

@@ -48,12 +48,14 @@ def proposal(p, nb, sd, f, prec):
 
 
 def inclusion_probs(p, nb, sd, f, lam, prec, dens, raw=False):
-    """pi_i = min(1, lam/2 (g0_i / max(D0, c0) + g1_i / max(D1, c1))), 0 where D0 < prec."""
+    """pi_i = min(1, lam p_i), p_i = (g0_i / max(D0, c0) + g1_i / max(D1, c1)) / 2; 0 where
+    D0 < prec or p_i < 1e-4 (outside the support)."""
     (g0, c0), (g1, c1) = proposal(p, nb, sd, f, prec)[0]
     T = proposal(p, nb, sd, f, prec)[1]
     D0 = dens[0].reshape(-1, 1).astype(np.float64)
     D1 = dens[1].reshape(-1, 1).astype(np.float64)
-    ratio = np.where(D0 >= prec, 0.5 * lam * (g0 / np.maximum(D0, c0) + g1 / np.maximum(D1, c1)), 0.0)
+    pr = 0.5 * (g0 / np.maximum(D0, c0) + g1 / np.maximum(D1, c1))
+    ratio = np.where((D0 >= prec) & (pr >= 1e-4), lam * pr, 0.0)
     if raw:
         return ratio, T.ids_sorted
     return np.minimum(1.0, ratio), T.ids_sorted

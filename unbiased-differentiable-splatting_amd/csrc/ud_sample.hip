@@ -1489,15 +1489,19 @@ __global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState*
 // only inside the reached subtree, which leaves the inclusion probabilities
 // unknown).  Every culled splat i is included at pixel q independently with the
 // defensive-mixture probability (Hesterberg 1995)
-//   pi_i = min(1, lambda/2 (g0_i(q) / N0(q) + g1_i(q) / N1(q))),
+//   pi_i = min(1, lambda p_i(q)),  p_i = (g0_i(q) / N0(q) + g1_i(q) / N1(q)) / 2,
 // g_b = c_b exp(-|q - s_i|^2 / 2 h_b^2) for the tree bandwidth h_0 = h and the
 // wide bandwidth h_1 = max(h, 99.9th percentile of the visible splats' own
 // bandwidths sqrt(lambda_max(V_i))), and N_b = max(D_b, c_b) with D_b the
 // fixed-order density of band b: the expected count stays near lambda, and EWA
 // weight / pi stays bounded for splats not wider than h_1.  Pixels with
-// D_0 < precision draw nothing.
+// D_0 < precision draw nothing.  The support is the set of pairs with
+// p >= 1e-4 (independent of the rate), so 1 / pi <= 1e4 / lambda: far tails of
+// splats wider than h_1, whose weights there would be multiplied by huge
+// 1 / pi, are outside it.
 // ---------------------------------------------------------------------------
 constexpr float kProposalHCap = 16.0f;
+constexpr float kProposalPMin = 1e-4f;   // support: pairs whose rate-free proposal p >= 1e-4
 
 // histogram of the visible splats' proposal bandwidths (1/16 px bins, the
 // last bin holds everything >= 16; degenerate splats are skipped)
@@ -1532,14 +1536,14 @@ __global__ void k_wide_band(ViewState* vs)
     vs->band[1] = make_band(hw, (double)vs->prec);
 }
 
-// mixture proposal s0 g0 + s1 g1 at (x, y)
-__device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float s0, float s1, float qx, float qy,
+// rate-free mixture proposal t0 g0 + t1 g1 at (x, y)
+__device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float t0, float t1, float qx, float qy,
                                            float x, float y)
 {
     float v0 = band_point(B0, qx, qy, x, y);
     float v1 = band_point(B1, qx, qy, x, y);
-    float m = s0 * v0;
-    m = m + s1 * v1;
+    float m = t0 * v0;
+    m = m + t1 * v1;
     return m;
 }
 
@@ -1584,9 +1588,8 @@ __global__ __launch_bounds__(256) void k_poisson(int W, int H, int kcap, float r
         const Band B0 = vs->band[0], B1 = vs->band[1];
         float lam = rate;
         if (tscale) lam = lam * tscale[(long)(i >> 4) * tsH + (j >> 4)];
-        const float half = 0.5f * lam;
-        const float s0 = half / fmaxf(D0, B0.c);
-        const float s1 = half / fmaxf(dens1[pix], B1.c);
+        const float t0 = 0.5f / fmaxf(D0, B0.c);
+        const float t1 = 0.5f / fmaxf(dens1[pix], B1.c);
         const float qx = (float)i, qy = (float)j;
         const uint32_t pixel = (uint32_t)pix;
         uint32_t draw = 0, blkId = 0xffffffffu;
@@ -1608,9 +1611,10 @@ __global__ __launch_bounds__(256) void k_poisson(int W, int H, int kcap, float r
             NodeGeo g = ld_geo(T, nd);
             float cx = fminf(fmaxf(qx, g.lox), g.hix);
             float cy = fminf(fmaxf(qy, g.loy), g.hiy);
-            float U = fminf(1.0f, mix_value(B0, B1, s0, s1, qx, qy, cx, cy));
+            float pB = mix_value(B0, B1, t0, t1, qx, qy, cx, cy);     // proposal bound over the node
+            float U = fminf(1.0f, lam * pB);
             bool descend = false;
-            if (U > 0.0f) {
+            if (pB >= kProposalPMin && U > 0.0f) {
                 const bool leaf = nd >= L0;
                 NodeLnk lk = ld_lnk(T, nd);
                 if (!leaf && (float)lk.count * U > 1.0f) descend = true;
@@ -1625,7 +1629,9 @@ __global__ __launch_bounds__(256) void k_poisson(int W, int H, int kcap, float r
                         if (!(r < (float)(last - pos))) break;
                         pos += (int)r + 1;
                         NodeGeo lg = ld_geo(T, L0 + pos);
-                        float pi = fminf(1.0f, mix_value(B0, B1, s0, s1, qx, qy, lg.cx, lg.cy));
+                        float pl = mix_value(B0, B1, t0, t1, qx, qy, lg.cx, lg.cy);
+                        if (!(pl >= kProposalPMin)) continue;     // outside the support
+                        float pi = fminf(1.0f, lam * pl);
                         float a = uniform();
                         if (a * U <= pi) {            // P = pi / U, exactly 1 when pi = U
                             if (ns < kcap) { oid[ns] = leafId[pos]; oip[ns] = 1.0f / pi; ns++; }

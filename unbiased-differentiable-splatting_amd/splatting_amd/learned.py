@@ -88,13 +88,19 @@ class RateNet:
                 hist.append((s, float(loss.item())))
         return hist
 
-    def tile_scale(self, feats, counts):
-        """Rate multipliers exp(prediction), rescaled so that the expected sample
-        count over the tiles, sum(scale * counts), is unchanged."""
-        s = torch.exp(self.forward(feats).clamp(-8.0, 8.0))
-        budget = counts.sum()
-        got = (s * counts).sum()
-        return torch.where(got > 0, s * (budget / got.clamp(min=1e-30)), s)
+    def tile_scale(self, feats, counts, lo=1.0 / 32, hi=4.0):
+        """Rate multipliers for tiles with features (..., 32) and plain-rate sample
+        counts (...): exp(prediction), rescaled so that the expected sample count
+        sum(scale * counts) is unchanged, and kept within [lo, hi]."""
+        shape = counts.shape
+        y = self.forward(feats.reshape(-1, N_IN)).reshape(shape).double()
+        c = counts.double()
+        tot = c.sum().clamp(min=1e-30)
+        y = y - (y * c).sum() / tot
+        s = torch.exp(y.clamp(-20.0, 20.0))
+        for _ in range(3):
+            s = (s * (tot / (s * c).sum().clamp(min=1e-30))).clamp(lo, hi)
+        return s.float().contiguous()
 
     def save(self, path):
         np.savez(path, params=self.params.cpu().numpy(), mean=self.mean.cpu().numpy(), std=self.std.cpu().numpy())
@@ -104,3 +110,216 @@ class RateNet:
         d = np.load(path)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(device)
         return cls(t(d["params"]), t(d["mean"]), t(d["std"]))
+
+
+# --------------------------------------------------------------------------- Poisson-rate frames
+class PoissonFrame:
+    """One fwd+bwd frame of the Horvitz-Thompson renderer on one view: world
+    positions / normals through the affine / linear transforms, Poisson-rate
+    sampling (optionally with per-tile rate multipliers), EWA weights times 1/pi,
+    interpolated albedo (shading "None", no occlusion, so the image estimate is
+    unbiased), L2 loss against ``target``, and the raw adjoints of every op.
+    Gradients are with respect to world positions / normals, std-devs and albedo."""
+
+    def __init__(self, scene, device, rate=10.0, kcap=40, precision=0.01, target=None):
+        from cudaExtensions import transform
+        self.sc = scene
+        self.dev = device
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(device)
+        self.M = t(scene.view)[None].contiguous()
+        self.Pw = t(scene.positions)[None].contiguous()
+        self.Nw = t(scene.normals)[None].contiguous()
+        self.S = t(scene.stddevs)[None].contiguous()
+        self.A = t(scene.albedo)[None].contiguous()
+        self.rate, self.kcap, self.prec = rate, kcap, precision
+        self.target = target
+        self.W, self.H, self.f = scene.width, scene.height, scene.focal
+        self.dens = torch.zeros((1, 2, self.W, self.H), dtype=torch.float32, device=device)
+        self._transform = transform
+
+    def camera(self):
+        tr = self._transform
+        P = tr.affineForward(self.M, self.Pw)
+        Nc = tr.linearForward(self.M, self.Nw)
+        nb = torch.where(Nc[..., 2:3] > 0, -Nc, Nc).contiguous()
+        return P, Nc, nb
+
+    def image(self, seed, tile_scale=None, rate=None, kcap=None):
+        """Sampled image estimate (1, W, H, 3) and the sample tensors."""
+        from cudaExtensions import sampling, splatting, interpolation
+        P, Nc, nb = self.camera()
+        ids, ipi = sampling.samplePoisson(self.W, self.H, self.f, P, nb, None, self.S, kcap or self.kcap, 0.0,
+                                          self.prec, seed=seed, rate=rate or self.rate, tileScale=tile_scale,
+                                          densities=self.dens)
+        w = splatting.forward(self.W, self.H, self.f, ids, P, Nc, self.S, 0.0)
+        wt = (w * ipi).contiguous()
+        img = interpolation.forward(ids, wt, self.A)
+        return img, (P, Nc, ids, ipi, wt)
+
+    def step(self, seed, tile_scale=None, rate=None, kcap=None):
+        """Returns (loss, image, grads (positions, normals, std-devs, albedo), dL/dimage)."""
+        from cudaExtensions import splatting, interpolation
+        tr = self._transform
+        img, (P, Nc, ids, ipi, wt) = self.image(seed, tile_scale, rate, kcap)
+        g = (2.0 * (img - self.target)).contiguous()
+        loss = ((img - self.target) ** 2).sum()
+        gW, gA = interpolation.backward(g, ids, wt, self.A)
+        gw = (gW * ipi).contiguous()
+        gP, gN, gS = splatting.backward(self.W, self.H, self.f, gw, ids, P, Nc, self.S, 0.0)
+        _, gPw = tr.affineBackward(gP, self.M, self.Pw)
+        _, gNw = tr.linearBackward(gN, self.M, self.Nw)
+        return loss, img, (gPw, gNw, gS, gA), g
+
+
+def mean_image(frame, seeds, rate=None):
+    """Average of the sampled image estimates over the given seeds."""
+    acc = None
+    for s in seeds:
+        img, _ = frame.image(s, rate=rate)
+        acc = img.double() if acc is None else acc + img.double()
+    return (acc / len(seeds)).float().contiguous()
+
+
+def tile_counts(ids):
+    """Number of samples per 16x16 tile, (bn, ceil(W/16), ceil(H/16))."""
+    bn, W, H, _ = ids.shape
+    c = (ids >= 0).sum(-1).float()
+    pw, ph = (-W) % 16, (-H) % 16
+    c = torch.nn.functional.pad(c, (0, ph, 0, pw))
+    return c.reshape(bn, c.shape[1] // 16, 16, c.shape[2] // 16, 16).sum((2, 4))
+
+
+def tile_sum(img):
+    bn, W, H = img.shape[:3]
+    x = img.reshape(bn, W, H, -1).sum(-1)
+    x = torch.nn.functional.pad(x, (0, (-H) % 16, 0, (-W) % 16))
+    return x.reshape(bn, x.shape[1] // 16, 16, x.shape[2] // 16, 16).sum((2, 4))
+
+
+def splat_tiles(frame):
+    """Tile index (ti * tH + tj) of every splat's screen centre, -1 outside the image."""
+    P, Nc, nb = frame.camera()
+    p = P[0]
+    sx = -(p[:, 0] * frame.f) / p[:, 2] + 0.5 * frame.W
+    sy = (p[:, 1] * frame.f) / p[:, 2] + 0.5 * frame.H
+    ok = (p[:, 2] < 0) & (sx >= 0) & (sx < frame.W) & (sy >= 0) & (sy < frame.H)
+    tH = (frame.H + 15) // 16
+    idx = (sx.clamp(0, frame.W - 1).long() >> 4) * tH + (sy.clamp(0, frame.H - 1).long() >> 4)
+    return torch.where(ok, idx, torch.full_like(idx, -1))
+
+
+def tile_variance(frame, seeds, rate=None):
+    """Per 16x16 tile: the summed per-parameter variance of the sampled gradient
+    over the splats centred in the tile (sampling is independent per pixel, so a
+    splat's gradient variance comes from the pixels around it), the mean sample
+    count, and the mean image gradient."""
+    s1 = s2 = cnt = g_img = None
+    for s in seeds:
+        _, _, grads, g = frame.step(s, rate=rate)
+        v = torch.cat([grads[0][0], grads[1][0], grads[2][0], grads[3][0]], dim=-1).double()   # (N, 11)
+        s1 = v if s1 is None else s1 + v
+        s2 = v * v if s2 is None else s2 + v * v
+        g_img = g if g_img is None else g_img + g
+        _, (P, Nc, ids, ipi, wt) = frame.image(s, rate=rate)
+        c = tile_counts(ids).double()
+        cnt = c if cnt is None else cnt + c
+    S = len(seeds)
+    mean = s1 / S
+    var = ((s2 - S * mean * mean) / (S - 1)).clamp(min=0).sum(-1)            # per splat
+    tiles = splat_tiles(frame)
+    V = torch.zeros(cnt.numel(), dtype=torch.float64, device=frame.dev)
+    ok = tiles >= 0
+    V.index_add_(0, tiles[ok], var[ok])
+    return V, (cnt / S).reshape(-1), g_img / S
+
+
+def variance_targets(frame, seeds, low=0.25):
+    """Plain-rate variance measurements of one scene at rate lambda and lambda *
+    low.  Under Poisson sampling a tile's gradient variance behaves like
+    V_t(s) = a_t / s + b_t in its rate multiplier s (saturated inclusions, pi = 1,
+    contribute only once s drops), so two rates give the marginal cost
+    a_t = (V_t(low) - V_t(1)) / (1 / low - 1).  The Neyman allocation of a fixed
+    budget sum_t s_t n_t (n_t = plain samples in the tile) minimises sum_t a_t / s_t
+    with s_t proportional to sqrt(a_t / n_t): the training target is
+    y_t = log sqrt(a_t / n_t) (centred per view), weighted by n_t.
+    Returns (features (T, 32), target (T,), weight (T,))."""
+    from cudaExtensions import sampling
+    V1, n, g_img = tile_variance(frame, seeds)
+    Vl, _, _ = tile_variance(frame, [s + 7919 for s in seeds], rate=frame.rate * low)
+    a = ((Vl - V1) / (1.0 / low - 1.0)).clamp(min=0)
+    P, Nc, nb = frame.camera()
+    feats = sampling.tileFeatures(frame.W, frame.H, frame.f, P, nb, frame.S, 0.0, frame.prec, frame.dens,
+                                  g_img.contiguous())
+    amax = float(a.max().item())
+    y = 0.5 * torch.log((a + 1e-9 * amax + 1e-30) / (n + 1e-3))
+    # only the allocation within a view matters: centre the targets (count-weighted)
+    y = y - (y * n).sum() / n.sum().clamp(min=1e-30)
+    return feats.reshape(-1, N_IN), y.float(), n.float()
+
+
+def gradient_variance(frame, seeds, tile_scale=None, rate=None, kcap=None):
+    """Mean over all parameters of the per-parameter variance of the sampled
+    gradient across seeds (raw adjoints), and the mean gradient."""
+    s1 = s2 = None
+    for s in seeds:
+        _, _, grads, _ = frame.step(s, tile_scale, rate, kcap)
+        x = torch.cat([gq.reshape(-1) for gq in grads]).double()
+        s1 = x if s1 is None else s1 + x
+        s2 = x * x if s2 is None else s2 + x * x
+    S = len(seeds)
+    mean = s1 / S
+    var = (s2 - S * mean * mean) / (S - 1)
+    return float(var.clamp(min=0).mean().item()), mean
+
+
+def make_target(scene, device, rate, kcap, precision=0.01, seeds=range(10_000, 10_016)):
+    """Target image: the mean Poisson-rate estimate of the scene with perturbed centres."""
+    from . import scenes
+    pert = scenes.Scene(scenes.perturbed_target_positions(scene), scene.normals, scene.stddevs, scene.albedo,
+                        scene.width, scene.height, scene.focal, scene.view)
+    fr = PoissonFrame(pert, device, rate=rate, kcap=kcap, precision=precision)
+    return mean_image(fr, list(seeds))
+
+
+def train_rate_net(device, scenes_k=range(20), seeds=8, rate=10.0, kcap=40, steps=3000, n_override=None,
+                   log=print):
+    """Trains a RateNet on plain-rate variance measurements of random ellipsoid
+    scenes (all on the GPU: sampling, rendering and the network's own kernels)."""
+    from . import scenes
+    F, Y, Wt = [], [], []
+    for k in scenes_k:
+        sc = scenes.training_scene(k, n_override)
+        target = make_target(sc, device, rate, kcap)
+        fr = PoissonFrame(sc, device, rate=rate, kcap=kcap, target=target)
+        f, y, w = variance_targets(fr, list(range(100 * k, 100 * k + seeds)))
+        F.append(f); Y.append(y); Wt.append(w)
+        log("rate net data: scene %d, %d tiles" % (k, f.shape[0]))
+    F, Y, Wt = torch.cat(F), torch.cat(Y), torch.cat(Wt)
+    net = RateNet.init(device, seed=0)
+    hist = net.fit(F, Y, Wt, steps=steps, lr=3e-3, log_every=max(steps // 6, 1))
+    for s, l in hist:
+        log("rate net training: step %d loss %.4f" % (s, l))
+    return net, hist
+
+
+DEFAULT_WEIGHTS = __file__.rsplit("/", 1)[0] + "/rate_net.npz"
+
+
+def main():
+    import argparse
+    ap = argparse.ArgumentParser(description="train the per-tile rate network on the GPU")
+    ap.add_argument("--scenes", type=int, default=20)
+    ap.add_argument("--seeds", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--splats", type=int, default=None)
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    args = ap.parse_args()
+    dev = torch.device("cuda:0")
+    net, hist = train_rate_net(dev, range(args.scenes), args.seeds, steps=args.steps, n_override=args.splats,
+                               log=lambda m: print(m, flush=True))
+    net.save(args.out)
+    print("saved", args.out, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -100,6 +100,14 @@ def config(idx, n_override=None):
     raise ValueError("unknown workload %r" % idx)
 
 
+def training_scene(k, n_override=None):
+    """The k-th random ellipsoid scene used to train the rate network: 64 ellipsoids with
+    random axes and centres, 1M surfels, 512x512 (focal 614.4), ellipsoid seed 1000 + k."""
+    per = 15625 if n_override is None else max(1, n_override // 64)
+    p, n, s, a = ellipsoids(64, per, seed=1000 + k)
+    return Scene(p, n, s, a, 512, 512, 1.2 * 512)
+
+
 def perturbed_target_positions(scene, sigma=0.01, seed=100):
     rng = np.random.default_rng(seed)
     return (scene.positions + rng.normal(0.0, sigma, size=scene.positions.shape)).astype(np.float32)
