@@ -150,6 +150,85 @@ def cpu_baseline(sc, rows=None, k=40, threads=None):
                       % (sc.n, t_tree, rows, W, t_sample + t_ops, W / float(rows))}
 
 
+def rates_comparison(dev, sc, steps, rate=10.0, kcap=40, seeds=32):
+    """Poisson-rate sampling with 1/pi weights on the same scene: the plain FMM-derived
+    rates against the per-tile learned rates at equal gradient variance.  The learned
+    multipliers come from the trained rate network (splatting_amd/rate_net.npz) fed
+    with the tile features of the previous frame; the rate that equalises the mean
+    per-parameter gradient variance is rate * V_learned / V_plain (variance scales
+    with 1 / rate), checked by a second measurement.  When the learned rates need
+    more samples than the plain ones for the same variance, the plain rates are used
+    (they are the network's fallback) and reported as the chosen path."""
+    import os
+    import torch
+    from cudaExtensions import sampling
+    from splatting_amd import learned as L
+
+    def timed(fn, n):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.time()
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return max(e0.elapsed_time(e1), (time.time() - t0) * 1e3) / n
+
+    target = L.make_target(sc, dev, rate, kcap)
+    fr = L.PoissonFrame(sc, dev, rate=rate, kcap=kcap, target=target)
+    counter = [0]
+
+    def plain_step():
+        counter[0] += 1
+        fr.step(1000 + counter[0])
+
+    plain_ms = timed(plain_step, steps)
+    seed_list = list(range(20_000, 20_000 + seeds))
+    v_plain, _ = L.gradient_variance(fr, seed_list)
+    out = {"estimator": "Horvitz-Thompson, linear splatting, L2 loss, raw adjoints",
+           "plain_rate": rate, "kcap": kcap, "plain_fps": 1000.0 / plain_ms, "plain_ms": plain_ms,
+           "plain_grad_variance": v_plain, "variance_seeds": seeds}
+    wpath = L.DEFAULT_WEIGHTS
+    if not os.path.exists(wpath):
+        out.update({"learned": None, "chosen": "plain (no trained rate network)",
+                    "equal_variance_fps": 1000.0 / plain_ms})
+        return out
+    net = L.RateNet.load(wpath, dev)
+    _, _, _, g = fr.step(99)
+    P, Nc, nb = fr.camera()
+    feats = sampling.tileFeatures(fr.W, fr.H, fr.f, P, nb, fr.S, 0.0, fr.prec, fr.dens, g)
+    _, (_, _, ids, _, _) = fr.image(98)
+    counts = L.tile_counts(ids)[0]
+    ts = net.tile_scale(feats[0], counts)[None].contiguous()
+    v_learned, _ = L.gradient_variance(fr, seed_list, tile_scale=ts, kcap=4 * kcap)
+    kappa = v_learned / v_plain
+    out.update({"learned_grad_variance_equal_budget": v_learned, "rate_factor_for_equal_variance": kappa,
+                "tile_scale_range": [float(ts.min().item()), float(ts.max().item())]})
+    if kappa < 1.0:
+        v_check, _ = L.gradient_variance(fr, seed_list, tile_scale=ts, rate=rate * kappa, kcap=4 * kcap)
+
+        def learned_step():
+            counter[0] += 1
+            _, _, _, gg = fr.step(5000 + counter[0], tile_scale=ts, rate=rate * kappa, kcap=4 * kcap)
+            Pq, Ncq, nbq = fr.camera()
+            f2 = sampling.tileFeatures(fr.W, fr.H, fr.f, Pq, nbq, fr.S, 0.0, fr.prec, fr.dens, gg)
+            net.forward(f2.reshape(-1, 32))
+
+        learned_ms = timed(learned_step, steps)
+        out.update({"learned_fps": 1000.0 / learned_ms, "learned_grad_variance_at_rate": v_check})
+        chosen = "learned" if learned_ms < plain_ms else "plain (fallback: learned slower)"
+        out.update({"chosen": chosen, "equal_variance_fps": 1000.0 / min(plain_ms, learned_ms),
+                    "speedup_vs_plain": plain_ms / min(plain_ms, learned_ms)})
+    else:
+        out.update({"learned_fps": None, "chosen": "plain (fallback: learned rates need %.2fx the samples)" % kappa,
+                    "equal_variance_fps": 1000.0 / plain_ms, "speedup_vs_plain": 1.0})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +236,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--splats", type=int, default=None, help="override the splat count (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-rates", action="store_true", help="skip the learned-vs-plain rates comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of known size (1 GiB read + 1 GiB written per launch), "
                          "to scale HBM traffic counters")
@@ -232,6 +312,9 @@ def main():
                      "frame_frac": frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "ops_ms": {k_: round(v, 4) for k_, v in sorted(ops.items())},
     }
+    if not args.no_rates:
+        print("bench: learned vs plain rates", file=sys.stderr, flush=True)
+        line["rates"] = rates_comparison(dev, sc, args.steps)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(sc)

@@ -1400,23 +1400,34 @@ __device__ __forceinline__ float band_point(const Band& B, float qx, float qy, f
 }
 
 // One lane per pixel, 8x8 pixel tile per wavefront, 4 tiles per workgroup.
-// The reference's explicit stack pushes the left then the right child of an
-// unaccepted node when the child's box meets the query disc, so it sums the
-// right subtree first; the same order is walked here without a stack, via
-// parent links.
+// The reference walks each pixel with an explicit stack: pop a node; evaluate
+// it if it is a leaf or carries an expansion, else push its left child and then
+// its right child, each only if its box meets the query disc
+// (transformRadixTree.cuh:15-69).  Here the wavefront runs that walk once for
+// the union of its 64 pixels: stack entries carry the mask of lanes for which
+// the node was pushed, so each lane sees its own nodes in its own order (the
+// shared stack is its stack with other lanes' entries interleaved, in the same
+// last-in-first-out order) and sums bit for bit like the reference, while node
+// records and coefficient blocks are read once per wavefront at uniform
+// addresses.  Karras trees on 64-bit keys are at most 64 levels deep, so the
+// stack never holds more than 65 entries.
+constexpr int kFmmStack = 96;
 __global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState* __restrict__ vs, int b,
                                                   const NodeGeo* geo, const NodeLnk* lnk,
                                                   const int* __restrict__ fixOff, const float* coef,
                                                   float* __restrict__ out)
 {
+    __shared__ int stkN[4][kFmmStack];
+    __shared__ unsigned long long stkM[4][kFmmStack];
     const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
     const int tilesY = (H + 7) >> 3;
     const int ntiles = ((W + 7) >> 3) * tilesY;
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wv >= ntiles) return;
+    const int wv = blockIdx.x * 4 + wid;
+    if (wv >= ntiles) return;                                // wave-uniform
     const int ti = wv / tilesY, tj = wv - (wv / tilesY) * tilesY;
     const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
-    if (i >= W || j >= H) return;
+    const bool valid = i < W && j < H;
     const int n = vs->nc;
     float w = 0.0f;
     if (n > 0) {
@@ -1427,60 +1438,62 @@ __global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState*
         const Band B = vs->band[b];
         const int L0 = n - 1;
         const float qx = (float)i, qy = (float)j;
-        int nd = 0;
-        while (true) {
+        int* sN = stkN[wid];
+        unsigned long long* sM = stkM[wid];
+        const unsigned long long all = __ballot(valid);
+        if (lane == 0) { sN[0] = 0; sM[0] = all; }
+        int sp = 1;
+        while (sp > 0) {
+            sp--;
+            __builtin_amdgcn_wave_barrier();
+            const int nd = __builtin_amdgcn_readfirstlane(sN[sp]);
+            const unsigned long long mask = sM[sp];
+            __builtin_amdgcn_wave_barrier();
+            const bool act = (mask >> lane) & 1ull;
             NodeGeo g = ld_geo(T, nd);
-            bool climb = true;
-            const int off = nd >= L0 ? -1 : fixOff[nd];
             if (nd >= L0) {
-                w = w + band_point(B, qx, qy, g.cx, g.cy);
-            } else if (off >= 0) {
-                float tx = (qx - g.cx) * B.is2h;
-                float ty = (qy - g.cy) * B.is2h;
-                float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
-                float hy[kFixedDegree + 1];
-                hy[0] = ey;
-                hy[1] = (2.0f * ty) * ey;
+                if (act) w = w + band_point(B, qx, qy, g.cx, g.cy);
+                continue;
+            }
+            const int off = __builtin_amdgcn_readfirstlane(fixOff[nd]);
+            if (off >= 0) {
+                if (act) {
+                    float tx = (qx - g.cx) * B.is2h;
+                    float ty = (qy - g.cy) * B.is2h;
+                    float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+                    float hy[kFixedDegree + 1];
+                    hy[0] = ey;
+                    hy[1] = (2.0f * ty) * ey;
 #pragma unroll
-                for (int q = 2; q <= kFixedDegree; q++)
-                    hy[q] = (2.0f * ty) * hy[q - 1] - (2.0f * (float)(q - 1)) * hy[q - 2];
-                const float UD_GLOBAL* cc = cf + off;
-                float v = 0.0f;
-                float hprev = 0.0f, ha = ex;
+                    for (int q = 2; q <= kFixedDegree; q++)
+                        hy[q] = (2.0f * ty) * hy[q - 1] - (2.0f * (float)(q - 1)) * hy[q - 2];
+                    const float UD_GLOBAL* cc = cf + off;
+                    float v = 0.0f;
+                    float hprev = 0.0f, ha = ex;
 #pragma unroll
-                for (int a = 0; a <= kFixedDegree; a++) {
-                    if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
-                    else if (a >= 2) {
-                        float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
-                        hprev = ha; ha = hn;
+                    for (int a = 0; a <= kFixedDegree; a++) {
+                        if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
+                        else if (a >= 2) {
+                            float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+                            hprev = ha; ha = hn;
+                        }
+#pragma unroll
+                        for (int bb = 0; bb <= kFixedDegree; bb++)
+                            v = v + (cc[a * (kFixedDegree + 1) + bb] * ha) * hy[bb];
                     }
-#pragma unroll
-                    for (int bb = 0; bb <= kFixedDegree; bb++)
-                        v = v + (cc[a * (kFixedDegree + 1) + bb] * ha) * hy[bb];
+                    w = w + v;
                 }
-                w = w + v;
-            } else {
-                NodeLnk lk = ld_lnk(T, nd);
-                if (box_meets_disc(ld_geo(T, lk.right), qx, qy, B.radius)) { nd = lk.right; climb = false; }
-                else if (box_meets_disc(ld_geo(T, lk.left), qx, qy, B.radius)) { nd = lk.left; climb = false; }
+                continue;
             }
-            if (!climb) continue;
-            // climb to the next unvisited left sibling (the right one was taken first)
-            bool done = true;
-            while (nd != 0) {
-                int p = T.lnk[nd].parent;
-                NodeLnk pl = ld_lnk(T, p);
-                if (nd == pl.right && box_meets_disc(ld_geo(T, pl.left), qx, qy, B.radius)) {
-                    nd = pl.left;
-                    done = false;
-                    break;
-                }
-                nd = p;
-            }
-            if (done) break;
+            NodeLnk lk = ld_lnk(T, nd);
+            const unsigned long long ml = __ballot(act && box_meets_disc(ld_geo(T, lk.left), qx, qy, B.radius));
+            const unsigned long long mr = __ballot(act && box_meets_disc(ld_geo(T, lk.right), qx, qy, B.radius));
+            if (sp + 2 > kFmmStack) break;                   // unreachable for trees of depth <= 64
+            if (ml) { if (lane == 0) { sN[sp] = lk.left; sM[sp] = ml; } sp++; }
+            if (mr) { if (lane == 0) { sN[sp] = lk.right; sM[sp] = mr; } sp++; }
         }
     }
-    out[(long)j + (long)H * i] = w;
+    if (valid) out[(long)j + (long)H * i] = w;
 }
 
 // ---------------------------------------------------------------------------
