@@ -1,14 +1,21 @@
-"""Benchmark: one fwd+bwd frame of Poisson-sampled EWA splatting per step.
+"""Benchmark: fwd+bwd frames/s of Poisson-rate splatting at equal gradient variance.
 
 Workload: 1M surfels on 64 random ellipsoids, one 512x512 view (focal 614.4),
-k = 40 samples per pixel, precision 0.01, smoothing 0, shading mode "None"
-(albedo interpolation), L2 loss (sum) against a target rendered from centres
-perturbed by N(0, 0.01).  One step = affine + linear transform, per-pixel
-sampling (LBVH + Hermite FGT + sequential Poisson), EWA weights, z-sort /
-compositing, interpolation, loss, and the backward of all of them.
+precision 0.01, smoothing 0, albedo interpolation without occlusion, L2 loss
+(sum) against a target rendered from centres perturbed by N(0, 0.01).  One
+step = affine + linear transform, Poisson-rate sampling with inclusion
+probabilities (LBVH, two fixed-order Gauss-transform densities, thinning at
+rate 10, about 8 samples per pixel, 1/pi weights), EWA weights, interpolation,
+loss, and the raw adjoints of all of them.  The per-tile learned rates replace
+the plain ones only when they reach the plain rates' gradient variance at a
+higher frame rate (the plain rates are their fallback); the chosen path is
+timed.  Next to it, the reference's own method (sequential Poisson sampling of
+k = 40 splats per pixel, z-sort / compositing, through the drop-in renderer) is
+timed on the same scene.
 
-Prints one JSON line (rank 0, N = 1 GPU).  Per-op device times are taken
-with HIP events on the stream the ops launch on (torch's current stream).
+Prints one JSON line (rank 0, N = 1 GPU).  Kernel and per-op device times are
+taken with HIP events on the stream the work launches on (torch's current
+stream).
 """
 import argparse
 import json
@@ -24,10 +31,10 @@ for _p in (PKG, ROOT):
 
 METRIC = "fwd+bwd frames/s at 1M splats 512² (learned vs plain rates, equal grad variance)"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the sampler kernel at the 1M-splat 512x512 k=40 frame: rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE on k_sample, each scaled by the same counter on a 1 GiB
-# device copy (bench.py --calibrate-copy); measured offline, not by this script
-TRAFFIC_BYTES = 3.374e9
+RATE, KCAP = 10.0, 40
+# HBM bytes per launch, measured offline (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
+# kernel, each scaled by the same counter on a 1 GiB device copy, bench.py --calibrate-copy)
+TRAFFIC_BYTES = {"k_sample": 3.374e9, "k_fmm_eval": None}
 
 
 def build_problem(dev, n_override=None):
@@ -150,83 +157,140 @@ def cpu_baseline(sc, rows=None, k=40, threads=None):
                       % (sc.n, t_tree, rows, W, t_sample + t_ops, W / float(rows))}
 
 
-def rates_comparison(dev, sc, steps, rate=10.0, kcap=40, seeds=32):
-    """Poisson-rate sampling with 1/pi weights on the same scene: the plain FMM-derived
-    rates against the per-tile learned rates at equal gradient variance.  The learned
-    multipliers come from the trained rate network (splatting_amd/rate_net.npz) fed
-    with the tile features of the previous frame; the rate that equalises the mean
-    per-parameter gradient variance is rate * V_learned / V_plain (variance scales
-    with 1 / rate), checked by a second measurement.  When the learned rates need
-    more samples than the plain ones for the same variance, the plain rates are used
-    (they are the network's fallback) and reported as the chosen path."""
-    import os
+def cpu_baseline_poisson(sc, rate=RATE, kcap=KCAP):
+    """Oracle (C, OpenMP) on the whole Poisson-rate frame: tree build, both
+    fixed-order densities and the thinning pass over every pixel, then EWA
+    weights, interpolation and their FP64 adjoints."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    p, n = sc.camera_space()
+    nb = np.where(n[:, 2:3] > 0, -n, n).astype(np.float32)
+    W, H, f = sc.width, sc.height, sc.focal
+    t0 = time.time()
+    ids, ipi, _, _ = O.poisson_view(W, H, f, p, nb, sc.stddevs, kcap, rate, 0.0, 0.01, seed=3)
+    t_sample = time.time() - t0
+    idx4 = ids[None]
+    p4, n4, s4, a4 = p[None].astype(np.float64), n[None].astype(np.float64), sc.stddevs[None].astype(np.float64), \
+        sc.albedo[None].astype(np.float64)
+    t1 = time.time()
+    w = O.splat_forward(W, H, f, idx4, p4, n4, s4)
+    wt = w * ipi[None]
+    img = O.interp_forward(idx4, wt, a4)
+    g = 2.0 * img
+    gW, _ = O.interp_backward(g, idx4, wt, a4)
+    O.splat_backward(W, H, f, gW * ipi[None], idx4, p4, n4, s4)
+    t_ops = time.time() - t1
+    frame = t_sample + t_ops
+    return {"value": 1.0 / frame, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": "the whole frame: oracle Poisson-rate sampling of all %d splats at rate %g over all %dx%d "
+                      "pixels (%.2fs) + EWA/interpolation fwd+bwd in FP64 (%.2fs)" % (sc.n, rate, W, H, t_sample, t_ops)}
+
+
+def timed_steps(fn, steps, kernel=None):
+    """Runs `steps` calls of fn between a device synchronisation on both sides;
+    returns (ms per step, summed ms and launches of the profiled sampling kernel)."""
+    import torch
+    from cudaExtensions import sampling
+    torch.cuda.synchronize()
+    if kernel:
+        sampling.profile_sampler_begin(kernel)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.time()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.time() - t0
+    k_ms, k_n = sampling.profile_sampler_end() if kernel else (0.0, 0)
+    return max(e0.elapsed_time(e1), wall * 1e3) / steps, k_ms, k_n
+
+
+def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
+    """The plain-rate step and, when the trained rate network reaches the plain
+    gradient variance with fewer samples, the learned-rate step at that rate.
+    Returns (chosen step, its name, the comparison record)."""
     import torch
     from cudaExtensions import sampling
     from splatting_amd import learned as L
-
-    def timed(fn, n):
-        for _ in range(2):
-            fn()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        t0 = time.time()
-        e0.record()
-        for _ in range(n):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        return max(e0.elapsed_time(e1), (time.time() - t0) * 1e3) / n
-
-    target = L.make_target(sc, dev, rate, kcap)
-    fr = L.PoissonFrame(sc, dev, rate=rate, kcap=kcap, target=target)
     counter = [0]
 
     def plain_step():
         counter[0] += 1
         fr.step(1000 + counter[0])
 
-    plain_ms = timed(plain_step, steps)
-    seed_list = list(range(20_000, 20_000 + seeds))
-    v_plain, _ = L.gradient_variance(fr, seed_list)
-    out = {"estimator": "Horvitz-Thompson, linear splatting, L2 loss, raw adjoints",
-           "plain_rate": rate, "kcap": kcap, "plain_fps": 1000.0 / plain_ms, "plain_ms": plain_ms,
-           "plain_grad_variance": v_plain, "variance_seeds": seeds}
-    wpath = L.DEFAULT_WEIGHTS
-    if not os.path.exists(wpath):
-        out.update({"learned": None, "chosen": "plain (no trained rate network)",
-                    "equal_variance_fps": 1000.0 / plain_ms})
-        return out
-    net = L.RateNet.load(wpath, dev)
-    _, _, _, g = fr.step(99)
-    P, Nc, nb = fr.camera()
-    feats = sampling.tileFeatures(fr.W, fr.H, fr.f, P, nb, fr.S, 0.0, fr.prec, fr.dens, g)
-    _, (_, _, ids, _, _) = fr.image(98)
-    counts = L.tile_counts(ids)[0]
-    ts = net.tile_scale(feats[0], counts)[None].contiguous()
-    v_learned, _ = L.gradient_variance(fr, seed_list, tile_scale=ts, kcap=4 * kcap)
-    kappa = v_learned / v_plain
-    out.update({"learned_grad_variance_equal_budget": v_learned, "rate_factor_for_equal_variance": kappa,
-                "tile_scale_range": [float(ts.min().item()), float(ts.max().item())]})
-    if kappa < 1.0:
-        v_check, _ = L.gradient_variance(fr, seed_list, tile_scale=ts, rate=rate * kappa, kcap=4 * kcap)
+    rec = {"estimator": "Horvitz-Thompson (1/pi weights), linear splatting, L2 loss, raw adjoints",
+           "plain_rate": RATE, "kcap": KCAP}
+    if not use_learned or not os.path.exists(L.DEFAULT_WEIGHTS):
+        rec["chosen"] = "plain (%s)" % ("learned rates not compared" if not use_learned else "no trained rate network")
+        return plain_step, "plain", rec
+    net = L.RateNet.load(L.DEFAULT_WEIGHTS, dev)
+    cmp = L.compare_rates(fr, net, list(range(20_000, 20_000 + seeds)))
+    ts = cmp.pop("tile_scale")
+    rec.update(cmp)
+    kappa = cmp["kappa"]
+    if kappa >= 1.0:
+        rec["chosen"] = "plain (fallback: the learned rates need %.2fx the samples for the plain variance)" % kappa
+        return plain_step, "plain", rec
+    kcl = cmp["kcap_learned"]
 
-        def learned_step():
-            counter[0] += 1
-            _, _, _, gg = fr.step(5000 + counter[0], tile_scale=ts, rate=rate * kappa, kcap=4 * kcap)
-            Pq, Ncq, nbq = fr.camera()
-            f2 = sampling.tileFeatures(fr.W, fr.H, fr.f, Pq, nbq, fr.S, 0.0, fr.prec, fr.dens, gg)
-            net.forward(f2.reshape(-1, 32))
+    def learned_step():
+        counter[0] += 1
+        _, _, _, g = fr.step(5000 + counter[0], tile_scale=ts, rate=RATE * kappa, kcap=kcl)
+        P, Nc, nb = fr.camera()
+        f2 = sampling.tileFeatures(fr.W, fr.H, fr.f, P, nb, fr.S, 0.0, fr.prec, fr.dens, g)
+        ts.copy_(net.tile_scale(f2[0], L.tile_counts(fr.last_ids)[0])[None])
 
-        learned_ms = timed(learned_step, steps)
-        out.update({"learned_fps": 1000.0 / learned_ms, "learned_grad_variance_at_rate": v_check})
-        chosen = "learned" if learned_ms < plain_ms else "plain (fallback: learned slower)"
-        out.update({"chosen": chosen, "equal_variance_fps": 1000.0 / min(plain_ms, learned_ms),
-                    "speedup_vs_plain": plain_ms / min(plain_ms, learned_ms)})
-    else:
-        out.update({"learned_fps": None, "chosen": "plain (fallback: learned rates need %.2fx the samples)" % kappa,
-                    "equal_variance_fps": 1000.0 / plain_ms, "speedup_vs_plain": 1.0})
-    return out
+    v_check, _ = L.gradient_variance(fr, list(range(40_000, 40_000 + seeds)), tile_scale=ts, rate=RATE * kappa,
+                                     kcap=kcl)
+    plain_ms, _, _ = timed_steps(plain_step, probe_steps)
+    learned_ms, _, _ = timed_steps(learned_step, probe_steps)
+    rec.update({"learned_rate": RATE * kappa, "learned_grad_variance_at_rate": v_check,
+                "probe_ms": {"plain": plain_ms, "learned": learned_ms}})
+    if learned_ms < plain_ms:
+        rec["chosen"] = "learned"
+        return learned_step, "learned", rec
+    rec["chosen"] = "plain (fallback: the learned step is not faster at equal variance)"
+    return plain_step, "plain", rec
+
+
+def reference_path(dev, args, sc_n):
+    """The reference's method on the same scene: sequential Poisson sampling of
+    k = 40 splats per pixel, EWA weights, z-sort / compositing and interpolation
+    through the drop-in renderer, fwd+bwd; the sampler kernel timed with HIP events."""
+    import torch
+    sc, r, params, cam, target = build_problem(dev, args.splats)
+    P, Nn, S, A = params
+    amb, lamp_d, lamp_i, M = cam
+    timer = OpTimer()
+
+    def step():
+        for t in params:
+            t.grad = None
+        image, _, _ = r(None, None, P, Nn, S, A, amb, lamp_d, lamp_i, transformMatrix=M)
+        loss = ((image - target) ** 2).sum()
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    timer.active = True
+    ms, ks_ms, ks_n = timed_steps(step, args.steps, "sample")
+    timer.active = False
+    ops = timer.summary(args.steps)
+    Np, Pp, k = sc.n, sc.width * sc.height, 40
+    nbytes = algorithmic_bytes(Np, Pp, k)
+    ks_avg = ks_ms / max(ks_n, 1)
+    achieved = nbytes["sample"] / (ks_avg * 1e-3) / 1e9
+    return {"method": "sequential Poisson sampling, k=40 per pixel, z-sort/compositing (reference semantics)",
+            "fps": 1000.0 / ms, "ms_per_step": ms,
+            "roofline": {"bound": "hbm", "kernel": "k_sample", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["k_sample"],
+                         "kernel_ms": ks_avg, "kernel_launches": ks_n, "algorithmic_bytes": nbytes["sample"]},
+            "ops_ms": {k_: round(v, 4) for k_, v in sorted(ops.items())}}
 
 
 def main():
@@ -236,7 +300,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--splats", type=int, default=None, help="override the splat count (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-rates", action="store_true", help="skip the learned-vs-plain rates comparison")
+    ap.add_argument("--no-rates", action="store_true", help="time the plain rates without the learned comparison")
+    ap.add_argument("--no-reference", action="store_true", help="skip the reference-method frame")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of known size (1 GiB read + 1 GiB written per launch), "
                          "to scale HBM traffic counters")
@@ -256,68 +321,55 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 3}))
         return
+    from splatting_amd import scenes
+    from splatting_amd import learned as L
     print("bench: building workload", file=sys.stderr, flush=True)
-    sc, r, params, cam, target = build_problem(dev, args.splats)
-    P, Nn, S, A = params
-    amb, lamp_d, lamp_i, M = cam
-    timer = OpTimer()
-
-    def step():
-        for t in params:
-            t.grad = None
-        image, _, _ = r(None, None, P, Nn, S, A, amb, lamp_d, lamp_i, transformMatrix=M)
-        loss = ((image - target) ** 2).sum()
-        loss.backward()
-        return loss
-
+    sc = scenes.config(1, args.splats)
+    target = L.make_target(sc, dev, RATE, KCAP)
+    fr = L.PoissonFrame(sc, dev, rate=RATE, kcap=KCAP, target=target)
+    print("bench: plain vs learned rates", file=sys.stderr, flush=True)
+    step, chosen, rates = poisson_paths(dev, fr, use_learned=not args.no_rates)
     print("bench: warmup", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    timer.active = True
-    from cudaExtensions import sampling
-    sampling.profile_sampler_begin()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.time()
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.time() - t0
-    timer.active = False
-    ks_ms, ks_n = sampling.profile_sampler_end()
-    dev_ms = e0.elapsed_time(e1)
-    ms_per_step = max(dev_ms, wall * 1e3) / args.steps
+    print("bench: timed steps (%s rates)" % chosen, file=sys.stderr, flush=True)
+    ms_per_step, fe_ms, fe_n = timed_steps(step, args.steps, "fmm_eval")
+    _, po_ms, po_n = timed_steps(step, max(args.steps // 4, 1), "poisson")
     fps = 1000.0 / ms_per_step
-    ops = timer.summary(args.steps)
-    Np, Pp, k = sc.n, sc.width * sc.height, 40
-    nbytes = algorithmic_bytes(Np, Pp, k)
-    # dominant kernel: the sampler kernel (one launch per view), timed with HIP events
-    # around each launch on its own stream; its algorithmic bytes are the sample op's
-    ks_avg = ks_ms / max(ks_n, 1)
-    achieved = nbytes["sample"] / (ks_avg * 1e-3) / 1e9
-    frame_bytes = 68 * Pp * k + 332 * Np + 24 * Pp
+    # dominant kernel: the fixed-order density evaluation (two launches per frame).  Its
+    # algorithmic bytes per launch: every node record it can reach (32 B geometry + 16 B
+    # links + 4 B expansion offset, 2n - 1 nodes for n splats in view) and one 4 B density
+    # per pixel; the expansion coefficients it reads are not counted (their number is only
+    # known on the device), so this is a lower bound
+    n_vis = int((L.splat_tiles(fr) >= 0).sum().item())
+    fe_bytes = 52 * (2 * n_vis - 1) + 4 * sc.width * sc.height
+    fe_avg = fe_ms / max(fe_n, 1)
+    achieved = fe_bytes / (fe_avg * 1e-3) / 1e9
     line = {
         "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "1M surfels on 64 ellipsoids, one 512x512 view, k=40, precision 0.01, plain rates",
-                   "splats": Np, "width": sc.width, "height": sc.height, "samples_per_pixel": k,
-                   "parallelism": "none (1 GPU)"},
-        "roofline": {"bound": "hbm", "kernel": "k_sample", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES,
-                     "kernel_ms": ks_avg, "kernel_launches": ks_n, "algorithmic_bytes": nbytes["sample"],
-                     "frame_frac": frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "ops_ms": {k_: round(v, 4) for k_, v in sorted(ops.items())},
+        "config": {"workload": "1M surfels on 64 ellipsoids, one 512x512 view, Poisson-rate sampling at rate %g "
+                               "(1/pi weights, about 8 samples per pixel), %s rates at the plain rates' gradient "
+                               "variance" % (RATE, chosen),
+                   "splats": sc.n, "splats_in_view": n_vis, "width": sc.width, "height": sc.height,
+                   "rate": RATE, "kcap": KCAP, "rates": chosen, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "kernel": "k_fmm_eval", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["k_fmm_eval"],
+                     "kernel_ms": fe_avg, "kernel_launches": fe_n, "algorithmic_bytes": fe_bytes,
+                     "next_kernel": {"kernel": "k_poisson", "kernel_ms": po_ms / max(po_n, 1)}},
+        "rates": rates,
     }
-    if not args.no_rates:
-        print("bench: learned vs plain rates", file=sys.stderr, flush=True)
-        line["rates"] = rates_comparison(dev, sc, args.steps)
+    if not args.no_reference:
+        print("bench: reference method", file=sys.stderr, flush=True)
+        del fr
+        line["reference_path"] = reference_path(dev, args, sc.n)
+        if not args.no_cpu_baseline:
+            print("bench: cpu baseline of the reference method", file=sys.stderr, flush=True)
+            line["reference_path"]["cpu_baseline"] = cpu_baseline(sc)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(sc)
+        line["cpu_baseline"] = cpu_baseline_poisson(sc)
     print(json.dumps(line))
 
 

@@ -122,11 +122,14 @@ int ude_adam_step(int n, float* params, const float* grad, float* m, float* v, f
                   float eps, int step, ude_stream_t stream);
 
 /* Profiling aid (no counterpart in the reference): between begin and end,
- * every launch of the sampler kernel (the per-pixel FGT descent + sequential
- * Poisson pass, one per view) is bracketed by HIP events on its stream.  end
- * waits for the recorded events and returns the summed kernel time and the
- * number of launches.  Host-thread-unsafe; meant for benchmarks. */
-int ude_profile_sampler_begin(void);
+ * every launch of one sampling kernel is bracketed by HIP events on its
+ * stream -- kernel 0: the per-pixel FGT descent + sequential Poisson pass of
+ * ude_sample (one launch per view); 1: the fixed-order density evaluation
+ * (two launches per view in ude_sample_poisson, one in ude_fmm_weights);
+ * 2: the Poisson-rate thinning pass of ude_sample_poisson.  end waits for the
+ * recorded events and returns the summed kernel time and the number of
+ * launches.  Host-thread-unsafe; meant for benchmarks. */
+int ude_profile_sampler_begin(int kernel);
 int ude_profile_sampler_end(double* totalMs, int* launches);
 
 /* sampling.covariance -- replaces cudaCovariance (sampling.cpp:96-134,

@@ -78,3 +78,26 @@ def test_tile_features(dev, cfg, sds):
     cols = [c for c in range(32) if c not in (3, 4)]
     assert err[..., cols].max() < 2e-4, np.unravel_index(np.argmax(err[..., cols]), err[..., cols].shape)
     assert err[..., 3:5].max() < 5e-2
+
+
+def test_learned_rates_against_plain(dev, record_property):
+    """What the trained rate network gains over plain rates on a scene it was not
+    trained on: gradient variance at the same expected sample count (kappa =
+    V_learned / V_plain; below 1 is a gain), measured over 32 seeds with the
+    plain measurement repeated on other seeds for the noise of the measurement.
+    The multipliers must keep the sample budget; their floor (0.75) bounds the
+    loss where the network errs, so kappa stays well below 1 / 0.75 plus noise."""
+    from splatting_amd import learned as L, scenes
+    sc = scenes.training_scene(100, n_override=200_000)
+    target = L.make_target(sc, dev, 10.0, 40, seeds=range(10_000, 10_004))
+    fr = L.PoissonFrame(sc, dev, rate=10.0, kcap=40, target=target)
+    net = L.RateNet.load(L.DEFAULT_WEIGHTS, dev)
+    r = L.compare_rates(fr, net, list(range(500, 532)))
+    ts = r.pop("tile_scale")
+    for k, v in r.items():
+        record_property(k, v)
+    print("learned vs plain:", {k: v for k, v in r.items()})
+    assert float(ts.min()) >= 0.75 - 1e-6 and float(ts.max()) <= 4.0 + 1e-6
+    assert abs(r["samples_per_pixel_learned"] / r["samples_per_pixel_plain"] - 1.0) < 0.05, r
+    assert not r["capacity_reached"], r
+    assert r["kappa"] < 2.0, r

@@ -1873,16 +1873,19 @@ int trie_entries(int k) { return 192 + 32 * num_launches(k); }
 // Optional timing of the sampler kernel: while enabled, every k_sample launch
 // is bracketed by a pair of HIP events recorded on its stream (single host
 // thread; events are pooled and reused across profiling windows).
+// HIP events bracketing the launches of one kernel (0 k_sample, 1 k_fmm_eval,
+// 2 k_poisson) while a benchmark asks for them
 struct SamplerTiming {
     bool on = false;
+    int kernel = 0;
     int used = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
 };
 SamplerTiming g_timing;
 
-bool timing_pair(hipEvent_t* a, hipEvent_t* b)
+bool timing_pair(int kernel, hipEvent_t* a, hipEvent_t* b)
 {
-    if (!g_timing.on) return false;
+    if (!g_timing.on || g_timing.kernel != kernel) return false;
     if (g_timing.used == (int)g_timing.pool.size()) {
         hipEvent_t x, y;
         if (hipEventCreate(&x) != hipSuccess) return false;
@@ -1902,7 +1905,7 @@ void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32
     unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
     size_t lds = (size_t)k * 64 * sizeof(unsigned long long);
     hipEvent_t e0, e1;
-    bool timed = timing_pair(&e0, &e1);
+    bool timed = timing_pair(0, &e0, &e1);
     if (timed) (void)hipEventRecord(e0, st);
     hipLaunchKernelGGL((k_sample<KMAX>), dim3(tiles), dim3(64), lds, st, W, H, k, num_launches(k), seedLo, seedHi,
                        view, (const ViewState*)w.vs, (const NodeGeo*)w.geo, (const NodeLnk*)w.lnk,
@@ -1918,7 +1921,11 @@ void fixed_density(int N, int W, int H, int b, const TreeWs& w, float* out, hipS
     k_fixed_select<<<grid1(N, 256), 256, 0, st>>>(w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixList[b]);
     k_fixed_moments<<<grid1(N, kFixedNodesPerBlock), 256, 0, st>>>(w.vs, b, w.geo, w.aux, w.fixList[b], w.fixOff[b],
                                                                     w.fixCoef[b]);
+    hipEvent_t e0, e1;
+    bool timed = timing_pair(1, &e0, &e1);
+    if (timed) (void)hipEventRecord(e0, st);
     k_fmm_eval<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixCoef[b], out);
+    if (timed) (void)hipEventRecord(e1, st);
 }
 
 } // namespace
@@ -2033,10 +2040,14 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         fixed_density(N, W, H, 0, w, d0, st);
         fixed_density(N, W, H, 1, w, d1, st);
         const float* ts = tileScale ? tileScale + (long)tsW * tsH * b : nullptr;
+        hipEvent_t e0, e1;
+        bool timed = timing_pair(2, &e0, &e1);
+        if (timed) (void)hipEventRecord(e0, st);
         k_poisson<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, kcap, (float)rate, ts, tsH, sLo, sHi, (uint32_t)b, w.vs,
                                                    w.geo, w.lnk, w.aux, w.idsA, d0, d1,
                                                    outIds + perView * kcap * b, outIpi + perView * kcap * b,
                                                    errFlag);
+        if (timed) (void)hipEventRecord(e1, st);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
@@ -2069,8 +2080,10 @@ int ude_tile_features(int bn, int N, int W, int H, double f, double smoothing, d
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
 }
 
-int ude_profile_sampler_begin(void)
+int ude_profile_sampler_begin(int kernel)
 {
+    if (kernel < 0 || kernel > 2) return UDE_ERR_ARG;
+    g_timing.kernel = kernel;
     g_timing.on = true;
     g_timing.used = 0;
     return UDE_OK;

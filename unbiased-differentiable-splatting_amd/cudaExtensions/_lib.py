@@ -37,7 +37,7 @@ _SIGS = {
     "ude_mlp_forward": (c_int, [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
     "ude_mlp_loss_grad": (c_int, [c_int] + [c_ptr] * 7 + [c_size, c_ptr]),
     "ude_adam_step": (c_int, [c_int] + [c_ptr] * 4 + [ctypes.c_float] * 4 + [c_int, c_ptr]),
-    "ude_profile_sampler_begin": (c_int, []),
+    "ude_profile_sampler_begin": (c_int, [c_int]),
     "ude_profile_sampler_end": (c_int, [c_ptr, c_ptr]),
     "ude_covariance": (c_int, [c_int] * 4 + [c_dbl] * 2 + [c_ptr] * 3 + [c_int, c_ptr, c_ptr]),
     "ude_splat_workspace_bytes": (c_size, [c_int] * 2),

@@ -179,13 +179,19 @@ def check_errors():
             raise RuntimeError("sampling.sample: descent exceeded the 63-level trace bound")
 
 
-def profile_sampler_begin():
-    """Start bracketing every sampler-kernel launch with HIP events (benchmarks)."""
-    _lib.check(_lib.lib().ude_profile_sampler_begin(), "profile_sampler_begin")
+_PROFILED = {"sample": 0, "fmm_eval": 1, "poisson": 2}
+
+
+def profile_sampler_begin(kernel="sample"):
+    """Start bracketing every launch of one sampling kernel with HIP events
+    (benchmarks): "sample" (the FGT descent + sequential Poisson pass of
+    ``sample``), "fmm_eval" (fixed-order density evaluation) or "poisson" (the
+    thinning pass of ``samplePoisson``)."""
+    _lib.check(_lib.lib().ude_profile_sampler_begin(_PROFILED[kernel]), "profile_sampler_begin")
 
 
 def profile_sampler_end():
-    """Stop; returns (summed sampler-kernel milliseconds, number of launches)."""
+    """Stop; returns (summed kernel milliseconds, number of launches)."""
     import ctypes
     ms = ctypes.c_double(0.0)
     n = ctypes.c_int(0)

@@ -88,7 +88,7 @@ class RateNet:
                 hist.append((s, float(loss.item())))
         return hist
 
-    def tile_scale(self, feats, counts, lo=1.0 / 32, hi=4.0):
+    def tile_scale(self, feats, counts, lo=0.75, hi=4.0):
         """Rate multipliers for tiles with features (..., 32) and plain-rate sample
         counts (...): exp(prediction), rescaled so that the expected sample count
         sum(scale * counts) is unchanged, and kept within [lo, hi]."""
@@ -154,6 +154,7 @@ class PoissonFrame:
         w = splatting.forward(self.W, self.H, self.f, ids, P, Nc, self.S, 0.0)
         wt = (w * ipi).contiguous()
         img = interpolation.forward(ids, wt, self.A)
+        self.last_ids = ids
         return img, (P, Nc, ids, ipi, wt)
 
     def step(self, seed, tile_scale=None, rate=None, kcap=None):
@@ -270,6 +271,51 @@ def gradient_variance(frame, seeds, tile_scale=None, rate=None, kcap=None):
     mean = s1 / S
     var = (s2 - S * mean * mean) / (S - 1)
     return float(var.clamp(min=0).mean().item()), mean
+
+
+def rate_features(frame, net, seed):
+    """One plain frame: tile features of its densities and image gradient, and the
+    learned tile multipliers for the plain per-tile sample counts."""
+    from cudaExtensions import sampling
+    _, _, _, g = frame.step(seed)
+    P, Nc, nb = frame.camera()
+    feats = sampling.tileFeatures(frame.W, frame.H, frame.f, P, nb, frame.S, 0.0, frame.prec, frame.dens, g)
+    counts = tile_counts(frame.last_ids)[0]
+    return feats, counts, net.tile_scale(feats[0], counts)[None].contiguous()
+
+
+def max_count(ids):
+    return int((ids >= 0).sum(-1).max().item())
+
+
+def compare_rates(frame, net, seeds, kcap_learned=None):
+    """Plain against learned per-tile rates on one frame at the same expected
+    sample count.  Gradient variances (mean per-parameter variance of the raw
+    adjoints) are measured over disjoint seed sets; a second plain measurement on
+    other seeds gives the noise of the measurement itself.  Variance behaves like
+    1 / rate, so the learned rates reach the plain variance at rate * kappa with
+    kappa = V_learned / V_plain.  Returns a dict of the measurements and the tile
+    multipliers."""
+    S = len(seeds)
+    base = seeds[0]
+    feats, counts, ts = rate_features(frame, net, base - 1)
+    kcl = kcap_learned or frame.kcap * 2
+    v_plain, _ = gradient_variance(frame, [base + s for s in range(S)])
+    v_rep, _ = gradient_variance(frame, [base + S + s for s in range(S)])
+    v_learned, _ = gradient_variance(frame, [base + 2 * S + s for s in range(S)], tile_scale=ts, kcap=kcl)
+    frame.image(base - 2)
+    n_plain = float((frame.last_ids >= 0).float().sum(-1).mean().item())
+    over_plain = max_count(frame.last_ids) >= frame.kcap
+    frame.image(base - 2, tile_scale=ts, kcap=kcl)
+    n_learned = float((frame.last_ids >= 0).float().sum(-1).mean().item())
+    over_learned = max_count(frame.last_ids) >= kcl
+    return {"plain_grad_variance": v_plain, "plain_grad_variance_repeat": v_rep,
+            "measurement_noise": abs(v_rep / v_plain - 1.0),
+            "learned_grad_variance_equal_budget": v_learned, "kappa": v_learned / v_plain,
+            "samples_per_pixel_plain": n_plain, "samples_per_pixel_learned": n_learned,
+            "kcap_plain": frame.kcap, "kcap_learned": kcl, "capacity_reached": over_plain or over_learned,
+            "tile_scale_range": [float(ts.min().item()), float(ts.max().item())], "variance_seeds": S,
+            "tile_scale": ts}
 
 
 def make_target(scene, device, rate, kcap, precision=0.01, seeds=range(10_000, 10_016)):
