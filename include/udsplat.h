@@ -59,8 +59,10 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
                     const float* pts, const float* nrm, const float* sd, int sdBatch,
                     float* out, void* ws, size_t wsBytes, ude_stream_t stream);
 
-/* Workspace for ude_sample_poisson. */
-size_t ude_poisson_workspace_bytes(int N, int W, int H);
+/* Workspace for ude_sample_poisson over bn views with every view's tree
+ * resident (bn per-view blocks); a workspace of one block (bn = 1) also
+ * works, view after view. */
+size_t ude_poisson_workspace_bytes(int bn, int N, int W, int H);
 
 /* Poisson-rate sampling with inclusion probabilities (no counterpart in the
  * reference, whose plain-Poisson kernel poissonRadixTree.cuh:70-200 is dead
@@ -76,11 +78,25 @@ size_t ude_poisson_workspace_bytes(int N, int W, int H);
  * splat ids (-1 padded) and 1 / pi_i (0 padded), so sum_s w_s a_s / pi_s is a
  * Horvitz-Thompson estimate of sum_i w_i a_i.  densOut (optional, (bn, 2, W, H))
  * receives D0 and D1.  A pixel with more than kcap inclusions sets bit 4 of
- * *errFlag. */
+ * *errFlag.  seedDev (optional, device memory): the Philox key is read there
+ * at run time instead of from seed, so a captured HIP graph draws fresh
+ * samples when the caller advances it between replays.  With a workspace of
+ * ude_poisson_workspace_bytes(bn, ...) bytes all views' trees are built
+ * first and each per-pixel pass (both densities, the thinning) is one launch
+ * over the whole batch. */
 int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double smoothing, double precision,
                        double rate, const float* tileScale, const float* pts, const float* nrm, const float* sd,
-                       int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, float* densOut, int* errFlag,
-                       void* ws, size_t wsBytes, ude_stream_t stream);
+                       int sdBatch, uint64_t seed, const uint64_t* seedDev, int32_t* outIds, float* outIpi,
+                       float* densOut, int* errFlag, void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* Launch shape of the per-pixel passes (no counterpart in the reference):
+ * kernel 0 the fixed-order density evaluation, 1 the Poisson-rate thinning.
+ * One wavefront per tileW x tileH pixel tile (tileW tileH = 64, powers of
+ * two), wavesPerBlock in {1, 2, 4, 8} per workgroup, xcdOrder 1 to give each
+ * of the 8 XCDs a contiguous band of tiles.  Results do not depend on it.
+ * Process-wide; default 8 x 8, 4, 0.  get writes the four values to cfg. */
+int ude_set_tile_config(int kernel, int tileW, int tileH, int wavesPerBlock, int xcdOrder);
+int ude_get_tile_config(int kernel, int* cfg);
 
 /* Features of 16x16 pixel tiles for the rate network (no counterpart in the
  * reference).  dens: (bn, 2, W, H) densities D0, D1 as returned by

@@ -1399,7 +1399,49 @@ __device__ __forceinline__ float band_point(const Band& B, float qx, float qy, f
     return v;
 }
 
-// One lane per pixel, 8x8 pixel tile per wavefront, 4 tiles per workgroup.
+// ---------------------------------------------------------------------------
+// Launch shape of the per-pixel passes (k_fmm_eval, k_poisson): one wavefront
+// per pixel tile of tw x 2^lth pixels (tw 2^lth = 64), wpb wavefronts per
+// workgroup, and optionally an XCD-grouped block order: workgroups are handed
+// to the 8 XCDs round-robin, so block b runs on XCD b mod 8; with xcd set,
+// block b takes the tile range of position b / 8 within XCD b mod 8's
+// contiguous share, so each XCD's L2 serves one band of the image (and of the
+// leaves under it).  A pixel's result does not depend on the shape.
+// The views of a batch are blockIdx.y; their workspaces lie vstride bytes apart.
+// ---------------------------------------------------------------------------
+struct TileCfg {
+    int tw, lth, wpb, xcd;
+};
+constexpr int kMaxWavesPerBlock = 8;
+
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// pixel (i, j) of this lane; false for a wavefront past the last tile (wave-uniform)
+__device__ __forceinline__ bool tile_lane(const TileCfg& c, int W, int H, int lane, int& i, int& j)
+{
+    const int blk = c.xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int tile = blk * c.wpb + (int)(threadIdx.x >> 6);
+    const int th = 1 << c.lth;
+    const int tilesY = (H + th - 1) >> c.lth;
+    const int ntiles = ((W + c.tw - 1) / c.tw) * tilesY;
+    if (tile >= ntiles) return false;
+    const int ti = tile / tilesY, tj = tile - ti * tilesY;
+    i = ti * c.tw + (lane >> c.lth);
+    j = (tj << c.lth) + (lane & (th - 1));
+    return true;
+}
+
+template <class T>
+__device__ __forceinline__ T* view_ptr(T* p, size_t stride)
+{
+    return (T*)((char*)p + (size_t)blockIdx.y * stride);
+}
+
+// One lane per pixel, one pixel tile per wavefront (TileCfg).
 // The reference walks each pixel with an explicit stack: pop a node; evaluate
 // it if it is a leaf or carries an expansion, else push its left child and then
 // its right child, each only if its box meets the query disc
@@ -1412,21 +1454,24 @@ __device__ __forceinline__ float band_point(const Band& B, float qx, float qy, f
 // addresses.  Karras trees on 64-bit keys are at most 64 levels deep, so the
 // stack never holds more than 65 entries.
 constexpr int kFmmStack = 96;
-__global__ __launch_bounds__(256) void k_fmm_eval(int W, int H, const ViewState* __restrict__ vs, int b,
-                                                  const NodeGeo* geo, const NodeLnk* lnk,
-                                                  const int* __restrict__ fixOff, const float* coef,
-                                                  float* __restrict__ out)
+__global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg, int W, int H,
+                                                                    const ViewState* __restrict__ vs, size_t vstride,
+                                                                    int b, const NodeGeo* geo, const NodeLnk* lnk,
+                                                                    const int* __restrict__ fixOff, const float* coef,
+                                                                    float* __restrict__ out, size_t outStride)
 {
-    __shared__ int stkN[4][kFmmStack];
-    __shared__ unsigned long long stkM[4][kFmmStack];
+    __shared__ int stkN[kMaxWavesPerBlock][kFmmStack];
+    __shared__ unsigned long long stkM[kMaxWavesPerBlock][kFmmStack];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    const int tilesY = (H + 7) >> 3;
-    const int ntiles = ((W + 7) >> 3) * tilesY;
-    const int wv = blockIdx.x * 4 + wid;
-    if (wv >= ntiles) return;                                // wave-uniform
-    const int ti = wv / tilesY, tj = wv - (wv / tilesY) * tilesY;
-    const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
+    int i, j;
+    if (!tile_lane(cfg, W, H, lane, i, j)) return;           // wave-uniform
+    vs = view_ptr(vs, vstride);
+    geo = view_ptr(geo, vstride);
+    lnk = view_ptr(lnk, vstride);
+    fixOff = view_ptr(fixOff, vstride);
+    coef = view_ptr(coef, vstride);
+    out = view_ptr(out, outStride);
     const bool valid = i < W && j < H;
     const int n = vs->nc;
     float w = 0.0f;
@@ -1560,7 +1605,7 @@ __device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float
     return m;
 }
 
-// One lane per pixel (8x8 pixel tile per wavefront, 4 per workgroup).  The walk
+// One lane per pixel (one pixel tile per wavefront, TileCfg).  The walk
 // visits nodes left to right; a node's bound U (the mixture at the point of its
 // box closest to q) is >= pi_i of each of its leaves.  U = 0: skip the
 // subtree.  A leaf, or a node with count U <= 1, is thinned: candidate leaves at
@@ -1569,22 +1614,40 @@ __device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float
 // probability pi_i, independently.  Uniforms are Philox4x32-10 words at counter
 // (draw >> 2, 0x80000000, pixel, view), word draw & 3, with a per-pixel draw
 // number.
-__global__ __launch_bounds__(256) void k_poisson(int W, int H, int kcap, float rate, const float* __restrict__ tscale,
-                                                 int tsH, uint32_t seedLo, uint32_t seedHi, uint32_t view,
-                                                 const ViewState* __restrict__ vs, const NodeGeo* geo,
-                                                 const NodeLnk* lnk, const NodeAux* aux,
-                                                 const int32_t* __restrict__ leafId, const float* __restrict__ dens0,
-                                                 const float* __restrict__ dens1, int32_t* __restrict__ outIds,
-                                                 float* __restrict__ outIpi, int* err)
+__global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg, int W, int H, int kcap, float rate,
+                                                                   const float* __restrict__ tscale, int tsH,
+                                                                   uint32_t seedLo, uint32_t seedHi,
+                                                                   const uint64_t* __restrict__ seedDev,
+                                                                   uint32_t view0, const ViewState* __restrict__ vs,
+                                                                   size_t vstride, const NodeGeo* geo,
+                                                                   const NodeLnk* lnk, const NodeAux* aux,
+                                                                   const int32_t* __restrict__ leafId,
+                                                                   const float* __restrict__ dens0,
+                                                                   const float* __restrict__ dens1, size_t densStride,
+                                                                   int32_t* __restrict__ outIds,
+                                                                   float* __restrict__ outIpi, int* err)
 {
     const int lane = threadIdx.x & 63;
-    const int tilesY = (H + 7) >> 3;
-    const int ntiles = ((W + 7) >> 3) * tilesY;
-    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wv >= ntiles) return;
-    const int ti = wv / tilesY, tj = wv - (wv / tilesY) * tilesY;
-    const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
+    int i, j;
+    if (!tile_lane(cfg, W, H, lane, i, j)) return;           // wave-uniform
     if (i >= W || j >= H) return;
+    const uint32_t view = view0 + blockIdx.y;
+    if (seedDev) {
+        const uint64_t sd = *seedDev;
+        seedLo = (uint32_t)sd;
+        seedHi = (uint32_t)(sd >> 32);
+    }
+    vs = view_ptr(vs, vstride);
+    geo = view_ptr(geo, vstride);
+    lnk = view_ptr(lnk, vstride);
+    aux = view_ptr(aux, vstride);
+    leafId = view_ptr(leafId, vstride);
+    dens0 = view_ptr(dens0, densStride);
+    dens1 = view_ptr(dens1, densStride);
+    const long perView = (long)W * H;
+    outIds += (long)blockIdx.y * perView * kcap;
+    outIpi += (long)blockIdx.y * perView * kcap;
+    if (tscale) tscale += (long)blockIdx.y * ((W + 15) / 16) * tsH;
     const long pix = (long)j + (long)H * i;
     int32_t* oid = outIds + pix * kcap;
     float* oip = outIpi + pix * kcap;
@@ -1914,17 +1977,35 @@ void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32
     if (timed) (void)hipEventRecord(e1, st);
 }
 
-// fixed-order density of band b over one view into out (W x H)
-void fixed_density(int N, int W, int H, int b, const TreeWs& w, float* out, hipStream_t st)
+// launch shapes of k_fmm_eval (0) and k_poisson (1); set by ude_set_tile_config
+TileCfg g_tile[2] = {{8, 3, 4, 0}, {8, 3, 4, 0}};
+
+dim3 tile_grid(const TileCfg& c, int W, int H, int views)
 {
-    unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
+    const int th = 1 << c.lth;
+    const long ntiles = (long)((W + c.tw - 1) / c.tw) * ((H + th - 1) / th);
+    return dim3((unsigned)((ntiles + c.wpb - 1) / c.wpb), (unsigned)views);
+}
+
+// degree-6 expansions of band b on one view's tree
+void fixed_expansions(int N, int b, const TreeWs& w, hipStream_t st)
+{
     k_fixed_select<<<grid1(N, 256), 256, 0, st>>>(w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixList[b]);
     k_fixed_moments<<<grid1(N, kFixedNodesPerBlock), 256, 0, st>>>(w.vs, b, w.geo, w.aux, w.fixList[b], w.fixOff[b],
                                                                     w.fixCoef[b]);
+}
+
+// band-b densities of `views` views in one launch: w is the first view's
+// workspace, the others follow vstride bytes apart; view v writes out + v outStride bytes
+void fmm_eval(int W, int H, int views, int b, const TreeWs& w, size_t vstride, float* out, size_t outStride,
+              hipStream_t st)
+{
+    const TileCfg c = g_tile[0];
     hipEvent_t e0, e1;
     bool timed = timing_pair(1, &e0, &e1);
     if (timed) (void)hipEventRecord(e0, st);
-    k_fmm_eval<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixCoef[b], out);
+    k_fmm_eval<<<tile_grid(c, W, H, views), 64 * c.wpb, 0, st>>>(c, W, H, w.vs, vstride, b, w.geo, w.lnk,
+                                                                  w.fixOff[b], w.fixCoef[b], out, outStride);
     if (timed) (void)hipEventRecord(e1, st);
 }
 
@@ -1996,21 +2077,22 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
         const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
         if (rc != UDE_OK) return rc;
-        fixed_density(N, W, H, 0, w, out + perView * b, st);
+        fixed_expansions(N, 0, w, st);
+        fmm_eval(W, H, 1, 0, w, 0, out + perView * b, 0, st);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
 }
 
-size_t ude_poisson_workspace_bytes(int N, int W, int H)
+size_t ude_poisson_workspace_bytes(int bn, int N, int W, int H)
 {
-    return carve(nullptr, N, W, H, 0, kModePoisson, nullptr) + 256;
+    return carve(nullptr, N, W, H, 0, kModePoisson, nullptr) * (size_t)(bn > 1 ? bn : 1) + 256;
 }
 
 int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double smoothing, double prec,
                        double rate, const float* tileScale, const float* pts, const float* nrm, const float* sd,
-                       int sdBatch, uint64_t seed, int32_t* outIds, float* outIpi, float* densOut, int* errFlag,
-                       void* ws, size_t wsBytes, hipStream_t st)
+                       int sdBatch, uint64_t seed, const uint64_t* seedDev, int32_t* outIds, float* outIpi,
+                       float* densOut, int* errFlag, void* ws, size_t wsBytes, hipStream_t st)
 {
     if (bn <= 0 || W <= 0 || H <= 0 || kcap <= 0 || N < 0 || !(rate >= 0.0)) return UDE_ERR_ARG;
     long perView = (long)W * H;
@@ -2021,36 +2103,68 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
     }
-    if (wsBytes < ude_poisson_workspace_bytes(N, W, H)) return UDE_ERR_WORKSPACE;
-    TreeWs w;
-    carve(ws, N, W, H, 0, kModePoisson, &w);
-    unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
+    // all views' trees resident at once when the workspace holds them: every
+    // per-pixel pass is then one launch over the whole batch; otherwise view by view
+    const size_t per = carve(nullptr, N, W, H, 0, kModePoisson, nullptr);
+    const bool batched = bn > 1 && wsBytes >= per * (size_t)bn + 256;
+    if (!batched && wsBytes < per + 256) return UDE_ERR_WORKSPACE;
+    const int span = batched ? bn : 1;
     const int tsW = (W + 15) / 16, tsH = (H + 15) / 16;
-    uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
-    for (int b = 0; b < bn; b++) {
-        const float* pb = pts + 3L * N * b;
-        const float* nb = nrm + 3L * N * b;
-        const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
-        int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
-        if (rc != UDE_OK) return rc;
-        k_wide_hist<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.idsA, w.hraw);
-        k_wide_band<<<1, 64, 0, st>>>(w.vs);
-        float* d0 = densOut ? densOut + perView * 2 * b : w.dens[0];
+    const uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
+    const TileCfg pc = g_tile[1];
+    for (int g = 0; g < bn; g += span) {
+        for (int v = 0; v < span; v++) {
+            const int b = g + v;
+            TreeWs wv;
+            carve((char*)ws + per * v, N, W, H, 0, kModePoisson, &wv);
+            const float* pb = pts + 3L * N * b;
+            const float* nb = nrm + 3L * N * b;
+            const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
+            int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, wv, st);
+            if (rc != UDE_OK) return rc;
+            k_wide_hist<<<grid1(N, 256), 256, 0, st>>>(wv.vs, wv.idsA, wv.hraw);
+            k_wide_band<<<1, 64, 0, st>>>(wv.vs);
+            fixed_expansions(N, 0, wv, st);
+            fixed_expansions(N, 1, wv, st);
+        }
+        TreeWs w;
+        carve(ws, N, W, H, 0, kModePoisson, &w);
+        float* d0 = densOut ? densOut + perView * 2 * g : w.dens[0];
         float* d1 = densOut ? d0 + perView : w.dens[1];
-        fixed_density(N, W, H, 0, w, d0, st);
-        fixed_density(N, W, H, 1, w, d1, st);
-        const float* ts = tileScale ? tileScale + (long)tsW * tsH * b : nullptr;
+        const size_t dstride = densOut ? sizeof(float) * 2 * perView : per;
+        fmm_eval(W, H, span, 0, w, per, d0, dstride, st);
+        fmm_eval(W, H, span, 1, w, per, d1, dstride, st);
+        const float* ts = tileScale ? tileScale + (long)tsW * tsH * g : nullptr;
         hipEvent_t e0, e1;
         bool timed = timing_pair(2, &e0, &e1);
         if (timed) (void)hipEventRecord(e0, st);
-        k_poisson<<<(tiles + 3) / 4, 256, 0, st>>>(W, H, kcap, (float)rate, ts, tsH, sLo, sHi, (uint32_t)b, w.vs,
-                                                   w.geo, w.lnk, w.aux, w.idsA, d0, d1,
-                                                   outIds + perView * kcap * b, outIpi + perView * kcap * b,
-                                                   errFlag);
+        k_poisson<<<tile_grid(pc, W, H, span), 64 * pc.wpb, 0, st>>>(
+            pc, W, H, kcap, (float)rate, ts, tsH, sLo, sHi, seedDev, (uint32_t)g, w.vs, per, w.geo, w.lnk, w.aux,
+            w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag);
         if (timed) (void)hipEventRecord(e1, st);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+int ude_set_tile_config(int kernel, int tileW, int tileH, int wavesPerBlock, int xcdOrder)
+{
+    if (kernel < 0 || kernel > 1) return UDE_ERR_ARG;
+    if (tileW <= 0 || tileH <= 0 || tileW * tileH != 64 || (tileW & (tileW - 1))) return UDE_ERR_ARG;
+    if (wavesPerBlock != 1 && wavesPerBlock != 2 && wavesPerBlock != 4 && wavesPerBlock != 8) return UDE_ERR_ARG;
+    if (xcdOrder != 0 && xcdOrder != 1) return UDE_ERR_ARG;
+    int lth = 0;
+    while ((1 << lth) < tileH) lth++;
+    g_tile[kernel] = TileCfg{tileW, lth, wavesPerBlock, xcdOrder};
+    return UDE_OK;
+}
+
+int ude_get_tile_config(int kernel, int* cfg)
+{
+    if (kernel < 0 || kernel > 1 || !cfg) return UDE_ERR_ARG;
+    const TileCfg c = g_tile[kernel];
+    cfg[0] = c.tw; cfg[1] = 1 << c.lth; cfg[2] = c.wpb; cfg[3] = c.xcd;
+    return UDE_OK;
 }
 
 size_t ude_tile_features_workspace_bytes(int W, int H)

@@ -27,8 +27,8 @@ _SIGS = {
     "ude_sample": (c_int, [c_int] * 5 + [c_dbl] * 3 + [c_ptr] * 3 + [c_int, ctypes.c_uint64, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "ude_fmm_weights_workspace_bytes": (c_size, [c_int] * 3),
     "ude_fmm_weights": (c_int, [c_int] * 4 + [c_dbl] * 3 + [c_ptr] * 3 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
-    "ude_poisson_workspace_bytes": (c_size, [c_int] * 3),
-    "ude_sample_poisson": (c_int, [c_int] * 5 + [c_dbl] * 4 + [c_ptr] * 4 + [c_int, ctypes.c_uint64] + [c_ptr] * 5 +
+    "ude_poisson_workspace_bytes": (c_size, [c_int] * 4),
+    "ude_sample_poisson": (c_int, [c_int] * 5 + [c_dbl] * 4 + [c_ptr] * 4 + [c_int, ctypes.c_uint64] + [c_ptr] * 6 +
                            [c_size, c_ptr]),
     "ude_tile_features_workspace_bytes": (c_size, [c_int] * 2),
     "ude_tile_features": (c_int, [c_int] * 4 + [c_dbl] * 3 + [c_ptr] * 3 + [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
@@ -37,6 +37,8 @@ _SIGS = {
     "ude_mlp_forward": (c_int, [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
     "ude_mlp_loss_grad": (c_int, [c_int] + [c_ptr] * 7 + [c_size, c_ptr]),
     "ude_adam_step": (c_int, [c_int] + [c_ptr] * 4 + [ctypes.c_float] * 4 + [c_int, c_ptr]),
+    "ude_set_tile_config": (c_int, [c_int] * 5),
+    "ude_get_tile_config": (c_int, [c_int, c_ptr]),
     "ude_profile_sampler_begin": (c_int, [c_int]),
     "ude_profile_sampler_end": (c_int, [c_ptr, c_ptr]),
     "ude_covariance": (c_int, [c_int] * 4 + [c_dbl] * 2 + [c_ptr] * 3 + [c_int, c_ptr, c_ptr]),

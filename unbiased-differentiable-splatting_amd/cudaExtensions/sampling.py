@@ -87,7 +87,8 @@ def approximateFMMWeights(width, height, focallength, points, normals, colors, s
 
 
 def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, numSamples, smoothing,
-                  precision=0.001, seed=None, rate=None, tileScale=None, densities=None):
+                  precision=0.001, seed=None, rate=None, tileScale=None, densities=None, seedTensor=None,
+                  batched=True):
     """Poisson-rate sampling with inclusion probabilities (no reference counterpart).
 
     Every visible splat i is included at pixel q independently with the
@@ -103,7 +104,12 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
     ``tileScale`` (bn, ceil(W/16), ceil(H/16)) set the sampling budget; a pixel
     with more than numSamples inclusions sets an error flag checked by
     check_errors().  ``densities`` (optional float32 (bn, 2, W, H)) receives D0 and
-    D1 (the input of tileFeatures)."""
+    D1 (the input of tileFeatures).  ``seedTensor`` (optional int64 (1,) on the
+    device) holds the Philox key instead of ``seed``; it is read when the kernel
+    runs, so a captured graph draws fresh samples after the caller advances it.
+    ``batched``: keep every view's tree resident and run each per-pixel pass as
+    one launch over all views (bn times the workspace of one view), else view
+    by view."""
     global last_error
     dev = _lib.require_gpu(points, normals, stdDevs)
     points = _lib.f32(points, "points")
@@ -122,8 +128,12 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
     ipi = torch.empty((bn, W, H, k), dtype=torch.float32, device=dev)
     err = torch.zeros(1, dtype=torch.int32, device=dev)
     L = _lib.lib()
-    ws = _lib.workspace(L.ude_poisson_workspace_bytes(N, W, H), dev)
-    s = _next_seed() if seed is None else int(seed)
+    ws = _lib.workspace(L.ude_poisson_workspace_bytes(bn if batched else 1, N, W, H), dev)
+    s = _next_seed() if seed is None and seedTensor is None else int(seed or 0)
+    if seedTensor is not None:
+        _lib.require_gpu(seedTensor)
+        if seedTensor.dtype != torch.int64 or seedTensor.numel() != 1:
+            raise ValueError("seedTensor must be a single int64 on the device")
     if densities is not None:
         _lib.require_gpu(densities)
         if densities.dtype != torch.float32 or tuple(densities.shape) != (bn, 2, W, H) or \
@@ -131,7 +141,8 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
             raise ValueError("densities must be a contiguous float32 tensor of shape (bn, 2, W, H)")
     rc = L.ude_sample_poisson(bn, N, W, H, k, float(focallength), float(smoothing), float(precision), rate,
                               _lib.ptr(ts), _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs),
-                              stdDevs.shape[0], s & 0xFFFFFFFFFFFFFFFF, _lib.ptr(ids), _lib.ptr(ipi),
+                              stdDevs.shape[0], s & 0xFFFFFFFFFFFFFFFF, _lib.ptr(seedTensor), _lib.ptr(ids),
+                              _lib.ptr(ipi),
                               _lib.ptr(densities), _lib.ptr(err), _lib.ptr(ws), ws.numel(), _lib.stream(dev))
     _lib.check(rc, "sampling.samplePoisson")
     last_error = err
@@ -188,6 +199,24 @@ def profile_sampler_begin(kernel="sample"):
     ``sample``), "fmm_eval" (fixed-order density evaluation) or "poisson" (the
     thinning pass of ``samplePoisson``)."""
     _lib.check(_lib.lib().ude_profile_sampler_begin(_PROFILED[kernel]), "profile_sampler_begin")
+
+
+def set_tile_config(kernel, tileW, tileH, wavesPerBlock, xcdOrder=0):
+    """Launch shape of a per-pixel pass ("fmm_eval" or "poisson"): one wavefront per
+    tileW x tileH pixel tile (tileW * tileH = 64), wavesPerBlock wavefronts per
+    workgroup, xcdOrder = 1 for XCD-contiguous tile bands.  Results do not depend
+    on it."""
+    kk = {"fmm_eval": 0, "poisson": 1}[kernel]
+    _lib.check(_lib.lib().ude_set_tile_config(kk, int(tileW), int(tileH), int(wavesPerBlock), int(xcdOrder)),
+               "set_tile_config")
+
+
+def get_tile_config(kernel):
+    import ctypes
+    kk = {"fmm_eval": 0, "poisson": 1}[kernel]
+    out = (ctypes.c_int * 4)()
+    _lib.check(_lib.lib().ude_get_tile_config(kk, ctypes.cast(out, ctypes.c_void_p)), "get_tile_config")
+    return tuple(out)
 
 
 def profile_sampler_end():

@@ -114,19 +114,23 @@ class RateNet:
 
 # --------------------------------------------------------------------------- Poisson-rate frames
 class PoissonFrame:
-    """One fwd+bwd frame of the Horvitz-Thompson renderer on one view: world
-    positions / normals through the affine / linear transforms, Poisson-rate
-    sampling (optionally with per-tile rate multipliers), EWA weights times 1/pi,
-    interpolated albedo (shading "None", no occlusion, so the image estimate is
-    unbiased), L2 loss against ``target``, and the raw adjoints of every op.
-    Gradients are with respect to world positions / normals, std-devs and albedo."""
+    """One fwd+bwd frame of the Horvitz-Thompson renderer over one or more views
+    (``scene.view`` (4, 4) or (V, 4, 4)): world positions / normals through the
+    affine / linear transforms, Poisson-rate sampling of all views (optionally
+    with per-tile rate multipliers), EWA weights times 1/pi, interpolated albedo
+    (shading "None", no occlusion, so the image estimate is unbiased), L2 loss
+    against ``target``, and the raw adjoints of every op.  Gradients are with
+    respect to world positions / normals, std-devs and albedo, summed over the
+    views."""
 
     def __init__(self, scene, device, rate=10.0, kcap=40, precision=0.01, target=None):
         from cudaExtensions import transform
         self.sc = scene
         self.dev = device
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(device)
-        self.M = t(scene.view)[None].contiguous()
+        view = np.asarray(scene.view, np.float32)
+        self.M = (t(view) if view.ndim == 3 else t(view)[None]).contiguous()
+        self.V = self.M.shape[0]
         self.Pw = t(scene.positions)[None].contiguous()
         self.Nw = t(scene.normals)[None].contiguous()
         self.S = t(scene.stddevs)[None].contiguous()
@@ -134,7 +138,7 @@ class PoissonFrame:
         self.rate, self.kcap, self.prec = rate, kcap, precision
         self.target = target
         self.W, self.H, self.f = scene.width, scene.height, scene.focal
-        self.dens = torch.zeros((1, 2, self.W, self.H), dtype=torch.float32, device=device)
+        self.dens = torch.zeros((self.V, 2, self.W, self.H), dtype=torch.float32, device=device)
         self._transform = transform
 
     def camera(self):
@@ -144,26 +148,27 @@ class PoissonFrame:
         nb = torch.where(Nc[..., 2:3] > 0, -Nc, Nc).contiguous()
         return P, Nc, nb
 
-    def image(self, seed, tile_scale=None, rate=None, kcap=None):
-        """Sampled image estimate (1, W, H, 3) and the sample tensors."""
+    def image(self, seed, tile_scale=None, rate=None, kcap=None, seed_tensor=None):
+        """Sampled image estimate (V, W, H, 3) and the sample tensors."""
         from cudaExtensions import sampling, splatting, interpolation
         P, Nc, nb = self.camera()
         ids, ipi = sampling.samplePoisson(self.W, self.H, self.f, P, nb, None, self.S, kcap or self.kcap, 0.0,
                                           self.prec, seed=seed, rate=rate or self.rate, tileScale=tile_scale,
-                                          densities=self.dens)
+                                          densities=self.dens, seedTensor=seed_tensor)
         w = splatting.forward(self.W, self.H, self.f, ids, P, Nc, self.S, 0.0)
         wt = (w * ipi).contiguous()
         img = interpolation.forward(ids, wt, self.A)
         self.last_ids = ids
         return img, (P, Nc, ids, ipi, wt)
 
-    def step(self, seed, tile_scale=None, rate=None, kcap=None):
-        """Returns (loss, image, grads (positions, normals, std-devs, albedo), dL/dimage)."""
+    def step(self, seed, tile_scale=None, rate=None, kcap=None, seed_tensor=None, with_loss=True):
+        """Returns (loss, image, grads (positions, normals, std-devs, albedo), dL/dimage);
+        loss is None without with_loss (a frame captured in a graph skips the reduction)."""
         from cudaExtensions import splatting, interpolation
         tr = self._transform
-        img, (P, Nc, ids, ipi, wt) = self.image(seed, tile_scale, rate, kcap)
+        img, (P, Nc, ids, ipi, wt) = self.image(seed, tile_scale, rate, kcap, seed_tensor)
         g = (2.0 * (img - self.target)).contiguous()
-        loss = ((img - self.target) ** 2).sum()
+        loss = ((img - self.target) ** 2).sum() if with_loss else None
         gW, gA = interpolation.backward(g, ids, wt, self.A)
         gw = (gW * ipi).contiguous()
         gP, gN, gS = splatting.backward(self.W, self.H, self.f, gw, ids, P, Nc, self.S, 0.0)

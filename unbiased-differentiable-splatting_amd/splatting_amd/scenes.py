@@ -108,6 +108,90 @@ def training_scene(k, n_override=None):
     return Scene(p, n, s, a, 512, 512, 1.2 * 512)
 
 
+def _box_faces(lo, hi, inward):
+    """The six faces of an axis-aligned box as (axis, value, lo, hi, normal sign)."""
+    faces = []
+    for ax in range(3):
+        for side, v in ((0, lo[ax]), (1, hi[ax])):
+            out = 1.0 if side == 1 else -1.0
+            faces.append((ax, v, lo, hi, -out if inward else out))
+    return faces
+
+
+def room_boxes(n_boxes=200, seed=3, size=(8.0, 6.0, 3.0)):
+    """n_boxes random axis-aligned boxes with sides U[0.2, 1.0] m inside the room: (lo, hi) arrays."""
+    rng = np.random.default_rng(seed)
+    side = rng.uniform(0.2, 1.0, size=(n_boxes, 3))
+    lo = rng.uniform(0.0, 1.0, size=(n_boxes, 3)) * (np.array(size) - side)
+    return lo, lo + side
+
+
+def box_room(n=2_000_000, n_boxes=200, seed=3, size=(8.0, 6.0, 3.0), sd_lo=0.01, sd_hi=0.03):
+    """Surfels area-uniform on the inside of a size[0] x size[1] x size[2] m room
+    (normals inward) and on the outside of the room_boxes (normals outward);
+    std-devs U[sd_lo, sd_hi], random albedo."""
+    rng = np.random.default_rng(seed + 1)
+    faces = _box_faces(np.zeros(3), np.array(size, np.float64), True)
+    for lo, hi in zip(*room_boxes(n_boxes, seed, size)):
+        faces += _box_faces(lo, hi, False)
+    area = np.array([np.prod([(f[3][a] - f[2][a]) for a in range(3) if a != f[0]]) for f in faces])
+    counts = rng.multinomial(n, area / area.sum())
+    P, Nn = [], []
+    for (ax, v, lo, hi, sgn), c in zip(faces, counts):
+        if c == 0:
+            continue
+        p = rng.uniform(lo, hi, size=(c, 3))
+        p[:, ax] = v
+        nrm = np.zeros((c, 3))
+        nrm[:, ax] = sgn
+        P.append(p)
+        Nn.append(nrm)
+    pos = np.concatenate(P)
+    nrm = np.concatenate(Nn)
+    perm = rng.permutation(n)
+    pos, nrm = pos[perm], nrm[perm]
+    sd = rng.uniform(sd_lo, sd_hi, size=(n, 2))
+    albedo = rng.uniform(0.0, 1.0, size=(n, 3))
+    return pos.astype(np.float32), nrm.astype(np.float32), sd.astype(np.float32), albedo.astype(np.float32)
+
+
+def look_matrix(centre, yaw, pitch):
+    """World -> camera matrix of a camera at `centre` looking along (yaw, pitch)
+    (z up); the camera looks down its -z axis."""
+    d = np.array([np.cos(pitch) * np.cos(yaw), np.cos(pitch) * np.sin(yaw), np.sin(pitch)])
+    r = np.cross(d, [0.0, 0.0, 1.0])
+    r /= np.linalg.norm(r)
+    u = np.cross(r, d)
+    R = np.stack([r, u, -d])
+    M = np.eye(4)
+    M[:3, :3] = R
+    M[:3, 3] = -R @ np.asarray(centre, np.float64)
+    return M.astype(np.float32)
+
+
+def room_cameras(n_views=16, seed=4, size=(8.0, 6.0, 3.0), margin=0.3, pitch=0.4, boxes=None):
+    """Cameras uniform inside the room, at least `margin` m from the walls and from
+    every box in `boxes` ((lo, hi) arrays), with yaw U[0, 2 pi) and pitch
+    U[-pitch, pitch]: (n_views, 4, 4) world -> camera matrices."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.full(3, margin), np.array(size) - margin
+    mats = []
+    while len(mats) < n_views:
+        c = rng.uniform(lo, hi)
+        yaw, pt = rng.uniform(0.0, 2.0 * np.pi), rng.uniform(-pitch, pitch)
+        if boxes is not None and np.any(np.all((c > boxes[0] - margin) & (c < boxes[1] + margin), axis=1)):
+            continue
+        mats.append(look_matrix(c, yaw, pt))
+    return np.stack(mats)
+
+
+def room_scene(n=2_000_000, n_views=16, width=640, height=480, focal=400.0):
+    """Batched-view workload: the 8 x 6 x 3 m box room with n surfels and 200 boxes,
+    seen by n_views cameras at width x height (Scene.view holds (n_views, 4, 4))."""
+    p, nrm, s, a = box_room(n)
+    return Scene(p, nrm, s, a, width, height, focal, room_cameras(n_views, boxes=room_boxes()))
+
+
 def perturbed_target_positions(scene, sigma=0.01, seed=100):
     rng = np.random.default_rng(seed)
     return (scene.positions + rng.normal(0.0, sigma, size=scene.positions.shape)).astype(np.float32)
