@@ -293,6 +293,61 @@ def reference_path(dev, args, sc_n):
             "ops_ms": {k_: round(v, 4) for k_, v in sorted(ops.items())}}
 
 
+def batched_views(dev, steps, warmup, n_splats=2_000_000, views=16):
+    """Batched-view workload: the 8 x 6 x 3 m box room (2M surfels on the walls and
+    on 200 boxes, std-devs U[0.01, 0.03]) seen by 16 cameras at 640x480 with uneven
+    culled counts; one fwd+bwd Poisson-rate frame over all views per step, with
+    every per-pixel pass one launch over the batch, launch shapes from the
+    autotuner, the whole frame replayed as one captured HIP graph (the Philox key
+    advances inside the graph)."""
+    import torch
+    from cudaExtensions import sampling
+    from splatting_amd import autotune, learned as L, scenes
+    sc = scenes.room_scene(n_splats, views)
+    fr = L.PoissonFrame(sc, dev, rate=RATE, kcap=KCAP)
+    fr.target = torch.zeros((fr.V, fr.W, fr.H, 3), dtype=torch.float32, device=dev)
+    P, Nc, nb = fr.camera()
+    in_front = (P[..., 2] < 0).sum(1).tolist()
+
+    def sample(batched=True):
+        sampling.samplePoisson(fr.W, fr.H, fr.f, P, nb, None, fr.S, KCAP, 0.0, fr.prec, seed=1, rate=RATE,
+                               densities=fr.dens, batched=batched)
+
+    saved = {k: sampling.get_tile_config(k) for k in autotune.KERNELS}
+    tuning = autotune.tune(sample)
+    seq_ms, _, _ = timed_steps(lambda: sample(False), 3)
+    bat_ms, _, _ = timed_steps(sample, 3)
+    for _ in range(warmup):
+        fr.step(7, with_loss=False)
+    eager_ms, _, _ = timed_steps(lambda: fr.step(11, with_loss=False), steps)
+    key = torch.tensor([1 << 20], dtype=torch.int64, device=dev)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fr.step(0, seed_tensor=key, with_loss=False)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        key.add_(1)
+        fr.step(0, seed_tensor=key, with_loss=False)
+    for _ in range(warmup):
+        graph.replay()
+    graph_ms, fe_ms, fe_n = timed_steps(graph.replay, steps, "fmm_eval")
+    sampling.check_errors()
+    for k, c in saved.items():
+        sampling.set_tile_config(k, *c)
+    best = min(eager_ms, graph_ms)
+    return {"workload": "box room 8x6x3 m, %d surfels on walls + 200 boxes, %d cameras at 640x480, "
+                        "Poisson-rate sampling at rate %g, fwd+bwd over all views per step" % (n_splats, views, RATE),
+            "views": views, "width": fr.W, "height": fr.H, "splats": n_splats, "splats_in_front_per_view": in_front,
+            "frames_per_s": 1000.0 / best, "views_per_s": views * 1000.0 / best,
+            "graph_ms_per_step": graph_ms, "eager_ms_per_step": eager_ms,
+            "sampling_ms": {"one_launch_per_pass": bat_ms, "view_by_view": seq_ms},
+            "fmm_eval_ms_per_launch": fe_ms / max(fe_n, 1), "autotune": tuning,
+            "memory_gb": torch.cuda.max_memory_allocated() / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,6 +357,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rates", action="store_true", help="time the plain rates without the learned comparison")
     ap.add_argument("--no-reference", action="store_true", help="skip the reference-method frame")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-view (box room) workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of known size (1 GiB read + 1 GiB written per launch), "
                          "to scale HBM traffic counters")
@@ -336,13 +392,13 @@ def main():
     ms_per_step, fe_ms, fe_n = timed_steps(step, args.steps, "fmm_eval")
     _, po_ms, po_n = timed_steps(step, max(args.steps // 4, 1), "poisson")
     fps = 1000.0 / ms_per_step
-    # dominant kernel: the fixed-order density evaluation (two launches per frame).  Its
-    # algorithmic bytes per launch: every node record it can reach (32 B geometry + 16 B
-    # links + 4 B expansion offset, 2n - 1 nodes for n splats in view) and one 4 B density
-    # per pixel; the expansion coefficients it reads are not counted (their number is only
-    # known on the device), so this is a lower bound
+    # dominant kernel: the fixed-order density evaluation (one launch per frame, both
+    # bands).  Its algorithmic bytes per launch, per band: every node record it can reach
+    # (32 B geometry + 16 B links + 4 B expansion offset, 2n - 1 nodes for n splats in
+    # view) and one 4 B density per pixel; the expansion coefficients it reads are not
+    # counted (their number is only known on the device), so this is a lower bound
     n_vis = int((L.splat_tiles(fr) >= 0).sum().item())
-    fe_bytes = 52 * (2 * n_vis - 1) + 4 * sc.width * sc.height
+    fe_bytes = 2 * (52 * (2 * n_vis - 1) + 4 * sc.width * sc.height)
     fe_avg = fe_ms / max(fe_n, 1)
     achieved = fe_bytes / (fe_avg * 1e-3) / 1e9
     line = {
@@ -360,6 +416,9 @@ def main():
                      "next_kernel": {"kernel": "k_poisson", "kernel_ms": po_ms / max(po_n, 1)}},
         "rates": rates,
     }
+    if not args.no_batched:
+        print("bench: batched views", file=sys.stderr, flush=True)
+        line["batched_views"] = batched_views(dev, max(args.steps // 2, 3), 2)
     if not args.no_reference:
         print("bench: reference method", file=sys.stderr, flush=True)
         del fr

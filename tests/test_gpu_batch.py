@@ -136,3 +136,40 @@ def test_graph_replays_match_eager(dev):
         assert torch.equal(got_ids, eids), r
         assert torch.equal(got_ipi, eipi), r
         assert (got_ids >= 0).sum() > 1000
+
+
+def test_multiview_frame_graph(dev):
+    """A whole fwd+bwd frame over 4 room views (uneven culled counts) captured as
+    one graph that advances its own Philox key: every replay equals the eager
+    frame of the same key (images bit for bit; gradients, which sum over views
+    with atomics, within 1e-4 relative)."""
+    from splatting_amd import learned as L, scenes
+    sc = scenes.Scene(*scenes.box_room(60_000, n_boxes=20), 96, 72, 60.0,
+                      scenes.room_cameras(4, boxes=scenes.room_boxes(20)))
+    fr = L.PoissonFrame(sc, dev, rate=6.0, kcap=32)
+    fr.target = torch.full((4, 96, 72, 3), 0.25, dtype=torch.float32, device=dev)
+    key = torch.tensor([500], dtype=torch.int64, device=dev)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fr.step(0, seed_tensor=key, with_loss=False)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        key.add_(1)
+        _, gimg, ggrads, _ = fr.step(0, seed_tensor=key, with_loss=False)
+    for r in range(3):
+        graph.replay()
+        torch.cuda.synchronize()
+        k = int(key.item())
+        img, grads = gimg.clone(), [x.clone() for x in ggrads]
+        _, eimg, egrads, _ = fr.step(k, with_loss=False)
+        torch.cuda.synchronize()
+        assert k == 501 + r
+        assert torch.equal(img, eimg), r
+        for a, b in zip(grads, egrads):
+            scale = b.abs().max().clamp(min=1e-30)
+            assert ((a - b).abs().max() / scale) < 1e-4, r
+        counts = (fr.last_ids >= 0).sum((1, 2, 3)).tolist()
+        assert min(counts) > 0 and len(set(counts)) > 1, counts

@@ -90,6 +90,7 @@ struct TreeWs {
     int* fixList[2];      // per band: accepted internal nodes
     float* fixCoef[2];    // per band: 49 coefficients per accepted node
     float* hraw;          // per splat id: proposal bandwidth sqrt(lambda_max(V))
+    int* esc;             // per node: next node in preorder after its subtree, -1 after the last
     float* dens[2];       // per pixel and band: density D_b(q)
 };
 
@@ -137,6 +138,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     w.coef = fixedOrder ? nullptr : (float*)take(4 * coefCap);
     w.coefCap = fixedOrder ? 0 : coefCap;
     w.hraw = nullptr;
+    w.esc = nullptr;
     for (int b = 0; b < 2; b++) { w.fixOff[b] = w.fixList[b] = nullptr; w.fixCoef[b] = w.dens[b] = nullptr; }
     const int nb = mode == kModePoisson ? 2 : mode == kModeFixed ? 1 : 0;
     for (int b = 0; b < nb; b++) {
@@ -146,6 +148,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     }
     if (mode == kModePoisson) {
         w.hraw = (float*)take(4 * n);
+        w.esc = (int*)take(4 * nn);
         w.dens[0] = (float*)take(4L * W * H);
         w.dens[1] = (float*)take(4L * W * H);
     }
@@ -1456,9 +1459,12 @@ __device__ __forceinline__ T* view_ptr(T* p, size_t stride)
 constexpr int kFmmStack = 96;
 __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg, int W, int H,
                                                                     const ViewState* __restrict__ vs, size_t vstride,
-                                                                    int b, const NodeGeo* geo, const NodeLnk* lnk,
-                                                                    const int* __restrict__ fixOff, const float* coef,
-                                                                    float* __restrict__ out, size_t outStride)
+                                                                    int nbands, const NodeGeo* geo, const NodeLnk* lnk,
+                                                                    const int* __restrict__ fixOff0,
+                                                                    const int* __restrict__ fixOff1,
+                                                                    const float* coef0, const float* coef1,
+                                                                    float* __restrict__ out, size_t outStride,
+                                                                    size_t bandStride)
 {
     __shared__ int stkN[kMaxWavesPerBlock][kFmmStack];
     __shared__ unsigned long long stkM[kMaxWavesPerBlock][kFmmStack];
@@ -1466,12 +1472,14 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
     const int wid = threadIdx.x >> 6;
     int i, j;
     if (!tile_lane(cfg, W, H, lane, i, j)) return;           // wave-uniform
-    vs = view_ptr(vs, vstride);
-    geo = view_ptr(geo, vstride);
-    lnk = view_ptr(lnk, vstride);
-    fixOff = view_ptr(fixOff, vstride);
-    coef = view_ptr(coef, vstride);
-    out = view_ptr(out, outStride);
+    const int b = (int)blockIdx.y % nbands;                  // band
+    const size_t v = blockIdx.y / nbands;                    // view
+    vs = (const ViewState*)((const char*)vs + v * vstride);
+    geo = (const NodeGeo*)((const char*)geo + v * vstride);
+    lnk = (const NodeLnk*)((const char*)lnk + v * vstride);
+    const int* fixOff = (const int*)((const char*)(b ? fixOff1 : fixOff0) + v * vstride);
+    const float* coef = (const float*)((const char*)(b ? coef1 : coef0) + v * vstride);
+    out = (float*)((char*)out + v * outStride + (size_t)b * bandStride);
     const bool valid = i < W && j < H;
     const int n = vs->nc;
     float w = 0.0f;
@@ -1605,6 +1613,23 @@ __device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float
     return m;
 }
 
+// escape pointer of every node: the right sibling of the first ancestor-or-self
+// that is a left child, -1 for the nodes on the rightmost path
+__global__ __launch_bounds__(256) void k_escape(const ViewState* __restrict__ vs, const NodeLnk* __restrict__ lnk,
+                                                int* __restrict__ esc)
+{
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = vs->nc;
+    if (nc <= 0 || nd >= 2 * nc - 1) return;
+    int e = nd, r = -1;
+    while (e != 0) {
+        const int p = lnk[e].parent;
+        if (lnk[p].left == e) { r = lnk[p].right; break; }
+        e = p;
+    }
+    esc[nd] = r;
+}
+
 // One lane per pixel (one pixel tile per wavefront, TileCfg).  The walk
 // visits nodes left to right; a node's bound U (the mixture at the point of its
 // box closest to q) is >= pi_i of each of its leaves.  U = 0: skip the
@@ -1613,7 +1638,7 @@ __device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float
 // log(1 - U))), each kept with probability pi_i / U, i.e. included with
 // probability pi_i, independently.  Uniforms are Philox4x32-10 words at counter
 // (draw >> 2, 0x80000000, pixel, view), word draw & 3, with a per-pixel draw
-// number.
+// number.  A skipped or thinned subtree is left through its escape pointer.
 __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg, int W, int H, int kcap, float rate,
                                                                    const float* __restrict__ tscale, int tsH,
                                                                    uint32_t seedLo, uint32_t seedHi,
@@ -1621,6 +1646,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                                                                    uint32_t view0, const ViewState* __restrict__ vs,
                                                                    size_t vstride, const NodeGeo* geo,
                                                                    const NodeLnk* lnk, const NodeAux* aux,
+                                                                   const int* __restrict__ esc,
                                                                    const int32_t* __restrict__ leafId,
                                                                    const float* __restrict__ dens0,
                                                                    const float* __restrict__ dens1, size_t densStride,
@@ -1641,6 +1667,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
     geo = view_ptr(geo, vstride);
     lnk = view_ptr(lnk, vstride);
     aux = view_ptr(aux, vstride);
+    esc = view_ptr(esc, vstride);
     leafId = view_ptr(leafId, vstride);
     dens0 = view_ptr(dens0, densStride);
     dens1 = view_ptr(dens1, densStride);
@@ -1717,14 +1744,8 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                 }
                 if (descend) { nd = lk.left; continue; }
             }
-            bool done = true;
-            while (nd != 0) {
-                int p = T.lnk[nd].parent;
-                int pl = T.lnk[p].left;
-                if (nd == pl) { nd = T.lnk[p].right; done = false; break; }
-                nd = p;
-            }
-            if (done) break;
+            nd = esc[nd];                             // past the subtree
+            if (nd < 0) break;
         }
     }
     for (int s = ns; s < kcap; s++) { oid[s] = -1; oip[s] = 0.0f; }
@@ -1995,17 +2016,19 @@ void fixed_expansions(int N, int b, const TreeWs& w, hipStream_t st)
                                                                     w.fixCoef[b]);
 }
 
-// band-b densities of `views` views in one launch: w is the first view's
-// workspace, the others follow vstride bytes apart; view v writes out + v outStride bytes
-void fmm_eval(int W, int H, int views, int b, const TreeWs& w, size_t vstride, float* out, size_t outStride,
-              hipStream_t st)
+// densities of bands 0 .. nbands-1 of `views` views in one launch: w is the
+// first view's workspace, the others follow vstride bytes apart; view v writes
+// band b to out + v outStride + b bandStride bytes
+void fmm_eval(int W, int H, int views, int nbands, const TreeWs& w, size_t vstride, float* out, size_t outStride,
+              size_t bandStride, hipStream_t st)
 {
     const TileCfg c = g_tile[0];
     hipEvent_t e0, e1;
     bool timed = timing_pair(1, &e0, &e1);
     if (timed) (void)hipEventRecord(e0, st);
-    k_fmm_eval<<<tile_grid(c, W, H, views), 64 * c.wpb, 0, st>>>(c, W, H, w.vs, vstride, b, w.geo, w.lnk,
-                                                                  w.fixOff[b], w.fixCoef[b], out, outStride);
+    k_fmm_eval<<<tile_grid(c, W, H, views * nbands), 64 * c.wpb, 0, st>>>(
+        c, W, H, w.vs, vstride, nbands, w.geo, w.lnk, w.fixOff[0], nbands > 1 ? w.fixOff[1] : nullptr, w.fixCoef[0],
+        nbands > 1 ? w.fixCoef[1] : nullptr, out, outStride, bandStride);
     if (timed) (void)hipEventRecord(e1, st);
 }
 
@@ -2078,7 +2101,7 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
         if (rc != UDE_OK) return rc;
         fixed_expansions(N, 0, w, st);
-        fmm_eval(W, H, 1, 0, w, 0, out + perView * b, 0, st);
+        fmm_eval(W, H, 1, 1, w, 0, out + perView * b, 0, 0, st);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
@@ -2124,6 +2147,7 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
             if (rc != UDE_OK) return rc;
             k_wide_hist<<<grid1(N, 256), 256, 0, st>>>(wv.vs, wv.idsA, wv.hraw);
             k_wide_band<<<1, 64, 0, st>>>(wv.vs);
+            k_escape<<<grid1(2L * N, 256), 256, 0, st>>>(wv.vs, wv.lnk, wv.esc);
             fixed_expansions(N, 0, wv, st);
             fixed_expansions(N, 1, wv, st);
         }
@@ -2132,15 +2156,14 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         float* d0 = densOut ? densOut + perView * 2 * g : w.dens[0];
         float* d1 = densOut ? d0 + perView : w.dens[1];
         const size_t dstride = densOut ? sizeof(float) * 2 * perView : per;
-        fmm_eval(W, H, span, 0, w, per, d0, dstride, st);
-        fmm_eval(W, H, span, 1, w, per, d1, dstride, st);
+        fmm_eval(W, H, span, 2, w, per, d0, dstride, (size_t)((char*)d1 - (char*)d0), st);
         const float* ts = tileScale ? tileScale + (long)tsW * tsH * g : nullptr;
         hipEvent_t e0, e1;
         bool timed = timing_pair(2, &e0, &e1);
         if (timed) (void)hipEventRecord(e0, st);
         k_poisson<<<tile_grid(pc, W, H, span), 64 * pc.wpb, 0, st>>>(
             pc, W, H, kcap, (float)rate, ts, tsH, sLo, sHi, seedDev, (uint32_t)g, w.vs, per, w.geo, w.lnk, w.aux,
-            w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag);
+            w.esc, w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag);
         if (timed) (void)hipEventRecord(e1, st);
     }
     hipError_t e = hipGetLastError();
