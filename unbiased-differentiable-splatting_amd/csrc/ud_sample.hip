@@ -69,6 +69,15 @@ struct NodeLnk {
 struct NodeAux {
     int off, first, last, pad;
 };
+// per internal node: what the fixed-order density walk needs of its two
+// children, so that one wave-uniform load serves a descent
+struct FmmRec {
+    float lbox[4], rbox[4];      // children's boxes (lo.x, lo.y, hi.x, hi.y)
+    float lc[2], rc[2];          // children's centres
+    int left, right;
+    int lfix[2], rfix[2];        // children's expansion offsets per band, -1 for none / leaves
+    int pad[2];
+};
 
 struct TreeWs {
     ViewState* vs;
@@ -91,6 +100,7 @@ struct TreeWs {
     float* fixCoef[2];    // per band: 49 coefficients per accepted node
     float* hraw;          // per splat id: proposal bandwidth sqrt(lambda_max(V))
     int* esc;             // per node: next node in preorder after its subtree, -1 after the last
+    FmmRec* frec;         // per internal node: children for the density walk
     float* dens[2];       // per pixel and band: density D_b(q)
 };
 
@@ -139,6 +149,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     w.coefCap = fixedOrder ? 0 : coefCap;
     w.hraw = nullptr;
     w.esc = nullptr;
+    w.frec = nullptr;
     for (int b = 0; b < 2; b++) { w.fixOff[b] = w.fixList[b] = nullptr; w.fixCoef[b] = w.dens[b] = nullptr; }
     const int nb = mode == kModePoisson ? 2 : mode == kModeFixed ? 1 : 0;
     for (int b = 0; b < nb; b++) {
@@ -146,6 +157,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
         w.fixList[b] = (int*)take(4 * n);
         w.fixCoef[b] = (float*)take(4L * kFixedCoefs * (n - 1 > 0 ? n - 1 : 1));
     }
+    if (nb > 0) w.frec = (FmmRec*)take(sizeof(FmmRec) * n);
     if (mode == kModePoisson) {
         w.hraw = (float*)take(4 * n);
         w.esc = (int*)take(4 * nn);
@@ -1444,7 +1456,46 @@ __device__ __forceinline__ T* view_ptr(T* p, size_t stride)
     return (T*)((char*)p + (size_t)blockIdx.y * stride);
 }
 
-// One lane per pixel, one pixel tile per wavefront (TileCfg).
+constexpr int kFmmStack = 96;
+
+// children records of the internal nodes for the density walk (after the
+// expansions of every band are selected)
+__global__ __launch_bounds__(256) void k_fmm_records(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
+                                                     const NodeLnk* __restrict__ lnk,
+                                                     const int* __restrict__ fixOff0,
+                                                     const int* __restrict__ fixOff1, FmmRec* __restrict__ rec)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int L0 = vs->nc - 1;
+    if (p >= L0) return;
+    const NodeLnk lk = lnk[p];
+    const NodeGeo gl = geo[lk.left], gr = geo[lk.right];
+    FmmRec r;
+    r.lbox[0] = gl.lox; r.lbox[1] = gl.loy; r.lbox[2] = gl.hix; r.lbox[3] = gl.hiy;
+    r.rbox[0] = gr.lox; r.rbox[1] = gr.loy; r.rbox[2] = gr.hix; r.rbox[3] = gr.hiy;
+    r.lc[0] = gl.cx; r.lc[1] = gl.cy; r.rc[0] = gr.cx; r.rc[1] = gr.cy;
+    r.left = lk.left; r.right = lk.right;
+    r.lfix[0] = lk.left < L0 ? fixOff0[lk.left] : -1;
+    r.rfix[0] = lk.right < L0 ? fixOff0[lk.right] : -1;
+    r.lfix[1] = (fixOff1 && lk.left < L0) ? fixOff1[lk.left] : -1;
+    r.rfix[1] = (fixOff1 && lk.right < L0) ? fixOff1[lk.right] : -1;
+    r.pad[0] = r.pad[1] = 0;
+    rec[p] = r;
+}
+
+__device__ __forceinline__ bool box4_meets_disc(const float* bx, float qx, float qy, float rad)
+{
+    float cx = fminf(fmaxf(qx, bx[0]), bx[2]);
+    float cy = fminf(fmaxf(qy, bx[1]), bx[3]);
+    float dx = qx - cx, dy = qy - cy;
+    float d2 = 0.0f;
+    d2 = d2 + dx * dx;
+    d2 = d2 + dy * dy;
+    return d2 < rad * rad;
+}
+
+// One lane per pixel, one pixel tile per wavefront (TileCfg), bands on
+// blockIdx.y (nbands per view).
 // The reference walks each pixel with an explicit stack: pop a node; evaluate
 // it if it is a leaf or carries an expansion, else push its left child and then
 // its right child, each only if its box meets the query disc
@@ -1452,14 +1503,15 @@ __device__ __forceinline__ T* view_ptr(T* p, size_t stride)
 // the union of its 64 pixels: stack entries carry the mask of lanes for which
 // the node was pushed, so each lane sees its own nodes in its own order (the
 // shared stack is its stack with other lanes' entries interleaved, in the same
-// last-in-first-out order) and sums bit for bit like the reference, while node
-// records and coefficient blocks are read once per wavefront at uniform
-// addresses.  Karras trees on 64-bit keys are at most 64 levels deep, so the
-// stack never holds more than 65 entries.
-constexpr int kFmmStack = 96;
+// last-in-first-out order) and sums bit for bit like the reference.  A pushed
+// entry also carries the child's centre and expansion offset from its parent's
+// FmmRec, so leaves and expansions need no node load and a descent needs one
+// wave-uniform record load.  Karras trees on 64-bit keys are at most 64 levels
+// deep, so the stack never holds more than 65 entries.
 __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg, int W, int H,
                                                                     const ViewState* __restrict__ vs, size_t vstride,
-                                                                    int nbands, const NodeGeo* geo, const NodeLnk* lnk,
+                                                                    int nbands, const NodeGeo* geo,
+                                                                    const FmmRec* __restrict__ frec,
                                                                     const int* __restrict__ fixOff0,
                                                                     const int* __restrict__ fixOff1,
                                                                     const float* coef0, const float* coef1,
@@ -1467,6 +1519,8 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
                                                                     size_t bandStride)
 {
     __shared__ int stkN[kMaxWavesPerBlock][kFmmStack];
+    __shared__ int stkF[kMaxWavesPerBlock][kFmmStack];
+    __shared__ float2 stkC[kMaxWavesPerBlock][kFmmStack];
     __shared__ unsigned long long stkM[kMaxWavesPerBlock][kFmmStack];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -1476,7 +1530,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
     const size_t v = blockIdx.y / nbands;                    // view
     vs = (const ViewState*)((const char*)vs + v * vstride);
     geo = (const NodeGeo*)((const char*)geo + v * vstride);
-    lnk = (const NodeLnk*)((const char*)lnk + v * vstride);
+    frec = (const FmmRec*)((const char*)frec + v * vstride);
     const int* fixOff = (const int*)((const char*)(b ? fixOff1 : fixOff0) + v * vstride);
     const float* coef = (const float*)((const char*)(b ? coef1 : coef0) + v * vstride);
     out = (float*)((char*)out + v * outStride + (size_t)b * bandStride);
@@ -1484,35 +1538,43 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
     const int n = vs->nc;
     float w = 0.0f;
     if (n > 0) {
-        TreeView T;
-        T.geo = (const NodeGeo UD_GLOBAL*)geo;
-        T.lnk = (const NodeLnk UD_GLOBAL*)lnk;
         const float UD_GLOBAL* cf = (const float UD_GLOBAL*)coef;
         const Band B = vs->band[b];
         const int L0 = n - 1;
         const float qx = (float)i, qy = (float)j;
         int* sN = stkN[wid];
+        int* sF = stkF[wid];
+        float2* sC = stkC[wid];
         unsigned long long* sM = stkM[wid];
         const unsigned long long all = __ballot(valid);
-        if (lane == 0) { sN[0] = 0; sM[0] = all; }
+        if (lane == 0) {
+            sN[0] = 0;
+            sF[0] = L0 > 0 ? fixOff[0] : -1;
+            sC[0] = make_float2(geo[0].cx, geo[0].cy);
+            sM[0] = all;
+        }
         int sp = 1;
         while (sp > 0) {
             sp--;
             __builtin_amdgcn_wave_barrier();
             const int nd = __builtin_amdgcn_readfirstlane(sN[sp]);
+            const int off = __builtin_amdgcn_readfirstlane(sF[sp]);
+            const float2 c = sC[sp];
             const unsigned long long mask = sM[sp];
             __builtin_amdgcn_wave_barrier();
             const bool act = (mask >> lane) & 1ull;
-            NodeGeo g = ld_geo(T, nd);
             if (nd >= L0) {
-                if (act) w = w + band_point(B, qx, qy, g.cx, g.cy);
+                if (act) w = w + band_point(B, qx, qy, c.x, c.y);
                 continue;
             }
-            const int off = __builtin_amdgcn_readfirstlane(fixOff[nd]);
             if (off >= 0) {
+                float co[kFixedCoefs];                      // wave-uniform: scalar loads
+                const float UD_GLOBAL* cc = cf + off;
+#pragma unroll
+                for (int q = 0; q < kFixedCoefs; q++) co[q] = cc[q];
                 if (act) {
-                    float tx = (qx - g.cx) * B.is2h;
-                    float ty = (qy - g.cy) * B.is2h;
+                    float tx = (qx - c.x) * B.is2h;
+                    float ty = (qy - c.y) * B.is2h;
                     float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
                     float hy[kFixedDegree + 1];
                     hy[0] = ey;
@@ -1520,8 +1582,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
 #pragma unroll
                     for (int q = 2; q <= kFixedDegree; q++)
                         hy[q] = (2.0f * ty) * hy[q - 1] - (2.0f * (float)(q - 1)) * hy[q - 2];
-                    const float UD_GLOBAL* cc = cf + off;
-                    float v = 0.0f;
+                    float val = 0.0f;
                     float hprev = 0.0f, ha = ex;
 #pragma unroll
                     for (int a = 0; a <= kFixedDegree; a++) {
@@ -1532,18 +1593,30 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
                         }
 #pragma unroll
                         for (int bb = 0; bb <= kFixedDegree; bb++)
-                            v = v + (cc[a * (kFixedDegree + 1) + bb] * ha) * hy[bb];
+                            val = val + (co[a * (kFixedDegree + 1) + bb] * ha) * hy[bb];
                     }
-                    w = w + v;
+                    w = w + val;
                 }
                 continue;
             }
-            NodeLnk lk = ld_lnk(T, nd);
-            const unsigned long long ml = __ballot(act && box_meets_disc(ld_geo(T, lk.left), qx, qy, B.radius));
-            const unsigned long long mr = __ballot(act && box_meets_disc(ld_geo(T, lk.right), qx, qy, B.radius));
+            const FmmRec r = frec[nd];
+            const unsigned long long ml = __ballot(act && box4_meets_disc(r.lbox, qx, qy, B.radius));
+            const unsigned long long mr = __ballot(act && box4_meets_disc(r.rbox, qx, qy, B.radius));
             if (sp + 2 > kFmmStack) break;                   // unreachable for trees of depth <= 64
-            if (ml) { if (lane == 0) { sN[sp] = lk.left; sM[sp] = ml; } sp++; }
-            if (mr) { if (lane == 0) { sN[sp] = lk.right; sM[sp] = mr; } sp++; }
+            if (ml) {
+                if (lane == 0) {
+                    sN[sp] = r.left; sF[sp] = b ? r.lfix[1] : r.lfix[0];
+                    sC[sp] = make_float2(r.lc[0], r.lc[1]); sM[sp] = ml;
+                }
+                sp++;
+            }
+            if (mr) {
+                if (lane == 0) {
+                    sN[sp] = r.right; sF[sp] = b ? r.rfix[1] : r.rfix[0];
+                    sC[sp] = make_float2(r.rc[0], r.rc[1]); sM[sp] = mr;
+                }
+                sp++;
+            }
         }
     }
     if (valid) out[(long)j + (long)H * i] = w;
@@ -2027,8 +2100,8 @@ void fmm_eval(int W, int H, int views, int nbands, const TreeWs& w, size_t vstri
     bool timed = timing_pair(1, &e0, &e1);
     if (timed) (void)hipEventRecord(e0, st);
     k_fmm_eval<<<tile_grid(c, W, H, views * nbands), 64 * c.wpb, 0, st>>>(
-        c, W, H, w.vs, vstride, nbands, w.geo, w.lnk, w.fixOff[0], nbands > 1 ? w.fixOff[1] : nullptr, w.fixCoef[0],
-        nbands > 1 ? w.fixCoef[1] : nullptr, out, outStride, bandStride);
+        c, W, H, w.vs, vstride, nbands, w.geo, w.frec, w.fixOff[0], nbands > 1 ? w.fixOff[1] : nullptr,
+        w.fixCoef[0], nbands > 1 ? w.fixCoef[1] : nullptr, out, outStride, bandStride);
     if (timed) (void)hipEventRecord(e1, st);
 }
 
@@ -2101,6 +2174,7 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
         if (rc != UDE_OK) return rc;
         fixed_expansions(N, 0, w, st);
+        k_fmm_records<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], nullptr, w.frec);
         fmm_eval(W, H, 1, 1, w, 0, out + perView * b, 0, 0, st);
     }
     hipError_t e = hipGetLastError();
@@ -2150,6 +2224,7 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
             k_escape<<<grid1(2L * N, 256), 256, 0, st>>>(wv.vs, wv.lnk, wv.esc);
             fixed_expansions(N, 0, wv, st);
             fixed_expansions(N, 1, wv, st);
+            k_fmm_records<<<grid1(N, 256), 256, 0, st>>>(wv.vs, wv.geo, wv.lnk, wv.fixOff[0], wv.fixOff[1], wv.frec);
         }
         TreeWs w;
         carve(ws, N, W, H, 0, kModePoisson, &w);
