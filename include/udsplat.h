@@ -78,7 +78,7 @@ size_t ude_poisson_workspace_bytes(int bn, int N, int W, int H);
  * splat ids (-1 padded) and 1 / pi_i (0 padded), so sum_s w_s a_s / pi_s is a
  * Horvitz-Thompson estimate of sum_i w_i a_i.  densOut (optional, (bn, 2, W, H))
  * receives D0 and D1.  A pixel with more than kcap inclusions sets bit 4 of
- * *errFlag.  seedDev (optional, device memory): the Philox key is read there
+ * *errFlag; a walk longer than the tree (not expected) stops and sets bit 8.  seedDev (optional, device memory): the Philox key is read there
  * at run time instead of from seed, so a captured HIP graph draws fresh
  * samples when the caller advances it between replays.  With a workspace of
  * ude_poisson_workspace_bytes(bn, ...) bytes all views' trees are built

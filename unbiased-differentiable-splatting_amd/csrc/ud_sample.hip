@@ -69,6 +69,13 @@ struct NodeLnk {
 struct NodeAux {
     int off, first, last, pad;
 };
+// per node: what the per-pixel Poisson walk reads of it in one load
+struct PNode {
+    float lox, loy, hix, hiy;    // box
+    int left;                    // left child, -1 for leaves
+    int esc;                     // next node in preorder after the subtree, -1 after the last
+    int first, last;             // sorted leaf range
+};
 // per internal node: what the fixed-order density walk needs of its two
 // children, so that one wave-uniform load serves a descent
 struct FmmRec {
@@ -99,7 +106,7 @@ struct TreeWs {
     int* fixList[2];      // per band: accepted internal nodes
     float* fixCoef[2];    // per band: 49 coefficients per accepted node
     float* hraw;          // per splat id: proposal bandwidth sqrt(lambda_max(V))
-    int* esc;             // per node: next node in preorder after its subtree, -1 after the last
+    PNode* pnode;         // per node: Poisson walk record
     FmmRec* frec;         // per internal node: children for the density walk
     float* dens[2];       // per pixel and band: density D_b(q)
 };
@@ -148,7 +155,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     w.coef = fixedOrder ? nullptr : (float*)take(4 * coefCap);
     w.coefCap = fixedOrder ? 0 : coefCap;
     w.hraw = nullptr;
-    w.esc = nullptr;
+    w.pnode = nullptr;
     w.frec = nullptr;
     for (int b = 0; b < 2; b++) { w.fixOff[b] = w.fixList[b] = nullptr; w.fixCoef[b] = w.dens[b] = nullptr; }
     const int nb = mode == kModePoisson ? 2 : mode == kModeFixed ? 1 : 0;
@@ -160,7 +167,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     if (nb > 0) w.frec = (FmmRec*)take(sizeof(FmmRec) * n);
     if (mode == kModePoisson) {
         w.hraw = (float*)take(4 * n);
-        w.esc = (int*)take(4 * nn);
+        w.pnode = (PNode*)take(sizeof(PNode) * nn);
         w.dens[0] = (float*)take(4L * W * H);
         w.dens[1] = (float*)take(4L * W * H);
     }
@@ -1686,10 +1693,11 @@ __device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float
     return m;
 }
 
-// escape pointer of every node: the right sibling of the first ancestor-or-self
-// that is a left child, -1 for the nodes on the rightmost path
-__global__ __launch_bounds__(256) void k_escape(const ViewState* __restrict__ vs, const NodeLnk* __restrict__ lnk,
-                                                int* __restrict__ esc)
+// Poisson walk records; the escape pointer of a node is the right sibling of
+// its first ancestor-or-self that is a left child, -1 on the rightmost path
+__global__ __launch_bounds__(256) void k_pois_nodes(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
+                                                    const NodeLnk* __restrict__ lnk, const NodeAux* __restrict__ aux,
+                                                    PNode* __restrict__ pnode)
 {
     const int nd = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = vs->nc;
@@ -1700,7 +1708,14 @@ __global__ __launch_bounds__(256) void k_escape(const ViewState* __restrict__ vs
         if (lnk[p].left == e) { r = lnk[p].right; break; }
         e = p;
     }
-    esc[nd] = r;
+    const NodeGeo g = geo[nd];
+    const NodeAux a = aux[nd];
+    PNode q;
+    q.lox = g.lox; q.loy = g.loy; q.hix = g.hix; q.hiy = g.hiy;
+    q.left = nd < nc - 1 ? lnk[nd].left : -1;
+    q.esc = r;
+    q.first = a.first; q.last = a.last;
+    pnode[nd] = q;
 }
 
 // One lane per pixel (one pixel tile per wavefront, TileCfg).  The walk
@@ -1717,9 +1732,8 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                                                                    uint32_t seedLo, uint32_t seedHi,
                                                                    const uint64_t* __restrict__ seedDev,
                                                                    uint32_t view0, const ViewState* __restrict__ vs,
-                                                                   size_t vstride, const NodeGeo* geo,
-                                                                   const NodeLnk* lnk, const NodeAux* aux,
-                                                                   const int* __restrict__ esc,
+                                                                   size_t vstride, const PNode* __restrict__ pnode,
+                                                                   const float2* __restrict__ leafXY,
                                                                    const int32_t* __restrict__ leafId,
                                                                    const float* __restrict__ dens0,
                                                                    const float* __restrict__ dens1, size_t densStride,
@@ -1737,10 +1751,8 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         seedHi = (uint32_t)(sd >> 32);
     }
     vs = view_ptr(vs, vstride);
-    geo = view_ptr(geo, vstride);
-    lnk = view_ptr(lnk, vstride);
-    aux = view_ptr(aux, vstride);
-    esc = view_ptr(esc, vstride);
+    pnode = view_ptr(pnode, vstride);
+    leafXY = view_ptr(leafXY, vstride);
     leafId = view_ptr(leafId, vstride);
     dens0 = view_ptr(dens0, densStride);
     dens1 = view_ptr(dens1, densStride);
@@ -1756,10 +1768,6 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
     const int n = vs->nc;
     const float D0 = n > 0 ? dens0[pix] : 0.0f;
     if (n > 0 && D0 >= vs->prec) {
-        TreeView T;
-        T.geo = (const NodeGeo UD_GLOBAL*)geo;
-        T.lnk = (const NodeLnk UD_GLOBAL*)lnk;
-        T.aux = (const NodeAux UD_GLOBAL*)aux;
         const int L0 = n - 1;
         const Band B0 = vs->band[0], B1 = vs->band[1];
         float lam = rate;
@@ -1783,29 +1791,29 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
             return u01(x);
         };
         int nd = 0;
+        int visits = 0;                               // a preorder walk visits each node at most once
         while (true) {
-            NodeGeo g = ld_geo(T, nd);
-            float cx = fminf(fmaxf(qx, g.lox), g.hix);
-            float cy = fminf(fmaxf(qy, g.loy), g.hiy);
+            if (++visits > 2 * n) { atomicOr(err, 8); break; }
+            const PNode pn = pnode[nd];
+            float cx = fminf(fmaxf(qx, pn.lox), pn.hix);
+            float cy = fminf(fmaxf(qy, pn.loy), pn.hiy);
             float pB = mix_value(B0, B1, t0, t1, qx, qy, cx, cy);     // proposal bound over the node
             float U = fminf(1.0f, lam * pB);
-            bool descend = false;
+            const int last = pn.last;
+            int next = pn.esc;                        // past the subtree
             if (pB >= kProposalPMin && U > 0.0f) {
                 const bool leaf = nd >= L0;
-                NodeLnk lk = ld_lnk(T, nd);
-                if (!leaf && (float)lk.count * U > 1.0f) descend = true;
+                if (!leaf && (float)(last - pn.first + 1) * U > 1.0f) next = pn.left;
                 else {
-                    int first = leaf ? nd - L0 : T.aux[nd].first;
-                    int last = leaf ? nd - L0 : T.aux[nd].last;
                     const float l1 = plog1mf(U);
-                    int pos = first - 1;
+                    int pos = pn.first - 1;
                     while (true) {
                         float u = uniform();
                         float r = plogf(u) / l1;          // failures before the next candidate
                         if (!(r < (float)(last - pos))) break;
                         pos += (int)r + 1;
-                        NodeGeo lg = ld_geo(T, L0 + pos);
-                        float pl = mix_value(B0, B1, t0, t1, qx, qy, lg.cx, lg.cy);
+                        const float2 lp = leafXY[pos];
+                        float pl = mix_value(B0, B1, t0, t1, qx, qy, lp.x, lp.y);
                         if (!(pl >= kProposalPMin)) continue;     // outside the support
                         float pi = fminf(1.0f, lam * pl);
                         float a = uniform();
@@ -1815,9 +1823,8 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                         }
                     }
                 }
-                if (descend) { nd = lk.left; continue; }
             }
-            nd = esc[nd];                             // past the subtree
+            nd = next;
             if (nd < 0) break;
         }
     }
@@ -2221,7 +2228,7 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
             if (rc != UDE_OK) return rc;
             k_wide_hist<<<grid1(N, 256), 256, 0, st>>>(wv.vs, wv.idsA, wv.hraw);
             k_wide_band<<<1, 64, 0, st>>>(wv.vs);
-            k_escape<<<grid1(2L * N, 256), 256, 0, st>>>(wv.vs, wv.lnk, wv.esc);
+            k_pois_nodes<<<grid1(2L * N, 256), 256, 0, st>>>(wv.vs, wv.geo, wv.lnk, wv.aux, wv.pnode);
             fixed_expansions(N, 0, wv, st);
             fixed_expansions(N, 1, wv, st);
             k_fmm_records<<<grid1(N, 256), 256, 0, st>>>(wv.vs, wv.geo, wv.lnk, wv.fixOff[0], wv.fixOff[1], wv.frec);
@@ -2237,8 +2244,8 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         bool timed = timing_pair(2, &e0, &e1);
         if (timed) (void)hipEventRecord(e0, st);
         k_poisson<<<tile_grid(pc, W, H, span), 64 * pc.wpb, 0, st>>>(
-            pc, W, H, kcap, (float)rate, ts, tsH, sLo, sHi, seedDev, (uint32_t)g, w.vs, per, w.geo, w.lnk, w.aux,
-            w.esc, w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag);
+            pc, W, H, kcap, (float)rate, ts, tsH, sLo, sHi, seedDev, (uint32_t)g, w.vs, per, w.pnode, w.leafXY,
+            w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag);
         if (timed) (void)hipEventRecord(e1, st);
     }
     hipError_t e = hipGetLastError();

@@ -183,6 +183,8 @@ def check_errors():
     """Synchronising check of the last sample() call's device error flag."""
     if last_error is not None:
         e = int(last_error.item())
+        if e & 8:
+            raise RuntimeError("sampling.samplePoisson: a pixel's tree walk exceeded its node bound")
         if e & 4:
             raise RuntimeError("sampling.samplePoisson: a pixel drew more than numSamples splats (raise "
                                "numSamples or lower the rate)")
