@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 RATE, KCAP = 10.0, 40
 # HBM bytes per launch, measured offline (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # kernel, each scaled by the same counter on a 1 GiB device copy, bench.py --calibrate-copy)
-TRAFFIC_BYTES = {"k_sample": 3.374e9, "k_fmm_eval": None}
+TRAFFIC_BYTES = {"k_sample": 3.374e9, "k_fmm_eval": 2.849e8}
 
 
 def build_problem(dev, n_override=None):
@@ -333,7 +333,7 @@ def batched_views(dev, steps, warmup, n_splats=2_000_000, views=16):
         fr.step(0, seed_tensor=key, with_loss=False)
     for _ in range(warmup):
         graph.replay()
-    graph_ms, fe_ms, fe_n = timed_steps(graph.replay, steps, "fmm_eval")
+    graph_ms, _, _ = timed_steps(graph.replay, steps)
     sampling.check_errors()
     for k, c in saved.items():
         sampling.set_tile_config(k, *c)
@@ -344,7 +344,7 @@ def batched_views(dev, steps, warmup, n_splats=2_000_000, views=16):
             "frames_per_s": 1000.0 / best, "views_per_s": views * 1000.0 / best,
             "graph_ms_per_step": graph_ms, "eager_ms_per_step": eager_ms,
             "sampling_ms": {"one_launch_per_pass": bat_ms, "view_by_view": seq_ms},
-            "fmm_eval_ms_per_launch": fe_ms / max(fe_n, 1), "autotune": tuning,
+            "autotune": tuning,
             "memory_gb": torch.cuda.max_memory_allocated() / 1e9}
 
 
@@ -393,12 +393,12 @@ def main():
     _, po_ms, po_n = timed_steps(step, max(args.steps // 4, 1), "poisson")
     fps = 1000.0 / ms_per_step
     # dominant kernel: the fixed-order density evaluation (one launch per frame, both
-    # bands).  Its algorithmic bytes per launch, per band: every node record it can reach
-    # (32 B geometry + 16 B links + 4 B expansion offset, 2n - 1 nodes for n splats in
-    # view) and one 4 B density per pixel; the expansion coefficients it reads are not
-    # counted (their number is only known on the device), so this is a lower bound
+    # bands).  Its algorithmic bytes per launch, per band: the children record of every
+    # internal node (80 B, n - 1 of them for n splats in view) and one 4 B density per
+    # pixel; the expansion coefficients it reads (196 B per accepted node, a count only
+    # the device knows) are not counted, so this is a lower bound
     n_vis = int((L.splat_tiles(fr) >= 0).sum().item())
-    fe_bytes = 2 * (52 * (2 * n_vis - 1) + 4 * sc.width * sc.height)
+    fe_bytes = 2 * (80 * (n_vis - 1) + 4 * sc.width * sc.height)
     fe_avg = fe_ms / max(fe_n, 1)
     achieved = fe_bytes / (fe_avg * 1e-3) / 1e9
     line = {
