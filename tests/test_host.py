@@ -152,3 +152,72 @@ def test_bench_algorithmic_bytes():
     b = bench.algorithmic_bytes(1_000_000, 512 * 512, 40)
     assert b["sample"] == 32 * 1_000_000 + 4 * 512 * 512 * 40
     assert all(v > 0 for v in b.values())
+
+
+# ------------------------------------------------------------------ box room and cameras
+def test_box_room_surfels_lie_on_faces():
+    from splatting_amd import scenes
+    p, n, s, a = scenes.box_room(20_000, n_boxes=12, seed=5)
+    assert p.shape == (20_000, 3) and n.shape == (20_000, 3) and s.shape == (20_000, 2) and a.shape == (20_000, 3)
+    assert np.all(np.abs(np.linalg.norm(n, axis=1) - 1) < 1e-6)
+    assert np.all((np.abs(n) == 1).sum(1) == 1)                   # axis-aligned normals
+    assert np.all((s >= 0.01) & (s <= 0.03))
+    lo, hi = scenes.room_boxes(12, seed=5)
+    ax = np.argmax(np.abs(n), axis=1)
+    on_face = np.zeros(len(p), bool)
+    room = np.array([8.0, 6.0, 3.0])
+    for q in range(3):
+        sel = ax == q
+        # room walls: normal points inward
+        on_face |= sel & (np.isclose(p[:, q], 0.0, atol=1e-5) & (n[:, q] > 0))
+        on_face |= sel & (np.isclose(p[:, q], room[q], atol=1e-4) & (n[:, q] < 0))
+        for b in range(len(lo)):
+            inside = np.all((p >= lo[b] - 1e-4) & (p <= hi[b] + 1e-4), axis=1)
+            on_face |= sel & inside & np.isclose(p[:, q], lo[b, q], atol=1e-4) & (n[:, q] < 0)
+            on_face |= sel & inside & np.isclose(p[:, q], hi[b, q], atol=1e-4) & (n[:, q] > 0)
+    assert on_face.all()
+
+
+def test_room_cameras_look_along_yaw_pitch_and_avoid_boxes():
+    from splatting_amd import scenes
+    M = scenes.look_matrix([1.0, 2.0, 1.5], 0.3, -0.2)
+    R = M[:3, :3].astype(np.float64)
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-6)
+    d = np.array([np.cos(-0.2) * np.cos(0.3), np.cos(-0.2) * np.sin(0.3), np.sin(-0.2)])
+    x = np.array([1.0, 2.0, 1.5]) + 2.0 * d                     # 2 m ahead of the camera
+    c = R @ x + M[:3, 3]
+    assert np.allclose(c, [0.0, 0.0, -2.0], atol=1e-5)
+    boxes = scenes.room_boxes(200)
+    V = scenes.room_cameras(16, boxes=boxes)
+    assert V.shape == (16, 4, 4)
+    centres = np.stack([-(m[:3, :3].T @ m[:3, 3]) for m in V.astype(np.float64)])
+    assert np.all((centres > 0.3 - 1e-4) & (centres < np.array([8.0, 6.0, 3.0]) - 0.3 + 1e-4))
+    for c in centres:
+        assert not np.any(np.all((c > boxes[0] - 0.3) & (c < boxes[1] + 0.3), axis=1))
+
+
+# ------------------------------------------------------------------ launch shapes and the tuner's cache
+def test_tile_config_roundtrip_and_autotune_cache(tmp_path):
+    from cudaExtensions import sampling
+    from splatting_amd import autotune
+    saved = {k: sampling.get_tile_config(k) for k in autotune.KERNELS}
+    try:
+        sampling.set_tile_config("poisson", 16, 4, 2, 1)
+        assert sampling.get_tile_config("poisson") == (16, 4, 2, 1)
+        with pytest.raises(RuntimeError):
+            sampling.set_tile_config("fmm_eval", 8, 4, 4)
+        assert autotune.workload_key(640, 480, 16, 2_000_000, 40) == "640x480_v16_n2000000_k40"
+        assert autotune.workload_key(512, 512, 1, 767_129, 40) == "512x512_v1_n770000_k40"
+        path = str(tmp_path / "tune.json")
+        res = {"fmm_eval": {"config": [4, 16, 8, 1], "ms": 1.0, "default_ms": 1.1, "candidates": 32},
+               "poisson": {"config": [32, 2, 1, 0], "ms": 2.0, "default_ms": 2.0, "candidates": 32}}
+        autotune.save(path, "k1", res)
+        assert autotune.load(path, "k1") == res and autotune.load(path, "k2") is None
+        calls = []
+        got = autotune.tune_cached(lambda: calls.append(1), path, "k1")
+        assert got == res and not calls                      # cached: nothing is run
+        assert sampling.get_tile_config("fmm_eval") == (4, 16, 8, 1)
+        assert sampling.get_tile_config("poisson") == (32, 2, 1, 0)
+    finally:
+        for k, c in saved.items():
+            sampling.set_tile_config(k, *c)
