@@ -66,3 +66,37 @@ def test_fmm_large_workload(dev):
     got = run_gpu(dev, sc.width, sc.height, sc.focal, p, nb, sd, 0.01)
     ref = O.fmm_weights_view(sc.width, sc.height, sc.focal, p, nb, sd, 0.0, 0.01)
     assert np.array_equal(got, ref)
+
+
+def test_fmm_large_workload_against_direct_sum(dev):
+    """1M surfels at 512x512: the GPU density at 4096 random pixels against the
+    FP64 direct Gaussian sum c sum_i exp(-|q - s_i|^2 / 2h^2) over every culled
+    splat (evaluated on the GPU in FP64): between the direct sum minus the mass of
+    the leaves beyond the query radius and the direct sum, up to the degree-6
+    expansion error of the accepted nodes (1e-4 of the largest density)."""
+    import math
+    sc, p, n, nb, sd = scene_inputs(1)
+    W, H = sc.width, sc.height
+    got = run_gpu(dev, W, H, sc.focal, p, nb, sd, 0.01)
+    T = O.Tree(W, H, sc.focal, p, nb, sd, 0.0, 0.01)
+    h, rad = float(T.h), float(T.radius)
+    c = 1.0 / (2.0 * math.pi * math.sqrt(h))
+    lv = torch.from_numpy(T.cen[T.nc - 1:].astype(np.float64)).to(dev)
+    rng = np.random.default_rng(4096)
+    q = np.stack([rng.integers(0, W, 4096), rng.integers(0, H, 4096)], 1)
+    qt = torch.from_numpy(q.astype(np.float64)).to(dev)
+    dens = torch.zeros(4096, dtype=torch.float64, device=dev)
+    far = torch.zeros_like(dens)
+    for s in range(0, lv.shape[0], 1 << 16):
+        d2 = ((qt[:, None, :] - lv[None, s:s + (1 << 16), :]) ** 2).sum(-1)
+        g = c * torch.exp(-d2 / (2 * h * h))
+        dens += g.sum(1)
+        far += (g * (d2 >= rad * rad)).sum(1)
+    dens, far = dens.cpu().numpy(), far.cpu().numpy()
+    img = got[q[:, 0], q[:, 1]].astype(np.float64)
+    tol = 1e-4 * dens.max()
+    print("direct-sum check: max over %.3g, max under-minus-far %.3g, max density %.3g, max far %.3g"
+          % ((img - dens).max(), (dens - far - img).max(), dens.max(), far.max()))
+    assert dens.max() > 1.0
+    assert np.all(img <= dens + tol), (img - dens).max()
+    assert np.all(img >= dens - far - tol), (dens - far - img).max()
