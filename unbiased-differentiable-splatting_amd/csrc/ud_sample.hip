@@ -1325,21 +1325,26 @@ __global__ void k_copy_err(const ViewState* vs, int* err)
 // ---------------------------------------------------------------------------
 
 // Internal nodes whose degree-6 truncation bound at band b meets the
-// precision carry an expansion: a 49-float block at fixOff[node].
+// precision, and whose parent's does not, carry an expansion: a 49-float
+// block at fixOff[node].
 __global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int b, const NodeGeo* __restrict__ geo,
                                                       const NodeLnk* __restrict__ lnk, int* fixOff, int* list)
 {
     const int n = vs->nc;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool acc = false;
-    if (t < n - 1) {
-        NodeGeo g = geo[t];
+    const float hb = vs->band[b].h, prec = vs->prec;
+    auto meets = [&](int nd) {
+        NodeGeo g = geo[nd];
         float mn = 0.0f;
         mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
         mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
-        float rr = mn / vs->band[b].h;
-        acc = hermite_error(rr, (float)lnk[t].count, kFixedDegree) <= vs->prec;
-    }
+        float rr = mn / hb;
+        return hermite_error(rr, (float)lnk[nd].count, kFixedDegree) <= prec;
+    };
+    bool acc = false;
+    // the walk from the root evaluates the first node on its path that meets the
+    // bound and skips its subtree, so only those nodes need an expansion
+    if (t < n - 1) acc = meets(t) && (t == 0 || !meets(lnk[t].parent));
     unsigned long long m = __ballot(acc);
     const int lane = threadIdx.x & 63;
     int base = 0;
