@@ -553,7 +553,8 @@ __global__ void k_interp_backward(long nslots, int W, int H, int k, int ab, int 
     for (int c = 0; c < C; c++) {
         float gc = g[(long)C * pix + c];
         acc += gc * attr[ao + c];
-        atomicAdd(gA + ao + c, gc * w);
+        const float v = gc * w;
+        if (v != 0.0f) atomicAdd(gA + ao + c, v);      // zero terms (occluded slots) add nothing
     }
     gW[t] = acc;
 }
