@@ -506,7 +506,7 @@ __global__ __launch_bounds__(256) void k_transform_backward(int affine, int bn, 
 // ---------------------------------------------------------------------------
 // Interpolation (interpolationCuda.cu:5-129).
 // ---------------------------------------------------------------------------
-__global__ void k_interp_forward(long npix, int W, int H, int k, int ab, int N, int C,
+__global__ void k_interp_forward(long npix, int W, int H, int k, int ab, int N, int C, int vec4,
                                  const int32_t* __restrict__ idx, const float* __restrict__ wts,
                                  const float* __restrict__ attr, float* __restrict__ out)
 {
@@ -516,12 +516,22 @@ __global__ void k_interp_forward(long npix, int W, int H, int k, int ab, int N, 
     const long abase = ab > 1 ? b * N : 0;
     if (C == 3) {
         float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
-        for (int s = 0; s < k; s++) {
-            int id = idx[pix * k + s];
-            if (id < 0) continue;
-            float w = wts[pix * k + s];
+        auto add = [&](int id, float w) {
+            if (id < 0) return;
             const float* a = attr + 3 * (abase + id);
             o0 += w * a[0]; o1 += w * a[1]; o2 += w * a[2];
+        };
+        if (vec4) {
+            // a pixel's k slots are contiguous and 16-byte aligned: read them four at a time
+            const int4* ip = (const int4*)(idx + pix * k);
+            const float4* wp = (const float4*)(wts + pix * k);
+            for (int s = 0; s < (k >> 2); s++) {
+                const int4 i4 = ip[s];
+                const float4 w4 = wp[s];
+                add(i4.x, w4.x); add(i4.y, w4.y); add(i4.z, w4.z); add(i4.w, w4.w);
+            }
+        } else {
+            for (int s = 0; s < k; s++) add(idx[pix * k + s], wts[pix * k + s]);
         }
         out[3 * pix] = o0; out[3 * pix + 1] = o1; out[3 * pix + 2] = o2;
         return;
@@ -846,7 +856,8 @@ int ude_interp_forward(int bn, int W, int H, int k, int ab, int N, int C, const 
 {
     if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0) return UDE_ERR_ARG;
     long npix = (long)bn * W * H;
-    k_interp_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, k, ab, N, C, idx, wts, attr, out);
+    const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)wts % 16 == 0);
+    k_interp_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, k, ab, N, C, vec4, idx, wts, attr, out);
     return launch_status();
 }
 
