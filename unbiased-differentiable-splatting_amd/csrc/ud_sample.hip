@@ -716,13 +716,27 @@ __global__ __launch_bounds__(256) void k_bottom_up_global(const ViewState* __res
     }
 }
 
-__global__ __launch_bounds__(256) void k_coef_sizes(ViewState* vs, const NodeGeo* __restrict__ geo, int* sizes,
-                                                    int* momList)
+// Coefficient block sizes.  FGT walks start at the root and at the children of
+// nodes with more than k sources (the capacity descent asks for their weights);
+// a walk stops at the first node with an expansion.  An internal node below an
+// expanded node, with no start in between, is therefore never evaluated and
+// gets no block.
+__global__ __launch_bounds__(256) void k_coef_sizes(ViewState* vs, const NodeGeo* __restrict__ geo,
+                                                    const NodeLnk* __restrict__ lnk, int k, int* sizes, int* momList)
 {
     int n = vs->nc;
     int nn = 2 * n - 1;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int d = t < nn ? geo[t].deg : -1;
+    if (d >= 0 && t < n - 1) {
+        int x = t;
+        while (true) {
+            const int p = lnk[x].parent;
+            if (p < 0 || lnk[p].count > k) break;           // x starts a walk
+            if (geo[p].deg >= 0) { d = -1; break; }          // every walk stops at p
+            x = p;
+        }
+    }
     if (t < nn) sizes[t] = d >= 0 ? (d + 1) * (d + 1) : 0;
     bool want = d >= 0 && t < n - 1;
     unsigned long long m = __ballot(want);
@@ -2015,10 +2029,10 @@ int build_structure(int N, int W, int H, double f, double smoothing, double prec
 }
 
 // Dynamic-degree Hermite moments of the sampler's tree.
-int build_moments(int N, const TreeWs& w, hipStream_t st)
+int build_moments(int N, int k, const TreeWs& w, hipStream_t st)
 {
     long nScan = (2L * N + kScanTile - 1) / kScanTile;
-    k_coef_sizes<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.geo, w.scanTmp, w.momList);
+    k_coef_sizes<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, k, w.scanTmp, w.momList);
     k_scan_tiles<<<(unsigned)nScan, 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks);
     k_scan_small<<<1, 1024, 0, st>>>(w.scanBlocks, (int)nScan, &w.vs->ncoef);
     k_scan_apply<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks, w.aux);
@@ -2146,7 +2160,7 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
         const float* nb = nrm + 3L * N * b;
         const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
-        if (rc == UDE_OK) rc = build_moments(N, w, st);
+        if (rc == UDE_OK) rc = build_moments(N, k, w, st);
         if (rc != UDE_OK) return rc;
         int32_t* ob = out + perView * b;
         if (k <= 16) launch_sample<16>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
