@@ -246,8 +246,22 @@ __device__ float iso_h(float Wf, float Hf, float ff, const float* p, const float
     return fminf(1.0f, iso_hraw(Wf, Hf, ff, p, n, sd, smoothing));
 }
 
-__global__ void k_zero_state(ViewState* vs)
+// Tree-build kernels run over a batch of views at once, view = blockIdx.y: the
+// views' workspaces lie ws bytes apart, their points / normals pts bytes apart
+// and their std-devs sd bytes apart (0 when shared).  A single view passes {}.
+struct VStride {
+    size_t ws, pts, sd;
+};
+
+template <class T>
+__device__ __forceinline__ T* vw(T* p, size_t stride)
 {
+    return p ? (T*)((char*)p + (size_t)blockIdx.y * stride) : p;
+}
+
+__global__ void k_zero_state(ViewState* vs, VStride vsd)
+{
+    vs = vw(vs, vsd.ws);
     if (threadIdx.x == 0) {
         vs->hsum = 0; vs->hcnt = 0; vs->nc = 0; vs->err = 0; vs->ncoef = 0; vs->nseeds = 0; vs->nmom = 0;
         vs->nfix[0] = 0; vs->nfix[1] = 0;
@@ -257,8 +271,13 @@ __global__ void k_zero_state(ViewState* vs)
 
 __global__ __launch_bounds__(256) void k_bandwidth(int N, float Wf, float Hf, float ff, float smoothing,
                                                    const float* __restrict__ pts, const float* __restrict__ nrm,
-                                                   const float* __restrict__ sd, ViewState* vs, float* hraw)
+                                                   const float* __restrict__ sd, ViewState* vs, float* hraw, VStride vsd)
 {
+    pts = vw(pts, vsd.pts);
+    nrm = vw(nrm, vsd.pts);
+    sd = vw(sd, vsd.sd);
+    vs = vw(vs, vsd.ws);
+    hraw = vw(hraw, vsd.ws);
     __shared__ unsigned long long ssum[4];
     __shared__ unsigned int scnt[4];
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -300,8 +319,9 @@ __device__ Band make_band(float h, double prec)
     return b;
 }
 
-__global__ void k_view_params(ViewState* vs, float Wf, float Hf, double prec)
+__global__ void k_view_params(ViewState* vs, float Wf, float Hf, double prec, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
     if (threadIdx.x != 0) return;
     double hd = ((double)vs->hsum * (1.0 / 1099511627776.0)) / (double)vs->hcnt;
     float h = (float)hd;
@@ -359,8 +379,11 @@ __device__ __forceinline__ int block_excl_scan(int v, int* sh /*[4]*/, int& tota
 
 __global__ __launch_bounds__(256) void k_cull_count(int N, float Wf, float Hf, float ff,
                                                     const float* __restrict__ pts, const ViewState* __restrict__ vs,
-                                                    int* __restrict__ counts)
+                                                    int* __restrict__ counts, VStride vsd)
 {
+    pts = vw(pts, vsd.pts);
+    vs = vw(vs, vsd.ws);
+    counts = vw(counts, vsd.ws);
     __shared__ int sh[4];
     ViewState v = *vs;
     int base = blockIdx.x * kCullTile;
@@ -376,8 +399,10 @@ __global__ __launch_bounds__(256) void k_cull_count(int N, float Wf, float Hf, f
 }
 
 // single-workgroup exclusive scan of n ints in place; total to *totalOut
-__global__ __launch_bounds__(1024) void k_scan_small(int* data, int n, int* totalOut)
+__global__ __launch_bounds__(1024) void k_scan_small(int* data, int n, int* totalOut, VStride vsd)
 {
+    data = vw(data, vsd.ws);
+    totalOut = vw(totalOut, vsd.ws);
     __shared__ int sh[16];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
@@ -405,8 +430,13 @@ __global__ __launch_bounds__(1024) void k_scan_small(int* data, int n, int* tota
 __global__ __launch_bounds__(256) void k_cull_scatter(int N, float Wf, float Hf, float ff,
                                                       const float* __restrict__ pts, const ViewState* __restrict__ vs,
                                                       const int* __restrict__ offs, uint32_t* __restrict__ keys,
-                                                      int32_t* __restrict__ ids)
+                                                      int32_t* __restrict__ ids, VStride vsd)
 {
+    pts = vw(pts, vsd.pts);
+    vs = vw(vs, vsd.ws);
+    offs = vw(offs, vsd.ws);
+    keys = vw(keys, vsd.ws);
+    ids = vw(ids, vsd.ws);
     __shared__ int sh[4];
     ViewState v = *vs;
     int base = blockIdx.x * kCullTile;
@@ -431,8 +461,11 @@ __global__ __launch_bounds__(256) void k_cull_scatter(int N, float Wf, float Hf,
 // LSD radix sort, 8-bit digits.  Histogram -> digit-major scan -> stable scatter.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sort_hist(const ViewState* __restrict__ vs, const uint32_t* __restrict__ keys,
-                                                   int shift, int nblk, int* __restrict__ hist)
+                                                   int shift, int nblk, int* __restrict__ hist, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    keys = vw(keys, vsd.ws);
+    hist = vw(hist, vsd.ws);
     __shared__ int h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -448,8 +481,10 @@ __global__ __launch_bounds__(256) void k_sort_hist(const ViewState* __restrict__
 
 // exclusive scan of hist[d][0..nblk) for each digit d (one workgroup per digit);
 // digit totals to tot[d]
-__global__ __launch_bounds__(256) void k_sort_scan_digits(int* hist, int nblk, int* tot)
+__global__ __launch_bounds__(256) void k_sort_scan_digits(int* hist, int nblk, int* tot, VStride vsd)
 {
+    hist = vw(hist, vsd.ws);
+    tot = vw(tot, vsd.ws);
     __shared__ int sh[4];
     __shared__ int carry;
     int* h = hist + (long)blockIdx.x * nblk;
@@ -470,8 +505,9 @@ __global__ __launch_bounds__(256) void k_sort_scan_digits(int* hist, int nblk, i
 }
 
 // digit bases: exclusive scan of the 256 digit totals (in place)
-__global__ __launch_bounds__(256) void k_sort_scan_top(int* tot)
+__global__ __launch_bounds__(256) void k_sort_scan_top(int* tot, VStride vsd)
 {
+    tot = vw(tot, vsd.ws);
     __shared__ int sh[4];
     int total;
     int v = tot[threadIdx.x];
@@ -482,8 +518,15 @@ __global__ __launch_bounds__(256) void k_sort_scan_top(int* tot)
 __global__ __launch_bounds__(256) void k_sort_scatter(const ViewState* __restrict__ vs, const uint32_t* __restrict__ kin,
                                                       const int32_t* __restrict__ iin, int shift, int nblk,
                                                       const int* __restrict__ hist, const int* __restrict__ digitBase,
-                                                      uint32_t* __restrict__ kout, int32_t* __restrict__ iout)
+                                                      uint32_t* __restrict__ kout, int32_t* __restrict__ iout, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    kin = vw(kin, vsd.ws);
+    iin = vw(iin, vsd.ws);
+    hist = vw(hist, vsd.ws);
+    digitBase = vw(digitBase, vsd.ws);
+    kout = vw(kout, vsd.ws);
+    iout = vw(iout, vsd.ws);
     __shared__ int run[256];
     __shared__ int wc[4][256];
     int n = vs->nc;
@@ -544,8 +587,16 @@ __global__ __launch_bounds__(256) void k_hierarchy(float Wf, float Hf, float ff,
                                                    const ViewState* __restrict__ vs, const uint32_t* __restrict__ keys,
                                                    const int32_t* __restrict__ ids, NodeGeo* __restrict__ geo,
                                                    NodeLnk* __restrict__ lnk, NodeAux* __restrict__ aux,
-                                                   float2* __restrict__ leafXY)
+                                                   float2* __restrict__ leafXY, VStride vsd)
 {
+    pts = vw(pts, vsd.pts);
+    vs = vw(vs, vsd.ws);
+    keys = vw(keys, vsd.ws);
+    ids = vw(ids, vsd.ws);
+    geo = vw(geo, vsd.ws);
+    lnk = vw(lnk, vsd.ws);
+    aux = vw(aux, vsd.ws);
+    leafXY = vw(leafXY, vsd.ws);
     int n = vs->nc;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -621,8 +672,10 @@ __device__ float hermite_error(float r, float nr, int p)
     return a * (c + d);
 }
 
-__global__ __launch_bounds__(256) void k_fill_i32(int* p, const ViewState* __restrict__ vs, int mode, int v)
+__global__ __launch_bounds__(256) void k_fill_i32(int* p, const ViewState* __restrict__ vs, int mode, int v, VStride vsd)
 {
+    p = vw(p, vsd.ws);
+    vs = vw(vs, vsd.ws);
     // mode 0: n = nc - 1 (fan-in flags)
     int n = vs->nc - 1;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -670,8 +723,14 @@ __device__ __forceinline__ void merge_node(int p, float h, float prec, NodeGeo* 
 // boundary hands its (complete) node to level 2.
 constexpr int kFanBlock = 1024;
 __global__ __launch_bounds__(kFanBlock) void k_bottom_up_local(ViewState* vs, NodeGeo* geo, NodeLnk* lnk,
-                                                               const NodeAux* __restrict__ aux, int* flags, int* seeds)
+                                                               const NodeAux* __restrict__ aux, int* flags, int* seeds, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    geo = vw(geo, vsd.ws);
+    lnk = vw(lnk, vsd.ws);
+    aux = vw(aux, vsd.ws);
+    flags = vw(flags, vsd.ws);
+    seeds = vw(seeds, vsd.ws);
     int n = vs->nc;
     int b0 = blockIdx.x * kFanBlock;
     int pos = b0 + threadIdx.x;
@@ -698,8 +757,13 @@ __global__ __launch_bounds__(kFanBlock) void k_bottom_up_local(ViewState* vs, No
 
 // Bottom-up pass, level 2: the few block-spanning nodes, device-scope fan-in.
 __global__ __launch_bounds__(256) void k_bottom_up_global(const ViewState* __restrict__ vs, NodeGeo* geo, NodeLnk* lnk,
-                                                          int* flags, const int* __restrict__ seeds)
+                                                          int* flags, const int* __restrict__ seeds, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    geo = vw(geo, vsd.ws);
+    lnk = vw(lnk, vsd.ws);
+    flags = vw(flags, vsd.ws);
+    seeds = vw(seeds, vsd.ws);
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= vs->nseeds) return;
     float h = vs->h, prec = vs->prec;
@@ -1306,8 +1370,11 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
 }
 
 // leaf splat ids into NodeAux.pad (so the sampler reads one record per leaf)
-__global__ __launch_bounds__(256) void k_leaf_ids(const ViewState* __restrict__ vs, const int32_t* __restrict__ ids, NodeAux* aux)
+__global__ __launch_bounds__(256) void k_leaf_ids(const ViewState* __restrict__ vs, const int32_t* __restrict__ ids, NodeAux* aux, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    ids = vw(ids, vsd.ws);
+    aux = vw(aux, vsd.ws);
     int n = vs->nc;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) aux[n - 1 + t].pad = ids[t];
@@ -1342,8 +1409,13 @@ __global__ void k_copy_err(const ViewState* vs, int* err)
 // precision, and whose parent's does not, carry an expansion: a 49-float
 // block at fixOff[node].
 __global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int b, const NodeGeo* __restrict__ geo,
-                                                      const NodeLnk* __restrict__ lnk, int* fixOff, int* list)
+                                                      const NodeLnk* __restrict__ lnk, int* fixOff, int* list, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    geo = vw(geo, vsd.ws);
+    lnk = vw(lnk, vsd.ws);
+    fixOff = vw(fixOff, vsd.ws);
+    list = vw(list, vsd.ws);
     const int n = vs->nc;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const float hb = vs->band[b].h, prec = vs->prec;
@@ -1381,8 +1453,14 @@ constexpr int kFixedNodesPerBlock = 36;
 __global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restrict__ vs, int b,
                                                        const NodeGeo* __restrict__ geo, const NodeAux* __restrict__ aux,
                                                        const int* __restrict__ list, const int* __restrict__ fixOff,
-                                                       float* __restrict__ coef)
+                                                       float* __restrict__ coef, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    geo = vw(geo, vsd.ws);
+    aux = vw(aux, vsd.ws);
+    list = vw(list, vsd.ws);
+    fixOff = vw(fixOff, vsd.ws);
+    coef = vw(coef, vsd.ws);
     const int t = threadIdx.x;
     if (t >= kFixedNodesPerBlock * (kFixedDegree + 1)) return;
     const long slot = (long)blockIdx.x * kFixedNodesPerBlock + t / (kFixedDegree + 1);
@@ -1476,11 +1554,6 @@ __device__ __forceinline__ bool tile_lane(const TileCfg& c, int W, int H, int la
     return true;
 }
 
-template <class T>
-__device__ __forceinline__ T* view_ptr(T* p, size_t stride)
-{
-    return (T*)((char*)p + (size_t)blockIdx.y * stride);
-}
 
 constexpr int kFmmStack = 96;
 
@@ -1489,8 +1562,14 @@ constexpr int kFmmStack = 96;
 __global__ __launch_bounds__(256) void k_fmm_records(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
                                                      const NodeLnk* __restrict__ lnk,
                                                      const int* __restrict__ fixOff0,
-                                                     const int* __restrict__ fixOff1, FmmRec* __restrict__ rec)
+                                                     const int* __restrict__ fixOff1, FmmRec* __restrict__ rec, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    geo = vw(geo, vsd.ws);
+    lnk = vw(lnk, vsd.ws);
+    fixOff0 = vw(fixOff0, vsd.ws);
+    fixOff1 = vw(fixOff1, vsd.ws);
+    rec = vw(rec, vsd.ws);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int L0 = vs->nc - 1;
     if (p >= L0) return;
@@ -1671,8 +1750,11 @@ constexpr float kProposalPMin = 1e-4f;   // support: pairs whose rate-free propo
 // histogram of the visible splats' proposal bandwidths (1/16 px bins, the
 // last bin holds everything >= 16; degenerate splats are skipped)
 __global__ __launch_bounds__(256) void k_wide_hist(ViewState* vs, const int32_t* __restrict__ ids,
-                                                   const float* __restrict__ hraw)
+                                                   const float* __restrict__ hraw, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    ids = vw(ids, vsd.ws);
+    hraw = vw(hraw, vsd.ws);
     __shared__ int sh[256];
     sh[threadIdx.x] = 0;
     __syncthreads();
@@ -1686,8 +1768,9 @@ __global__ __launch_bounds__(256) void k_wide_hist(ViewState* vs, const int32_t*
 }
 
 // wide band: h_1 = max(h, 99.9th percentile of the histogram, rounded up to 1/16 px)
-__global__ void k_wide_band(ViewState* vs)
+__global__ void k_wide_band(ViewState* vs, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
     if (threadIdx.x != 0) return;
     long total = 0;
     for (int b = 0; b < 256; b++) total += vs->hhist[b];
@@ -1716,8 +1799,13 @@ __device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float
 // its first ancestor-or-self that is a left child, -1 on the rightmost path
 __global__ __launch_bounds__(256) void k_pois_nodes(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
                                                     const NodeLnk* __restrict__ lnk, const NodeAux* __restrict__ aux,
-                                                    PNode* __restrict__ pnode)
+                                                    PNode* __restrict__ pnode, VStride vsd)
 {
+    vs = vw(vs, vsd.ws);
+    geo = vw(geo, vsd.ws);
+    lnk = vw(lnk, vsd.ws);
+    aux = vw(aux, vsd.ws);
+    pnode = vw(pnode, vsd.ws);
     const int nd = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = vs->nc;
     if (nc <= 0 || nd >= 2 * nc - 1) return;
@@ -1769,12 +1857,12 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         seedLo = (uint32_t)sd;
         seedHi = (uint32_t)(sd >> 32);
     }
-    vs = view_ptr(vs, vstride);
-    pnode = view_ptr(pnode, vstride);
-    leafXY = view_ptr(leafXY, vstride);
-    leafId = view_ptr(leafId, vstride);
-    dens0 = view_ptr(dens0, densStride);
-    dens1 = view_ptr(dens1, densStride);
+    vs = vw(vs, vstride);
+    pnode = vw(pnode, vstride);
+    leafXY = vw(leafXY, vstride);
+    leafId = vw(leafId, vstride);
+    dens0 = vw(dens0, densStride);
+    dens1 = vw(dens1, densStride);
     const long perView = (long)W * H;
     outIds += (long)blockIdx.y * perView * kcap;
     outIpi += (long)blockIdx.y * perView * kcap;
@@ -1996,34 +2084,34 @@ inline unsigned grid1(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 // Tree structure of one view: bandwidth, cull, sort, hierarchy, bottom-up
 // boxes / centres / counts / dynamic degrees.
 int build_structure(int N, int W, int H, double f, double smoothing, double prec, const float* pts, const float* nrm,
-                    const float* sd, const TreeWs& w, hipStream_t st)
+                    const float* sd, const TreeWs& w, hipStream_t st, int views = 1, VStride vsd = VStride{})
 {
     float Wf = (float)W, Hf = (float)H, ff = (float)f;
     long nCull = (N + kCullTile - 1) / kCullTile;
     long nSort = (N + kSortTile - 1) / kSortTile;
-    k_zero_state<<<1, 64, 0, st>>>(w.vs);
-    k_bandwidth<<<grid1(N, 256), 256, 0, st>>>(N, Wf, Hf, ff, (float)smoothing, pts, nrm, sd, w.vs, w.hraw);
-    k_view_params<<<1, 64, 0, st>>>(w.vs, Wf, Hf, prec);
-    k_cull_count<<<(unsigned)nCull, 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts);
-    k_scan_small<<<1, 1024, 0, st>>>(w.cullCounts, (int)nCull, &w.vs->nc);
-    k_cull_scatter<<<(unsigned)nCull, 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts, w.keysA, w.idsA);
+    k_zero_state<<<dim3(1, views), 64, 0, st>>>(w.vs, vsd);
+    k_bandwidth<<<dim3(grid1(N, 256), views), 256, 0, st>>>(N, Wf, Hf, ff, (float)smoothing, pts, nrm, sd, w.vs, w.hraw, vsd);
+    k_view_params<<<dim3(1, views), 64, 0, st>>>(w.vs, Wf, Hf, prec, vsd);
+    k_cull_count<<<dim3((unsigned)nCull, views), 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts, vsd);
+    k_scan_small<<<dim3(1, views), 1024, 0, st>>>(w.cullCounts, (int)nCull, &w.vs->nc, vsd);
+    k_cull_scatter<<<dim3((unsigned)nCull, views), 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts, w.keysA, w.idsA, vsd);
     uint32_t* kin = w.keysA; uint32_t* kout = w.keysB;
     int32_t* iin = w.idsA; int32_t* iout = w.idsB;
     for (int shift = 0; shift < 32; shift += 8) {
-        k_sort_hist<<<(unsigned)nSort, 256, 0, st>>>(w.vs, kin, shift, (int)nSort, w.sortHist);
-        k_sort_scan_digits<<<256, 256, 0, st>>>(w.sortHist, (int)nSort, w.scanBlocks);
-        k_sort_scan_top<<<1, 256, 0, st>>>(w.scanBlocks);
-        k_sort_scatter<<<(unsigned)nSort, 256, 0, st>>>(w.vs, kin, iin, shift, (int)nSort, w.sortHist, w.scanBlocks,
-                                                        kout, iout);
+        k_sort_hist<<<dim3((unsigned)nSort, views), 256, 0, st>>>(w.vs, kin, shift, (int)nSort, w.sortHist, vsd);
+        k_sort_scan_digits<<<dim3(256, views), 256, 0, st>>>(w.sortHist, (int)nSort, w.scanBlocks, vsd);
+        k_sort_scan_top<<<dim3(1, views), 256, 0, st>>>(w.scanBlocks, vsd);
+        k_sort_scatter<<<dim3((unsigned)nSort, views), 256, 0, st>>>(w.vs, kin, iin, shift, (int)nSort, w.sortHist, w.scanBlocks,
+                                                        kout, iout, vsd);
         uint32_t* tk = kin; kin = kout; kout = tk;
         int32_t* ti = iin; iin = iout; iout = ti;
     }
     // after 4 passes the sorted data is back in keysA / idsA
-    k_hierarchy<<<grid1(N, 256), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux, w.leafXY);
-    k_leaf_ids<<<grid1(N, 256), 256, 0, st>>>(w.vs, iin, w.aux);
-    k_fill_i32<<<grid1(N, 256), 256, 0, st>>>(w.flags, w.vs, 0, 0);
-    k_bottom_up_local<<<grid1(N, kFanBlock), kFanBlock, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.flags, w.seeds);
-    k_bottom_up_global<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.flags, w.seeds);
+    k_hierarchy<<<dim3(grid1(N, 256), views), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux, w.leafXY, vsd);
+    k_leaf_ids<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, iin, w.aux, vsd);
+    k_fill_i32<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.flags, w.vs, 0, 0, vsd);
+    k_bottom_up_local<<<dim3(grid1(N, kFanBlock), views), kFanBlock, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.flags, w.seeds, vsd);
+    k_bottom_up_global<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.flags, w.seeds, vsd);
     hipError_t e0 = hipGetLastError();
     return e0 == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
 }
@@ -2034,7 +2122,7 @@ int build_moments(int N, int k, const TreeWs& w, hipStream_t st)
     long nScan = (2L * N + kScanTile - 1) / kScanTile;
     k_coef_sizes<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, k, w.scanTmp, w.momList);
     k_scan_tiles<<<(unsigned)nScan, 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks);
-    k_scan_small<<<1, 1024, 0, st>>>(w.scanBlocks, (int)nScan, &w.vs->ncoef);
+    k_scan_small<<<1, 1024, 0, st>>>(w.scanBlocks, (int)nScan, &w.vs->ncoef, VStride{});
     k_scan_apply<<<grid1(2L * N, 256), 256, 0, st>>>(w.vs, w.scanTmp, w.scanBlocks, w.aux);
     k_check_coef<<<1, 64, 0, st>>>(w.vs, w.coefCap);
     // enough workgroups for max(N leaves / 256, N nodes / 15)
@@ -2108,11 +2196,11 @@ dim3 tile_grid(const TileCfg& c, int W, int H, int views)
 }
 
 // degree-6 expansions of band b on one view's tree
-void fixed_expansions(int N, int b, const TreeWs& w, hipStream_t st)
+void fixed_expansions(int N, int b, const TreeWs& w, hipStream_t st, int views = 1, VStride vsd = VStride{})
 {
-    k_fixed_select<<<grid1(N, 256), 256, 0, st>>>(w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixList[b]);
-    k_fixed_moments<<<grid1(N, kFixedNodesPerBlock), 256, 0, st>>>(w.vs, b, w.geo, w.aux, w.fixList[b], w.fixOff[b],
-                                                                    w.fixCoef[b]);
+    k_fixed_select<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixList[b], vsd);
+    k_fixed_moments<<<dim3(grid1(N, kFixedNodesPerBlock), views), 256, 0, st>>>(w.vs, b, w.geo, w.aux, w.fixList[b],
+                                                                                 w.fixOff[b], w.fixCoef[b], vsd);
 }
 
 // densities of bands 0 .. nbands-1 of `views` views in one launch: w is the
@@ -2200,7 +2288,7 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
         if (rc != UDE_OK) return rc;
         fixed_expansions(N, 0, w, st);
-        k_fmm_records<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], nullptr, w.frec);
+        k_fmm_records<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], nullptr, w.frec, VStride{});
         fmm_eval(W, H, 1, 1, w, 0, out + perView * b, 0, 0, st);
     }
     hipError_t e = hipGetLastError();
@@ -2236,24 +2324,22 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
     const uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
     const TileCfg pc = g_tile[1];
     for (int g = 0; g < bn; g += span) {
-        for (int v = 0; v < span; v++) {
-            const int b = g + v;
-            TreeWs wv;
-            carve((char*)ws + per * v, N, W, H, 0, kModePoisson, &wv);
-            const float* pb = pts + 3L * N * b;
-            const float* nb = nrm + 3L * N * b;
-            const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
-            int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, wv, st);
-            if (rc != UDE_OK) return rc;
-            k_wide_hist<<<grid1(N, 256), 256, 0, st>>>(wv.vs, wv.idsA, wv.hraw);
-            k_wide_band<<<1, 64, 0, st>>>(wv.vs);
-            k_pois_nodes<<<grid1(2L * N, 256), 256, 0, st>>>(wv.vs, wv.geo, wv.lnk, wv.aux, wv.pnode);
-            fixed_expansions(N, 0, wv, st);
-            fixed_expansions(N, 1, wv, st);
-            k_fmm_records<<<grid1(N, 256), 256, 0, st>>>(wv.vs, wv.geo, wv.lnk, wv.fixOff[0], wv.fixOff[1], wv.frec);
-        }
+        // every view of the group in one launch per build stage (view = blockIdx.y)
         TreeWs w;
         carve(ws, N, W, H, 0, kModePoisson, &w);
+        const VStride vsd{per, sizeof(float) * 3 * (size_t)N, sdBatch > 1 ? sizeof(float) * 2 * (size_t)N : 0};
+        const float* pb = pts + 3L * N * g;
+        const float* nb = nrm + 3L * N * g;
+        const float* sb = sd + (sdBatch > 1 ? 2L * N * g : 0);
+        int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st, span, vsd);
+        if (rc != UDE_OK) return rc;
+        k_wide_hist<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.idsA, w.hraw, vsd);
+        k_wide_band<<<dim3(1, span), 64, 0, st>>>(w.vs, vsd);
+        k_pois_nodes<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.pnode, vsd);
+        fixed_expansions(N, 0, w, st, span, vsd);
+        fixed_expansions(N, 1, w, st, span, vsd);
+        k_fmm_records<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], w.fixOff[1], w.frec,
+                                                                 vsd);
         float* d0 = densOut ? densOut + perView * 2 * g : w.dens[0];
         float* d1 = densOut ? d0 + perView : w.dens[1];
         const size_t dstride = densOut ? sizeof(float) * 2 * perView : per;
