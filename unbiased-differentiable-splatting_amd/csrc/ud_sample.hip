@@ -1408,41 +1408,58 @@ __global__ void k_copy_err(const ViewState* vs, int* err)
 // Internal nodes whose degree-6 truncation bound at band b meets the
 // precision, and whose parent's does not, carry an expansion: a 49-float
 // block at fixOff[node].
-__global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int b, const NodeGeo* __restrict__ geo,
-                                                      const NodeLnk* __restrict__ lnk, int* fixOff, int* list, VStride vsd)
+__global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int nbands, const NodeGeo* __restrict__ geo,
+                                                      const NodeLnk* __restrict__ lnk, int* fixOff0, int* fixOff1,
+                                                      int* list0, int* list1, VStride vsd)
 {
     vs = vw(vs, vsd.ws);
     geo = vw(geo, vsd.ws);
     lnk = vw(lnk, vsd.ws);
-    fixOff = vw(fixOff, vsd.ws);
-    list = vw(list, vsd.ws);
+    fixOff0 = vw(fixOff0, vsd.ws);
+    fixOff1 = vw(fixOff1, vsd.ws);
+    list0 = vw(list0, vsd.ws);
+    list1 = vw(list1, vsd.ws);
+    __shared__ int wcnt[2][4];
+    __shared__ int bbase[2];
     const int n = vs->nc;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const float hb = vs->band[b].h, prec = vs->prec;
-    auto meets = [&](int nd) {
-        NodeGeo g = geo[nd];
-        float mn = 0.0f;
-        mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
-        mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
-        float rr = mn / hb;
-        return hermite_error(rr, (float)lnk[nd].count, kFixedDegree) <= prec;
-    };
-    bool acc = false;
-    // the walk from the root evaluates the first node on its path that meets the
-    // bound and skips its subtree, so only those nodes need an expansion
-    if (t < n - 1) acc = meets(t) && (t == 0 || !meets(lnk[t].parent));
-    unsigned long long m = __ballot(acc);
-    const int lane = threadIdx.x & 63;
-    int base = 0;
-    if (m != 0ull) {
-        int leader = __ffsll((long long)m) - 1;
-        if (lane == leader) base = atomicAdd(&vs->nfix[b], __popcll(m));
-        base = __shfl(base, leader, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const float prec = vs->prec;
+    bool acc[2] = { false, false };
+    unsigned long long m[2] = { 0ull, 0ull };
+    for (int b = 0; b < nbands; b++) {
+        const float hb = vs->band[b].h;
+        auto meets = [&](int nd) {
+            NodeGeo g = geo[nd];
+            float mn = 0.0f;
+            mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
+            mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
+            float rr = mn / hb;
+            return hermite_error(rr, (float)lnk[nd].count, kFixedDegree) <= prec;
+        };
+        // the walk from the root evaluates the first node on its path that meets the
+        // bound and skips its subtree, so only those nodes need an expansion
+        if (t < n - 1) acc[b] = meets(t) && (t == 0 || !meets(lnk[t].parent));
+        m[b] = __ballot(acc[b]);
+        if (lane == 0) wcnt[b][wid] = __popcll(m[b]);
     }
-    if (t < n - 1) {
-        int slot = base + __popcll(m & ((1ull << lane) - 1ull));
-        fixOff[t] = acc ? slot * kFixedCoefs : -1;
-        if (acc) list[slot] = t;
+    __syncthreads();
+    // one slot reservation per band and workgroup
+    if (threadIdx.x < nbands) {
+        const int b = threadIdx.x;
+        const int tot = wcnt[b][0] + wcnt[b][1] + wcnt[b][2] + wcnt[b][3];
+        bbase[b] = tot ? atomicAdd(&vs->nfix[b], tot) : 0;
+    }
+    __syncthreads();
+    if (t >= n - 1) return;
+    for (int b = 0; b < nbands; b++) {
+        int base = bbase[b];
+        for (int q = 0; q < wid; q++) base += wcnt[b][q];
+        const int slot = base + __popcll(m[b] & ((1ull << lane) - 1ull));
+        int* fixOff = b ? fixOff1 : fixOff0;
+        int* list = b ? list1 : list0;
+        fixOff[t] = acc[b] ? slot * kFixedCoefs : -1;
+        if (acc[b]) list[slot] = t;
     }
 }
 
@@ -2195,12 +2212,16 @@ dim3 tile_grid(const TileCfg& c, int W, int H, int views)
     return dim3((unsigned)((ntiles + c.wpb - 1) / c.wpb), (unsigned)views);
 }
 
-// degree-6 expansions of band b on one view's tree
-void fixed_expansions(int N, int b, const TreeWs& w, hipStream_t st, int views = 1, VStride vsd = VStride{})
+// degree-6 expansions of bands 0 .. nbands-1 on one view's tree (or a batch)
+void fixed_expansions(int N, int nbands, const TreeWs& w, hipStream_t st, int views = 1, VStride vsd = VStride{})
 {
-    k_fixed_select<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, b, w.geo, w.lnk, w.fixOff[b], w.fixList[b], vsd);
-    k_fixed_moments<<<dim3(grid1(N, kFixedNodesPerBlock), views), 256, 0, st>>>(w.vs, b, w.geo, w.aux, w.fixList[b],
-                                                                                 w.fixOff[b], w.fixCoef[b], vsd);
+    k_fixed_select<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, nbands, w.geo, w.lnk, w.fixOff[0],
+                                                                nbands > 1 ? w.fixOff[1] : nullptr, w.fixList[0],
+                                                                nbands > 1 ? w.fixList[1] : nullptr, vsd);
+    for (int b = 0; b < nbands; b++)
+        k_fixed_moments<<<dim3(grid1(N, kFixedNodesPerBlock), views), 256, 0, st>>>(w.vs, b, w.geo, w.aux,
+                                                                                     w.fixList[b], w.fixOff[b],
+                                                                                     w.fixCoef[b], vsd);
 }
 
 // densities of bands 0 .. nbands-1 of `views` views in one launch: w is the
@@ -2287,7 +2308,7 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
         const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
         if (rc != UDE_OK) return rc;
-        fixed_expansions(N, 0, w, st);
+        fixed_expansions(N, 1, w, st);
         k_fmm_records<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], nullptr, w.frec, VStride{});
         fmm_eval(W, H, 1, 1, w, 0, out + perView * b, 0, 0, st);
     }
@@ -2336,8 +2357,7 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         k_wide_hist<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.idsA, w.hraw, vsd);
         k_wide_band<<<dim3(1, span), 64, 0, st>>>(w.vs, vsd);
         k_pois_nodes<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.pnode, vsd);
-        fixed_expansions(N, 0, w, st, span, vsd);
-        fixed_expansions(N, 1, w, st, span, vsd);
+        fixed_expansions(N, 2, w, st, span, vsd);
         k_fmm_records<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], w.fixOff[1], w.frec,
                                                                  vsd);
         float* d0 = densOut ? densOut + perView * 2 * g : w.dens[0];
