@@ -802,14 +802,21 @@ __global__ __launch_bounds__(256) void k_coef_sizes(ViewState* vs, const NodeGeo
         }
     }
     if (t < nn) sizes[t] = d >= 0 ? (d + 1) * (d + 1) : 0;
-    bool want = d >= 0 && t < n - 1;
-    unsigned long long m = __ballot(want);
-    if (m == 0ull) return;
-    const int lane = threadIdx.x & 63;
-    int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&vs->nmom, __popcll(m));
-    base = __shfl(base, leader, 64);
+    // list the internal nodes that need moments: one slot reservation per workgroup
+    __shared__ int wcnt[4];
+    __shared__ int bbase;
+    const bool want = d >= 0 && t < n - 1;
+    const unsigned long long m = __ballot(want);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) wcnt[wid] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        bbase = tot ? atomicAdd(&vs->nmom, tot) : 0;
+    }
+    __syncthreads();
+    int base = bbase;
+    for (int q = 0; q < wid; q++) base += wcnt[q];
     if (want) momList[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
 }
 
