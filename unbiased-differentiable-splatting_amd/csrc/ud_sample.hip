@@ -283,10 +283,12 @@ __global__ __launch_bounds__(256) void k_bandwidth(int N, float Wf, float Hf, fl
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long q = 0;
     unsigned int c = 0;
-    if (t < N) {
+    // grid-stride over the splats (a capped grid keeps the two device-wide atomics
+    // per workgroup few; the fixed-point sum does not depend on the grouping)
+    for (; t < N; t += gridDim.x * blockDim.x) {
         float hk = iso_h(Wf, Hf, ff, pts + 3L * t, nrm + 3L * t, sd + 2L * t, smoothing);
         if (hraw) hraw[t] = prop_h(Wf, Hf, ff, pts + 3L * t, nrm + 3L * t, sd + 2L * t, smoothing);
-        if (isfinite(hk)) { q = (unsigned long long)((double)hk * 1099511627776.0); c = 1; }
+        if (isfinite(hk)) { q += (unsigned long long)((double)hk * 1099511627776.0); c += 1; }
     }
     for (int o = 32; o > 0; o >>= 1) {
         q += __shfl_xor(q, o, 64);
@@ -1782,8 +1784,8 @@ __global__ __launch_bounds__(256) void k_wide_hist(ViewState* vs, const int32_t*
     __shared__ int sh[256];
     sh[threadIdx.x] = 0;
     __syncthreads();
-    int pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos < vs->nc) {
+    const int nc = vs->nc;
+    for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < nc; pos += gridDim.x * blockDim.x) {
         float hi = hraw[ids[pos]];
         if (hi >= 0.0f) atomicAdd(&sh[hi < kProposalHCap ? (int)(hi * 16.0f) : 255], 1);
     }
@@ -2114,7 +2116,9 @@ int build_structure(int N, int W, int H, double f, double smoothing, double prec
     long nCull = (N + kCullTile - 1) / kCullTile;
     long nSort = (N + kSortTile - 1) / kSortTile;
     k_zero_state<<<dim3(1, views), 64, 0, st>>>(w.vs, vsd);
-    k_bandwidth<<<dim3(grid1(N, 256), views), 256, 0, st>>>(N, Wf, Hf, ff, (float)smoothing, pts, nrm, sd, w.vs, w.hraw, vsd);
+    const unsigned bwBlocks = grid1(N, 256) < 512u ? grid1(N, 256) : 512u;
+    k_bandwidth<<<dim3(bwBlocks, views), 256, 0, st>>>(N, Wf, Hf, ff, (float)smoothing, pts, nrm, sd, w.vs, w.hraw,
+                                                       vsd);
     k_view_params<<<dim3(1, views), 64, 0, st>>>(w.vs, Wf, Hf, prec, vsd);
     k_cull_count<<<dim3((unsigned)nCull, views), 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts, vsd);
     k_scan_small<<<dim3(1, views), 1024, 0, st>>>(w.cullCounts, (int)nCull, &w.vs->nc, vsd);
@@ -2361,7 +2365,8 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         const float* sb = sd + (sdBatch > 1 ? 2L * N * g : 0);
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st, span, vsd);
         if (rc != UDE_OK) return rc;
-        k_wide_hist<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.idsA, w.hraw, vsd);
+        k_wide_hist<<<dim3(grid1(N, 256) < 256u ? grid1(N, 256) : 256u, span), 256, 0, st>>>(w.vs, w.idsA, w.hraw,
+                                                                                          vsd);
         k_wide_band<<<dim3(1, span), 64, 0, st>>>(w.vs, vsd);
         k_pois_nodes<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.pnode, vsd);
         fixed_expansions(N, 2, w, st, span, vsd);
