@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 RATE, KCAP = 10.0, 40
 # HBM bytes per launch, measured offline (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # kernel, each scaled by the same counter on a 1 GiB device copy, bench.py --calibrate-copy)
-TRAFFIC_BYTES = {"k_sample": 3.374e9, "k_fmm_eval": 2.849e8}
+TRAFFIC_BYTES = {"k_sample": 3.374e9, "k_fmm_eval": 2.849e8, "k_poisson": None}
 
 
 def build_problem(dev, n_override=None):
@@ -390,17 +390,30 @@ def main():
         step()
     print("bench: timed steps (%s rates)" % chosen, file=sys.stderr, flush=True)
     ms_per_step, fe_ms, fe_n = timed_steps(step, args.steps, "fmm_eval")
-    _, po_ms, po_n = timed_steps(step, max(args.steps // 4, 1), "poisson")
+    _, po_ms, po_n = timed_steps(step, max(args.steps // 2, 1), "poisson")
     fps = 1000.0 / ms_per_step
-    # dominant kernel: the fixed-order density evaluation (one launch per frame, both
-    # bands).  Its algorithmic bytes per launch, per band: the children record of every
-    # internal node (80 B, n - 1 of them for n splats in view) and one 4 B density per
-    # pixel; the expansion coefficients it reads (196 B per accepted node, a count only
-    # the device knows) are not counted, so this is a lower bound
+    # The two per-pixel passes dominate the frame; the roofline line is the slower one.
+    # Algorithmic bytes per launch, n = splats in view, P = pixels:
+    #   k_fmm_eval (both bands): per band the 80 B children record of each of the n - 1
+    #     internal nodes and one 4 B density per pixel; the expansion coefficients it
+    #     reads (196 B per accepted node, a count only the device knows) are not counted,
+    #     so this is a lower bound;
+    #   k_poisson: the 32 B walk record of each of the 2n - 1 nodes, 8 B position and
+    #     4 B id per leaf, two 4 B densities per pixel and 8 B (index, 1/pi) per slot.
     n_vis = int((L.splat_tiles(fr) >= 0).sum().item())
-    fe_bytes = 2 * (80 * (n_vis - 1) + 4 * sc.width * sc.height)
-    fe_avg = fe_ms / max(fe_n, 1)
-    achieved = fe_bytes / (fe_avg * 1e-3) / 1e9
+    P = sc.width * sc.height
+    kern = {
+        "k_fmm_eval": {"kernel_ms": fe_ms / max(fe_n, 1), "kernel_launches": fe_n,
+                       "algorithmic_bytes": 2 * (80 * (n_vis - 1) + 4 * P)},
+        "k_poisson": {"kernel_ms": po_ms / max(po_n, 1), "kernel_launches": po_n,
+                      "algorithmic_bytes": 32 * (2 * n_vis - 1) + 12 * n_vis + 8 * P + 8 * P * KCAP},
+    }
+    for name, k in kern.items():
+        k["achieved"] = k["algorithmic_bytes"] / (k["kernel_ms"] * 1e-3) / 1e9
+        k["traffic"] = TRAFFIC_BYTES[name]
+    dom = max(kern, key=lambda nm: kern[nm]["kernel_ms"])
+    other = [nm for nm in kern if nm != dom][0]
+    achieved = kern[dom]["achieved"]
     line = {
         "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -410,10 +423,11 @@ def main():
                                "variance" % (RATE, chosen),
                    "splats": sc.n, "splats_in_view": n_vis, "width": sc.width, "height": sc.height,
                    "rate": RATE, "kcap": KCAP, "rates": chosen, "parallelism": "none (1 GPU)"},
-        "roofline": {"bound": "hbm", "kernel": "k_fmm_eval", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["k_fmm_eval"],
-                     "kernel_ms": fe_avg, "kernel_launches": fe_n, "algorithmic_bytes": fe_bytes,
-                     "next_kernel": {"kernel": "k_poisson", "kernel_ms": po_ms / max(po_n, 1)}},
+        "roofline": dict({"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS},
+                         **{k_: v for k_, v in kern[dom].items() if k_ != "achieved"},
+                         next_kernel=dict({"kernel": other, "frac": kern[other]["achieved"] / HBM_PEAK_GBS},
+                                          **kern[other])),
         "rates": rates,
     }
     if not args.no_batched:

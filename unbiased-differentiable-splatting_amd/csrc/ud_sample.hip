@@ -1876,7 +1876,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
     const int lane = threadIdx.x & 63;
     int i, j;
     if (!tile_lane(cfg, W, H, lane, i, j)) return;           // wave-uniform
-    if (i >= W || j >= H) return;
+    const bool valid = i < W && j < H;
     const uint32_t view = view0 + blockIdx.y;
     if (seedDev) {
         const uint64_t sd = *seedDev;
@@ -1893,14 +1893,14 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
     outIds += (long)blockIdx.y * perView * kcap;
     outIpi += (long)blockIdx.y * perView * kcap;
     if (tscale) tscale += (long)blockIdx.y * ((W + 15) / 16) * tsH;
-    const long pix = (long)j + (long)H * i;
+    const long pix = valid ? (long)j + (long)H * i : 0;
     int32_t* oid = outIds + pix * kcap;
     float* oip = outIpi + pix * kcap;
     int ns = 0;
     bool over = false;
     const int n = vs->nc;
-    const float D0 = n > 0 ? dens0[pix] : 0.0f;
-    if (n > 0 && D0 >= vs->prec) {
+    const float D0 = (n > 0 && valid) ? dens0[pix] : 0.0f;
+    if (valid && n > 0 && D0 >= vs->prec) {
         const int L0 = n - 1;
         const Band B0 = vs->band[0], B1 = vs->band[1];
         float lam = rate;
@@ -1961,7 +1961,24 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
             if (nd < 0) break;
         }
     }
-    for (int s = ns; s < kcap; s++) { oid[s] = -1; oip[s] = 0.0f; }
+    // empty slots: a row of the tile (2^lth pixels) is contiguous in memory and is
+    // padded by the whole wavefront, so the stores coalesce
+    const int th = 1 << cfg.lth;
+    for (int r = 0; r < (64 >> cfg.lth); r++) {
+        const int l0 = r << cfg.lth;
+        const int ir = __shfl(i, l0, 64), jr = __shfl(j, l0, 64);
+        const int cnt = ir < W ? (H - jr < th ? H - jr : th) : 0;          // wave-uniform
+        if (cnt <= 0) continue;
+        int32_t* rid = outIds + ((long)jr + (long)H * ir) * kcap;
+        float* rip = outIpi + ((long)jr + (long)H * ir) * kcap;
+        const int total = cnt * kcap;
+        for (int e0 = 0; e0 < total; e0 += 64) {
+            const int e = e0 + lane;
+            const int c = e < total ? e / kcap : 0;
+            const int nsc = __shfl(ns, l0 + c, 64);
+            if (e < total && e - c * kcap >= nsc) { rid[e] = -1; rip[e] = 0.0f; }
+        }
+    }
     if (over) atomicOr(err, 4);
 }
 
