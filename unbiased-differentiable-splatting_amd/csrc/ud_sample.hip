@@ -98,6 +98,7 @@ struct TreeWs {
     int* seeds;
     int* momList;
     float* trie;
+    int2* reach;          // per launch and pixel lane: leaf range the capacity descent reached
     float2* leafXY;       // sorted position -> screen position of the leaf
     float* coef;
     long coefCap;
@@ -151,6 +152,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     w.momList = (int*)take(4 * n);
     long lanes = (long)((W + 7) / 8) * ((H + 7) / 8) * 64;
     w.trie = fixedOrder ? nullptr : (float*)take(16L * trie_entries(k) * lanes);
+    w.reach = fixedOrder ? nullptr : (int2*)take(8L * 16 * lanes);
     w.leafXY = (float2*)take(8 * n);
     w.coef = fixedOrder ? nullptr : (float*)take(4 * coefCap);
     w.coefCap = fixedOrder ? 0 : coefCap;
@@ -1142,13 +1144,10 @@ __device__ __forceinline__ void heap_replace_top(unsigned long long* list, int l
     list[i * 64 + lane] = x;
 }
 
-template <int KMAX>
-__global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint32_t seedLo, uint32_t seedHi,
-                                               uint32_t view, const ViewState* __restrict__ vs,
-                                               const NodeGeo* geo, const NodeLnk* lnk, const NodeAux* aux,
-                                               const float* coef, float* trieWs, int tcap,
-                                               const float2* __restrict__ leafXY, const int32_t* __restrict__ leafId,
-                                               int32_t* __restrict__ out, int* __restrict__ errOut)
+__global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, const ViewState* __restrict__ vs,
+                                                    const NodeGeo* geo, const NodeLnk* lnk, const NodeAux* aux,
+                                                    const float* coef, float* trieWs, int tcap,
+                                                    int2* __restrict__ reach, int* __restrict__ errOut)
 {
     TreeView T;
     T.geo = (const NodeGeo UD_GLOBAL*)geo;
@@ -1159,7 +1158,6 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
     T.is2h = vs->is2h;
     const int TMAX = tcap;
     const int DFSMAX = tcap - L * 32;                 // entries the walk may create
-    extern __shared__ unsigned long long list[];
     const int lane = threadIdx.x;
     TrieRef tr;
     tr.stride = (long)gridDim.x * 64;
@@ -1169,17 +1167,13 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
     const int ti = tile / tilesY, tj = tile - (tile / tilesY) * tilesY;
     const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
     const bool valid = (i < W) && (j < H);
-    // per-lane bounded max-heap of the k smallest (key, draw number) words; the
-    // padding word (key +inf, draw 0xffffffff) is larger than any real entry
-    const unsigned long long empty = ((unsigned long long)0x7f800000u << 32) | 0xffffffffull;
-    for (int s = 0; s < k; s++) list[s * 64 + lane] = empty;
-    int firstPos[16];                                   // per launch: first leaf position drawn
+    int2* rch = reach + (long)blockIdx.x * L * 64 + lane;   // [launch][lane] of this tile
+    int drawn = 0;                                      // launches whose leaf range is recorded
     const int n = vs->nc;
     const float prec = vs->prec;
     const bool runTree = n > 0 && vs->err == 0;          // wave-uniform
     const int L0 = n - 1;
     const float qx = (float)i, qy = (float)j;
-    const uint32_t pixel = (uint32_t)(j + H * i);
     const float unset = __builtin_nanf("");              // weight not computed yet
     // trie fields: cap, wl, wr, kids = (left entry + 1) | (right entry + 1) << 16
     int nt = 1;
@@ -1321,60 +1315,95 @@ __global__ __launch_bounds__(64) void k_sample(int W, int H, int k, int L, uint3
                     tr.set(ce, 0, fmaxf(0.0f, tr.get(ce, 0) - prob));
                     prob = prob * pp[dd];
                 }
-                // sequential Poisson keys over the leaves of the reached subtree: in
-                // a Karras tree they are the contiguous sorted range
-                // [first, first + count), in the left-to-right order nextLeaf visits
-                // (sequentialPoissonDynamicTree.cuh:16-53), and every leaf carries the
-                // single Hermite coefficient 1, so E(leaf) = e^{-tx^2} e^{-ty^2}.
+                // the reached subtree's leaves are the contiguous sorted range
+                // [first, first + count); k_sample_draw keys them
                 const int cnt = T.lnk[nd].count;
                 const int first = (nd >= L0) ? nd - L0 : T.aux[nd].first;
-                firstPos[la] = first;
-                for (int kk0 = 0; kk0 < cnt; kk0 += 4) {
-                    u32x4 c4 = { (uint32_t)(kk0 >> 2), (uint32_t)la, pixel, view };
-                    u32x4 rnd = philox4x32_10(c4, seedLo, seedHi);
-                    float2 xy[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        int pos = first + kk0 + q;
-                        bool ok = kk0 + q < cnt;
-                        xy[q] = ok ? leafXY[pos] : make_float2(0.0f, 0.0f);
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        if (kk0 + q >= cnt) break;
-                        float tx = (qx - xy[q].x) * T.is2h;
-                        float ty = (qy - xy[q].y) * T.is2h;
-                        float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
-                        float wgt = 0.0f;
-                        wgt = wgt + (1.0f * ex) * ey;
-                        float p = fmaxf(0.0f, wgt);
-                        uint32_t x = (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w;
-                        float key = u01(x) / p;
-                        if (!(key < __builtin_huge_valf())) continue;
-                        // (key, draw number) in one ordered word: keys are >= 0, so
-                        // their bit patterns order like the floats
-                        unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) |
-                                                 (uint32_t)((la << 16) | (kk0 + q));
-                        if (ent < list[lane]) heap_replace_top(list, lane, k, ent);
-                    }
-                }
+                rch[(long)la * 64] = make_int2(first, cnt);
+                drawn = la + 1;
             }
         }
         if (bad) atomicOr(errOut, 1);
     }
-    __builtin_amdgcn_wave_barrier();
-    if (valid) {
-        // heap -> ascending order (heapsort), then draw numbers -> splat ids
-        for (int end = k - 1; end > 0; end--) {
-            unsigned long long x = list[end * 64 + lane];
-            list[end * 64 + lane] = list[lane];
-            heap_replace_top(list, lane, end, x);
+    for (int la = drawn; la < L; la++) rch[(long)la * 64] = make_int2(0, 0);
+}
+
+// Sequential Poisson keys over the leaves each launch reached (the walk's
+// capacity descents do not depend on the keys, so they run first, in
+// k_sample_walk, at the occupancy their long dependent loads need): in a Karras
+// tree the leaves of a subtree are the contiguous sorted range [first, first +
+// count), in the left-to-right order nextLeaf visits
+// (sequentialPoissonDynamicTree.cuh:16-53), and every leaf carries the single
+// Hermite coefficient 1, so E(leaf) = e^{-tx^2} e^{-ty^2}.  Key u / E with u the
+// Philox word of (leaf ordinal, launch, pixel, view); the k smallest (key, draw
+// number) pairs in order are the sample, kept in a per-lane bounded max-heap in
+// LDS and heap-sorted at the end.
+__global__ __launch_bounds__(64) void k_sample_draw(int W, int H, int k, int L, uint32_t seedLo, uint32_t seedHi,
+                                                    uint32_t view, const ViewState* __restrict__ vs,
+                                                    const float2* __restrict__ leafXY,
+                                                    const int32_t* __restrict__ leafId,
+                                                    const int2* __restrict__ reach, int32_t* __restrict__ out)
+{
+    extern __shared__ unsigned long long list[];
+    const int lane = threadIdx.x;
+    const int tilesY = (H + 7) >> 3;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int ti = tile / tilesY, tj = tile - (tile / tilesY) * tilesY;
+    const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
+    const bool valid = (i < W) && (j < H);
+    if (!valid) return;
+    // the padding word (key +inf, draw 0xffffffff) is larger than any real entry
+    const unsigned long long empty = ((unsigned long long)0x7f800000u << 32) | 0xffffffffull;
+    for (int s = 0; s < k; s++) list[s * 64 + lane] = empty;
+    const float is2h = vs->is2h;
+    const float qx = (float)i, qy = (float)j;
+    const uint32_t pixel = (uint32_t)(j + H * i);
+    const int2* rch = reach + (long)blockIdx.x * L * 64 + lane;
+    int firstPos[16];                                   // per launch: first leaf position drawn
+    for (int la = 0; la < L; la++) {
+        const int2 fc = rch[(long)la * 64];
+        const int first = fc.x, cnt = fc.y;
+        firstPos[la] = first;
+        for (int kk0 = 0; kk0 < cnt; kk0 += 4) {
+            u32x4 c4 = { (uint32_t)(kk0 >> 2), (uint32_t)la, pixel, view };
+            u32x4 rnd = philox4x32_10(c4, seedLo, seedHi);
+            float2 xy[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int pos = first + kk0 + q;
+                bool ok = kk0 + q < cnt;
+                xy[q] = ok ? leafXY[pos] : make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (kk0 + q >= cnt) break;
+                float tx = (qx - xy[q].x) * is2h;
+                float ty = (qy - xy[q].y) * is2h;
+                float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+                float wgt = 0.0f;
+                wgt = wgt + (1.0f * ex) * ey;
+                float p = fmaxf(0.0f, wgt);
+                uint32_t x = (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w;
+                float key = u01(x) / p;
+                if (!(key < __builtin_huge_valf())) continue;
+                // (key, draw number) in one ordered word: keys are >= 0, so
+                // their bit patterns order like the floats
+                unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) |
+                                         (uint32_t)((la << 16) | (kk0 + q));
+                if (ent < list[lane]) heap_replace_top(list, lane, k, ent);
+            }
         }
-        int32_t* o = out + ((long)j + (long)H * i) * k;
-        for (int s = 0; s < k; s++) {
-            uint32_t dn = (uint32_t)(list[s * 64 + lane] & 0xffffffffull);
-            o[s] = dn == 0xffffffffu ? -1 : leafId[firstPos[dn >> 16] + (int)(dn & 0xffffu)];
-        }
+    }
+    // heap -> ascending order (heapsort), then draw numbers -> splat ids
+    for (int end = k - 1; end > 0; end--) {
+        unsigned long long x = list[end * 64 + lane];
+        list[end * 64 + lane] = list[lane];
+        heap_replace_top(list, lane, end, x);
+    }
+    int32_t* o = out + ((long)j + (long)H * i) * k;
+    for (int s = 0; s < k; s++) {
+        uint32_t dn = (uint32_t)(list[s * 64 + lane] & 0xffffffffull);
+        o[s] = dn == 0xffffffffu ? -1 : leafId[firstPos[dn >> 16] + (int)(dn & 0xffffu)];
     }
 }
 
@@ -2214,19 +2243,18 @@ bool timing_pair(int kernel, hipEvent_t* a, hipEvent_t* b)
     return true;
 }
 
-template <int KMAX>
 void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32_t view, const TreeWs& w,
                    int32_t* out, int* err, hipStream_t st)
 {
     unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
     size_t lds = (size_t)k * 64 * sizeof(unsigned long long);
+    const int L = num_launches(k);
     hipEvent_t e0, e1;
     bool timed = timing_pair(0, &e0, &e1);
     if (timed) (void)hipEventRecord(e0, st);
-    hipLaunchKernelGGL((k_sample<KMAX>), dim3(tiles), dim3(64), lds, st, W, H, k, num_launches(k), seedLo, seedHi,
-                       view, (const ViewState*)w.vs, (const NodeGeo*)w.geo, (const NodeLnk*)w.lnk,
-                       (const NodeAux*)w.aux, (const float*)w.coef, w.trie, trie_entries(k),
-                       (const float2*)w.leafXY, (const int32_t*)w.idsA, out, err);
+    k_sample_walk<<<tiles, 64, 0, st>>>(W, H, k, L, w.vs, w.geo, w.lnk, w.aux, w.coef, w.trie, trie_entries(k),
+                                        w.reach, err);
+    k_sample_draw<<<tiles, 64, lds, st>>>(W, H, k, L, seedLo, seedHi, view, w.vs, w.leafXY, w.idsA, w.reach, out);
     if (timed) (void)hipEventRecord(e1, st);
 }
 
@@ -2300,11 +2328,7 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
         if (rc == UDE_OK) rc = build_moments(N, k, w, st);
         if (rc != UDE_OK) return rc;
         int32_t* ob = out + perView * b;
-        if (k <= 16) launch_sample<16>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
-        else if (k <= 32) launch_sample<32>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
-        else if (k <= 64) launch_sample<64>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
-        else if (k <= 128) launch_sample<128>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
-        else launch_sample<256>(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
+        launch_sample(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
         k_copy_err<<<1, 64, 0, st>>>(w.vs, errFlag);
     }
     hipError_t e = hipGetLastError();
