@@ -1103,10 +1103,20 @@ __device__ __forceinline__ float key_of(unsigned long long v) { return __uint_as
 // entry index issues one coalesced access.
 // per-depth arrays of the current DFS path (private memory; every index is
 // wave-uniform)
+// Accumulators of the current DFS path, one per depth and lane: the first
+// kDepthLds depths in LDS ([depth][lane], indices are wave-uniform), deeper ones
+// (rare: a Karras tree is rarely more than ~25 levels deep above its k-source
+// nodes) in private memory.
+constexpr int kDepthLds = 28;
 struct DepthF {
-    float a[kMaxDepth];
-    __device__ float get(int i) const { return a[i]; }
-    __device__ void set(int i, float v) { a[i] = v; }
+    float* lds;                     // this wavefront's [kDepthLds][64] slice, offset by the lane
+    float a[kMaxDepth - kDepthLds];
+    __device__ float get(int i) const { return i < kDepthLds ? lds[i * 64] : a[i - kDepthLds]; }
+    __device__ void set(int i, float v)
+    {
+        if (i < kDepthLds) lds[i * 64] = v;
+        else a[i - kDepthLds] = v;
+    }
 };
 struct DepthI {
     int a[kMaxDepth];
@@ -1190,7 +1200,9 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
         __shared__ unsigned long long s_walk[kMaxDepth];
         __shared__ float s_coef[(kMaxDegree + 1) * (kMaxDegree + 1)];
         // per-depth accumulators and trie entries of the current path
+        __shared__ float s_pacc[kDepthLds * 64];
         DepthF pacc;
+        pacc.lds = s_pacc + lane;
         DepthI pent;
         int big = 0;                 // leading path nodes with more than k sources
         uint64_t rightBits = 0;      // bit d: node at depth d is its parent's right child
