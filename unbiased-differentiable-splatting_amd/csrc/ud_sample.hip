@@ -994,6 +994,32 @@ __device__ __forceinline__ float eval_node(const TreeView& T, const NodeGeo& g, 
 // lanes of a wavefront for the same node (uniform p and block), so the block
 // is fetched with one coalesced load per 64 coefficients and then read with
 // LDS broadcasts.  Lanes with act == false only help with the load.
+// the degree-P series sum_a sum_b c[a][b] h_a(tx) h_b(ty) with the loops of
+// eval_node unrolled exactly (same operation order)
+template <int P>
+__device__ __forceinline__ float hermite_series(const float* sc, float tx, float ty, float ex, float ey)
+{
+    float hy[P + 1];
+    hy[0] = ey;
+    if (P >= 1) hy[P >= 1 ? 1 : 0] = (2.0f * ty) * ey;
+#pragma unroll
+    for (int n = 2; n <= P; n++) hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
+    float w = 0.0f, hprev = 0.0f, ha = ex;
+#pragma unroll
+    for (int a = 0; a <= P; a++) {
+        if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
+        else if (a >= 2) {
+            float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+            hprev = ha; ha = hn;
+        }
+#pragma unroll
+        for (int b = 0; b <= P; b++) w = w + (sc[a * (P + 1) + b] * ha) * hy[b];
+    }
+    return w;
+}
+
+// eval_node for a wave-uniform node: the coefficient block is staged in LDS
+// (read by broadcast) and the series is unrolled for the node's degree
 __device__ __forceinline__ float eval_node_staged(const TreeView& T, const NodeGeo& g, int nd, float qx, float qy,
                                                   bool act, float* sc, int lane)
 {
@@ -1007,9 +1033,13 @@ __device__ __forceinline__ float eval_node_staged(const TreeView& T, const NodeG
         float tx = (qx - g.cx) * T.is2h;
         float ty = (qy - g.cy) * T.is2h;
         float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
-        if (p == 0) {
-            w = w + (sc[0] * ex) * ey;
-        } else {
+        switch (p) {
+#define UD_SERIES(P) case P: w = hermite_series<P>(sc, tx, ty, ex, ey); break;
+        UD_SERIES(0) UD_SERIES(1) UD_SERIES(2) UD_SERIES(3) UD_SERIES(4) UD_SERIES(5) UD_SERIES(6)
+        UD_SERIES(7) UD_SERIES(8)
+#undef UD_SERIES
+        default: {
+            // degrees 9 .. 16 (few nodes): loops to the maximum degree, predicated
             float hy[kMaxDegree + 1];
             hy[0] = ey;
             hy[1] = (2.0f * ty) * ey;
@@ -1028,6 +1058,7 @@ __device__ __forceinline__ float eval_node_staged(const TreeView& T, const NodeG
                 for (int b = 0; b <= kMaxDegree; b++)
                     if (b <= p) w = w + (row[b] * ha) * hy[b];
             }
+        }
         }
     }
     __builtin_amdgcn_wave_barrier();
