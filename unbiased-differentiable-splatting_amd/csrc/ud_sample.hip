@@ -1020,12 +1020,12 @@ __device__ __forceinline__ float hermite_series(const float* sc, float tx, float
 
 // eval_node for a wave-uniform node: the coefficient block is staged in LDS
 // (read by broadcast) and the series is unrolled for the node's degree
-__device__ __forceinline__ float eval_node_staged(const TreeView& T, const NodeGeo& g, int nd, float qx, float qy,
+__device__ __forceinline__ float eval_node_staged(const TreeView& T, const NodeGeo& g, int off, float qx, float qy,
                                                   bool act, float* sc, int lane)
 {
     const int p = g.deg;
     const int np = (p + 1) * (p + 1);
-    const float UD_GLOBAL* c = T.coef + T.aux[nd].off;
+    const float UD_GLOBAL* c = T.coef + off;
     for (int q = lane; q < np; q += 64) sc[q] = c[q];
     __builtin_amdgcn_wave_barrier();
     float w = 0.0f;
@@ -1249,10 +1249,11 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
             nd = __builtin_amdgcn_readfirstlane(nd);
             NodeGeo g = ld_geo(T, nd);
             NodeLnk lk = ld_lnk(T, nd);        // issued together with the geometry
+            const int coefOff = T.aux[nd].off;   // and the coefficient offset
             float wv = 0.0f;
             bool walk = false;
             if (g.deg >= 0) {
-                wv = eval_node_staged(T, g, nd, qx, qy, act, s_coef, lane);
+                wv = eval_node_staged(T, g, coefOff, qx, qy, act, s_coef, lane);
                 if (act) {
                     // ancestors whose weight a descent can compare: depth 0 and every
                     // node whose parent has more than k sources (a prefix of the path,
