@@ -33,8 +33,9 @@ METRIC = "fwd+bwd frames/s at 1M splats 512² (learned vs plain rates, equal gra
 HBM_PEAK_GBS = 8000.0
 RATE, KCAP = 10.0, 40
 # HBM bytes per launch, measured offline (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
-# kernel, each scaled by the same counter on a 1 GiB device copy, bench.py --calibrate-copy)
-TRAFFIC_BYTES = {"k_sample": 3.374e9, "k_fmm_eval": 2.849e8, "k_poisson": None}
+# kernel, each scaled by the same counter on a 1 GiB device copy, bench.py --calibrate-copy);
+# k_sample is its two kernels, the walk with the capacity descents and the keyed draws
+TRAFFIC_BYTES = {"k_sample": 3.298e9, "k_fmm_eval": 2.441e8, "k_poisson": 5.171e8}
 
 
 def build_problem(dev, n_override=None):
@@ -287,7 +288,8 @@ def reference_path(dev, args, sc_n):
     achieved = nbytes["sample"] / (ks_avg * 1e-3) / 1e9
     return {"method": "sequential Poisson sampling, k=40 per pixel, z-sort/compositing (reference semantics)",
             "fps": 1000.0 / ms, "ms_per_step": ms,
-            "roofline": {"bound": "hbm", "kernel": "k_sample", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "roofline": {"bound": "hbm", "kernel": "k_sample (k_sample_walk + k_sample_draw)", "achieved": achieved,
+                         "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["k_sample"],
                          "kernel_ms": ks_avg, "kernel_launches": ks_n, "algorithmic_bytes": nbytes["sample"]},
             "ops_ms": {k_: round(v, 4) for k_, v in sorted(ops.items())}}

@@ -139,8 +139,8 @@ int ude_adam_step(int n, float* params, const float* grad, float* m, float* v, f
 
 /* Profiling aid (no counterpart in the reference): between begin and end,
  * every launch of one sampling kernel is bracketed by HIP events on its
- * stream -- kernel 0: the per-pixel FGT descent + sequential Poisson pass of
- * ude_sample (one launch per view); 1: the fixed-order density evaluation
+ * stream -- kernel 0: the per-pixel FGT walk, capacity descents and keyed
+ * draws of ude_sample (one pair of launches per view, bracketed together); 1: the fixed-order density evaluation
  * (two launches per view in ude_sample_poisson, one in ude_fmm_weights);
  * 2: the Poisson-rate thinning pass of ude_sample_poisson.  end waits for the
  * recorded events and returns the summed kernel time and the number of

@@ -197,7 +197,7 @@ _PROFILED = {"sample": 0, "fmm_eval": 1, "poisson": 2}
 
 def profile_sampler_begin(kernel="sample"):
     """Start bracketing every launch of one sampling kernel with HIP events
-    (benchmarks): "sample" (the FGT descent + sequential Poisson pass of
+    (benchmarks): "sample" (the FGT walk, descents and keyed draws of
     ``sample``), "fmm_eval" (fixed-order density evaluation) or "poisson" (the
     thinning pass of ``samplePoisson``)."""
     _lib.check(_lib.lib().ude_profile_sampler_begin(_PROFILED[kernel]), "profile_sampler_begin")
