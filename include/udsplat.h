@@ -140,9 +140,10 @@ int ude_adam_step(int n, float* params, const float* grad, float* m, float* v, f
 /* Profiling aid (no counterpart in the reference): between begin and end,
  * every launch of one sampling kernel is bracketed by HIP events on its
  * stream -- kernel 0: the per-pixel FGT walk, capacity descents and keyed
- * draws of ude_sample (one pair of launches per view, bracketed together); 1: the fixed-order density evaluation
- * (two launches per view in ude_sample_poisson, one in ude_fmm_weights);
- * 2: the Poisson-rate thinning pass of ude_sample_poisson.  end waits for the
+ * draws of ude_sample (one pair of launches per view, bracketed together);
+ * 1: the fixed-order density evaluation (one launch per batch of views for
+ * both bands in ude_sample_poisson, one per view in ude_fmm_weights); 2: the
+ * Poisson-rate thinning pass of ude_sample_poisson.  end waits for the
  * recorded events and returns the summed kernel time and the number of
  * launches.  Host-thread-unsafe; meant for benchmarks. */
 int ude_profile_sampler_begin(int kernel);
