@@ -299,9 +299,9 @@ def batched_views(dev, steps, warmup, n_splats=2_000_000, views=16):
     """Batched-view workload: the 8 x 6 x 3 m box room (2M surfels on the walls and
     on 200 boxes, std-devs U[0.01, 0.03]) seen by 16 cameras at 640x480 with uneven
     culled counts; one fwd+bwd Poisson-rate frame over all views per step, with
-    every per-pixel pass one launch over the batch, launch shapes from the
-    autotuner, the whole frame replayed as one captured HIP graph (the Philox key
-    advances inside the graph)."""
+    every stage one launch over the batch, launch shapes and the Hermite order of
+    the densities from the autotuner, the whole frame replayed as one captured HIP
+    graph (the Philox key advances inside the graph)."""
     import torch
     from cudaExtensions import sampling
     from splatting_amd import autotune, learned as L, scenes
@@ -317,6 +317,7 @@ def batched_views(dev, steps, warmup, n_splats=2_000_000, views=16):
 
     saved = {k: sampling.get_tile_config(k) for k in autotune.KERNELS}
     tuning = autotune.tune(sample)
+    tuning["degree"] = autotune.tune_degree(sample)
     seq_ms, _, _ = timed_steps(lambda: sample(False), 3)
     bat_ms, _, _ = timed_steps(sample, 3)
     for _ in range(warmup):
@@ -339,6 +340,7 @@ def batched_views(dev, steps, warmup, n_splats=2_000_000, views=16):
     sampling.check_errors()
     for k, c in saved.items():
         sampling.set_tile_config(k, *c)
+    sampling.set_fixed_degree(6)
     best = min(eager_ms, graph_ms)
     return {"workload": "box room 8x6x3 m, %d surfels on walls + 200 boxes, %d cameras at 640x480, "
                         "Poisson-rate sampling at rate %g, fwd+bwd over all views per step" % (n_splats, views, RATE),

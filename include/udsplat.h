@@ -89,6 +89,15 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
                        int sdBatch, uint64_t seed, const uint64_t* seedDev, int32_t* outIds, float* outIpi,
                        float* densOut, int* errFlag, void* ws, size_t wsBytes, ude_stream_t stream);
 
+/* Order of the fixed-order Hermite expansions behind ude_fmm_weights and the
+ * densities of ude_sample_poisson (no counterpart in the reference, whose
+ * order is a compile-time constant, hermitefmm.cuh:11): 4 .. 8, default 6.
+ * A node carries an expansion when its order-p truncation bound meets the
+ * precision, so every order keeps the stated error bound; lower orders accept
+ * fewer, smaller nodes.  Process-wide; read when a call starts. */
+int ude_set_fixed_degree(int p);
+int ude_get_fixed_degree(void);
+
 /* Launch shape of the per-pixel passes (no counterpart in the reference):
  * kernel 0 the fixed-order density evaluation, 1 the Poisson-rate thinning.
  * One wavefront per tileW x tileH pixel tile (tileW tileH = 64, powers of

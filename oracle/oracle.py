@@ -44,6 +44,8 @@ def lib():
         _lib.o_fmm_weights_view.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 4
         _lib.o_poisson_view.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 4 + [ctypes.c_void_p] * 4 +
                                        [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3)
+        _lib.o_set_fixed_degree.restype = ctypes.c_int
+        _lib.o_set_fixed_degree.argtypes = [ctypes.c_int]
         _lib.o_poisson_wide_h.restype = ctypes.c_float
         _lib.o_poisson_wide_h.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3
         _lib.o_sample_view_rows.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 +
@@ -323,6 +325,13 @@ def sample(W, H, f, pts, nrm, sd, k, smoothing=0.0, precision=0.01, seed=0):
         sdb = sd[b] if sd.shape[0] > 1 else sd[0]
         outs.append(sample_view(W, H, f, pts[b], nrm[b], sdb, k, smoothing, precision, seed, b))
     return np.stack(outs)
+
+
+def set_fixed_degree(p):
+    """Degree of the fixed-order expansions (densities of approximateFMMWeights and
+    of Poisson-rate sampling): 4 .. 8, default 6."""
+    if lib().o_set_fixed_degree(ctypes.c_int(int(p))) != 0:
+        raise ValueError("fixed degree must be in 4 .. 8")
 
 
 def fmm_weights_view(W, H, f, pts, nrm, sd, smoothing=0.0, precision=0.01):

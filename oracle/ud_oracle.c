@@ -1229,7 +1229,15 @@ done:
 /* first).  Output x-major (j + H i); the reference writes j + W i, which is */
 /* the same image for W == H and out of bounds otherwise.                    */
 /* ------------------------------------------------------------------------ */
-#define O_FIXED_P 6
+#define O_FIXED_PMAX 8
+/* degree of the fixed-order expansions (6 unless set; 4 .. 8) */
+static int g_fixed_p = 6;
+int o_set_fixed_degree(int p)
+{
+    if (p < 4 || p > O_FIXED_PMAX) return -1;
+    g_fixed_p = p;
+    return 0;
+}
 
 static int o_box_meets_disc(const OTree* T, int nd, float qx, float qy, float rad)
 {
@@ -1244,8 +1252,9 @@ static int o_box_meets_disc(const OTree* T, int nd, float qx, float qy, float ra
 }
 
 /* Fixed-order tree of one view: the sampler's tree structure plus, for up to
- * two bandwidths, degree-6 moments (times the leaf coefficient c = 1 / (2 pi
- * sqrt(h))) at the internal nodes whose truncation bound meets the precision.
+ * two bandwidths, degree-p moments (times the leaf coefficient c = 1 / (2 pi
+ * sqrt(h))) at the internal nodes whose truncation bound meets the precision
+ * (p = g_fixed_p, fixed when the tree is built).
  * Bandwidth 0 is the tree's h; bandwidth 1 (Poisson-rate sampling only) is the
  * wide proposal bandwidth hw = max(h, 99.9th percentile of the visible splats'
  * h_i = sqrt(lambda_max(V_i)), rounded up to 1/16 px, at most 16) (degenerate
@@ -1262,6 +1271,7 @@ typedef struct {
     float* fb;
     OBand band[2];
     int nband;
+    int p;
     int ok;
 } OFixed;
 
@@ -1282,14 +1292,14 @@ static void o_band_build(OFixed* F, OBand* B, float h, double prec)
 {
     const OTree* T = &F->T;
     const int nc = T->nc, L0 = nc - 1;
-    const int P1 = O_FIXED_P + 1;
+    const int P = F->p, P1 = P + 1;
     B->h = h;
     B->is2h = 1.0f / sqrtf((2.0f * h) * h);
     B->c = (float)(1.0 / (2.0 * O_PI * (double)sqrtf(h)));
     B->radius = o_band_radius(h, prec);
     B->c6 = (float*)malloc(sizeof(float) * (size_t)P1 * P1 * (nc > 1 ? nc - 1 : 1));
     B->acc6 = (char*)calloc(nc > 1 ? nc - 1 : 1, 1);
-    double ifact[O_FIXED_P + 1];
+    double ifact[O_FIXED_PMAX + 1];
     double fa = 1.0;
     ifact[0] = 1.0;
     for (int a = 1; a < P1; a++) { fa = fa * a; ifact[a] = 1.0 / fa; }
@@ -1301,15 +1311,15 @@ static void o_band_build(OFixed* F, OBand* B, float h, double prec)
             mn = fmaxf(mn, fmaxf(cc - b[d], b[2 + d] - cc));
         }
         float rr = mn / h;
-        B->acc6[nd] = o_hermite_error(rr, (float)T->count[nd], O_FIXED_P) <= T->prec;
+        B->acc6[nd] = o_hermite_error(rr, (float)T->count[nd], P) <= T->prec;
         if (!B->acc6[nd]) continue;
-        double acc[(O_FIXED_P + 1) * (O_FIXED_P + 1)];
+        double acc[(O_FIXED_PMAX + 1) * (O_FIXED_PMAX + 1)];
         for (int q = 0; q < P1 * P1; q++) acc[q] = 0.0;
         for (int pos = T->first[nd]; pos <= T->last[nd]; pos++) {
             int lf = L0 + pos;
             float tx = (T->cen[2 * lf] - T->cen[2 * nd]) * B->is2h;
             float ty = (T->cen[2 * lf + 1] - T->cen[2 * nd + 1]) * B->is2h;
-            double mx[O_FIXED_P + 1], my[O_FIXED_P + 1], px = 1.0, py = 1.0;
+            double mx[O_FIXED_PMAX + 1], my[O_FIXED_PMAX + 1], px = 1.0, py = 1.0;
             for (int a = 0; a < P1; a++) {
                 mx[a] = px * ifact[a]; my[a] = py * ifact[a];
                 px = px * (double)tx; py = py * (double)ty;
@@ -1346,6 +1356,7 @@ static int o_fixed_build(int N, int W, int H, double f, double smoothing, double
     if (rc < 0) return rc;
     if (T->nc == 0) return 0;
     F->nband = nband;
+    F->p = g_fixed_p;
     o_band_build(F, &F->band[0], T->h, prec);
     if (nband > 1) {
         /* 99.9th percentile of the visible splats' bandwidths, on a 1/16 px grid */
@@ -1389,7 +1400,7 @@ static float o_fixed_eval(const OFixed* F, int b, float qx, float qy)
     const OTree* T = &F->T;
     const OBand* B = &F->band[b];
     if (!F->ok) return 0.0f;
-    const int P1 = O_FIXED_P + 1;
+    const int P1 = F->p + 1;
     const int L0 = T->nc - 1;
     int stack[256];
     int sp = 0;
@@ -1402,7 +1413,7 @@ static float o_fixed_eval(const OFixed* F, int b, float qx, float qy)
             float tx = (qx - T->cen[2 * nd]) * B->is2h;
             float ty = (qy - T->cen[2 * nd + 1]) * B->is2h;
             float ex = oexpf(-(tx * tx)), ey = oexpf(-(ty * ty));
-            float hx[O_FIXED_P + 1], hy[O_FIXED_P + 1];
+            float hx[O_FIXED_PMAX + 1], hy[O_FIXED_PMAX + 1];
             hx[0] = ex; hy[0] = ey;
             hx[1] = (2.0f * tx) * ex; hy[1] = (2.0f * ty) * ey;
             for (int n = 2; n < P1; n++) {

@@ -100,3 +100,19 @@ def test_fmm_large_workload_against_direct_sum(dev):
     assert dens.max() > 1.0
     assert np.all(img <= dens + tol), (img - dens).max()
     assert np.all(img >= dens - far - tol), (dens - far - img).max()
+
+
+@pytest.mark.parametrize("p", [4, 5, 7, 8])
+def test_fmm_other_orders_bit_exact(dev, p):
+    from cudaExtensions import sampling
+    sc, p_, n, nb, sd = scene_inputs(0, sd_scale=2.0)
+    sampling.set_fixed_degree(p)
+    O.set_fixed_degree(p)
+    try:
+        got = run_gpu(dev, sc.width, sc.height, sc.focal, p_, nb, sd, 0.01)
+        ref = O.fmm_weights_view(sc.width, sc.height, sc.focal, p_, nb, sd, 0.0, 0.01)
+    finally:
+        sampling.set_fixed_degree(6)
+        O.set_fixed_degree(6)
+    assert sampling.get_fixed_degree() == 6
+    assert np.array_equal(got, ref)

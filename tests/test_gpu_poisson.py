@@ -112,3 +112,18 @@ def test_poisson_gradients_unbiased(dev):
     # parameters whose sampled gradient never varies must match exactly (up to FP32 sums)
     dm = (~m).cpu().numpy()
     assert np.allclose(mean.cpu().numpy()[dm], dense.cpu().numpy()[dm], rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("p", [4, 8])
+def test_poisson_other_orders_bit_exact(dev, p):
+    from cudaExtensions import sampling
+    sc, p_, n, nb, sd = scene_inputs(0, sd_scale=2.0)
+    sampling.set_fixed_degree(p)
+    O.set_fixed_degree(p)
+    try:
+        ids, ipi = gpu_poisson(dev, sc.width, sc.height, sc.focal, p_[None], nb[None], sd[None], 40, 8.0, seed=12)
+        rid, ripi, _, over = O.poisson_view(sc.width, sc.height, sc.focal, p_, nb, sd, 40, 8.0, 0.0, 0.01, seed=12)
+    finally:
+        sampling.set_fixed_degree(6)
+        O.set_fixed_degree(6)
+    assert over == 0 and np.array_equal(ids[0], rid) and np.array_equal(ipi[0], ripi)

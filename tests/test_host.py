@@ -221,3 +221,20 @@ def test_tile_config_roundtrip_and_autotune_cache(tmp_path):
     finally:
         for k, c in saved.items():
             sampling.set_tile_config(k, *c)
+
+
+def test_autotune_degree_picks_the_fastest_order():
+    import time as _t
+    from cudaExtensions import sampling
+    from splatting_amd import autotune
+    cost = {4: 0.004, 5: 0.001, 6: 0.003, 7: 0.006, 8: 0.008}
+    try:
+        r = autotune.tune_degree(lambda: _t.sleep(cost[sampling.get_fixed_degree()]), repeats=2, sync=lambda: None)
+        assert r["degree"] == 5 and sampling.get_fixed_degree() == 5
+        assert set(r["table"]) == set(cost)
+        autotune.apply({"degree": {"degree": 7}})
+        assert sampling.get_fixed_degree() == 7
+        with pytest.raises(RuntimeError):
+            sampling.set_fixed_degree(9)
+    finally:
+        sampling.set_fixed_degree(6)

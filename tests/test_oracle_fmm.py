@@ -74,3 +74,21 @@ def test_fmm_single_splat_and_empty():
     z = np.zeros((0, 3), np.float32)
     assert np.all(O.fmm_weights_view(8, 8, 9.6, z, z, np.zeros((0, 2), np.float32)) == 0)
     assert np.all(O.fmm_weights_view(sc.width, sc.height, sc.focal, -p, nb, sd) == 0)
+
+
+@pytest.mark.parametrize("p", [4, 5, 7, 8])
+def test_fmm_density_bracket_other_orders(p):
+    """Every expansion order keeps the bracket: a node expands only where its
+    order-p truncation bound meets the precision."""
+    sc, p_, n, nb, sd = scene_inputs(0, sd_scale=2.0)
+    O.set_fixed_degree(p)
+    try:
+        img = O.fmm_weights_view(sc.width, sc.height, sc.focal, p_, nb, sd, 0.0, 0.01)
+    finally:
+        O.set_fixed_degree(6)
+    dens, far, T = direct(sc, p_, nb, sd, 0.01)
+    tol = 5e-5
+    assert np.all(img <= dens + tol)
+    assert np.all(img >= dens - far - tol)
+    with pytest.raises(ValueError):
+        O.set_fixed_degree(3)

@@ -33,8 +33,9 @@ constexpr int kMaxDegree = 16;
 constexpr int kMaxDepth = 64;        // descent trace bound (reference STACKSIZE)
 constexpr int kCullTile = 1024;      // elements per workgroup in the cull pass
 constexpr int kSortTile = 2048;      // elements per workgroup in a sort pass
-constexpr int kFixedDegree = 6;      // approximateFMMWeights expansion order (hermitefmm.cuh:11)
-constexpr int kFixedCoefs = (kFixedDegree + 1) * (kFixedDegree + 1);
+constexpr int kFixedDegree = 6;      // default expansion order of the fixed-order densities (hermitefmm.cuh:11)
+constexpr int kFixedPMin = 4, kFixedPMax = 8;              // orders ude_set_fixed_degree accepts
+constexpr int kFixedCoefsMax = (kFixedPMax + 1) * (kFixedPMax + 1);
 constexpr int kScanTile = 1024;      // elements per workgroup in a scan pass
 
 // Gaussian bandwidth of a fixed-order density: h, 1 / sqrt(2 h^2), the leaf
@@ -164,7 +165,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     for (int b = 0; b < nb; b++) {
         w.fixOff[b] = (int*)take(4 * n);
         w.fixList[b] = (int*)take(4 * n);
-        w.fixCoef[b] = (float*)take(4L * kFixedCoefs * (n - 1 > 0 ? n - 1 : 1));
+        w.fixCoef[b] = (float*)take(4L * kFixedCoefsMax * (n - 1 > 0 ? n - 1 : 1));
     }
     if (nb > 0) w.frec = (FmmRec*)take(sizeof(FmmRec) * n);
     if (mode == kModePoisson) {
@@ -1490,7 +1491,7 @@ __global__ void k_copy_err(const ViewState* vs, int* err)
 // Internal nodes whose degree-6 truncation bound at band b meets the
 // precision, and whose parent's does not, carry an expansion: a 49-float
 // block at fixOff[node].
-__global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int nbands, const NodeGeo* __restrict__ geo,
+__global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int nbands, int p, const NodeGeo* __restrict__ geo,
                                                       const NodeLnk* __restrict__ lnk, int* fixOff0, int* fixOff1,
                                                       int* list0, int* list1, VStride vsd)
 {
@@ -1517,7 +1518,7 @@ __global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int nbands,
             mn = fmaxf(mn, fmaxf(g.cx - g.lox, g.hix - g.cx));
             mn = fmaxf(mn, fmaxf(g.cy - g.loy, g.hiy - g.cy));
             float rr = mn / hb;
-            return hermite_error(rr, (float)lnk[nd].count, kFixedDegree) <= prec;
+            return hermite_error(rr, (float)lnk[nd].count, p) <= prec;
         };
         // the walk from the root evaluates the first node on its path that meets the
         // bound and skips its subtree, so only those nodes need an expansion
@@ -1540,15 +1541,15 @@ __global__ __launch_bounds__(256) void k_fixed_select(ViewState* vs, int nbands,
         const int slot = base + __popcll(m[b] & ((1ull << lane) - 1ull));
         int* fixOff = b ? fixOff1 : fixOff0;
         int* list = b ? list1 : list0;
-        fixOff[t] = acc[b] ? slot * kFixedCoefs : -1;
+        fixOff[t] = acc[b] ? slot * (p + 1) * (p + 1) : -1;
         if (acc[b]) list[slot] = t;
     }
 }
 
-// Degree-6 moments at band b, scaled by the band's leaf coefficient: thread
+// Degree-P moments at band b, scaled by the band's leaf coefficient: thread
 // (node, row a) sums row a over the node's leaf range in FP64 (leaf order).
-// 36 nodes x 7 rows per 256-thread workgroup.
-constexpr int kFixedNodesPerBlock = 36;
+// 256 / (P + 1) nodes x (P + 1) rows per 256-thread workgroup.
+template <int P>
 __global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restrict__ vs, int b,
                                                        const NodeGeo* __restrict__ geo, const NodeAux* __restrict__ aux,
                                                        const int* __restrict__ list, const int* __restrict__ fixOff,
@@ -1561,9 +1562,9 @@ __global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restri
     fixOff = vw(fixOff, vsd.ws);
     coef = vw(coef, vsd.ws);
     const int t = threadIdx.x;
-    if (t >= kFixedNodesPerBlock * (kFixedDegree + 1)) return;
-    const long slot = (long)blockIdx.x * kFixedNodesPerBlock + t / (kFixedDegree + 1);
-    const int ax = t % (kFixedDegree + 1);
+    if (t >= (256 / (P + 1)) * (P + 1)) return;
+    const long slot = (long)blockIdx.x * (256 / (P + 1)) + t / (P + 1);
+    const int ax = t % (P + 1);
     if (slot >= vs->nfix[b]) return;
     const int nd = list[slot];
     const int L0 = vs->nc - 1;
@@ -1573,9 +1574,9 @@ __global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restri
     NodeAux a = aux[nd];
     const int off = fixOff[nd];
     const double ifx = kInvFact[ax];
-    double acc[kFixedDegree + 1];
+    double acc[P + 1];
 #pragma unroll
-    for (int bb = 0; bb <= kFixedDegree; bb++) acc[bb] = 0.0;
+    for (int bb = 0; bb <= P; bb++) acc[bb] = 0.0;
     for (int pos = a.first; pos <= a.last; pos++) {
         NodeGeo lf = geo[L0 + pos];
         float tx = (lf.cx - g.cx) * is2h;
@@ -1585,14 +1586,14 @@ __global__ __launch_bounds__(256) void k_fixed_moments(const ViewState* __restri
         double mx = px * ifx;
         double py = 1.0;
 #pragma unroll
-        for (int bb = 0; bb <= kFixedDegree; bb++) {
+        for (int bb = 0; bb <= P; bb++) {
             acc[bb] += mx * (py * ifact_c(bb));
             py = py * (double)ty;
         }
     }
 #pragma unroll
-    for (int bb = 0; bb <= kFixedDegree; bb++)
-        coef[off + ax * (kFixedDegree + 1) + bb] = (float)(c * acc[bb]);
+    for (int bb = 0; bb <= P; bb++)
+        coef[off + ax * (P + 1) + bb] = (float)(c * acc[bb]);
 }
 
 __device__ __forceinline__ bool box_meets_disc(const NodeGeo& g, float qx, float qy, float rad)
@@ -1712,6 +1713,7 @@ __device__ __forceinline__ bool box4_meets_disc(const float* bx, float qx, float
 // FmmRec, so leaves and expansions need no node load and a descent needs one
 // wave-uniform record load.  Karras trees on 64-bit keys are at most 64 levels
 // deep, so the stack never holds more than 65 entries.
+template <int P>
 __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg, int W, int H,
                                                                     const ViewState* __restrict__ vs, size_t vstride,
                                                                     int nbands, const NodeGeo* geo,
@@ -1772,32 +1774,32 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
                 continue;
             }
             if (off >= 0) {
-                float co[kFixedCoefs];                      // wave-uniform: scalar loads
+                float co[(P + 1) * (P + 1)];                      // wave-uniform: scalar loads
                 const float UD_GLOBAL* cc = cf + off;
 #pragma unroll
-                for (int q = 0; q < kFixedCoefs; q++) co[q] = cc[q];
+                for (int q = 0; q < (P + 1) * (P + 1); q++) co[q] = cc[q];
                 if (act) {
                     float tx = (qx - c.x) * B.is2h;
                     float ty = (qy - c.y) * B.is2h;
                     float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
-                    float hy[kFixedDegree + 1];
+                    float hy[P + 1];
                     hy[0] = ey;
                     hy[1] = (2.0f * ty) * ey;
 #pragma unroll
-                    for (int q = 2; q <= kFixedDegree; q++)
+                    for (int q = 2; q <= P; q++)
                         hy[q] = (2.0f * ty) * hy[q - 1] - (2.0f * (float)(q - 1)) * hy[q - 2];
                     float val = 0.0f;
                     float hprev = 0.0f, ha = ex;
 #pragma unroll
-                    for (int a = 0; a <= kFixedDegree; a++) {
+                    for (int a = 0; a <= P; a++) {
                         if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
                         else if (a >= 2) {
                             float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
                             hprev = ha; ha = hn;
                         }
 #pragma unroll
-                        for (int bb = 0; bb <= kFixedDegree; bb++)
-                            val = val + (co[a * (kFixedDegree + 1) + bb] * ha) * hy[bb];
+                        for (int bb = 0; bb <= P; bb++)
+                            val = val + (co[a * (P + 1) + bb] * ha) * hy[bb];
                     }
                     w = w + val;
                 }
@@ -2313,32 +2315,48 @@ dim3 tile_grid(const TileCfg& c, int W, int H, int views)
 }
 
 // degree-6 expansions of bands 0 .. nbands-1 on one view's tree (or a batch)
-void fixed_expansions(int N, int nbands, const TreeWs& w, hipStream_t st, int views = 1, VStride vsd = VStride{})
+void fixed_expansions(int N, int nbands, int p, const TreeWs& w, hipStream_t st, int views = 1,
+                      VStride vsd = VStride{})
 {
-    k_fixed_select<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, nbands, w.geo, w.lnk, w.fixOff[0],
+    k_fixed_select<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, nbands, p, w.geo, w.lnk, w.fixOff[0],
                                                                 nbands > 1 ? w.fixOff[1] : nullptr, w.fixList[0],
                                                                 nbands > 1 ? w.fixList[1] : nullptr, vsd);
-    for (int b = 0; b < nbands; b++)
-        k_fixed_moments<<<dim3(grid1(N, kFixedNodesPerBlock), views), 256, 0, st>>>(w.vs, b, w.geo, w.aux,
-                                                                                     w.fixList[b], w.fixOff[b],
-                                                                                     w.fixCoef[b], vsd);
+    for (int b = 0; b < nbands; b++) {
+        const dim3 grid(grid1(N, 256 / (p + 1)), views);
+        switch (p) {
+#define UD_MOMENTS(P) case P: k_fixed_moments<P><<<grid, 256, 0, st>>>(w.vs, b, w.geo, w.aux, w.fixList[b], \
+                                                                        w.fixOff[b], w.fixCoef[b], vsd); break;
+        UD_MOMENTS(4) UD_MOMENTS(5) UD_MOMENTS(6) UD_MOMENTS(7) UD_MOMENTS(8)
+#undef UD_MOMENTS
+        default: break;
+        }
+    }
 }
 
 // densities of bands 0 .. nbands-1 of `views` views in one launch: w is the
 // first view's workspace, the others follow vstride bytes apart; view v writes
 // band b to out + v outStride + b bandStride bytes
-void fmm_eval(int W, int H, int views, int nbands, const TreeWs& w, size_t vstride, float* out, size_t outStride,
-              size_t bandStride, hipStream_t st)
+void fmm_eval(int W, int H, int views, int nbands, int p, const TreeWs& w, size_t vstride, float* out,
+              size_t outStride, size_t bandStride, hipStream_t st)
 {
     const TileCfg c = g_tile[0];
     hipEvent_t e0, e1;
     bool timed = timing_pair(1, &e0, &e1);
     if (timed) (void)hipEventRecord(e0, st);
-    k_fmm_eval<<<tile_grid(c, W, H, views * nbands), 64 * c.wpb, 0, st>>>(
-        c, W, H, w.vs, vstride, nbands, w.geo, w.frec, w.fixOff[0], nbands > 1 ? w.fixOff[1] : nullptr,
-        w.fixCoef[0], nbands > 1 ? w.fixCoef[1] : nullptr, out, outStride, bandStride);
+    const dim3 grid = tile_grid(c, W, H, views * nbands);
+    switch (p) {
+#define UD_EVAL(P) case P: k_fmm_eval<P><<<grid, 64 * c.wpb, 0, st>>>(                                             \
+        c, W, H, w.vs, vstride, nbands, w.geo, w.frec, w.fixOff[0], nbands > 1 ? w.fixOff[1] : nullptr,            \
+        w.fixCoef[0], nbands > 1 ? w.fixCoef[1] : nullptr, out, outStride, bandStride); break;
+    UD_EVAL(4) UD_EVAL(5) UD_EVAL(6) UD_EVAL(7) UD_EVAL(8)
+#undef UD_EVAL
+    default: break;
+    }
     if (timed) (void)hipEventRecord(e1, st);
 }
+
+// degree of the fixed-order expansions; set by ude_set_fixed_degree
+int g_fixed_p = kFixedDegree;
 
 } // namespace
 
@@ -2396,6 +2414,7 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
         return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
     }
     if (wsBytes < ude_fmm_weights_workspace_bytes(N, W, H)) return UDE_ERR_WORKSPACE;
+    const int p = g_fixed_p;
     TreeWs w;
     carve(ws, N, W, H, 0, kModeFixed, &w);
     for (int b = 0; b < bn; b++) {
@@ -2404,9 +2423,9 @@ int ude_fmm_weights(int bn, int N, int W, int H, double f, double smoothing, dou
         const float* sb = sd + (sdBatch > 1 ? 2L * N * b : 0);
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st);
         if (rc != UDE_OK) return rc;
-        fixed_expansions(N, 1, w, st);
+        fixed_expansions(N, 1, p, w, st);
         k_fmm_records<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], nullptr, w.frec, VStride{});
-        fmm_eval(W, H, 1, 1, w, 0, out + perView * b, 0, 0, st);
+        fmm_eval(W, H, 1, 1, p, w, 0, out + perView * b, 0, 0, st);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
@@ -2440,6 +2459,7 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
     const int tsW = (W + 15) / 16, tsH = (H + 15) / 16;
     const uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
     const TileCfg pc = g_tile[1];
+    const int p = g_fixed_p;
     for (int g = 0; g < bn; g += span) {
         // every view of the group in one launch per build stage (view = blockIdx.y)
         TreeWs w;
@@ -2454,13 +2474,13 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
                                                                                           vsd);
         k_wide_band<<<dim3(1, span), 64, 0, st>>>(w.vs, vsd);
         k_pois_nodes<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.pnode, vsd);
-        fixed_expansions(N, 2, w, st, span, vsd);
+        fixed_expansions(N, 2, p, w, st, span, vsd);
         k_fmm_records<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], w.fixOff[1], w.frec,
                                                                  vsd);
         float* d0 = densOut ? densOut + perView * 2 * g : w.dens[0];
         float* d1 = densOut ? d0 + perView : w.dens[1];
         const size_t dstride = densOut ? sizeof(float) * 2 * perView : per;
-        fmm_eval(W, H, span, 2, w, per, d0, dstride, (size_t)((char*)d1 - (char*)d0), st);
+        fmm_eval(W, H, span, 2, p, w, per, d0, dstride, (size_t)((char*)d1 - (char*)d0), st);
         const float* ts = tileScale ? tileScale + (long)tsW * tsH * g : nullptr;
         hipEvent_t e0, e1;
         bool timed = timing_pair(2, &e0, &e1);
@@ -2472,6 +2492,18 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+int ude_set_fixed_degree(int p)
+{
+    if (p < kFixedPMin || p > kFixedPMax) return UDE_ERR_ARG;
+    g_fixed_p = p;
+    return UDE_OK;
+}
+
+int ude_get_fixed_degree(void)
+{
+    return g_fixed_p;
 }
 
 int ude_set_tile_config(int kernel, int tileW, int tileH, int wavesPerBlock, int xcdOrder)

@@ -203,6 +203,17 @@ def profile_sampler_begin(kernel="sample"):
     _lib.check(_lib.lib().ude_profile_sampler_begin(_PROFILED[kernel]), "profile_sampler_begin")
 
 
+def set_fixed_degree(p):
+    """Order of the fixed-order Hermite expansions behind approximateFMMWeights and
+    samplePoisson's densities: 4 .. 8 (default 6).  Every order keeps the
+    precision bound; lower orders expand fewer, smaller nodes."""
+    _lib.check(_lib.lib().ude_set_fixed_degree(int(p)), "set_fixed_degree")
+
+
+def get_fixed_degree():
+    return int(_lib.lib().ude_get_fixed_degree())
+
+
 def set_tile_config(kernel, tileW, tileH, wavesPerBlock, xcdOrder=0):
     """Launch shape of a per-pixel pass ("fmm_eval" or "poisson"): one wavefront per
     tileW x tileH pixel tile (tileW * tileH = 64), wavesPerBlock wavefronts per

@@ -12,11 +12,18 @@ draws are its own), so any choice is safe and the tuner only times.
 ``tune(run)`` calls ``run()`` -- one call of the real workload, e.g.
 ``samplePoisson`` on the batch of views to be rendered -- under every
 candidate shape, times each kernel's launches with HIP events on their stream,
-keeps the fastest shape per kernel and leaves it set.  Results can be cached in
-a JSON file keyed by the workload (``workload_key``).
+keeps the fastest shape per kernel and leaves it set.  ``tune_degree(run)``
+picks the order of the fixed-order Hermite expansions (4 .. 8) the same way,
+timing whole calls: every order keeps the precision bound (a node expands only
+where its truncation bound at that order meets it), lower orders expand fewer,
+smaller nodes with cheaper series, so the fastest order depends on the
+workload.  Unlike the shapes, the order changes the densities (within the
+bound) and therefore the drawn samples.  Results can be cached in a JSON file
+keyed by the workload (``workload_key``).
 """
 import json
 import os
+import time
 
 SHAPES = [(8, 8), (16, 4), (4, 16), (32, 2)]
 WAVES = [1, 2, 4, 8]
@@ -67,11 +74,43 @@ def tune(run, kernels=KERNELS, repeats=2, shapes=SHAPES, waves=WAVES, xcd=XCD, l
     return out
 
 
+DEGREES = (4, 5, 6, 7, 8)
+
+
+def tune_degree(run, degrees=DEGREES, repeats=3, sync=None, log=None):
+    """Times whole calls of ``run`` at every expansion order and sets the fastest.
+    Returns {"degree": p, "ms": best ms per call, "table": {p: ms}}."""
+    from cudaExtensions import sampling
+    if sync is None:
+        import torch
+        sync = torch.cuda.synchronize
+    table = {}
+    for p in degrees:
+        sampling.set_fixed_degree(p)
+        run()
+        sync()
+        best = float("inf")
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            run()
+            sync()
+            best = min(best, (time.perf_counter() - t0) * 1e3)
+        table[p] = best
+    p = min(table, key=table.get)
+    sampling.set_fixed_degree(p)
+    if log:
+        log("autotune fixed degree: %d (%s)" % (p, ", ".join("%d: %.3f ms" % kv for kv in sorted(table.items()))))
+    return {"degree": p, "ms": table[p], "table": table}
+
+
 def apply(result):
     """Sets the shapes of a tune() result (or of a cached one)."""
     from cudaExtensions import sampling
     for kernel, r in result.items():
-        sampling.set_tile_config(kernel, *r["config"])
+        if kernel == "degree":
+            sampling.set_fixed_degree(r["degree"])
+        else:
+            sampling.set_tile_config(kernel, *r["config"])
 
 
 def load(path, key):
