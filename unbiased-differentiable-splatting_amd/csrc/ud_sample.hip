@@ -1,9 +1,11 @@
-// ud_sample.hip -- sampling.sample on gfx950: LBVH build, Hermite FGT and the
-// per-pixel sequential-Poisson sampler (reference: samplingCuda.cu:197-290,
-// buildDynamicRadixTree.cuh, evaluateDynamicRadixTree.cuh,
-// sequentialPoissonDynamicTree.cuh).
+// ud_sample.hip -- the sampling module on gfx950: LBVH build, Hermite FGT,
+// the per-pixel sequential-Poisson sampler (sampling.sample; reference:
+// samplingCuda.cu:197-290, buildDynamicRadixTree.cuh,
+// evaluateDynamicRadixTree.cuh, sequentialPoissonDynamicTree.cuh), the
+// fixed-order densities (approximateFMMWeights; transformRadixTree.cuh) and
+// Poisson-rate sampling with inclusion probabilities (no reference kernel).
 //
-// Every stage runs on the device with sizes kept in device memory, so one view
+// Every stage runs on the device with sizes kept in device memory, so a view
 // costs no host synchronisation:
 //   1 k_bandwidth     per-splat isotropic h_k -> fixed-point sum (2^-40 units)
 //   2 k_view_params   h, query radius, screen bounds
@@ -12,12 +14,19 @@
 //                     wavefront ballot ranking)
 //   5 k_hierarchy     Karras 2012 split on 64-bit keys (morton << 32 | pos)
 //   6 k_bottom_up     atomic fan-in: boxes, counts, centres, Hermite degree
-//   7 k_coef_*        coefficient slots (deg+1)^2, device-wide exclusive scan
+//   7 k_coef_*        coefficient slots (deg+1)^2 of the reachable expanded
+//                     nodes, device-wide exclusive scan
 //   8 k_moments       Hermite moments per node over its contiguous leaf range,
 //                     accumulated in FP64 in leaf order
-//   9 k_sample        per pixel: FGT-guided capacity descent, sequential
-//                     Poisson keys u/p from Philox4x32-10, top-k in LDS
-// All FP32 arithmetic on the tree and in the sampler follows a fixed operation
+//   9 k_sample_walk   per pixel tile: union FGT walk, capacity descents
+//  10 k_sample_draw   per pixel: sequential Poisson keys u/p from
+//                     Philox4x32-10, top-k in LDS
+// The fixed-order and Poisson-rate paths share stages 1-6 and add expansion
+// selection and moments per band (k_fixed_*), walk records (k_fmm_records,
+// k_pois_nodes), the density walk (k_fmm_eval) and the thinning pass
+// (k_poisson).  Batches of views run each stage as one launch (views on
+// blockIdx.y) where the workspace holds them all.
+// All FP32 arithmetic on the tree and in the samplers follows a fixed operation
 // order (no contraction) so that index sets are reproducible bit for bit.
 #include "ud_device.h"
 #include "../../include/udsplat.h"
@@ -1094,20 +1103,6 @@ __device__ __noinline__ float tree_weight(const TreeView& T, int entry, float qx
     return w;
 }
 
-// nextLeaf (sequentialPoissonDynamicTree.cuh:16-53)
-__device__ __forceinline__ int next_leaf(const TreeView& T, int L0, int sp, int idx)
-{
-    if (idx >= L0) {
-        while (true) {
-            int pr = T.lnk[idx].parent;
-            if (pr == sp) break;
-            if (idx == T.lnk[pr].left) { idx = T.lnk[pr].right; break; }
-            idx = pr;
-        }
-    }
-    while (idx < L0) idx = T.lnk[idx].left;
-    return idx;
-}
 
 __device__ __forceinline__ float key_of(unsigned long long v) { return __uint_as_float((uint32_t)(v >> 32)); }
 
@@ -2261,11 +2256,10 @@ int num_launches(int k)
 // trie entries per lane: room for the walk plus 32 new entries per descent
 int trie_entries(int k) { return 192 + 32 * num_launches(k); }
 
-// Optional timing of the sampler kernel: while enabled, every k_sample launch
-// is bracketed by a pair of HIP events recorded on its stream (single host
+// Optional timing of one sampling kernel (0 the walk + draw pair of
+// sampling.sample, 1 k_fmm_eval, 2 k_poisson): while enabled, its launches are
+// bracketed by pairs of HIP events recorded on their stream (single host
 // thread; events are pooled and reused across profiling windows).
-// HIP events bracketing the launches of one kernel (0 k_sample, 1 k_fmm_eval,
-// 2 k_poisson) while a benchmark asks for them
 struct SamplerTiming {
     bool on = false;
     int kernel = 0;
