@@ -221,6 +221,31 @@ int ude_interp_forward(int bn, int W, int H, int k, int ab, int N, int C, const 
 int ude_interp_backward(int bn, int W, int H, int k, int ab, int N, int C, const float* g, const int32_t* idx,
                         const float* wts, const float* attr, float* gW, float* gA, ude_stream_t stream);
 
+/* ---- Horvitz-Thompson image estimate ------------------------------------ */
+
+/* No reference counterpart: splatting.forward, the 1 / pi weighting and
+ * interpolation.forward of a three-channel attribute fused per pixel, and their
+ * adjoints fused per sample, for the unbiased renderer over samplePoisson's
+ * output.  idx / ipi: (bn, W, H, k) with each pixel's samples first, then -1
+ * (ids) padding; a pixel's slots after its first -1 are not read.  attr:
+ * (ab, N, 3), ab in {1, bn}; pts / nrm (bn, N, 3); sd (sdBatch, N, 2).
+ * The image equals the unfused ops' bit for bit; the gradients equal theirs up
+ * to the order of atomic sums.  No per-slot tensor is stored. */
+size_t ude_ht_workspace_bytes(int bn, int N);
+
+/* out: (bn, W, H, 3) = sum_s min(w_s, 1) ipi_s attr[idx_s]. */
+int ude_ht_forward(int bn, int N, int W, int H, int k, double f, double smoothing, const int32_t* idx,
+                   const float* ipi, const float* pts, const float* nrm, const float* sd, int sdBatch, int ab,
+                   const float* attr, float* out, void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* Adjoint of ude_ht_forward for gradImage (bn, W, H, 3).  gPts, gNrm fully
+ * written; gSd as in ude_splat_backward; gAttr fully written if ab == bn,
+ * accumulated into (caller zeroed) if ab == 1 < bn. */
+int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothing, const float* gradImage,
+                    const int32_t* idx, const float* ipi, const float* pts, const float* nrm, const float* sd,
+                    int sdBatch, int ab, const float* attr, float* gPts, float* gNrm, float* gSd, float* gAttr,
+                    void* ws, size_t wsBytes, ude_stream_t stream);
+
 /* ---- shading ------------------------------------------------------------ */
 
 /* shading.forwardSh -- replaces cudaShadingShForward (shading.cpp:141-168,

@@ -152,6 +152,11 @@ __global__ void k_splat_forward(long nslots, int N, int W, int H, int k,
     out[t] = fminf(w, 1.0f);
 }
 
+// floats per (view, splat) record of the backward passes: splatting.backward's
+// six weight moments, and the fused estimator's moments + attribute gradient
+constexpr int kMomRec = 8;
+constexpr int kHtRec = 16;
+
 // splatting.backward, slot pass: accumulate per-splat moments in the whitened
 // coordinates z = T d = (u, v):
 //   m0 = sum g w, (m1u, m1v) = sum g w z, (m2uu, m2uv, m2vv) = sum g w z z^T
@@ -175,7 +180,7 @@ __global__ void k_splat_moments(long nslots, int N, int W, int H, int k,
     if (!(w <= 1.0f)) return;
     float gw = gin[t] * w;
     if (gw == 0.0f) return;
-    float* m = mom + 8 * si;
+    float* m = mom + kMomRec * si;
     atomicAdd(m + 0, gw);
     atomicAdd(m + 1, gw * dx);
     atomicAdd(m + 2, gw * dy);
@@ -185,18 +190,28 @@ __global__ void k_splat_moments(long nslots, int N, int W, int H, int k,
 }
 
 // splatting.backward, splat pass: chain the moments through V(A, sigma),
-// A(p, n, s(p)) and s(p) in FP64 with forward-mode derivatives of A.
+// A(p, n, s(p)) and s(p) in FP64 with forward-mode derivatives of A.  A record
+// holds the six moments at [0, 6); records of kHtRec floats (the fused
+// estimator's) also hold the attribute gradient of the (view, splat) at
+// [8, 11), written to gAttr (ab == bn) or added into it (ab == 1 < bn).
 __global__ void k_splat_finalize(int bn, int N, int W, int H, double f, double smoothing,
                                  const float* __restrict__ pts, const float* __restrict__ nrm,
                                  const float* __restrict__ sd, int sdBatch, const SplatRec* __restrict__ rec,
-                                 const float* __restrict__ mom,
+                                 const float* __restrict__ mom, int recLen, int ab, float* __restrict__ gAttr,
                                  float* __restrict__ gPts, float* __restrict__ gNrm, float* __restrict__ gSd)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)bn * N) return;
     long b = t / N, id = t % N;
     long so = 2 * ((sdBatch > 1 ? b * N : 0) + id);
-    const float* m = mom + 8 * t;
+    const float* m = mom + (long)recLen * t;
+    if (gAttr) {
+        float* ga = gAttr + 3 * ((ab > 1 ? b * N : 0) + id);
+        for (int c = 0; c < 3; c++) {
+            if (ab == bn) ga[c] = m[8 + c];
+            else if (m[8 + c] != 0.0f) atomicAdd(ga + c, m[8 + c]);
+        }
+    }
     double m0 = m[0], m1x = m[1], m1y = m[2], m2xx = m[3], m2xy = m[4], m2yy = m[5];
     if (m0 == 0.0 && m1x == 0.0 && m1y == 0.0 && m2xx == 0.0 && m2xy == 0.0 && m2yy == 0.0) {
         for (int c = 0; c < 3; c++) { gPts[3 * t + c] = 0.0f; gNrm[3 * t + c] = 0.0f; }
@@ -263,6 +278,157 @@ __global__ void k_splat_finalize(int bn, int N, int W, int H, double f, double s
     }
     if (sdBatch > 1) { gSd[so] = (float)gs0; gSd[so + 1] = (float)gs1; }
     else { atomicAdd(gSd + so, (float)gs0); atomicAdd(gSd + so + 1, (float)gs1); }
+}
+
+// ---------------------------------------------------------------------------
+// Horvitz-Thompson image estimate (no reference counterpart; it composes
+// splatting.forward, the 1 / pi weighting and interpolation.forward of a
+// three-channel attribute, and their adjoints).  A pixel's samples come first
+// in its slots, then -1 padding -- the layout sampling.samplePoisson writes --
+// so a pixel's slots are read up to its first -1.  Each slot is computed with
+// the FP32 operations of the unfused ops (w = min(ewa, 1), wt = w * ipi,
+// gW = sum_c g_c a_c, gw = gW * ipi, moments of gw * ewa), so the image equals
+// the unfused composition bit for bit and the gradients differ from it only in
+// the order of their atomic sums.  Neither pass stores a per-slot tensor.
+// ---------------------------------------------------------------------------
+
+// one thread per pixel; the channel sums run over the slots in order
+__global__ __launch_bounds__(256) void k_ht_forward(long npix, int N, int W, int H, int k, int ab, int vec4,
+                                                    const int32_t* __restrict__ idx, const float* __restrict__ ipi,
+                                                    const SplatRec* __restrict__ rec,
+                                                    const float* __restrict__ attr, float* __restrict__ out)
+{
+    const long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    const long b = pix / ((long)W * H);
+    const int r = (int)(pix - b * (long)W * H);
+    const int i = r / H, j = r - (r / H) * H;
+    const SplatRec* rv = rec + b * N;
+    const float* av = attr + 3 * (ab > 1 ? b * N : 0);
+    const float px = (float)i, py = (float)j;
+    float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+    auto add = [&](int id, float q) {
+        if (id < 0) return false;
+        const SplatRec s = rv[id];
+        float u, v;
+        const float wt = fminf(ewa_eval(s, px, py, u, v), 1.0f) * q;
+        const float* a = av + 3 * id;
+        o0 += wt * a[0]; o1 += wt * a[1]; o2 += wt * a[2];
+        return true;
+    };
+    if (vec4) {
+        const int4* ip = (const int4*)(idx + pix * k);
+        const float4* qp = (const float4*)(ipi + pix * k);
+        for (int c = 0; c < (k >> 2); c++) {
+            const int4 i4 = ip[c];
+            const float4 q4 = qp[c];
+            if (!add(i4.x, q4.x) || !add(i4.y, q4.y) || !add(i4.z, q4.z) || !add(i4.w, q4.w)) break;
+        }
+    } else {
+        for (int s = 0; s < k; s++)
+            if (!add(idx[pix * k + s], ipi[pix * k + s])) break;
+    }
+    out[3 * pix] = o0; out[3 * pix + 1] = o1; out[3 * pix + 2] = o2;
+}
+
+// The adjoint of one slot: interpolation.backward's attribute gradient g wt and
+// weight gradient gW, times 1 / pi, into splatting.backward's moments.  v[0, 6)
+// are the weight moments (zero for a slot whose unclamped weight exceeds 1),
+// v[6, 9) the attribute gradient.
+__device__ __forceinline__ void ht_slot_adjoint(const SplatRec& s, float px, float py, float g0, float g1, float g2,
+                                                const float* a, float q, float v[9])
+{
+    float u, w;
+    const float we = ewa_eval(s, px, py, u, w);
+    const float wt = fminf(we, 1.0f) * q;
+    v[6] = g0 * wt;
+    v[7] = g1 * wt;
+    v[8] = g2 * wt;
+    float acc = 0.0f;
+    acc += g0 * a[0];
+    acc += g1 * a[1];
+    acc += g2 * a[2];
+    const float gw = (we <= 1.0f) ? (acc * q) * we : 0.0f;
+    v[0] = gw;
+    v[1] = gw * u;
+    v[2] = gw * w;
+    v[3] = (gw * u) * u;
+    v[4] = (gw * u) * w;
+    v[5] = (gw * w) * w;
+}
+
+__device__ __forceinline__ void ht_add_record(float* m, const float v[9])
+{
+    for (int c = 0; c < 6; c++)
+        if (v[c] != 0.0f) atomicAdd(m + c, v[c]);
+    for (int c = 0; c < 3; c++)
+        if (v[6 + c] != 0.0f) atomicAdd(m + 8 + c, v[6 + c]);
+}
+
+// One workgroup per 16 x 16 pixel tile (tiles on blockIdx.x, views on
+// blockIdx.y), one thread per pixel over its samples in slot order.  A splat is
+// sampled by about 2.5 pixels of a tile on average, so the tile first sums its
+// samples per splat in an LDS hash table (open addressing on the splat id) and
+// then adds each splat's sums to its record with one set of device atomics; a
+// sample that finds no free entry within kHtProbes adds its own values directly.
+constexpr int kHtTile = 16;
+constexpr int kHtSlots = 2048;
+constexpr int kHtProbes = 32;
+
+__global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W, int H, int k, int ab,
+                                                                   const float* __restrict__ g,
+                                                                   const int32_t* __restrict__ idx,
+                                                                   const float* __restrict__ ipi,
+                                                                   const SplatRec* __restrict__ rec,
+                                                                   const float* __restrict__ attr,
+                                                                   float* __restrict__ mom)
+{
+    __shared__ int key[kHtSlots];
+    __shared__ float sum[9][kHtSlots];
+    for (int e = threadIdx.x; e < kHtSlots; e += blockDim.x) {
+        key[e] = -1;
+        for (int c = 0; c < 9; c++) sum[c][e] = 0.0f;
+    }
+    __syncthreads();
+    const long b = blockIdx.y;
+    const int tilesY = (H + kHtTile - 1) / kHtTile;
+    const int ti = (int)blockIdx.x / tilesY, tj = (int)blockIdx.x - ti * tilesY;
+    const int i = ti * kHtTile + (int)(threadIdx.x / kHtTile), j = tj * kHtTile + (int)(threadIdx.x % kHtTile);
+    float* mv = mom + (long)kHtRec * b * N;
+    if (i < W && j < H) {
+        const long pix = b * W * H + j + (long)H * i;
+        const float px = (float)i, py = (float)j;
+        const float g0 = g[3 * pix], g1 = g[3 * pix + 1], g2 = g[3 * pix + 2];
+        const float* av = attr + 3 * (ab > 1 ? b * N : 0);
+        const SplatRec* rv = rec + b * N;
+        for (int s = 0; s < k; s++) {
+            const int id = idx[pix * k + s];
+            if (id < 0) break;
+            float v[9];
+            ht_slot_adjoint(rv[id], px, py, g0, g1, g2, av + 3 * id, ipi[pix * k + s], v);
+            int e = (int)(((uint32_t)id * 2654435761u) >> 21);            // 11 bits: kHtSlots entries
+            int slot = -1;
+            for (int t = 0; t < kHtProbes; t++) {
+                const int old = atomicCAS(&key[e], -1, id);
+                if (old == -1 || old == id) { slot = e; break; }
+                e = (e + 1) & (kHtSlots - 1);
+            }
+            if (slot >= 0) {
+                for (int c = 0; c < 9; c++)
+                    if (v[c] != 0.0f) atomicAdd(&sum[c][slot], v[c]);
+            } else {
+                ht_add_record(mv + (long)kHtRec * id, v);
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kHtSlots; e += blockDim.x) {
+        const int id = key[e];
+        if (id < 0) continue;
+        float v[9];
+        for (int c = 0; c < 9; c++) v[c] = sum[c][e];
+        ht_add_record(mv + (long)kHtRec * id, v);
+    }
 }
 
 // sampling.covariance (samplingCuda.cu:323-361), FP64 internally
@@ -761,7 +927,7 @@ extern "C" {
 
 size_t ude_splat_workspace_bytes(int bn, int N)
 {
-    return (size_t)bn * N * (sizeof(SplatRec) + 8 * sizeof(float)) + 256;
+    return (size_t)bn * N * (sizeof(SplatRec) + kMomRec * sizeof(float)) + 256;
 }
 
 int ude_splat_forward(int bn, int N, int W, int H, int k, double f, double smoothing,
@@ -793,10 +959,10 @@ int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoo
     SplatRec* rec = (SplatRec*)ws;
     float* mom = (float*)(rec + (long)bn * N);
     k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
-    k_fill_f32<<<grid1(8L * bn * N, 256), 256, 0, st>>>(mom, 8L * bn * N, 0.0f);
+    k_fill_f32<<<grid1((long)kMomRec * bn * N, 256), 256, 0, st>>>(mom, (long)kMomRec * bn * N, 0.0f);
     k_splat_moments<<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, gradIn, rec, mom);
     k_splat_finalize<<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
-                                                              rec, mom, gPts, gNrm, gSd);
+                                                              rec, mom, kMomRec, 1, nullptr, gPts, gNrm, gSd);
     return launch_status();
 }
 
@@ -867,6 +1033,48 @@ int ude_interp_backward(int bn, int W, int H, int k, int ab, int N, int C, const
     if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0) return UDE_ERR_ARG;
     long nslots = (long)bn * W * H * k;
     k_interp_backward<<<grid1(nslots, 256), 256, 0, st>>>(nslots, W, H, k, ab, N, C, g, idx, wts, attr, gW, gA);
+    return launch_status();
+}
+
+size_t ude_ht_workspace_bytes(int bn, int N)
+{
+    return (size_t)bn * N * (sizeof(SplatRec) + kHtRec * sizeof(float)) + 256;
+}
+
+int ude_ht_forward(int bn, int N, int W, int H, int k, double f, double smoothing, const int32_t* idx,
+                   const float* ipi, const float* pts, const float* nrm, const float* sd, int sdBatch, int ab,
+                   const float* attr, float* out, void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0 || (ab != 1 && ab != bn)) return UDE_ERR_ARG;
+    const long npix = (long)bn * W * H;
+    if (N == 0) {
+        k_fill_f32<<<grid1(3 * npix, 256), 256, 0, st>>>(out, 3 * npix, 0.0f);
+        return launch_status();
+    }
+    if (wsBytes < ude_ht_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
+    SplatRec* rec = (SplatRec*)ws;
+    k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+    const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)ipi % 16 == 0);
+    k_ht_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, N, W, H, k, ab, vec4, idx, ipi, rec, attr, out);
+    return launch_status();
+}
+
+int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothing, const float* gradImage,
+                    const int32_t* idx, const float* ipi, const float* pts, const float* nrm, const float* sd,
+                    int sdBatch, int ab, const float* attr, float* gPts, float* gNrm, float* gSd, float* gAttr,
+                    void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0 || (ab != 1 && ab != bn)) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    if (wsBytes < ude_ht_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
+    SplatRec* rec = (SplatRec*)ws;
+    float* mom = (float*)(rec + (long)bn * N);
+    k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+    k_fill_f32<<<grid1((long)kHtRec * bn * N, 256), 256, 0, st>>>(mom, (long)kHtRec * bn * N, 0.0f);
+    const dim3 tiles((unsigned)(((W + kHtTile - 1) / kHtTile) * ((H + kHtTile - 1) / kHtTile)), (unsigned)bn);
+    k_ht_backward<<<tiles, kHtTile * kHtTile, 0, st>>>(N, W, H, k, ab, gradImage, idx, ipi, rec, attr, mom);
+    k_splat_finalize<<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
+                                                              rec, mom, kHtRec, ab, gAttr, gPts, gNrm, gSd);
     return launch_status();
 }
 

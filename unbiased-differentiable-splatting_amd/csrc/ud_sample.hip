@@ -696,13 +696,11 @@ __global__ __launch_bounds__(256) void k_fill_i32(int* p, const ViewState* __res
     if (t < n) p[t] = v;
 }
 
-// Merge the two (complete) children of internal node p
-// (buildDynamicRadixTree.cuh:126-223, with the children's own centres).
-__device__ __forceinline__ void merge_node(int p, float h, float prec, NodeGeo* geo, NodeLnk* lnk)
+// The record of an internal node from its two complete children, left then
+// right (buildDynamicRadixTree.cuh:126-223, with the children's own centres).
+__device__ __forceinline__ NodeGeo merge_geo(const NodeGeo& gl, int cl, const NodeGeo& gr, int cr, float h,
+                                             float prec)
 {
-    NodeLnk L = lnk[p];
-    NodeGeo gl = geo[L.left], gr = geo[L.right];
-    int cl = lnk[L.left].count, cr = lnk[L.right].count;
     NodeGeo g;
     g.lox = fminf(gl.lox, gr.lox); g.loy = fminf(gl.loy, gr.loy);
     g.hix = fmaxf(gl.hix, gr.hix); g.hiy = fmaxf(gl.hiy, gr.hiy);
@@ -726,18 +724,77 @@ __device__ __forceinline__ void merge_node(int p, float h, float prec, NodeGeo* 
     }
     g.deg = deg;
     g.r2 = trunc_r2(h, prec, cn);
-    geo[p] = g;
-    lnk[p].count = cn;
+    return g;
+}
+
+// One fan-in step at parent p of node nd, whose record g (count cnt) the thread
+// holds: the second arrival loads the sibling's record (the sibling's count is
+// p's leaf count minus cnt), merges, stores p and continues from it.
+__device__ __forceinline__ void merge_up(int p, const NodeLnk& L, const NodeAux& a, int& nd, NodeGeo& g, int& cnt,
+                                         float h, float prec, NodeGeo* geo, NodeLnk* lnk)
+{
+    const bool isLeft = L.left == nd;
+    const NodeGeo gs = geo[isLeft ? L.right : L.left];
+    const int cs = (a.last - a.first + 1) - cnt;
+    const NodeGeo gl = isLeft ? g : gs, gr = isLeft ? gs : g;
+    const int cl = isLeft ? cnt : cs, cr = isLeft ? cs : cnt;
+    const NodeGeo m = merge_geo(gl, cl, gr, cr, h, prec);
+    geo[p] = m;
+    lnk[p].count = cl + cr;
+    g = m;
+    cnt = cl + cr;
+    nd = p;
 }
 
 // Bottom-up pass, level 1: one workgroup per block of 1024 sorted leaves.  The
 // second thread to reach a node whose leaf range lies inside the block merges
-// it; both arrivals come from the same workgroup, so workgroup-scope fences
-// order the exchange.  A thread that reaches a node spanning the block's
-// boundary hands its (complete) node to level 2.
+// it.  Such a node's index lies inside the block too (a Karras node's index is
+// one end of its range), so its arrival counter is an LDS word; both arrivals
+// come from the same workgroup, so workgroup-scope fences order the exchange
+// of the children's records.  A thread that reaches a node spanning the
+// block's boundary hands its (complete) node to level 2.
 constexpr int kFanBlock = 1024;
 __global__ __launch_bounds__(kFanBlock) void k_bottom_up_local(ViewState* vs, NodeGeo* geo, NodeLnk* lnk,
-                                                               const NodeAux* __restrict__ aux, int* flags, int* seeds, VStride vsd)
+                                                               const NodeAux* __restrict__ aux, int* seeds, VStride vsd)
+{
+    __shared__ int arrivals[kFanBlock];
+    vs = vw(vs, vsd.ws);
+    geo = vw(geo, vsd.ws);
+    lnk = vw(lnk, vsd.ws);
+    aux = vw(aux, vsd.ws);
+    seeds = vw(seeds, vsd.ws);
+    arrivals[threadIdx.x] = 0;
+    __syncthreads();
+    int n = vs->nc;
+    int b0 = blockIdx.x * kFanBlock;
+    int pos = b0 + threadIdx.x;
+    if (pos >= n || n < 2) return;
+    float h = vs->h, prec = vs->prec;
+    int nd = (n - 1) + pos;
+    NodeGeo g = geo[nd];
+    int cnt = 1;
+    int p = lnk[nd].parent;
+    while (p >= 0) {
+        const NodeAux a = aux[p];
+        const NodeLnk L = lnk[p];
+        if (a.first < b0 || a.last >= b0 + kFanBlock) {
+            int q = atomicAdd(&vs->nseeds, 1);
+            seeds[q] = nd;
+            return;
+        }
+        __threadfence_block();
+        int old = atomicAdd(&arrivals[p - b0], 1);
+        if (old == 0) return;
+        __threadfence_block();
+        merge_up(p, L, a, nd, g, cnt, h, prec, geo, lnk);
+        p = L.parent;
+    }
+}
+
+// Bottom-up pass, level 2: the few block-spanning nodes, device-scope fan-in.
+__global__ __launch_bounds__(256) void k_bottom_up_global(const ViewState* __restrict__ vs, NodeGeo* geo, NodeLnk* lnk,
+                                                          const NodeAux* __restrict__ aux, int* flags,
+                                                          const int* __restrict__ seeds, VStride vsd)
 {
     vs = vw(vs, vsd.ws);
     geo = vw(geo, vsd.ws);
@@ -745,52 +802,22 @@ __global__ __launch_bounds__(kFanBlock) void k_bottom_up_local(ViewState* vs, No
     aux = vw(aux, vsd.ws);
     flags = vw(flags, vsd.ws);
     seeds = vw(seeds, vsd.ws);
-    int n = vs->nc;
-    int b0 = blockIdx.x * kFanBlock;
-    int pos = b0 + threadIdx.x;
-    if (pos >= n || n < 2) return;
-    float h = vs->h, prec = vs->prec;
-    int nd = (n - 1) + pos;
-    int p = lnk[nd].parent;
-    while (p >= 0) {
-        NodeAux a = aux[p];
-        if (a.first < b0 || a.last >= b0 + kFanBlock) {
-            int q = atomicAdd(&vs->nseeds, 1);
-            seeds[q] = nd;
-            return;
-        }
-        __threadfence_block();
-        int old = atomicAdd(&flags[p], 1);
-        if (old == 0) return;
-        __threadfence_block();
-        merge_node(p, h, prec, geo, lnk);
-        nd = p;
-        p = lnk[p].parent;
-    }
-}
-
-// Bottom-up pass, level 2: the few block-spanning nodes, device-scope fan-in.
-__global__ __launch_bounds__(256) void k_bottom_up_global(const ViewState* __restrict__ vs, NodeGeo* geo, NodeLnk* lnk,
-                                                          int* flags, const int* __restrict__ seeds, VStride vsd)
-{
-    vs = vw(vs, vsd.ws);
-    geo = vw(geo, vsd.ws);
-    lnk = vw(lnk, vsd.ws);
-    flags = vw(flags, vsd.ws);
-    seeds = vw(seeds, vsd.ws);
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= vs->nseeds) return;
     float h = vs->h, prec = vs->prec;
     int nd = seeds[t];
+    NodeGeo g = geo[nd];
+    int cnt = lnk[nd].count;
     int p = lnk[nd].parent;
     while (p >= 0) {
+        const NodeAux a = aux[p];
+        const NodeLnk L = lnk[p];
         __threadfence();
         int old = atomicAdd(&flags[p], 1);
         if (old == 0) return;
         __threadfence();
-        merge_node(p, h, prec, geo, lnk);
-        nd = p;
-        p = lnk[p].parent;
+        merge_up(p, L, a, nd, g, cnt, h, prec, geo, lnk);
+        p = L.parent;
     }
 }
 
@@ -2225,8 +2252,8 @@ int build_structure(int N, int W, int H, double f, double smoothing, double prec
     k_hierarchy<<<dim3(grid1(N, 256), views), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux, w.leafXY, vsd);
     k_leaf_ids<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, iin, w.aux, vsd);
     k_fill_i32<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.flags, w.vs, 0, 0, vsd);
-    k_bottom_up_local<<<dim3(grid1(N, kFanBlock), views), kFanBlock, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.flags, w.seeds, vsd);
-    k_bottom_up_global<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.flags, w.seeds, vsd);
+    k_bottom_up_local<<<dim3(grid1(N, kFanBlock), views), kFanBlock, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.seeds, vsd);
+    k_bottom_up_global<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.flags, w.seeds, vsd);
     hipError_t e0 = hipGetLastError();
     return e0 == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
 }

@@ -1,0 +1,77 @@
+"""estimator -- the Horvitz-Thompson image estimate over Poisson-rate samples
+and its adjoint, each fused into one per-pixel pass (no reference counterpart:
+it composes splatting.forward, the 1 / pi weighting and interpolation.forward
+of a three-channel attribute, splatting.cpp:43-82 and interpolation.cpp:31-58,
+with their backward ops).
+
+``indices`` / ``invPi`` are sampling.samplePoisson's outputs: per pixel its
+samples first, then -1 (0) padding; a pixel's slots after its first -1 are not
+read.  The image equals
+
+    interpolation.forward(indices, splatting.forward(...) * invPi, attribute)
+
+bit for bit; the gradients equal the unfused backward ops' up to the order of
+their atomic sums.  No per-slot tensor is stored in either pass.
+"""
+import torch
+
+from . import _lib
+
+
+def _inputs(indices, invPi, points, normals, stdDevs, attribute):
+    indices = _lib.i32(indices, "indices")
+    invPi = _lib.f32(invPi, "invPi")
+    points = _lib.f32(points, "points")
+    normals = _lib.f32(normals, "normals")
+    stdDevs = _lib.f32(stdDevs, "stdDevs")
+    attribute = _lib.f32(attribute, "attribute")
+    if invPi.shape != indices.shape:
+        raise ValueError("invPi %s must match indices %s" % (tuple(invPi.shape), tuple(indices.shape)))
+    if attribute.dim() != 3 or attribute.shape[2] != 3:
+        raise ValueError("attribute must be (1 | bn, N, 3), got %s" % (tuple(attribute.shape),))
+    return indices, invPi, points, normals, stdDevs, attribute
+
+
+def forward(width, height, focallength, indices, invPi, points, normals, stdDevs, attribute, smoothing=0.0):
+    """image (bn, W, H, 3) = sum over a pixel's samples of min(w, 1) / pi * attribute."""
+    dev = _lib.require_gpu(indices, invPi, points, normals, stdDevs, attribute)
+    indices, invPi, points, normals, stdDevs, attribute = _inputs(indices, invPi, points, normals, stdDevs,
+                                                                  attribute)
+    bn, W, H, k = indices.shape
+    N = stdDevs.shape[1]
+    out = torch.empty((bn, W, H, 3), dtype=torch.float32, device=dev)
+    L = _lib.lib()
+    ws = _lib.workspace(L.ude_ht_workspace_bytes(bn, N), dev)
+    rc = L.ude_ht_forward(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(indices), _lib.ptr(invPi),
+                          _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs), stdDevs.shape[0],
+                          attribute.shape[0], _lib.ptr(attribute), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                          _lib.stream(dev))
+    _lib.check(rc, "estimator.forward")
+    return out
+
+
+def backward(width, height, focallength, gradImage, indices, invPi, points, normals, stdDevs, attribute,
+             smoothing=0.0):
+    """[gradPoints, gradNormals, gradStdDevs, gradAttribute] of sum(gradImage * image)."""
+    dev = _lib.require_gpu(gradImage, indices, invPi, points, normals, stdDevs, attribute)
+    gradImage = _lib.f32(gradImage, "gradImage")
+    indices, invPi, points, normals, stdDevs, attribute = _inputs(indices, invPi, points, normals, stdDevs,
+                                                                  attribute)
+    bn, W, H, k = indices.shape
+    if tuple(gradImage.shape) != (bn, W, H, 3):
+        raise ValueError("gradImage must be %s, got %s" % ((bn, W, H, 3), tuple(gradImage.shape)))
+    N = stdDevs.shape[1]
+    gp = torch.empty(points.shape, dtype=torch.float32, device=dev)
+    gn = torch.empty(normals.shape, dtype=torch.float32, device=dev)
+    gs = (torch.zeros if stdDevs.shape[0] == 1 else torch.empty)(stdDevs.shape, dtype=torch.float32, device=dev)
+    ga = (torch.empty if attribute.shape[0] == bn else torch.zeros)(attribute.shape, dtype=torch.float32,
+                                                                    device=dev)
+    L = _lib.lib()
+    ws = _lib.workspace(L.ude_ht_workspace_bytes(bn, N), dev)
+    rc = L.ude_ht_backward(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(gradImage),
+                           _lib.ptr(indices), _lib.ptr(invPi), _lib.ptr(points), _lib.ptr(normals),
+                           _lib.ptr(stdDevs), stdDevs.shape[0], attribute.shape[0], _lib.ptr(attribute),
+                           _lib.ptr(gp), _lib.ptr(gn), _lib.ptr(gs), _lib.ptr(ga), _lib.ptr(ws), ws.numel(),
+                           _lib.stream(dev))
+    _lib.check(rc, "estimator.backward")
+    return [gp, gn, gs, ga]

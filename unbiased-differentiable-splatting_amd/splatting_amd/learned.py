@@ -149,29 +149,26 @@ class PoissonFrame:
         return P, Nc, nb
 
     def image(self, seed, tile_scale=None, rate=None, kcap=None, seed_tensor=None):
-        """Sampled image estimate (V, W, H, 3) and the sample tensors."""
-        from cudaExtensions import sampling, splatting, interpolation
+        """Sampled image estimate (V, W, H, 3) (EWA weights, 1/pi and albedo
+        interpolation fused in one pass) and the sample tensors."""
+        from cudaExtensions import sampling, estimator
         P, Nc, nb = self.camera()
         ids, ipi = sampling.samplePoisson(self.W, self.H, self.f, P, nb, None, self.S, kcap or self.kcap, 0.0,
                                           self.prec, seed=seed, rate=rate or self.rate, tileScale=tile_scale,
                                           densities=self.dens, seedTensor=seed_tensor)
-        w = splatting.forward(self.W, self.H, self.f, ids, P, Nc, self.S, 0.0)
-        wt = (w * ipi).contiguous()
-        img = interpolation.forward(ids, wt, self.A)
+        img = estimator.forward(self.W, self.H, self.f, ids, ipi, P, Nc, self.S, self.A)
         self.last_ids = ids
-        return img, (P, Nc, ids, ipi, wt)
+        return img, (P, Nc, ids, ipi)
 
     def step(self, seed, tile_scale=None, rate=None, kcap=None, seed_tensor=None, with_loss=True):
         """Returns (loss, image, grads (positions, normals, std-devs, albedo), dL/dimage);
         loss is None without with_loss (a frame captured in a graph skips the reduction)."""
-        from cudaExtensions import splatting, interpolation
+        from cudaExtensions import estimator
         tr = self._transform
-        img, (P, Nc, ids, ipi, wt) = self.image(seed, tile_scale, rate, kcap, seed_tensor)
+        img, (P, Nc, ids, ipi) = self.image(seed, tile_scale, rate, kcap, seed_tensor)
         g = (2.0 * (img - self.target)).contiguous()
         loss = ((img - self.target) ** 2).sum() if with_loss else None
-        gW, gA = interpolation.backward(g, ids, wt, self.A)
-        gw = (gW * ipi).contiguous()
-        gP, gN, gS = splatting.backward(self.W, self.H, self.f, gw, ids, P, Nc, self.S, 0.0)
+        gP, gN, gS, gA = estimator.backward(self.W, self.H, self.f, g, ids, ipi, P, Nc, self.S, self.A)
         _, gPw = tr.affineBackward(gP, self.M, self.Pw)
         _, gNw = tr.linearBackward(gN, self.M, self.Nw)
         return loss, img, (gPw, gNw, gS, gA), g
@@ -226,7 +223,7 @@ def tile_variance(frame, seeds, rate=None):
         s1 = v if s1 is None else s1 + v
         s2 = v * v if s2 is None else s2 + v * v
         g_img = g if g_img is None else g_img + g
-        _, (P, Nc, ids, ipi, wt) = frame.image(s, rate=rate)
+        _, (P, Nc, ids, ipi) = frame.image(s, rate=rate)
         c = tile_counts(ids).double()
         cnt = c if cnt is None else cnt + c
     S = len(seeds)

@@ -8,12 +8,14 @@ sum_s w_s a_s / pi_s (EWA weights w, albedo a, inclusion probabilities pi,
 no occlusion).  The sample set is a pure function of the splats, the Philox key
 and the rates, so with ``recompute=True`` the backward draws it again from the
 same key instead of keeping the (bn, W, H, kcap) index and 1/pi tensors alive
-between the passes: the forward saves only the per-splat inputs.  Gradients are
-the raw adjoints (no clipping), as unbiasedness requires.
+between the passes: the forward saves only the per-splat inputs.  Both passes
+run the fused estimator (EWA weight, 1/pi and interpolation per sample, no
+per-slot tensor).  Gradients are the raw adjoints (no clipping), as
+unbiasedness requires.
 """
 import torch as th
 
-from cudaExtensions import interpolation, sampling, splatting
+from cudaExtensions import estimator, sampling
 
 from .extensions import AffineTransform, LinearTransform
 
@@ -21,9 +23,7 @@ from .extensions import AffineTransform, LinearTransform
 def _draw(P, Nc, S, args):
     W, H, f, kcap, prec, rate, seed, ts = args
     nb = th.where(Nc[..., 2:3] > 0, -Nc, Nc).contiguous()
-    ids, ipi = sampling.samplePoisson(W, H, f, P, nb, None, S, kcap, 0.0, prec, seed=seed, rate=rate, tileScale=ts)
-    w = splatting.forward(W, H, f, ids, P, Nc, S, 0.0)
-    return ids, ipi, (w * ipi).contiguous()
+    return sampling.samplePoisson(W, H, f, P, nb, None, S, kcap, 0.0, prec, seed=seed, rate=rate, tileScale=ts)
 
 
 class PoissonRender(th.autograd.Function):
@@ -33,27 +33,25 @@ class PoissonRender(th.autograd.Function):
         P, Nc, S, A = (x.detach().contiguous() for x in (points, normals, stdDevs, albedo))
         args = (int(width), int(height), float(focallength), int(numSamples), float(precision), float(rate),
                 int(seed), tileScale)
-        ids, ipi, wt = _draw(P, Nc, S, args)
-        img = interpolation.forward(ids, wt, A)
+        ids, ipi = _draw(P, Nc, S, args)
+        img = estimator.forward(args[0], args[1], args[2], ids, ipi, P, Nc, S, A)
         ctx.args = args
         ctx.recompute = bool(recompute)
         if ctx.recompute:
             ctx.save_for_backward(P, Nc, S, A)
         else:
-            ctx.save_for_backward(P, Nc, S, A, ids, ipi, wt)
+            ctx.save_for_backward(P, Nc, S, A, ids, ipi)
         return img
 
     @staticmethod
     def backward(ctx, gradImage):
         if ctx.recompute:
             P, Nc, S, A = ctx.saved_tensors
-            ids, ipi, wt = _draw(P, Nc, S, ctx.args)
+            ids, ipi = _draw(P, Nc, S, ctx.args)
         else:
-            P, Nc, S, A, ids, ipi, wt = ctx.saved_tensors
+            P, Nc, S, A, ids, ipi = ctx.saved_tensors
         W, H, f = ctx.args[:3]
-        gW, gA = interpolation.backward(gradImage.contiguous(), ids, wt, A)
-        gw = (gW * ipi).contiguous()
-        gP, gN, gS = splatting.backward(W, H, f, gw, ids, P, Nc, S, 0.0)
+        gP, gN, gS, gA = estimator.backward(W, H, f, gradImage.contiguous(), ids, ipi, P, Nc, S, A)
         return gP, gN, gS, gA, None, None, None, None, None, None, None, None, None
 
 
