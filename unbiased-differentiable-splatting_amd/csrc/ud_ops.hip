@@ -194,11 +194,13 @@ __global__ void k_splat_moments(long nslots, int N, int W, int H, int k,
 // holds the six moments at [0, 6); records of kHtRec floats (the fused
 // estimator's) also hold the attribute gradient of the (view, splat) at
 // [8, 11), written to gAttr (ab == bn) or added into it (ab == 1 < bn).
-__global__ void k_splat_finalize(int bn, int N, int W, int H, double f, double smoothing,
-                                 const float* __restrict__ pts, const float* __restrict__ nrm,
-                                 const float* __restrict__ sd, int sdBatch, const SplatRec* __restrict__ rec,
-                                 const float* __restrict__ mom, int recLen, int ab, float* __restrict__ gAttr,
-                                 float* __restrict__ gPts, float* __restrict__ gNrm, float* __restrict__ gSd)
+__global__ __launch_bounds__(128) void k_splat_finalize(int bn, int N, int W, int H, double f, double smoothing,
+                                                        const float* __restrict__ pts, const float* __restrict__ nrm,
+                                                        const float* __restrict__ sd, int sdBatch,
+                                                        const SplatRec* __restrict__ rec,
+                                                        const float* __restrict__ mom, int recLen, int ab,
+                                                        float* __restrict__ gAttr, float* __restrict__ gPts,
+                                                        float* __restrict__ gNrm, float* __restrict__ gSd)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)bn * N) return;
@@ -292,6 +294,35 @@ __global__ void k_splat_finalize(int bn, int N, int W, int H, double f, double s
 // the order of their atomic sums.  Neither pass stores a per-slot tensor.
 // ---------------------------------------------------------------------------
 
+// Four consecutive slots of a pixel (k % 4 == 0, 16-byte aligned rows): the
+// records and attributes of all four are requested before any is used, so a
+// lane has eight gathers in flight instead of two; ids past the first -1 are
+// read as splat 0 (in bounds) and never used.
+struct SlotQuad {
+    int id[4];
+    float q[4];
+    SplatRec r[4];
+    float a[4][3];
+};
+
+__device__ __forceinline__ int slot_quad(const int4* ip, const float4* qp, int c, const SplatRec* rv,
+                                         const float* av, SlotQuad& Q)
+{
+    const int4 i4 = ip[c];
+    const float4 q4 = qp[c];
+    Q.id[0] = i4.x; Q.id[1] = i4.y; Q.id[2] = i4.z; Q.id[3] = i4.w;
+    Q.q[0] = q4.x; Q.q[1] = q4.y; Q.q[2] = q4.z; Q.q[3] = q4.w;
+    int n = 0;
+    while (n < 4 && Q.id[n] >= 0) n++;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int id = u < n ? Q.id[u] : 0;
+        Q.r[u] = rv[id];
+        Q.a[u][0] = av[3 * id]; Q.a[u][1] = av[3 * id + 1]; Q.a[u][2] = av[3 * id + 2];
+    }
+    return n;                                    // samples in this quad, in slot order
+}
+
 // one thread per pixel; the channel sums run over the slots in order
 __global__ __launch_bounds__(256) void k_ht_forward(long npix, int N, int W, int H, int k, int ab, int vec4,
                                                     const int32_t* __restrict__ idx, const float* __restrict__ ipi,
@@ -307,26 +338,28 @@ __global__ __launch_bounds__(256) void k_ht_forward(long npix, int N, int W, int
     const float* av = attr + 3 * (ab > 1 ? b * N : 0);
     const float px = (float)i, py = (float)j;
     float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
-    auto add = [&](int id, float q) {
-        if (id < 0) return false;
-        const SplatRec s = rv[id];
+    auto add = [&](const SplatRec& sr, const float* a, float q) {
         float u, v;
-        const float wt = fminf(ewa_eval(s, px, py, u, v), 1.0f) * q;
-        const float* a = av + 3 * id;
+        const float wt = fminf(ewa_eval(sr, px, py, u, v), 1.0f) * q;
         o0 += wt * a[0]; o1 += wt * a[1]; o2 += wt * a[2];
-        return true;
     };
     if (vec4) {
         const int4* ip = (const int4*)(idx + pix * k);
         const float4* qp = (const float4*)(ipi + pix * k);
         for (int c = 0; c < (k >> 2); c++) {
-            const int4 i4 = ip[c];
-            const float4 q4 = qp[c];
-            if (!add(i4.x, q4.x) || !add(i4.y, q4.y) || !add(i4.z, q4.z) || !add(i4.w, q4.w)) break;
+            SlotQuad Q;
+            const int n = slot_quad(ip, qp, c, rv, av, Q);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (u < n) add(Q.r[u], Q.a[u], Q.q[u]);
+            if (n < 4) break;
         }
     } else {
-        for (int s = 0; s < k; s++)
-            if (!add(idx[pix * k + s], ipi[pix * k + s])) break;
+        for (int s = 0; s < k; s++) {
+            const int id = idx[pix * k + s];
+            if (id < 0) break;
+            add(rv[id], av + 3 * id, ipi[pix * k + s]);
+        }
     }
     out[3 * pix] = o0; out[3 * pix + 1] = o1; out[3 * pix + 2] = o2;
 }
@@ -375,7 +408,7 @@ constexpr int kHtTile = 16;
 constexpr int kHtSlots = 2048;
 constexpr int kHtProbes = 32;
 
-__global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W, int H, int k, int ab,
+__global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W, int H, int k, int ab, int vec4,
                                                                    const float* __restrict__ g,
                                                                    const int32_t* __restrict__ idx,
                                                                    const float* __restrict__ ipi,
@@ -401,11 +434,9 @@ __global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W,
         const float g0 = g[3 * pix], g1 = g[3 * pix + 1], g2 = g[3 * pix + 2];
         const float* av = attr + 3 * (ab > 1 ? b * N : 0);
         const SplatRec* rv = rec + b * N;
-        for (int s = 0; s < k; s++) {
-            const int id = idx[pix * k + s];
-            if (id < 0) break;
+        auto sample = [&](int id, const SplatRec& sr, const float* a, float q) {
             float v[9];
-            ht_slot_adjoint(rv[id], px, py, g0, g1, g2, av + 3 * id, ipi[pix * k + s], v);
+            ht_slot_adjoint(sr, px, py, g0, g1, g2, a, q, v);
             int e = (int)(((uint32_t)id * 2654435761u) >> 21);            // 11 bits: kHtSlots entries
             int slot = -1;
             for (int t = 0; t < kHtProbes; t++) {
@@ -418,6 +449,24 @@ __global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W,
                     if (v[c] != 0.0f) atomicAdd(&sum[c][slot], v[c]);
             } else {
                 ht_add_record(mv + (long)kHtRec * id, v);
+            }
+        };
+        if (vec4) {
+            const int4* ip = (const int4*)(idx + pix * k);
+            const float4* qp = (const float4*)(ipi + pix * k);
+            for (int c = 0; c < (k >> 2); c++) {
+                SlotQuad Q;
+                const int n = slot_quad(ip, qp, c, rv, av, Q);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (u < n) sample(Q.id[u], Q.r[u], Q.a[u], Q.q[u]);
+                if (n < 4) break;
+            }
+        } else {
+            for (int s = 0; s < k; s++) {
+                const int id = idx[pix * k + s];
+                if (id < 0) break;
+                sample(id, rv[id], av + 3 * id, ipi[pix * k + s]);
             }
         }
     }
@@ -640,19 +689,20 @@ __global__ __launch_bounds__(256) void k_transform_backward(int affine, int bn, 
 {
     __shared__ float red[4][12];
     int b = blockIdx.y;
-    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
     float acc[12];
     for (int q = 0; q < 12; q++) acc[q] = 0.0f;
-    if (x < N) {
-        const float* m = M + 16 * b;
+    const float* m = M + 16 * b;
+    // grid-stride over the points: few workgroups per view, so the 12 matrix
+    // sums see few atomics (they all land on the same 12 words)
+    for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += (long)gridDim.x * blockDim.x) {
         long vo = 3 * ((vb > 1 ? (long)b * N : 0) + x);
         long go = 3 * ((long)b * N + x);
         float g0 = g[go], g1 = g[go + 1], g2 = g[go + 2];
         float a[3] = { v[vo], v[vo + 1], v[vo + 2] };
         float gg[3] = { g0, g1, g2 };
         for (int r = 0; r < 3; r++) {
-            for (int c = 0; c < 3; c++) acc[4 * r + c] = gg[r] * a[c];
-            acc[4 * r + 3] = affine ? gg[r] : 0.0f;
+            for (int c = 0; c < 3; c++) acc[4 * r + c] += gg[r] * a[c];
+            if (affine) acc[4 * r + 3] += gg[r];
         }
         float gv[3];
         for (int c = 0; c < 3; c++) gv[c] = m[c] * g0 + m[4 + c] * g1 + m[8 + c] * g2;
@@ -1012,7 +1062,8 @@ int ude_transform_backward(int affine, int bn, int vb, int N, const float* g, co
 {
     if (bn <= 0 || N < 0 || (vb != 1 && vb != bn)) return UDE_ERR_ARG;
     if (N == 0) return UDE_OK;
-    dim3 grid(grid1(N, 256), bn);
+    const unsigned blocks = grid1(N, 256) < 512u ? grid1(N, 256) : 512u;
+    dim3 grid(blocks, bn);
     k_transform_backward<<<grid, 256, 0, st>>>(affine, bn, vb, N, g, M, v, gM, gV);
     return launch_status();
 }
@@ -1072,7 +1123,8 @@ int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothi
     k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
     k_fill_f32<<<grid1((long)kHtRec * bn * N, 256), 256, 0, st>>>(mom, (long)kHtRec * bn * N, 0.0f);
     const dim3 tiles((unsigned)(((W + kHtTile - 1) / kHtTile) * ((H + kHtTile - 1) / kHtTile)), (unsigned)bn);
-    k_ht_backward<<<tiles, kHtTile * kHtTile, 0, st>>>(N, W, H, k, ab, gradImage, idx, ipi, rec, attr, mom);
+    const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)ipi % 16 == 0);
+    k_ht_backward<<<tiles, kHtTile * kHtTile, 0, st>>>(N, W, H, k, ab, vec4, gradImage, idx, ipi, rec, attr, mom);
     k_splat_finalize<<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
                                                               rec, mom, kHtRec, ab, gAttr, gPts, gNrm, gSd);
     return launch_status();
