@@ -1503,12 +1503,20 @@ static float o_puniform(uint32_t seed_lo, uint32_t seed_hi, uint32_t view, uint3
 }
 
 /* mixture proposal at point (x, y): t0 g0 + t1 g1 */
-static float o_mix(const OFixed* F, float t0, float t1, float qx, float qy, float x, float y)
+/* proposal mixture at q for a point s: k0 e^{-a0 d2} + k1 e^{-a1 d2} with
+ * d2 = |q - s|^2, k_b = t_b c_b, a_b = is2h_b^2 = 1 / 2h_b^2 (one exponential
+ * per band; this build's definition, no reference counterpart) */
+typedef struct { float k0, a0, k1, a1; } OMix;
+
+static float o_mix(const OMix* M, float qx, float qy, float x, float y)
 {
-    float v0 = o_band_point(&F->band[0], qx, qy, x, y);
-    float v1 = o_band_point(&F->band[1], qx, qy, x, y);
-    float m = t0 * v0;
-    m = m + t1 * v1;
+    const float dx = qx - x, dy = qy - y;
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    const float e0 = oexpf(-(d2 * M->a0));
+    const float e1 = oexpf(-(d2 * M->a1));
+    float m = M->k0 * e0;
+    m = m + M->k1 * e1;
     return m;
 }
 
@@ -1526,6 +1534,9 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
      * D_b (truncated at the query radius) misses the nearby splats it normalises */
     const float t0 = 0.5f / fmaxf(D0, F->band[0].c);
     const float t1 = 0.5f / fmaxf(D1, F->band[1].c);
+    OMix M;
+    M.k0 = t0 * F->band[0].c; M.k1 = t1 * F->band[1].c;
+    M.a0 = F->band[0].is2h * F->band[0].is2h; M.a1 = F->band[1].is2h * F->band[1].is2h;
     const int L0 = T->nc - 1;
     const uint32_t pixel = (uint32_t)(j + H * i);
     uint32_t draw = 0;
@@ -1535,7 +1546,7 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
         const float* b = T->box + 4 * nd;
         float cx = fminf(fmaxf(qx, b[0]), b[2]);
         float cy = fminf(fmaxf(qy, b[1]), b[3]);
-        float pB = o_mix(F, t0, t1, qx, qy, cx, cy);     /* proposal bound over the node */
+        float pB = o_mix(&M, qx, qy, cx, cy);            /* proposal bound over the node */
         float U = fminf(1.0f, lambda * pB);
         int descend = 0;
         if (pB >= O_PMIN && U > 0.0f) {
@@ -1553,7 +1564,7 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
                     if (!(r < (float)(last - pos))) break;
                     pos += (int)r + 1;
                     int lf = L0 + pos;
-                    float pl = o_mix(F, t0, t1, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]);
+                    float pl = o_mix(&M, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]);
                     if (!(pl >= O_PMIN)) continue;       /* outside the support */
                     float pi = fminf(1.0f, lambda * pl);
                     float a = o_puniform(seed_lo, seed_hi, view, pixel, draw++);

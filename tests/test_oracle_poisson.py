@@ -40,10 +40,10 @@ def proposal(p, nb, sd, f, prec):
     out = []
     for h in hs:
         c = 1.0 / (2.0 * math.pi * math.sqrt(h))
-        # per-axis Gaussian factors; the FP32 exponential flushes arguments below -87 to 0
-        t2 = ((q[:, None, :] - lv[None, :, :]) / (math.sqrt(2.0) * float(h))) ** 2
+        # one exponential of |q - s|^2 / 2h^2; the FP32 exponential flushes arguments below -87 to 0
+        t2 = (((q[:, None, :] - lv[None, :, :]) / (math.sqrt(2.0) * float(h))) ** 2).sum(-1)
         e = np.where(t2 <= 87.0, np.exp(-t2), 0.0)
-        out.append((c * e[..., 0] * e[..., 1], c))
+        out.append((c * e, c))
     return out, T
 
 

@@ -10,7 +10,7 @@
 //   1 k_bandwidth     per-splat isotropic h_k -> fixed-point sum (2^-40 units)
 //   2 k_view_params   h, query radius, screen bounds
 //   3 k_cull_*        order-preserving compaction of visible splats + Morton
-//   4 k_sort_*        4-pass LSD radix sort of 30-bit Morton keys (stable,
+//   4 k_sort_*        3-pass LSD radix sort of 20-bit Morton keys (stable,
 //                     wavefront ballot ranking)
 //   5 k_hierarchy     Karras 2012 split on 64-bit keys (morton << 32 | pos)
 //   6 k_bottom_up     atomic fan-in: boxes, counts, centres, Hermite degree
@@ -361,12 +361,18 @@ __device__ __forceinline__ bool visible(const ViewState& v, float Wf, float Hf, 
     return vis;
 }
 
-__device__ __forceinline__ uint32_t expand_bits(uint32_t v)
+// 10-bit cell coordinates interleaved into a 20-bit key (x bits above y bits).
+// The reference spreads the bits three apart (a 3D code with z = 0,
+// buildRadixTree.cuh:18-54); the compact code orders the splats identically and
+// its highest differing bit is a monotone function of the spread one, so the
+// Karras hierarchy on it is the same tree, and the sort needs three 8-bit
+// passes instead of four.
+__device__ __forceinline__ uint32_t expand_bits2(uint32_t v)
 {
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
     return v;
 }
 
@@ -374,7 +380,7 @@ __device__ __forceinline__ uint32_t morton2(float ux, float uy)
 {
     float cx = fminf(fmaxf(ux * 1024.0f, 0.0f), 1023.0f);
     float cy = fminf(fmaxf(uy * 1024.0f, 0.0f), 1023.0f);
-    return 2u * expand_bits((uint32_t)cx) + expand_bits((uint32_t)cy);
+    return 2u * expand_bits2((uint32_t)cx) + expand_bits2((uint32_t)cy);
 }
 
 // exclusive scan of one int per thread over a 256-thread workgroup
@@ -1908,14 +1914,23 @@ __global__ void k_wide_band(ViewState* vs, VStride vsd)
 }
 
 // rate-free mixture proposal t0 g0 + t1 g1 at (x, y)
-__device__ __forceinline__ float mix_value(const Band& B0, const Band& B1, float t0, float t1, float qx, float qy,
-                                           float x, float y)
+// The proposal mixture at squared distance d2: k0 e^{-a0 d2} + k1 e^{-a1 d2},
+// with k_b = t_b c_b and a_b = is2h_b^2 = 1 / 2h_b^2 (one exponential per band).
+__device__ __forceinline__ float mix_d2(float k0, float a0, float k1, float a1, float d2)
 {
-    float v0 = band_point(B0, qx, qy, x, y);
-    float v1 = band_point(B1, qx, qy, x, y);
-    float m = t0 * v0;
-    m = m + t1 * v1;
+    const float e0 = pexpf(-(d2 * a0));
+    const float e1 = pexpf(-(d2 * a1));
+    float m = k0 * e0;
+    m = m + k1 * e1;
     return m;
+}
+
+__device__ __forceinline__ float dist2(float qx, float qy, float x, float y)
+{
+    const float dx = qx - x, dy = qy - y;
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    return d2;
 }
 
 // Poisson walk records; the escape pointer of a node is the right sibling of
@@ -2004,6 +2019,8 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         if (tscale) lam = lam * tscale[(long)(i >> 4) * tsH + (j >> 4)];
         const float t0 = 0.5f / fmaxf(D0, B0.c);
         const float t1 = 0.5f / fmaxf(dens1[pix], B1.c);
+        const float k0 = t0 * B0.c, k1 = t1 * B1.c;
+        const float a0 = B0.is2h * B0.is2h, a1 = B1.is2h * B1.is2h;
         const float qx = (float)i, qy = (float)j;
         const uint32_t pixel = (uint32_t)pix;
         uint32_t draw = 0, blkId = 0xffffffffu;
@@ -2027,7 +2044,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
             const PNode pn = pnode[nd];
             float cx = fminf(fmaxf(qx, pn.lox), pn.hix);
             float cy = fminf(fmaxf(qy, pn.loy), pn.hiy);
-            float pB = mix_value(B0, B1, t0, t1, qx, qy, cx, cy);     // proposal bound over the node
+            float pB = mix_d2(k0, a0, k1, a1, dist2(qx, qy, cx, cy));   // proposal bound over the node
             float U = fminf(1.0f, lam * pB);
             const int last = pn.last;
             int next = pn.esc;                        // past the subtree
@@ -2043,7 +2060,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                         if (!(r < (float)(last - pos))) break;
                         pos += (int)r + 1;
                         const float2 lp = leafXY[pos];
-                        float pl = mix_value(B0, B1, t0, t1, qx, qy, lp.x, lp.y);
+                        float pl = mix_d2(k0, a0, k1, a1, dist2(qx, qy, lp.x, lp.y));
                         if (!(pl >= kProposalPMin)) continue;     // outside the support
                         float pi = fminf(1.0f, lam * pl);
                         float a = uniform();
@@ -2236,10 +2253,10 @@ int build_structure(int N, int W, int H, double f, double smoothing, double prec
     k_view_params<<<dim3(1, views), 64, 0, st>>>(w.vs, Wf, Hf, prec, vsd);
     k_cull_count<<<dim3((unsigned)nCull, views), 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts, vsd);
     k_scan_small<<<dim3(1, views), 1024, 0, st>>>(w.cullCounts, (int)nCull, &w.vs->nc, vsd);
-    k_cull_scatter<<<dim3((unsigned)nCull, views), 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts, w.keysA, w.idsA, vsd);
-    uint32_t* kin = w.keysA; uint32_t* kout = w.keysB;
-    int32_t* iin = w.idsA; int32_t* iout = w.idsB;
-    for (int shift = 0; shift < 32; shift += 8) {
+    k_cull_scatter<<<dim3((unsigned)nCull, views), 256, 0, st>>>(N, Wf, Hf, ff, pts, w.vs, w.cullCounts, w.keysB, w.idsB, vsd);
+    uint32_t* kin = w.keysB; uint32_t* kout = w.keysA;
+    int32_t* iin = w.idsB; int32_t* iout = w.idsA;
+    for (int shift = 0; shift < 24; shift += 8) {                  // 20-bit keys
         k_sort_hist<<<dim3((unsigned)nSort, views), 256, 0, st>>>(w.vs, kin, shift, (int)nSort, w.sortHist, vsd);
         k_sort_scan_digits<<<dim3(256, views), 256, 0, st>>>(w.sortHist, (int)nSort, w.scanBlocks, vsd);
         k_sort_scan_top<<<dim3(1, views), 256, 0, st>>>(w.scanBlocks, vsd);
@@ -2248,7 +2265,7 @@ int build_structure(int N, int W, int H, double f, double smoothing, double prec
         uint32_t* tk = kin; kin = kout; kout = tk;
         int32_t* ti = iin; iin = iout; iout = ti;
     }
-    // after 4 passes the sorted data is back in keysA / idsA
+    // after 3 passes the sorted data is in keysA / idsA
     k_hierarchy<<<dim3(grid1(N, 256), views), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux, w.leafXY, vsd);
     k_leaf_ids<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, iin, w.aux, vsd);
     k_fill_i32<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.flags, w.vs, 0, 0, vsd);
