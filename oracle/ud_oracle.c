@@ -1417,13 +1417,17 @@ static float o_fixed_eval(const OFixed* F, int b, float qx, float qy)
             hx[0] = ex; hy[0] = ey;
             hx[1] = (2.0f * tx) * ex; hy[1] = (2.0f * ty) * ey;
             for (int n = 2; n < P1; n++) {
-                hx[n] = (2.0f * tx) * hx[n - 1] - (2.0f * (float)(n - 1)) * hx[n - 2];
-                hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
+                hx[n] = fmaf(2.0f * tx, hx[n - 1], -((2.0f * (float)(n - 1)) * hx[n - 2]));
+                hy[n] = fmaf(2.0f * ty, hy[n - 1], -((2.0f * (float)(n - 1)) * hy[n - 2]));
             }
+            /* sum_a h_a(tx) sum_b C_ab h_b(ty), each product-sum one fused multiply-add */
             const float* cc = B->c6 + (size_t)nd * P1 * P1;
             float v = 0.0f;
-            for (int a = 0; a < P1; a++)
-                for (int bb = 0; bb < P1; bb++) v = v + (cc[a * P1 + bb] * hx[a]) * hy[bb];
+            for (int a = 0; a < P1; a++) {
+                float sa = 0.0f;
+                for (int bb = 0; bb < P1; bb++) sa = fmaf(cc[a * P1 + bb], hy[bb], sa);
+                v = fmaf(hx[a], sa, v);
+            }
             w = w + v;
             continue;
         }

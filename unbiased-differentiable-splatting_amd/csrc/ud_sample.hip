@@ -1741,6 +1741,33 @@ __device__ __forceinline__ bool box4_meets_disc(const float* bx, float qx, float
 // FmmRec, so leaves and expansions need no node load and a descent needs one
 // wave-uniform record load.  Karras trees on 64-bit keys are at most 64 levels
 // deep, so the stack never holds more than 65 entries.
+// A fixed-order expansion at scaled offset t = (q - c) / sqrt(2 h^2):
+// sum_a h_a(tx) sum_b C_ab h_b(ty), Hermite functions by their three-term
+// recurrence, every product-sum one fused multiply-add (the oracle uses fmaf in
+// the same order).
+template <int P>
+__device__ __forceinline__ float fixed_series(const float* co, float tx, float ty)
+{
+    const float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+    float hx[P + 1], hy[P + 1];
+    hx[0] = ex; hy[0] = ey;
+    hx[1] = (2.0f * tx) * ex; hy[1] = (2.0f * ty) * ey;
+#pragma unroll
+    for (int q = 2; q <= P; q++) {
+        hx[q] = __builtin_fmaf(2.0f * tx, hx[q - 1], -((2.0f * (float)(q - 1)) * hx[q - 2]));
+        hy[q] = __builtin_fmaf(2.0f * ty, hy[q - 1], -((2.0f * (float)(q - 1)) * hy[q - 2]));
+    }
+    float val = 0.0f;
+#pragma unroll
+    for (int a = 0; a <= P; a++) {
+        float sa = 0.0f;
+#pragma unroll
+        for (int b = 0; b <= P; b++) sa = __builtin_fmaf(co[a * (P + 1) + b], hy[b], sa);
+        val = __builtin_fmaf(hx[a], sa, val);
+    }
+    return val;
+}
+
 template <int P>
 __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg, int W, int H,
                                                                     const ViewState* __restrict__ vs, size_t vstride,
@@ -1806,31 +1833,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
                 const float UD_GLOBAL* cc = cf + off;
 #pragma unroll
                 for (int q = 0; q < (P + 1) * (P + 1); q++) co[q] = cc[q];
-                if (act) {
-                    float tx = (qx - c.x) * B.is2h;
-                    float ty = (qy - c.y) * B.is2h;
-                    float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
-                    float hy[P + 1];
-                    hy[0] = ey;
-                    hy[1] = (2.0f * ty) * ey;
-#pragma unroll
-                    for (int q = 2; q <= P; q++)
-                        hy[q] = (2.0f * ty) * hy[q - 1] - (2.0f * (float)(q - 1)) * hy[q - 2];
-                    float val = 0.0f;
-                    float hprev = 0.0f, ha = ex;
-#pragma unroll
-                    for (int a = 0; a <= P; a++) {
-                        if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
-                        else if (a >= 2) {
-                            float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
-                            hprev = ha; ha = hn;
-                        }
-#pragma unroll
-                        for (int bb = 0; bb <= P; bb++)
-                            val = val + (co[a * (P + 1) + bb] * ha) * hy[bb];
-                    }
-                    w = w + val;
-                }
+                if (act) w = w + fixed_series<P>(co, (qx - c.x) * B.is2h, (qy - c.y) * B.is2h);
                 continue;
             }
             const FmmRec r = frec[nd];
