@@ -1560,11 +1560,11 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
             else {
                 int first = leaf ? nd - L0 : T->first[nd];
                 int last = leaf ? nd - L0 : T->last[nd];
-                float l1 = o_log1m(U);
+                float il1 = 1.0f / o_log1m(U);
                 int pos = first - 1;
                 while (1) {
                     float u = o_puniform(seed_lo, seed_hi, view, pixel, draw++);
-                    float r = ologf(u) / l1;             /* failures before the next candidate */
+                    float r = ologf(u) * il1;            /* failures before the next candidate */
                     if (!(r < (float)(last - pos))) break;
                     pos += (int)r + 1;
                     int lf = L0 + pos;

@@ -103,10 +103,11 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        uint32_t lo0 = 0xD2511F53u * c.x;
-        uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
-        uint32_t lo1 = 0xCD9E8D57u * c.z;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+        // one 32 x 32 -> 64-bit product per multiplier (a single v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         u32x4 n;
         n.x = hi1 ^ c.y ^ k0;
         n.y = lo1;

@@ -2055,11 +2055,11 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                 const bool leaf = nd >= L0;
                 if (!leaf && (float)(last - pn.first + 1) * U > 1.0f) next = pn.left;
                 else {
-                    const float l1 = plog1mf(U);
+                    const float il1 = 1.0f / plog1mf(U);
                     int pos = pn.first - 1;
                     while (true) {
                         float u = uniform();
-                        float r = plogf(u) / l1;          // failures before the next candidate
+                        float r = plogf(u) * il1;         // failures before the next candidate
                         if (!(r < (float)(last - pos))) break;
                         pos += (int)r + 1;
                         const float2 lp = leafXY[pos];
