@@ -1497,16 +1497,18 @@ static float o_log1m(float u)
     return ologf(1.0f - u);
 }
 
-static float o_puniform(uint32_t seed_lo, uint32_t seed_hi, uint32_t view, uint32_t pixel, uint32_t draw)
+/* the Philox block of one geometric-skip step of a pixel: counter (step,
+ * 0x80000000, pixel, view); word 0 draws the skip, word 1 the acceptance test */
+static void o_pblock(uint32_t seed_lo, uint32_t seed_hi, uint32_t view, uint32_t pixel, uint32_t step, float u[2])
 {
-    uint32_t ctr[4] = { draw >> 2, 0x80000000u, pixel, view };
+    uint32_t ctr[4] = { step, 0x80000000u, pixel, view };
     uint32_t key[2] = { seed_lo, seed_hi };
     uint32_t o[4];
     o_philox4x32_10(ctr, key, o);
-    return (float)((o[draw & 3u] >> 8) + 1u) * 5.9604644775390625e-08f;
+    u[0] = (float)((o[0] >> 8) + 1u) * 5.9604644775390625e-08f;
+    u[1] = (float)((o[1] >> 8) + 1u) * 5.9604644775390625e-08f;
 }
 
-/* mixture proposal at point (x, y): t0 g0 + t1 g1 */
 /* proposal mixture at q for a point s: k0 e^{-a0 d2} + k1 e^{-a1 d2} with
  * d2 = |q - s|^2, k_b = t_b c_b, a_b = is2h_b^2 = 1 / 2h_b^2 (one exponential
  * per band; this build's definition, no reference counterpart) */
@@ -1543,7 +1545,7 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
     M.a0 = F->band[0].is2h * F->band[0].is2h; M.a1 = F->band[1].is2h * F->band[1].is2h;
     const int L0 = T->nc - 1;
     const uint32_t pixel = (uint32_t)(j + H * i);
-    uint32_t draw = 0;
+    uint32_t step = 0;
     int ns = 0, over = 0;
     int nd = 0;
     while (1) {
@@ -1563,7 +1565,9 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
                 float il1 = 1.0f / o_log1m(U);
                 int pos = first - 1;
                 while (1) {
-                    float u = o_puniform(seed_lo, seed_hi, view, pixel, draw++);
+                    float uu[2];
+                    o_pblock(seed_lo, seed_hi, view, pixel, step++, uu);
+                    float u = uu[0];
                     float r = ologf(u) * il1;            /* failures before the next candidate */
                     if (!(r < (float)(last - pos))) break;
                     pos += (int)r + 1;
@@ -1571,7 +1575,7 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
                     float pl = o_mix(&M, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]);
                     if (!(pl >= O_PMIN)) continue;       /* outside the support */
                     float pi = fminf(1.0f, lambda * pl);
-                    float a = o_puniform(seed_lo, seed_hi, view, pixel, draw++);
+                    float a = uu[1];
                     if (a * U <= pi) {          /* P = pi / U (exactly 1 when pi = U) */
                         if (ns < kcap) { ids[ns] = T->ids[pos]; ipi[ns] = 1.0f / pi; ns++; }
                         else over = 1;

@@ -1973,8 +1973,9 @@ __global__ __launch_bounds__(256) void k_pois_nodes(const ViewState* __restrict_
 // rate U by geometric skips over its contiguous leaf range (skip = floor(log u /
 // log(1 - U))), each kept with probability pi_i / U, i.e. included with
 // probability pi_i, independently.  Uniforms are Philox4x32-10 words at counter
-// (draw >> 2, 0x80000000, pixel, view), word draw & 3, with a per-pixel draw
-// number.  A skipped or thinned subtree is left through its escape pointer.
+// (step, 0x80000000, pixel, view), one block per geometric-skip step of the
+// pixel (word x the skip, word y the acceptance test).  A skipped or thinned
+// subtree is left through its escape pointer.
 __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg, int W, int H, int kcap, float rate,
                                                                    const float* __restrict__ tscale, int tsH,
                                                                    uint32_t seedLo, uint32_t seedHi,
@@ -2026,20 +2027,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         const float a0 = B0.is2h * B0.is2h, a1 = B1.is2h * B1.is2h;
         const float qx = (float)i, qy = (float)j;
         const uint32_t pixel = (uint32_t)pix;
-        uint32_t draw = 0, blkId = 0xffffffffu;
-        u32x4 blk = { 0u, 0u, 0u, 0u };
-        auto uniform = [&]() {
-            uint32_t bi = draw >> 2;
-            if (bi != blkId) {
-                blkId = bi;
-                u32x4 ctr = { bi, 0x80000000u, pixel, view };
-                blk = philox4x32_10(ctr, seedLo, seedHi);
-            }
-            uint32_t q = draw & 3u;
-            uint32_t x = q == 0 ? blk.x : q == 1 ? blk.y : q == 2 ? blk.z : blk.w;
-            draw++;
-            return u01(x);
-        };
+        uint32_t step = 0;                            // geometric-skip steps of this pixel
         int nd = 0;
         int visits = 0;                               // a preorder walk visits each node at most once
         while (true) {
@@ -2058,7 +2046,12 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                     const float il1 = 1.0f / plog1mf(U);
                     int pos = pn.first - 1;
                     while (true) {
-                        float u = uniform();
+                        // one Philox block per step, computed by every lane of the
+                        // wavefront in the same iteration: word x draws the skip,
+                        // word y the acceptance test
+                        const u32x4 blk = philox4x32_10(u32x4{ step, 0x80000000u, pixel, view }, seedLo, seedHi);
+                        step++;
+                        float u = u01(blk.x);
                         float r = plogf(u) * il1;         // failures before the next candidate
                         if (!(r < (float)(last - pos))) break;
                         pos += (int)r + 1;
@@ -2066,7 +2059,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                         float pl = mix_d2(k0, a0, k1, a1, dist2(qx, qy, lp.x, lp.y));
                         if (!(pl >= kProposalPMin)) continue;     // outside the support
                         float pi = fminf(1.0f, lam * pl);
-                        float a = uniform();
+                        float a = u01(blk.y);
                         if (a * U <= pi) {            // P = pi / U, exactly 1 when pi = U
                             if (ns < kcap) { oid[ns] = leafId[pos]; oip[ns] = 1.0f / pi; ns++; }
                             else over = true;
