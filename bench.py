@@ -246,7 +246,7 @@ def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
         ts.copy_(net.tile_scale(f2[0], L.tile_counts(fr.last_ids)[0])[None])
 
     v_check, _ = L.gradient_variance(fr, list(range(40_000, 40_000 + seeds)), tile_scale=ts, rate=RATE * kappa,
-                                     kcap=kcl)
+                                     kcap=kcl, groups=4)
     plain_ms, _, _ = timed_steps(plain_step, probe_steps)
     learned_ms, _, _ = timed_steps(learned_step, probe_steps)
     rec.update({"learned_rate": RATE * kappa, "learned_grad_variance_at_rate": v_check,

@@ -3,7 +3,9 @@ FP64 central differences (step 1e-4) on 32 random parameters: 256 splats of
 the sphere generator at 64x64, one Poisson-rate sample set (indices and 1/pi
 held fixed, as the estimator treats them), loss sum_q G_q . image_q with a
 random G.  The differenced forward is the oracle's FP64 EWA weight and
-interpolation, so the check is independent of the GPU forward."""
+interpolation, so the check is independent of the GPU forward.  A parameter whose
+step moves a slot across the weight clamp min(w, 1) is passed over (the loss has
+a kink there)."""
 import numpy as np
 import pytest
 import torch
@@ -36,22 +38,26 @@ def test_adjoint_matches_central_differences(dev):
     def loss(prm):
         wf = O.splat_forward(W, H, f, idn, prm[0], prm[1], prm[2])
         img = O.interp_forward(idn, wf * ipn, prm[3])
-        return float((G.astype(np.float64) * img).sum())
+        return float((G.astype(np.float64) * img).sum()), wf >= 1.0
 
     drawn = np.unique(idn[idn >= 0])
     assert drawn.size > 50
     h = 1e-4
     checked = 0
-    for q in range(32):
+    for q in range(200):
+        if checked == 32:
+            break
         g = int(rng.integers(0, 4))
         sid = int(rng.choice(drawn))
         c = int(rng.integers(0, params[g].shape[-1]))
         x0 = params[g][0, sid, c]
         params[g][0, sid, c] = x0 + h
-        fp = loss(params)
+        fp, cp = loss(params)
         params[g][0, sid, c] = x0 - h
-        fm = loss(params)
+        fm, cm = loss(params)
         params[g][0, sid, c] = x0
+        if not np.array_equal(cp, cm):
+            continue            # the step crosses the clamp min(w, 1) of a slot: no derivative there
         fd = (fp - fm) / (2 * h)
         got = grads[g][sid, c]
         scale = max(abs(fd), abs(got), 1e-3 * np.abs(grads[g]).max())

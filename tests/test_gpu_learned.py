@@ -83,8 +83,10 @@ def test_tile_features(dev, cfg, sds):
 def test_learned_rates_against_plain(dev, record_property):
     """What the trained rate network gains over plain rates on a scene it was not
     trained on: gradient variance at the same expected sample count (kappa =
-    V_learned / V_plain; below 1 is a gain), measured over 32 seeds with the
-    plain measurement repeated on other seeds for the noise of the measurement.
+    V_learned / V_plain; below 1 is a gain), each variance the median over eight
+    groups of 16 seeds (the estimator's heavy tails can put a single seed far out),
+    with the plain measurement repeated on other seeds for the noise of the
+    measurement.
     The multipliers must keep the sample budget; their floor (0.75) bounds the
     loss where the network errs, so kappa stays well below 1 / 0.75 plus noise."""
     from splatting_amd import learned as L, scenes
@@ -92,7 +94,7 @@ def test_learned_rates_against_plain(dev, record_property):
     target = L.make_target(sc, dev, 10.0, 40, seeds=range(10_000, 10_004))
     fr = L.PoissonFrame(sc, dev, rate=10.0, kcap=40, target=target)
     net = L.RateNet.load(L.DEFAULT_WEIGHTS, dev)
-    r = L.compare_rates(fr, net, list(range(500, 532)))
+    r = L.compare_rates(fr, net, list(range(500, 628)), groups=8)
     ts = r.pop("tile_scale")
     for k, v in r.items():
         record_property(k, v)

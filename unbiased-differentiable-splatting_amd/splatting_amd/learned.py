@@ -260,19 +260,29 @@ def variance_targets(frame, seeds, low=0.25):
     return feats.reshape(-1, N_IN), y.float(), n.float()
 
 
-def gradient_variance(frame, seeds, tile_scale=None, rate=None, kcap=None):
+def gradient_variance(frame, seeds, tile_scale=None, rate=None, kcap=None, groups=1):
     """Mean over all parameters of the per-parameter variance of the sampled
-    gradient across seeds (raw adjoints), and the mean gradient."""
-    s1 = s2 = None
-    for s in seeds:
-        _, _, grads, _ = frame.step(s, tile_scale, rate, kcap)
-        x = torch.cat([gq.reshape(-1) for gq in grads]).double()
-        s1 = x if s1 is None else s1 + x
-        s2 = x * x if s2 is None else s2 + x * x
+    gradient across seeds (raw adjoints), and the mean gradient.  With groups > 1
+    the seeds are split into that many consecutive groups and the median of the
+    groups' values is returned: the estimator's heavy tails (a rare pair with a
+    small inclusion probability and a large weight) can put one seed's gradient
+    far out, and the median of groups keeps such a seed from deciding the value."""
     S = len(seeds)
-    mean = s1 / S
-    var = (s2 - S * mean * mean) / (S - 1)
-    return float(var.clamp(min=0).mean().item()), mean
+    G = max(1, min(int(groups), S // 2))
+    bounds = [S * q // G for q in range(G + 1)]
+    vals, total = [], None
+    for q in range(G):
+        s1 = s2 = None
+        for s in seeds[bounds[q]:bounds[q + 1]]:
+            _, _, grads, _ = frame.step(s, tile_scale, rate, kcap)
+            x = torch.cat([gq.reshape(-1) for gq in grads]).double()
+            s1 = x if s1 is None else s1 + x
+            s2 = x * x if s2 is None else s2 + x * x
+        n = bounds[q + 1] - bounds[q]
+        mean = s1 / n
+        vals.append(float(((s2 - n * mean * mean) / (n - 1)).clamp(min=0).mean().item()))
+        total = s1 if total is None else total + s1
+    return float(np.median(vals)), total / S
 
 
 def rate_features(frame, net, seed):
@@ -290,21 +300,22 @@ def max_count(ids):
     return int((ids >= 0).sum(-1).max().item())
 
 
-def compare_rates(frame, net, seeds, kcap_learned=None):
+def compare_rates(frame, net, seeds, kcap_learned=None, groups=8):
     """Plain against learned per-tile rates on one frame at the same expected
     sample count.  Gradient variances (mean per-parameter variance of the raw
     adjoints) are measured over disjoint seed sets; a second plain measurement on
     other seeds gives the noise of the measurement itself.  Variance behaves like
     1 / rate, so the learned rates reach the plain variance at rate * kappa with
     kappa = V_learned / V_plain.  Returns a dict of the measurements and the tile
-    multipliers."""
+    multipliers.  Each variance is the median over `groups` seed groups."""
     S = len(seeds)
     base = seeds[0]
     feats, counts, ts = rate_features(frame, net, base - 1)
     kcl = kcap_learned or frame.kcap * 2
-    v_plain, _ = gradient_variance(frame, [base + s for s in range(S)])
-    v_rep, _ = gradient_variance(frame, [base + S + s for s in range(S)])
-    v_learned, _ = gradient_variance(frame, [base + 2 * S + s for s in range(S)], tile_scale=ts, kcap=kcl)
+    G = groups                             # median of seed groups (heavy tails)
+    v_plain, _ = gradient_variance(frame, [base + s for s in range(S)], groups=G)
+    v_rep, _ = gradient_variance(frame, [base + S + s for s in range(S)], groups=G)
+    v_learned, _ = gradient_variance(frame, [base + 2 * S + s for s in range(S)], tile_scale=ts, kcap=kcl, groups=G)
     frame.image(base - 2)
     n_plain = float((frame.last_ids >= 0).float().sum(-1).mean().item())
     over_plain = max_count(frame.last_ids) >= frame.kcap
@@ -317,6 +328,7 @@ def compare_rates(frame, net, seeds, kcap_learned=None):
             "samples_per_pixel_plain": n_plain, "samples_per_pixel_learned": n_learned,
             "kcap_plain": frame.kcap, "kcap_learned": kcl, "capacity_reached": over_plain or over_learned,
             "tile_scale_range": [float(ts.min().item()), float(ts.max().item())], "variance_seeds": S,
+            "variance_seed_groups": G,
             "tile_scale": ts}
 
 
