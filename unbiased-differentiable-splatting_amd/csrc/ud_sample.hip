@@ -23,7 +23,7 @@
 //                     Philox4x32-10, top-k in LDS
 // The fixed-order and Poisson-rate paths share stages 1-6 and add expansion
 // selection and moments per band (k_fixed_*), walk records (k_fmm_records,
-// k_pois_nodes), the density walk (k_fmm_eval) and the thinning pass
+// k_walk_records), the density walk (k_fmm_eval) and the thinning pass
 // (k_poisson).  Batches of views run each stage as one launch (views on
 // blockIdx.y) where the workspace holds them all.
 // All FP32 arithmetic on the tree and in the samplers follows a fixed operation
@@ -77,7 +77,7 @@ struct NodeLnk {
     int left, right, parent, count;
 };
 struct NodeAux {
-    int off, first, last, pad;
+    int off, first, last, pad;   // pad: a leaf's splat id
 };
 // per node: what the per-pixel Poisson walk reads of it in one load
 struct PNode {
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(float Wf, float Hf, float ff,
                                                    const ViewState* __restrict__ vs, const uint32_t* __restrict__ keys,
                                                    const int32_t* __restrict__ ids, NodeGeo* __restrict__ geo,
                                                    NodeLnk* __restrict__ lnk, NodeAux* __restrict__ aux,
-                                                   float2* __restrict__ leafXY, VStride vsd)
+                                                   float2* __restrict__ leafXY, int* __restrict__ flags, VStride vsd)
 {
     pts = vw(pts, vsd.pts);
     vs = vw(vs, vsd.ws);
@@ -617,6 +617,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(float Wf, float Hf, float ff,
     lnk = vw(lnk, vsd.ws);
     aux = vw(aux, vsd.ws);
     leafXY = vw(leafXY, vsd.ws);
+    flags = vw(flags, vsd.ws);
     int n = vs->nc;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -634,11 +635,12 @@ __global__ __launch_bounds__(256) void k_hierarchy(float Wf, float Hf, float ff,
         geo[nd] = g;
         leafXY[t] = make_float2(sx, sy);
         lnk[nd].left = -1; lnk[nd].right = -1; lnk[nd].count = 1;
-        NodeAux a; a.off = 0; a.first = t; a.last = t; a.pad = 0;
+        NodeAux a; a.off = 0; a.first = t; a.last = t; a.pad = ids[t];   // pad: the leaf's splat id
         aux[nd] = a;
     }
     if (t == 0) lnk[0].parent = -1;
     if (t >= n - 1) return;
+    flags[t] = 0;                                  // fan-in arrival count of internal node t
     // internal node t
     int i0, j0;
     if (t == 0) { i0 = 0; j0 = n - 1; }
@@ -690,16 +692,6 @@ __device__ float hermite_error(float r, float nr, int p)
     float c = b * b;
     float d = 2.0f * (1.0f - rp) * b;
     return a * (c + d);
-}
-
-__global__ __launch_bounds__(256) void k_fill_i32(int* p, const ViewState* __restrict__ vs, int mode, int v, VStride vsd)
-{
-    p = vw(p, vsd.ws);
-    vs = vw(vs, vsd.ws);
-    // mode 0: n = nc - 1 (fan-in flags)
-    int n = vs->nc - 1;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = v;
 }
 
 // The record of an internal node from its two complete children, left then
@@ -1480,17 +1472,6 @@ __global__ __launch_bounds__(64) void k_sample_draw(int W, int H, int k, int L, 
     }
 }
 
-// leaf splat ids into NodeAux.pad (so the sampler reads one record per leaf)
-__global__ __launch_bounds__(256) void k_leaf_ids(const ViewState* __restrict__ vs, const int32_t* __restrict__ ids, NodeAux* aux, VStride vsd)
-{
-    vs = vw(vs, vsd.ws);
-    ids = vw(ids, vsd.ws);
-    aux = vw(aux, vsd.ws);
-    int n = vs->nc;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) aux[n - 1 + t].pad = ids[t];
-}
-
 __global__ void k_fill_f32(float* p, long n, float v)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1685,8 +1666,28 @@ __device__ __forceinline__ bool tile_lane(const TileCfg& c, int W, int H, int la
 
 constexpr int kFmmStack = 96;
 
-// children records of the internal nodes for the density walk (after the
-// expansions of every band are selected)
+// children record of internal node p for the density walk (after the expansions
+// of every band are selected)
+__device__ __forceinline__ FmmRec fmm_record(int p, int L0, const NodeGeo* __restrict__ geo,
+                                             const NodeLnk* __restrict__ lnk, const int* __restrict__ fixOff0,
+                                             const int* __restrict__ fixOff1)
+{
+    const NodeLnk lk = lnk[p];
+    const NodeGeo gl = geo[lk.left], gr = geo[lk.right];
+    FmmRec r;
+    r.lbox[0] = gl.lox; r.lbox[1] = gl.loy; r.lbox[2] = gl.hix; r.lbox[3] = gl.hiy;
+    r.rbox[0] = gr.lox; r.rbox[1] = gr.loy; r.rbox[2] = gr.hix; r.rbox[3] = gr.hiy;
+    r.lc[0] = gl.cx; r.lc[1] = gl.cy; r.rc[0] = gr.cx; r.rc[1] = gr.cy;
+    r.left = lk.left; r.right = lk.right;
+    r.lfix[0] = lk.left < L0 ? fixOff0[lk.left] : -1;
+    r.rfix[0] = lk.right < L0 ? fixOff0[lk.right] : -1;
+    r.lfix[1] = (fixOff1 && lk.left < L0) ? fixOff1[lk.left] : -1;
+    r.rfix[1] = (fixOff1 && lk.right < L0) ? fixOff1[lk.right] : -1;
+    r.pad[0] = r.pad[1] = 0;
+    return r;
+}
+
+// children records of the internal nodes for the density walk
 __global__ __launch_bounds__(256) void k_fmm_records(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
                                                      const NodeLnk* __restrict__ lnk,
                                                      const int* __restrict__ fixOff0,
@@ -1701,19 +1702,7 @@ __global__ __launch_bounds__(256) void k_fmm_records(const ViewState* __restrict
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int L0 = vs->nc - 1;
     if (p >= L0) return;
-    const NodeLnk lk = lnk[p];
-    const NodeGeo gl = geo[lk.left], gr = geo[lk.right];
-    FmmRec r;
-    r.lbox[0] = gl.lox; r.lbox[1] = gl.loy; r.lbox[2] = gl.hix; r.lbox[3] = gl.hiy;
-    r.rbox[0] = gr.lox; r.rbox[1] = gr.loy; r.rbox[2] = gr.hix; r.rbox[3] = gr.hiy;
-    r.lc[0] = gl.cx; r.lc[1] = gl.cy; r.rc[0] = gr.cx; r.rc[1] = gr.cy;
-    r.left = lk.left; r.right = lk.right;
-    r.lfix[0] = lk.left < L0 ? fixOff0[lk.left] : -1;
-    r.rfix[0] = lk.right < L0 ? fixOff0[lk.right] : -1;
-    r.lfix[1] = (fixOff1 && lk.left < L0) ? fixOff1[lk.left] : -1;
-    r.rfix[1] = (fixOff1 && lk.right < L0) ? fixOff1[lk.right] : -1;
-    r.pad[0] = r.pad[1] = 0;
-    rec[p] = r;
+    rec[p] = fmm_record(p, L0, geo, lnk, fixOff0, fixOff1);
 }
 
 __device__ __forceinline__ bool box4_meets_disc(const float* bx, float qx, float qy, float rad)
@@ -1936,34 +1925,45 @@ __device__ __forceinline__ float dist2(float qx, float qy, float x, float y)
     return d2;
 }
 
-// Poisson walk records; the escape pointer of a node is the right sibling of
-// its first ancestor-or-self that is a left child, -1 on the rightmost path
-__global__ __launch_bounds__(256) void k_pois_nodes(const ViewState* __restrict__ vs, const NodeGeo* __restrict__ geo,
-                                                    const NodeLnk* __restrict__ lnk, const NodeAux* __restrict__ aux,
-                                                    PNode* __restrict__ pnode, VStride vsd)
+// Walk records of the Poisson pass, and (for internal nodes) the density walk's
+// children records, in one launch after the expansions are selected.  The
+// escape pointer of a node is the right sibling of its first ancestor-or-self
+// that is a left child (-1 on the rightmost path).  That sibling's leaf range
+// starts at m = last + 1, and in a Karras tree a right child's index is the
+// first leaf of its range, so it is internal node m when node m's range starts
+// at m and leaf m otherwise (no climb over parent links).
+__global__ __launch_bounds__(256) void k_walk_records(const ViewState* __restrict__ vs,
+                                                      const NodeGeo* __restrict__ geo,
+                                                      const NodeLnk* __restrict__ lnk,
+                                                      const NodeAux* __restrict__ aux,
+                                                      const int* __restrict__ fixOff0,
+                                                      const int* __restrict__ fixOff1, PNode* __restrict__ pnode,
+                                                      FmmRec* __restrict__ frec, VStride vsd)
 {
     vs = vw(vs, vsd.ws);
     geo = vw(geo, vsd.ws);
     lnk = vw(lnk, vsd.ws);
     aux = vw(aux, vsd.ws);
+    fixOff0 = vw(fixOff0, vsd.ws);
+    fixOff1 = vw(fixOff1, vsd.ws);
     pnode = vw(pnode, vsd.ws);
+    frec = vw(frec, vsd.ws);
     const int nd = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = vs->nc;
     if (nc <= 0 || nd >= 2 * nc - 1) return;
-    int e = nd, r = -1;
-    while (e != 0) {
-        const int p = lnk[e].parent;
-        if (lnk[p].left == e) { r = lnk[p].right; break; }
-        e = p;
-    }
+    const int L0 = nc - 1;
     const NodeGeo g = geo[nd];
     const NodeAux a = aux[nd];
+    const int m = a.last + 1;
+    int esc = -1;
+    if (m < nc) esc = (m < L0 && aux[m].first == m) ? m : L0 + m;
     PNode q;
     q.lox = g.lox; q.loy = g.loy; q.hix = g.hix; q.hiy = g.hiy;
-    q.left = nd < nc - 1 ? lnk[nd].left : -1;
-    q.esc = r;
+    q.left = nd < L0 ? lnk[nd].left : -1;
+    q.esc = esc;
     q.first = a.first; q.last = a.last;
     pnode[nd] = q;
+    if (nd < L0) frec[nd] = fmm_record(nd, L0, geo, lnk, fixOff0, fixOff1);
 }
 
 // One lane per pixel (one pixel tile per wavefront, TileCfg).  The walk
@@ -2262,9 +2262,8 @@ int build_structure(int N, int W, int H, double f, double smoothing, double prec
         int32_t* ti = iin; iin = iout; iout = ti;
     }
     // after 3 passes the sorted data is in keysA / idsA
-    k_hierarchy<<<dim3(grid1(N, 256), views), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux, w.leafXY, vsd);
-    k_leaf_ids<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, iin, w.aux, vsd);
-    k_fill_i32<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.flags, w.vs, 0, 0, vsd);
+    k_hierarchy<<<dim3(grid1(N, 256), views), 256, 0, st>>>(Wf, Hf, ff, pts, w.vs, kin, iin, w.geo, w.lnk, w.aux, w.leafXY,
+                                                            w.flags, vsd);
     k_bottom_up_local<<<dim3(grid1(N, kFanBlock), views), kFanBlock, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.seeds, vsd);
     k_bottom_up_global<<<dim3(grid1(N, 256), views), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.flags, w.seeds, vsd);
     hipError_t e0 = hipGetLastError();
@@ -2507,10 +2506,9 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         k_wide_hist<<<dim3(grid1(N, 256) < 256u ? grid1(N, 256) : 256u, span), 256, 0, st>>>(w.vs, w.idsA, w.hraw,
                                                                                           vsd);
         k_wide_band<<<dim3(1, span), 64, 0, st>>>(w.vs, vsd);
-        k_pois_nodes<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.pnode, vsd);
         fixed_expansions(N, 2, p, w, st, span, vsd);
-        k_fmm_records<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.fixOff[0], w.fixOff[1], w.frec,
-                                                                 vsd);
+        k_walk_records<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.fixOff[0],
+                                                                       w.fixOff[1], w.pnode, w.frec, vsd);
         float* d0 = densOut ? densOut + perView * 2 * g : w.dens[0];
         float* d1 = densOut ? d0 + perView : w.dens[1];
         const size_t dstride = densOut ? sizeof(float) * 2 * perView : per;
