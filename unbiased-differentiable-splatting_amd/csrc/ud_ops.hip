@@ -471,12 +471,21 @@ __global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W,
         }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < kHtSlots; e += blockDim.x) {
-        const int id = key[e];
-        if (id < 0) continue;
-        float v[9];
-        for (int c = 0; c < 9; c++) v[c] = sum[c][e];
-        ht_add_record(mv + (long)kHtRec * id, v);
+    // Flush: nine consecutive lanes add the nine sums of one entry, so one atomic
+    // instruction covers seven records (one 64 B line each) instead of 64 lines;
+    // L2 serves a line's dwords of one instruction together (about 9x faster).
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int per = kHtSlots / (kHtTile * kHtTile / 64);
+    const int sub = lane / 9, c = lane - 9 * sub;
+    const int field = c < 6 ? c : c + 2;          // moments at [0, 6), attribute sums at [8, 11)
+    for (int e = wv * per + sub; e < (wv + 1) * per + sub; e += 7) {
+        if (lane < 63 && e < (wv + 1) * per) {
+            const int id = key[e];
+            if (id >= 0) {
+                const float x = sum[c][e];
+                if (x != 0.0f) atomicAdd(mv + (long)kHtRec * id + field, x);
+            }
+        }
     }
 }
 
@@ -1089,7 +1098,14 @@ int ude_interp_backward(int bn, int W, int H, int k, int ab, int N, int C, const
 
 size_t ude_ht_workspace_bytes(int bn, int N)
 {
-    return (size_t)bn * N * (sizeof(SplatRec) + kHtRec * sizeof(float)) + 256;
+    return (size_t)bn * N * (sizeof(SplatRec) + kHtRec * sizeof(float)) + 512;
+}
+
+// backward records after the splat records, on a 128 B boundary (one record per 64 B half-line)
+static float* ht_records(void* ws, int bn, int N)
+{
+    const uintptr_t a = (uintptr_t)((SplatRec*)ws + (long)bn * N);
+    return (float*)((a + 127) & ~(uintptr_t)127);
 }
 
 int ude_ht_forward(int bn, int N, int W, int H, int k, double f, double smoothing, const int32_t* idx,
@@ -1119,7 +1135,7 @@ int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothi
     if (N == 0) return UDE_OK;
     if (wsBytes < ude_ht_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
     SplatRec* rec = (SplatRec*)ws;
-    float* mom = (float*)(rec + (long)bn * N);
+    float* mom = ht_records(ws, bn, N);
     k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
     k_fill_f32<<<grid1((long)kHtRec * bn * N, 256), 256, 0, st>>>(mom, (long)kHtRec * bn * N, 0.0f);
     const dim3 tiles((unsigned)(((W + kHtTile - 1) / kHtTile) * ((H + kHtTile - 1) / kHtTile)), (unsigned)bn);
