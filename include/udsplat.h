@@ -175,8 +175,8 @@ int ude_splat_forward(int bn, int N, int W, int H, int k, double f, double smoot
 
 /* splatting.backward -- replaces cudaSplattingBackward (splatting.cpp:84-129,
  * splattingCuda.cu:114-234).  gPts, gNrm: (bn, N, 3), fully written.
- * gSd: (sdBatch, N, 2); fully written if sdBatch == bn > 1, accumulated into
- * (caller zeroed) if sdBatch == 1. */
+ * gSd: (sdBatch, N, 2); fully written if sdBatch == bn, accumulated into
+ * (caller zeroed) if sdBatch == 1 < bn. */
 int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoothing,
                        const float* gradIn, const int32_t* idx, const float* pts, const float* nrm,
                        const float* sd, int sdBatch, float* gPts, float* gNrm, float* gSd,
@@ -240,11 +240,13 @@ int ude_ht_forward(int bn, int N, int W, int H, int k, double f, double smoothin
 
 /* Adjoint of ude_ht_forward for gradImage (bn, W, H, 3).  gPts, gNrm fully
  * written; gSd as in ude_splat_backward; gAttr fully written if ab == bn,
- * accumulated into (caller zeroed) if ab == 1 < bn. */
+ * accumulated into (caller zeroed) if ab == 1 < bn.  prepared != 0: ws is the
+ * workspace ude_ht_forward used for the same splats, and its per-splat records
+ * are reused instead of recomputed. */
 int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothing, const float* gradImage,
                     const int32_t* idx, const float* ipi, const float* pts, const float* nrm, const float* sd,
                     int sdBatch, int ab, const float* attr, float* gPts, float* gNrm, float* gSd, float* gAttr,
-                    void* ws, size_t wsBytes, ude_stream_t stream);
+                    int prepared, void* ws, size_t wsBytes, ude_stream_t stream);
 
 /* ---- shading ------------------------------------------------------------ */
 

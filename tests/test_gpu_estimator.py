@@ -57,6 +57,12 @@ def check(dev, W, H, f, ids, ipi, P, NB, SD, A, g, what):
         assert a.shape == b.shape, (what, name)
         assert_close(a.cpu().numpy(), b.cpu().numpy(), RTOL_UNFUSED, "%s grad %s vs unfused" % (what, name))
         assert_close(a.cpu().numpy(), r, RTOL_ORACLE, "%s grad %s vs oracle" % (what, name))
+    # the backward reusing the forward's per-splat records
+    ws = estimator.workspace(ids, SD)
+    assert torch.equal(estimator.forward(W, H, f, ids, ipi, P, NB, SD, A, workspace=ws), img), what
+    pgrads = estimator.backward(W, H, f, g, ids, ipi, P, NB, SD, A, workspace=ws, prepared=True)
+    for name, a, b in zip(("points", "normals", "std-devs", "attribute"), pgrads, grads):
+        assert_close(a.cpu().numpy(), b.cpu().numpy(), RTOL_UNFUSED, "%s grad %s, prepared" % (what, name))
 
 
 def poisson_views(dev, views, kcap=40, rate=8.0):
@@ -140,3 +146,7 @@ def test_estimate_empty_and_bad_arguments(dev):
         estimator.forward(W, H, f, ids, ipi[..., :8].contiguous(), P, NB, SD, A)
     with pytest.raises(RuntimeError):
         estimator.forward(W, H, f, ids, ipi, P, NB, SD, torch.cat([A, A, A]).contiguous())
+    with pytest.raises(ValueError):
+        estimator.backward(W, H, f, g, ids, ipi, P, NB, SD, A, prepared=True)
+    with pytest.raises(ValueError):
+        estimator.forward(W, H, f, ids, ipi, P, NB, SD, A, workspace=torch.empty(16, dtype=torch.uint8, device=dev))

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(128) void k_splat_finalize(int bn, int N, int W, in
     double m0 = m[0], m1x = m[1], m1y = m[2], m2xx = m[3], m2xy = m[4], m2yy = m[5];
     if (m0 == 0.0 && m1x == 0.0 && m1y == 0.0 && m2xx == 0.0 && m2xy == 0.0 && m2yy == 0.0) {
         for (int c = 0; c < 3; c++) { gPts[3 * t + c] = 0.0f; gNrm[3 * t + c] = 0.0f; }
-        if (sdBatch > 1) { gSd[so] = 0.0f; gSd[so + 1] = 0.0f; }
+        if (sdBatch == bn) { gSd[so] = 0.0f; gSd[so + 1] = 0.0f; }
         return;
     }
     D6 p[3], n[3];
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(128) void k_splat_finalize(int bn, int N, int W, in
         gPts[3 * t + c] = (float)gp;
         gNrm[3 * t + c] = (float)gn;
     }
-    if (sdBatch > 1) { gSd[so] = (float)gs0; gSd[so + 1] = (float)gs1; }
+    if (sdBatch == bn) { gSd[so] = (float)gs0; gSd[so + 1] = (float)gs1; }     // one writer per std-dev
     else { atomicAdd(gSd + so, (float)gs0); atomicAdd(gSd + so + 1, (float)gs1); }
 }
 
@@ -1129,14 +1129,15 @@ int ude_ht_forward(int bn, int N, int W, int H, int k, double f, double smoothin
 int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothing, const float* gradImage,
                     const int32_t* idx, const float* ipi, const float* pts, const float* nrm, const float* sd,
                     int sdBatch, int ab, const float* attr, float* gPts, float* gNrm, float* gSd, float* gAttr,
-                    void* ws, size_t wsBytes, hipStream_t st)
+                    int prepared, void* ws, size_t wsBytes, hipStream_t st)
 {
     if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0 || (ab != 1 && ab != bn)) return UDE_ERR_ARG;
     if (N == 0) return UDE_OK;
     if (wsBytes < ude_ht_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
     SplatRec* rec = (SplatRec*)ws;
     float* mom = ht_records(ws, bn, N);
-    k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+    if (!prepared)
+        k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
     k_fill_f32<<<grid1((long)kHtRec * bn * N, 256), 256, 0, st>>>(mom, (long)kHtRec * bn * N, 0.0f);
     const dim3 tiles((unsigned)(((W + kHtTile - 1) / kHtTile) * ((H + kHtTile - 1) / kHtTile)), (unsigned)bn);
     const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)ipi % 16 == 0);

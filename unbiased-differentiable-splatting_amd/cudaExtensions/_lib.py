@@ -55,7 +55,8 @@ _SIGS = {
     "ude_interp_backward": (c_int, [c_int] * 7 + [c_ptr] * 7),
     "ude_ht_workspace_bytes": (c_size, [c_int] * 2),
     "ude_ht_forward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] * 2 + [c_ptr] * 3 + [c_size, c_ptr]),
-    "ude_ht_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 6 + [c_int] * 2 + [c_ptr] * 6 + [c_size, c_ptr]),
+    "ude_ht_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 6 + [c_int] * 2 + [c_ptr] * 5 + [c_int, c_ptr,
+                                                                                                    c_size, c_ptr]),
     "ude_sh_forward": (c_int, [c_int] * 4 + [c_ptr] * 5),
     "ude_sh_backward": (c_int, [c_int] * 4 + [c_ptr] * 8),
     "ude_lambert_forward": (c_int, [c_int] * 5 + [c_ptr] * 6),

@@ -32,7 +32,24 @@ def _inputs(indices, invPi, points, normals, stdDevs, attribute):
     return indices, invPi, points, normals, stdDevs, attribute
 
 
-def forward(width, height, focallength, indices, invPi, points, normals, stdDevs, attribute, smoothing=0.0):
+def workspace(indices, stdDevs):
+    """A workspace for forward and backward of these shapes; passing the same one to
+    forward and then to backward(..., prepared=True) reuses the per-splat records."""
+    bn = indices.shape[0]
+    return _lib.workspace(_lib.lib().ude_ht_workspace_bytes(bn, stdDevs.shape[1]), indices.device)
+
+
+def _ws(ws, bn, N, dev):
+    need = _lib.lib().ude_ht_workspace_bytes(bn, N)
+    if ws is None:
+        return _lib.workspace(need, dev)
+    if ws.dtype != torch.uint8 or ws.numel() < need or ws.device != dev:
+        raise ValueError("workspace must be a uint8 tensor of at least %d bytes on %s" % (need, dev))
+    return ws
+
+
+def forward(width, height, focallength, indices, invPi, points, normals, stdDevs, attribute, smoothing=0.0,
+            workspace=None):
     """image (bn, W, H, 3) = sum over a pixel's samples of min(w, 1) / pi * attribute."""
     dev = _lib.require_gpu(indices, invPi, points, normals, stdDevs, attribute)
     indices, invPi, points, normals, stdDevs, attribute = _inputs(indices, invPi, points, normals, stdDevs,
@@ -41,7 +58,7 @@ def forward(width, height, focallength, indices, invPi, points, normals, stdDevs
     N = stdDevs.shape[1]
     out = torch.empty((bn, W, H, 3), dtype=torch.float32, device=dev)
     L = _lib.lib()
-    ws = _lib.workspace(L.ude_ht_workspace_bytes(bn, N), dev)
+    ws = _ws(workspace, bn, N, dev)
     rc = L.ude_ht_forward(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(indices), _lib.ptr(invPi),
                           _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs), stdDevs.shape[0],
                           attribute.shape[0], _lib.ptr(attribute), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
@@ -51,8 +68,9 @@ def forward(width, height, focallength, indices, invPi, points, normals, stdDevs
 
 
 def backward(width, height, focallength, gradImage, indices, invPi, points, normals, stdDevs, attribute,
-             smoothing=0.0):
-    """[gradPoints, gradNormals, gradStdDevs, gradAttribute] of sum(gradImage * image)."""
+             smoothing=0.0, workspace=None, prepared=False):
+    """[gradPoints, gradNormals, gradStdDevs, gradAttribute] of sum(gradImage * image).
+    prepared=True: `workspace` is the one forward used on the same splats."""
     dev = _lib.require_gpu(gradImage, indices, invPi, points, normals, stdDevs, attribute)
     gradImage = _lib.f32(gradImage, "gradImage")
     indices, invPi, points, normals, stdDevs, attribute = _inputs(indices, invPi, points, normals, stdDevs,
@@ -63,15 +81,17 @@ def backward(width, height, focallength, gradImage, indices, invPi, points, norm
     N = stdDevs.shape[1]
     gp = torch.empty(points.shape, dtype=torch.float32, device=dev)
     gn = torch.empty(normals.shape, dtype=torch.float32, device=dev)
-    gs = (torch.zeros if stdDevs.shape[0] == 1 else torch.empty)(stdDevs.shape, dtype=torch.float32, device=dev)
+    gs = (torch.empty if stdDevs.shape[0] == bn else torch.zeros)(stdDevs.shape, dtype=torch.float32, device=dev)
     ga = (torch.empty if attribute.shape[0] == bn else torch.zeros)(attribute.shape, dtype=torch.float32,
                                                                     device=dev)
+    if prepared and workspace is None:
+        raise ValueError("prepared=True needs the workspace forward used")
     L = _lib.lib()
-    ws = _lib.workspace(L.ude_ht_workspace_bytes(bn, N), dev)
+    ws = _ws(workspace, bn, N, dev)
     rc = L.ude_ht_backward(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(gradImage),
                            _lib.ptr(indices), _lib.ptr(invPi), _lib.ptr(points), _lib.ptr(normals),
                            _lib.ptr(stdDevs), stdDevs.shape[0], attribute.shape[0], _lib.ptr(attribute),
-                           _lib.ptr(gp), _lib.ptr(gn), _lib.ptr(gs), _lib.ptr(ga), _lib.ptr(ws), ws.numel(),
-                           _lib.stream(dev))
+                           _lib.ptr(gp), _lib.ptr(gn), _lib.ptr(gs), _lib.ptr(ga), int(bool(prepared)), _lib.ptr(ws),
+                           ws.numel(), _lib.stream(dev))
     _lib.check(rc, "estimator.backward")
     return [gp, gn, gs, ga]

@@ -38,7 +38,7 @@ def backward(width, height, focallength, gradIn, indices, points, normals, stdDe
     N = stdDevs.shape[1]
     gp = torch.empty(points.shape, dtype=torch.float32, device=dev)
     gn = torch.empty(normals.shape, dtype=torch.float32, device=dev)
-    gs = (torch.zeros if stdDevs.shape[0] == 1 else torch.empty)(stdDevs.shape, dtype=torch.float32, device=dev)
+    gs = (torch.empty if stdDevs.shape[0] == bn else torch.zeros)(stdDevs.shape, dtype=torch.float32, device=dev)
     L = _lib.lib()
     ws = _lib.workspace(L.ude_splat_workspace_bytes(bn, N), dev)
     rc = L.ude_splat_backward(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(gradIn),

@@ -140,6 +140,7 @@ class PoissonFrame:
         self.W, self.H, self.f = scene.width, scene.height, scene.focal
         self.dens = torch.zeros((self.V, 2, self.W, self.H), dtype=torch.float32, device=device)
         self._transform = transform
+        self._ws = None              # fused-estimator workspace shared by forward and backward
 
     def camera(self):
         tr = self._transform
@@ -156,7 +157,9 @@ class PoissonFrame:
         ids, ipi = sampling.samplePoisson(self.W, self.H, self.f, P, nb, None, self.S, kcap or self.kcap, 0.0,
                                           self.prec, seed=seed, rate=rate or self.rate, tileScale=tile_scale,
                                           densities=self.dens, seedTensor=seed_tensor)
-        img = estimator.forward(self.W, self.H, self.f, ids, ipi, P, Nc, self.S, self.A)
+        if self._ws is None:
+            self._ws = estimator.workspace(ids, self.S)
+        img = estimator.forward(self.W, self.H, self.f, ids, ipi, P, Nc, self.S, self.A, workspace=self._ws)
         self.last_ids = ids
         return img, (P, Nc, ids, ipi)
 
@@ -168,7 +171,8 @@ class PoissonFrame:
         img, (P, Nc, ids, ipi) = self.image(seed, tile_scale, rate, kcap, seed_tensor)
         g = (2.0 * (img - self.target)).contiguous()
         loss = ((img - self.target) ** 2).sum() if with_loss else None
-        gP, gN, gS, gA = estimator.backward(self.W, self.H, self.f, g, ids, ipi, P, Nc, self.S, self.A)
+        gP, gN, gS, gA = estimator.backward(self.W, self.H, self.f, g, ids, ipi, P, Nc, self.S, self.A,
+                                            workspace=self._ws, prepared=True)
         _, gPw = tr.affineBackward(gP, self.M, self.Pw)
         _, gNw = tr.linearBackward(gN, self.M, self.Nw)
         return loss, img, (gPw, gNw, gS, gA), g
