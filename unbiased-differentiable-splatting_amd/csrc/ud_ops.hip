@@ -161,32 +161,56 @@ constexpr int kHtRec = 16;
 // coordinates z = T d = (u, v):
 //   m0 = sum g w, (m1u, m1v) = sum g w z, (m2uu, m2uv, m2vv) = sum g w z z^T
 // for slots whose unclamped weight is <= 1 (splattingCuda.cu:173).
-__global__ void k_splat_moments(long nslots, int N, int W, int H, int k,
-                                const int32_t* __restrict__ idx, const float* __restrict__ gin,
-                                const SplatRec* __restrict__ rec, float* __restrict__ mom)
+__global__ __launch_bounds__(256) void k_splat_moments(long nslots, int N, int W, int H, int k,
+                                                       const int32_t* __restrict__ idx,
+                                                       const float* __restrict__ gin,
+                                                       const SplatRec* __restrict__ rec, float* __restrict__ mom)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nslots) return;
-    int id = idx[t];
-    if (id < 0) return;
-    long pix = t / k;
-    long b = pix / ((long)W * H);
-    int r = (int)(pix - b * (long)W * H);
-    int i = r / H, j = r - (r / H) * H;
-    long si = b * N + id;
-    SplatRec s = rec[si];
-    float dx, dy;
-    float w = ewa_eval(s, (float)i, (float)j, dx, dy);
-    if (!(w <= 1.0f)) return;
-    float gw = gin[t] * w;
-    if (gw == 0.0f) return;
-    float* m = mom + kMomRec * si;
-    atomicAdd(m + 0, gw);
-    atomicAdd(m + 1, gw * dx);
-    atomicAdd(m + 2, gw * dy);
-    atomicAdd(m + 3, (gw * dx) * dx);
-    atomicAdd(m + 4, (gw * dx) * dy);
-    atomicAdd(m + 5, (gw * dy) * dy);
+    // each lane computes its slot's six sums; the wavefront then adds them with six
+    // consecutive lanes per record, so one atomic instruction covers ten records'
+    // lines instead of 64 (L2 serves the dwords of one line in one instruction
+    // together)
+    __shared__ float sv[4][64][6];
+    __shared__ long sr[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long si = -1;
+    float v[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+    const int id = t < nslots ? idx[t] : -1;
+    if (id >= 0) {
+        long pix = t / k;
+        long b = pix / ((long)W * H);
+        int r = (int)(pix - b * (long)W * H);
+        int i = r / H, j = r - (r / H) * H;
+        const long s_ = b * N + id;
+        SplatRec s = rec[s_];
+        float dx, dy;
+        float w = ewa_eval(s, (float)i, (float)j, dx, dy);
+        if (w <= 1.0f) {
+            float gw = gin[t] * w;
+            if (gw != 0.0f) {
+                si = s_;
+                v[0] = gw;
+                v[1] = gw * dx;
+                v[2] = gw * dy;
+                v[3] = (gw * dx) * dx;
+                v[4] = (gw * dx) * dy;
+                v[5] = (gw * dy) * dy;
+            }
+        }
+    }
+    for (int c = 0; c < 6; c++) sv[wv][lane][c] = v[c];
+    sr[wv][lane] = si;
+    __syncthreads();
+    const int sub = lane / 6, c = lane - 6 * sub;
+    for (int q = 0; q < 64; q += 10) {
+        const int slot = q + sub;
+        if (lane < 60 && slot < 64) {
+            const long s_ = sr[wv][slot];
+            const float x = sv[wv][slot][c];
+            if (s_ >= 0 && x != 0.0f) atomicAdd(mom + kMomRec * s_ + c, x);
+        }
+    }
 }
 
 // splatting.backward, splat pass: chain the moments through V(A, sigma),
@@ -771,27 +795,53 @@ __global__ void k_interp_forward(long npix, int W, int H, int k, int ab, int N, 
     }
 }
 
-__global__ void k_interp_backward(long nslots, int W, int H, int k, int ab, int N, int C,
-                                  const float* __restrict__ g, const int32_t* __restrict__ idx,
-                                  const float* __restrict__ wts, const float* __restrict__ attr,
-                                  float* __restrict__ gW, float* __restrict__ gA)
+__global__ __launch_bounds__(256) void k_interp_backward(long nslots, int W, int H, int k, int ab, int N, int C,
+                                                         const float* __restrict__ g,
+                                                         const int32_t* __restrict__ idx,
+                                                         const float* __restrict__ wts,
+                                                         const float* __restrict__ attr, float* __restrict__ gW,
+                                                         float* __restrict__ gA)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nslots) return;
-    int id = idx[t];
-    if (id < 0) { gW[t] = 0.0f; return; }
-    long pix = t / k;
-    long b = pix / ((long)W * H);
-    long ao = (long)C * ((ab > 1 ? b * N : 0) + id);
-    float w = wts[t];
-    float acc = 0.0f;
-    for (int c = 0; c < C; c++) {
-        float gc = g[(long)C * pix + c];
-        acc += gc * attr[ao + c];
-        const float v = gc * w;
-        if (v != 0.0f) atomicAdd(gA + ao + c, v);      // zero terms (occluded slots) add nothing
+    // three-channel attributes: the wavefront adds the slots' g w with three
+    // consecutive lanes per attribute row (21 rows per atomic instruction)
+    __shared__ float sv[4][64][3];
+    __shared__ long sa[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long ao = -1;
+    float v3[3] = { 0.0f, 0.0f, 0.0f };
+    if (t < nslots) {
+        const int id = idx[t];
+        if (id < 0) gW[t] = 0.0f;
+        else {
+            long pix = t / k;
+            long b = pix / ((long)W * H);
+            ao = (long)C * ((ab > 1 ? b * N : 0) + id);
+            float w = wts[t];
+            float acc = 0.0f;
+            for (int c = 0; c < C; c++) {
+                float gc = g[(long)C * pix + c];
+                acc += gc * attr[ao + c];
+                const float v = gc * w;
+                if (C == 3) v3[c] = v;
+                else if (v != 0.0f) atomicAdd(gA + ao + c, v);      // zero terms (occluded slots) add nothing
+            }
+            gW[t] = acc;
+        }
     }
-    gW[t] = acc;
+    if (C != 3) return;                                // block-uniform
+    for (int c = 0; c < 3; c++) sv[wv][lane][c] = v3[c];
+    sa[wv][lane] = ao;
+    __syncthreads();
+    const int sub = lane / 3, c = lane - 3 * sub;
+    for (int q = 0; q < 64; q += 21) {
+        const int slot = q + sub;
+        if (lane < 63 && slot < 64) {
+            const long a = sa[wv][slot];
+            const float x = sv[wv][slot][c];
+            if (a >= 0 && x != 0.0f) atomicAdd(gA + a + c, x);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
