@@ -715,10 +715,27 @@ __device__ __forceinline__ NodeGeo merge_geo(const NodeGeo& gl, int cl, const No
     float rr = mn / h;
     int deg;
     if (gl.deg < 0 && gr.deg < 0) deg = -1;
+    else if (!(rr < 1.0f)) deg = -1;
     else {
+        // the lowest degree from the children's up whose hermiteError meets the
+        // precision; the error's power r^d and factorial d! are carried from one
+        // degree to the next, with the same products as hermite_error evaluates
         int d = gl.deg > gr.deg ? gl.deg : gr.deg;
-        while (d <= kMaxDegree && hermite_error(rr, (float)cn, d) > prec && rr < 1.0f) d++;
-        deg = (rr < 1.0f && d <= kMaxDegree) ? d : -1;
+        const float om = 1.0f - rr;
+        const float a = (float)cn / (om * om);
+        float rp = 1.0f, fac = 1.0f;
+        for (int i = 0; i < d; i++) rp = rp * rr;
+        for (int i = 1; i <= d; i++) fac = fac * (float)i;
+        while (d <= kMaxDegree) {
+            const float b = rp / sqrtf(fac);
+            const float c = b * b;
+            const float e = 2.0f * (1.0f - rp) * b;
+            if (!(a * (c + e) > prec)) break;
+            d++;
+            rp = rp * rr;
+            fac = fac * (float)d;
+        }
+        deg = d <= kMaxDegree ? d : -1;
     }
     g.deg = deg;
     g.r2 = trunc_r2(h, prec, cn);
@@ -747,46 +764,93 @@ __device__ __forceinline__ void merge_up(int p, const NodeLnk& L, const NodeAux&
 // Bottom-up pass, level 1: one workgroup per block of 1024 sorted leaves.  The
 // second thread to reach a node whose leaf range lies inside the block merges
 // it.  Such a node's index lies inside the block too (a Karras node's index is
-// one end of its range), so its arrival counter is an LDS word; both arrivals
-// come from the same workgroup, so workgroup-scope fences order the exchange
-// of the children's records.  A thread that reaches a node spanning the
-// block's boundary hands its (complete) node to level 2.
+// one end of its range), so the block's internal nodes are staged in LDS first
+// (links and leaf ranges in, merged records out, one arrival counter each) and
+// the fan-in never waits on device memory: both arrivals come from the same
+// workgroup, so workgroup-scope fences order the exchange.  Merged records are
+// also stored to device memory for the later passes.  A thread that reaches a
+// node spanning the block's boundary hands its (complete) node to level 2.
 constexpr int kFanBlock = 1024;
 __global__ __launch_bounds__(kFanBlock) void k_bottom_up_local(ViewState* vs, NodeGeo* geo, NodeLnk* lnk,
                                                                const NodeAux* __restrict__ aux, int* seeds, VStride vsd)
 {
     __shared__ int arrivals[kFanBlock];
+    __shared__ float2 sLeaf[kFanBlock];          // the block's leaf positions
+    __shared__ NodeGeo sNode[kFanBlock];         // merged records of internal nodes b0 + i
+    __shared__ int4 sLink[kFanBlock];            // left, right, parent, first leaf of node b0 + i
+    __shared__ int sLast[kFanBlock];             // last leaf of node b0 + i
+    __shared__ int sSeeds[kFanBlock];            // nodes handed to level 2
+    __shared__ int nSeeds, seedBase;
     vs = vw(vs, vsd.ws);
     geo = vw(geo, vsd.ws);
     lnk = vw(lnk, vsd.ws);
     aux = vw(aux, vsd.ws);
     seeds = vw(seeds, vsd.ws);
-    arrivals[threadIdx.x] = 0;
+    const int n = vs->nc;
+    const int L0 = n - 1;
+    const int b0 = blockIdx.x * kFanBlock;
+    const int tid = threadIdx.x;
+    const int pos = b0 + tid;
+    arrivals[tid] = 0;
+    if (tid == 0) nSeeds = 0;
+    NodeGeo g;
+    int p = -1;
+    if (pos < n && n >= 2) {
+        g = geo[L0 + pos];
+        sLeaf[tid] = make_float2(g.cx, g.cy);
+        p = lnk[L0 + pos].parent;
+    }
+    if (pos < L0) {
+        const NodeLnk L = lnk[pos];
+        const NodeAux a = aux[pos];
+        sLink[tid] = make_int4(L.left, L.right, L.parent, a.first);
+        sLast[tid] = a.last;
+    }
     __syncthreads();
-    int n = vs->nc;
-    int b0 = blockIdx.x * kFanBlock;
-    int pos = b0 + threadIdx.x;
-    if (pos >= n || n < 2) return;
-    float h = vs->h, prec = vs->prec;
-    int nd = (n - 1) + pos;
-    NodeGeo g = geo[nd];
+    const float h = vs->h, prec = vs->prec;
+    int nd = L0 + pos;
     int cnt = 1;
-    int p = lnk[nd].parent;
+    if (pos >= n || n < 2) p = -1;
     while (p >= 0) {
-        const NodeAux a = aux[p];
-        const NodeLnk L = lnk[p];
-        if (a.first < b0 || a.last >= b0 + kFanBlock) {
-            int q = atomicAdd(&vs->nseeds, 1);
-            seeds[q] = nd;
-            return;
+        const int pi = p - b0;
+        const int4 Lk = (pi >= 0 && pi < kFanBlock) ? sLink[pi] : make_int4(0, 0, 0, -1);
+        const int last = (pi >= 0 && pi < kFanBlock) ? sLast[pi] : b0 + kFanBlock;
+        if (Lk.w < b0 || last >= b0 + kFanBlock) {   // p spans the block's boundary
+            sSeeds[atomicAdd(&nSeeds, 1)] = nd;       // at most one per thread
+            break;
         }
         __threadfence_block();
-        int old = atomicAdd(&arrivals[p - b0], 1);
-        if (old == 0) return;
+        int old = atomicAdd(&arrivals[pi], 1);
+        if (old == 0) break;
         __threadfence_block();
-        merge_up(p, L, a, nd, g, cnt, h, prec, geo, lnk);
-        p = L.parent;
+        const bool isLeft = Lk.x == nd;
+        const int sib = isLeft ? Lk.y : Lk.x;
+        NodeGeo gs;
+        if (sib >= L0) {                             // a leaf: its box and centre are its position
+            const float2 q = sLeaf[sib - L0 - b0];
+            gs.lox = q.x; gs.loy = q.y; gs.hix = q.x; gs.hiy = q.y; gs.cx = q.x; gs.cy = q.y;
+            gs.r2 = 0.0f; gs.deg = 0;
+        } else {
+            gs = sNode[sib - b0];
+        }
+        const int cs = (last - Lk.w + 1) - cnt;
+        const NodeGeo gl = isLeft ? g : gs, gr = isLeft ? gs : g;
+        const int cl = isLeft ? cnt : cs, cr = isLeft ? cs : cnt;
+        const NodeGeo m = merge_geo(gl, cl, gr, cr, h, prec);
+        sNode[pi] = m;
+        geo[p] = m;
+        lnk[p].count = cl + cr;
+        g = m;
+        cnt = cl + cr;
+        nd = p;
+        p = Lk.z;
     }
+    // one device-wide slot reservation per workgroup (thousands of single-word
+    // atomics on one counter would serialize)
+    __syncthreads();
+    if (tid == 0) seedBase = nSeeds ? atomicAdd(&vs->nseeds, nSeeds) : 0;
+    __syncthreads();
+    if (tid < nSeeds) seeds[seedBase + tid] = sSeeds[tid];
 }
 
 // Bottom-up pass, level 2: the few block-spanning nodes, device-scope fan-in.
