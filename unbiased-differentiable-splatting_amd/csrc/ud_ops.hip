@@ -353,7 +353,9 @@ __global__ __launch_bounds__(256) void k_ht_forward(long npix, int N, int W, int
                                                     const SplatRec* __restrict__ rec,
                                                     const float* __restrict__ attr, float* __restrict__ out)
 {
-    const long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-grouped block order: the columns a block shades share their splats with
+    // the neighbouring blocks', which then run on the same XCD
+    const long pix = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     const long b = pix / ((long)W * H);
     const int r = (int)(pix - b * (long)W * H);

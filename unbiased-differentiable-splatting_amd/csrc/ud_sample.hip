@@ -1706,11 +1706,6 @@ struct TileCfg {
 };
 constexpr int kMaxWavesPerBlock = 8;
 
-__device__ __forceinline__ int xcd_block(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
-}
 
 // pixel (i, j) of this lane; false for a wavefront past the last tile (wave-uniform)
 __device__ __forceinline__ bool tile_lane(const TileCfg& c, int W, int H, int lane, int& i, int& j)
