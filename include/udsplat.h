@@ -248,6 +248,14 @@ int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothi
                     int sdBatch, int ab, const float* attr, float* gPts, float* gNrm, float* gSd, float* gAttr,
                     int prepared, void* ws, size_t wsBytes, ude_stream_t stream);
 
+/* ---- L2 image loss -------------------------------------------------------- */
+
+/* No reference counterpart (its scripts use torch ops): grad = 2 (image - target)
+ * and sums[0] = sum (image - target)^2 over n floats, accumulated in FP64 in a
+ * fixed order; sums holds 1 + 1024 doubles (the rest is scratch).  Records no
+ * memset (usable under graph capture). */
+int ude_l2_grad(long n, const float* image, const float* target, float* grad, double* sums, ude_stream_t stream);
+
 /* ---- shading ------------------------------------------------------------ */
 
 /* shading.forwardSh -- replaces cudaShadingShForward (shading.cpp:141-168,

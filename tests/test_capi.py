@@ -56,6 +56,8 @@ def test_python_bindings_match_header():
                 assert ca is ctypes.c_uint64, (name, decl)
             elif decl.startswith("size_t"):
                 assert ca is ctypes.c_size_t, (name, decl)
+            elif decl.startswith("long "):
+                assert ca is ctypes.c_long, (name, decl)
             else:
                 assert decl.startswith("int ") and ca is ctypes.c_int, (name, decl)
 

@@ -150,3 +150,17 @@ def test_estimate_empty_and_bad_arguments(dev):
         estimator.backward(W, H, f, g, ids, ipi, P, NB, SD, A, prepared=True)
     with pytest.raises(ValueError):
         estimator.forward(W, H, f, ids, ipi, P, NB, SD, A, workspace=torch.empty(16, dtype=torch.uint8, device=dev))
+
+
+def test_l2_loss_and_gradient(dev):
+    """estimator.l2: the gradient equals 2 (image - target) bit for bit, the loss the
+    FP64 sum of squares (also for an empty image)."""
+    from cudaExtensions import estimator
+    rng = np.random.default_rng(3)
+    for shape in [(1, 64, 48, 3), (2, 129, 7, 3), (1, 0, 5, 3)]:
+        a = t(rng.normal(size=shape).astype(np.float32), dev)
+        b = t(rng.normal(size=shape).astype(np.float32), dev)
+        loss, g = estimator.l2(a, b)
+        assert torch.equal(g, 2.0 * (a - b))
+        ref = float(((a - b).double() ** 2).sum())
+        assert abs(float(loss) - ref) <= 1e-12 * max(ref, 1.0), (shape, float(loss), ref)

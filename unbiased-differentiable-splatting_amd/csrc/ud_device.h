@@ -96,16 +96,6 @@ __device__ __forceinline__ float plog1mf(float u)
 // ---------------------------------------------------------------------------
 // Philox4x32-10 counter-based generator (Salmon et al., SC'11).
 // ---------------------------------------------------------------------------
-// Workgroups are handed to the 8 XCDs round-robin (block b on XCD b mod 8).
-// xcd_block(b, nb) renumbers them so that XCD x runs a contiguous share of the
-// nb blocks (a bijection): neighbouring blocks, which read the same splats or
-// tree nodes, then share one XCD's L2.
-__device__ __forceinline__ int xcd_block(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
-}
-
 struct u32x4 { uint32_t x, y, z, w; };
 
 __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)

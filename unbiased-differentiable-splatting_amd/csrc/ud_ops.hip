@@ -355,7 +355,7 @@ __global__ __launch_bounds__(256) void k_ht_forward(long npix, int N, int W, int
 {
     // XCD-grouped block order: the columns a block shades share their splats with
     // the neighbouring blocks', which then run on the same XCD
-    const long pix = (long)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
+    const long pix = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     const long b = pix / ((long)W * H);
     const int r = (int)(pix - b * (long)W * H);
@@ -513,6 +513,42 @@ __global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W,
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// L2 image loss of a frame (no reference counterpart; the reference's scripts
+// compute it with torch ops): grad = 2 (image - target) and loss =
+// sum (image - target)^2 in one pass, the per-workgroup FP64 sums added by one
+// workgroup in a fixed order (a reproducible loss), and no memset, so a captured
+// frame can include it.
+// ---------------------------------------------------------------------------
+constexpr int kL2Blocks = 1024;
+
+__global__ __launch_bounds__(256) void k_l2_grad(long n, const float* __restrict__ a, const float* __restrict__ b,
+                                                 float* __restrict__ grad, double* __restrict__ part)
+{
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const float d = a[i] - b[i];
+        grad[i] = 2.0f * d;
+        acc += (double)d * (double)d;
+    }
+    acc = wave_sum_d(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_sum_parts(int nb, const double* __restrict__ part, double* __restrict__ out)
+{
+    __shared__ double red[4];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) acc += part[i];
+    acc = wave_sum_d(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 // sampling.covariance (samplingCuda.cu:323-361), FP64 internally
@@ -1196,6 +1232,19 @@ int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothi
     k_ht_backward<<<tiles, kHtTile * kHtTile, 0, st>>>(N, W, H, k, ab, vec4, gradImage, idx, ipi, rec, attr, mom);
     k_splat_finalize<<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
                                                               rec, mom, kHtRec, ab, gAttr, gPts, gNrm, gSd);
+    return launch_status();
+}
+
+int ude_l2_grad(long n, const float* image, const float* target, float* grad, double* sums, hipStream_t st)
+{
+    if (n < 0) return UDE_ERR_ARG;
+    if (n == 0) {
+        k_sum_parts<<<1, 256, 0, st>>>(0, sums + 1, sums);
+        return launch_status();
+    }
+    const unsigned blocks = grid1(n, 256) < (unsigned)kL2Blocks ? grid1(n, 256) : (unsigned)kL2Blocks;
+    k_l2_grad<<<blocks, 256, 0, st>>>(n, image, target, grad, sums + 1);
+    k_sum_parts<<<1, 256, 0, st>>>((int)blocks, sums + 1, sums);
     return launch_status();
 }
 

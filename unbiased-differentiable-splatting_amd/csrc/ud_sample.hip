@@ -1710,7 +1710,7 @@ constexpr int kMaxWavesPerBlock = 8;
 // pixel (i, j) of this lane; false for a wavefront past the last tile (wave-uniform)
 __device__ __forceinline__ bool tile_lane(const TileCfg& c, int W, int H, int lane, int& i, int& j)
 {
-    const int blk = c.xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int blk = c.xcd ? (int)xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int tile = blk * c.wpb + (int)(threadIdx.x >> 6);
     const int th = 1 << c.lth;
     const int tilesY = (H + th - 1) >> c.lth;

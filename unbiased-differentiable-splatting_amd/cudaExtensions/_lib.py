@@ -57,6 +57,7 @@ _SIGS = {
     "ude_ht_forward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] * 2 + [c_ptr] * 3 + [c_size, c_ptr]),
     "ude_ht_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 6 + [c_int] * 2 + [c_ptr] * 5 + [c_int, c_ptr,
                                                                                                     c_size, c_ptr]),
+    "ude_l2_grad": (c_int, [ctypes.c_long] + [c_ptr] * 5),
     "ude_sh_forward": (c_int, [c_int] * 4 + [c_ptr] * 5),
     "ude_sh_backward": (c_int, [c_int] * 4 + [c_ptr] * 8),
     "ude_lambert_forward": (c_int, [c_int] * 5 + [c_ptr] * 6),

@@ -95,3 +95,19 @@ def backward(width, height, focallength, gradImage, indices, invPi, points, norm
                            ws.numel(), _lib.stream(dev))
     _lib.check(rc, "estimator.backward")
     return [gp, gn, gs, ga]
+
+
+def l2(image, target):
+    """(loss, grad) of the L2 image loss sum (image - target)^2: grad = 2 (image -
+    target) in float32, loss a float64 0-d tensor summed in a fixed order."""
+    dev = _lib.require_gpu(image, target)
+    image = _lib.f32(image, "image")
+    target = _lib.f32(target, "target")
+    if image.shape != target.shape:
+        raise ValueError("target %s must match image %s" % (tuple(target.shape), tuple(image.shape)))
+    grad = torch.empty_like(image)
+    sums = torch.empty(1 + 1024, dtype=torch.float64, device=dev)
+    rc = _lib.lib().ude_l2_grad(image.numel(), _lib.ptr(image), _lib.ptr(target), _lib.ptr(grad), _lib.ptr(sums),
+                                _lib.stream(dev))
+    _lib.check(rc, "estimator.l2")
+    return sums[0], grad

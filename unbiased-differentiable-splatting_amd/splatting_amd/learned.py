@@ -165,12 +165,13 @@ class PoissonFrame:
 
     def step(self, seed, tile_scale=None, rate=None, kcap=None, seed_tensor=None, with_loss=True):
         """Returns (loss, image, grads (positions, normals, std-devs, albedo), dL/dimage);
-        loss is None without with_loss (a frame captured in a graph skips the reduction)."""
+        loss (a float64 0-d tensor) is None without with_loss."""
         from cudaExtensions import estimator
         tr = self._transform
         img, (P, Nc, ids, ipi) = self.image(seed, tile_scale, rate, kcap, seed_tensor)
-        g = (2.0 * (img - self.target)).contiguous()
-        loss = ((img - self.target) ** 2).sum() if with_loss else None
+        loss, g = estimator.l2(img, self.target)          # one pass, no memset (graph-safe)
+        if not with_loss:
+            loss = None
         gP, gN, gS, gA = estimator.backward(self.W, self.H, self.f, g, ids, ipi, P, Nc, self.S, self.A,
                                             workspace=self._ws, prepared=True)
         _, gPw = tr.affineBackward(gP, self.M, self.Pw)
