@@ -1877,7 +1877,9 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
                 continue;
             }
             if (off >= 0) {
-                float co[(P + 1) * (P + 1)];                      // wave-uniform: scalar loads
+                // wave-uniform block, fetched by broadcast vector loads issued
+                // together (faster here than scalar loads or staging in LDS)
+                float co[(P + 1) * (P + 1)];
                 const float UD_GLOBAL* cc = cf + off;
 #pragma unroll
                 for (int q = 0; q < (P + 1) * (P + 1); q++) co[q] = cc[q];
