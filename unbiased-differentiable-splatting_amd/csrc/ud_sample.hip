@@ -1723,7 +1723,7 @@ __device__ __forceinline__ bool tile_lane(const TileCfg& c, int W, int H, int la
 }
 
 
-constexpr int kFmmStack = 96;
+constexpr int kFmmStack = 136;
 
 // children record of internal node p for the density walk (after the expansions
 // of every band are selected)
@@ -1787,8 +1787,11 @@ __device__ __forceinline__ bool box4_meets_disc(const float* bx, float qx, float
 // last-in-first-out order) and sums bit for bit like the reference.  A pushed
 // entry also carries the child's centre and expansion offset from its parent's
 // FmmRec, so leaves and expansions need no node load and a descent needs one
-// wave-uniform record load.  Karras trees on 64-bit keys are at most 64 levels
-// deep, so the stack never holds more than 65 entries.
+// wave-uniform record load.  Two internal entries on top of the stack are
+// expanded together (two record loads in flight), their children pushed in the
+// order that popping one after the other leaves, so the accumulation order is
+// unchanged.  Karras trees on 64-bit keys are at most 64 levels deep; a level
+// keeps at most two entries pending, so the stack stays within 2 * 64 + 4.
 // A fixed-order expansion at scaled offset t = (q - c) / sqrt(2 h^2):
 // sum_a h_a(tx) sum_b C_ab h_b(ty), Hermite functions by their three-term
 // recurrence, every product-sum one fused multiply-add (the oracle uses fmaf in
@@ -1868,8 +1871,12 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
             __builtin_amdgcn_wave_barrier();
             const int nd = __builtin_amdgcn_readfirstlane(sN[sp]);
             const int off = __builtin_amdgcn_readfirstlane(sF[sp]);
+            const int sp2 = sp > 0 ? sp - 1 : 0;
+            const int nd2 = __builtin_amdgcn_readfirstlane(sN[sp2]);
+            const int off2 = __builtin_amdgcn_readfirstlane(sF[sp2]);
             const float2 c = sC[sp];
             const unsigned long long mask = sM[sp];
+            const unsigned long long mask2 = sM[sp2];
             __builtin_amdgcn_wave_barrier();
             const bool act = (mask >> lane) & 1ull;
             if (nd >= L0) {
@@ -1886,10 +1893,34 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
                 if (act) w = w + fixed_series<P>(co, (qx - c.x) * B.is2h, (qy - c.y) * B.is2h);
                 continue;
             }
+            // an internal node takes the entry below it along when that one is an
+            // internal node without an expansion too: both records load together,
+            // and the lower entry's children are pushed below the upper entry's,
+            // which is the stack a pop of each in turn leaves
+            const bool pair = sp > 0 && nd2 < L0 && off2 < 0;
+            if (pair) sp--;
             const FmmRec r = frec[nd];
+            const FmmRec r2 = frec[pair ? nd2 : nd];
+            const bool act2 = pair && ((mask2 >> lane) & 1ull);
             const unsigned long long ml = __ballot(act && box4_meets_disc(r.lbox, qx, qy, B.radius));
             const unsigned long long mr = __ballot(act && box4_meets_disc(r.rbox, qx, qy, B.radius));
-            if (sp + 2 > kFmmStack) break;                   // unreachable for trees of depth <= 64
+            const unsigned long long ml2 = __ballot(act2 && box4_meets_disc(r2.lbox, qx, qy, B.radius));
+            const unsigned long long mr2 = __ballot(act2 && box4_meets_disc(r2.rbox, qx, qy, B.radius));
+            if (sp + 4 > kFmmStack) break;                   // unreachable for trees of depth <= 64
+            if (ml2) {
+                if (lane == 0) {
+                    sN[sp] = r2.left; sF[sp] = b ? r2.lfix[1] : r2.lfix[0];
+                    sC[sp] = make_float2(r2.lc[0], r2.lc[1]); sM[sp] = ml2;
+                }
+                sp++;
+            }
+            if (mr2) {
+                if (lane == 0) {
+                    sN[sp] = r2.right; sF[sp] = b ? r2.rfix[1] : r2.rfix[0];
+                    sC[sp] = make_float2(r2.rc[0], r2.rc[1]); sM[sp] = mr2;
+                }
+                sp++;
+            }
             if (ml) {
                 if (lane == 0) {
                     sN[sp] = r.left; sF[sp] = b ? r.lfix[1] : r.lfix[0];
