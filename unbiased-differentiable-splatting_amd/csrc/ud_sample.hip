@@ -1787,9 +1787,10 @@ __device__ __forceinline__ bool box4_meets_disc(const float* bx, float qx, float
 // last-in-first-out order) and sums bit for bit like the reference.  A pushed
 // entry also carries the child's centre and expansion offset from its parent's
 // FmmRec, so leaves and expansions need no node load and a descent needs one
-// wave-uniform record load.  Two internal entries on top of the stack are
-// expanded together (two record loads in flight), their children pushed in the
-// order that popping one after the other leaves, so the accumulation order is
+// wave-uniform record load.  An internal entry just below the top is handled
+// in the same step as the top entry (two record loads in flight, or a record
+// load under a leaf's or expansion's term), its children pushed in the order
+// that popping one after the other leaves, so the accumulation order is
 // unchanged.  Karras trees on 64-bit keys are at most 64 levels deep; a level
 // keeps at most two entries pending, so the stack stays within 2 * 64 + 4.
 // A fixed-order expansion at scaled offset t = (q - c) / sqrt(2 h^2):
@@ -1879,31 +1880,41 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_fmm_eval(TileCfg cfg
             const unsigned long long mask2 = sM[sp2];
             __builtin_amdgcn_wave_barrier();
             const bool act = (mask >> lane) & 1ull;
-            if (nd >= L0) {
-                if (act) w = w + band_point(B, qx, qy, c.x, c.y);
-                continue;
-            }
-            if (off >= 0) {
-                // wave-uniform block, fetched by broadcast vector loads issued
-                // together (faster here than scalar loads or staging in LDS)
-                float co[(P + 1) * (P + 1)];
-                const float UD_GLOBAL* cc = cf + off;
+            // the term of a leaf or an expansion on top of the stack
+            auto add_top = [&]() {
+                if (nd >= L0) {
+                    if (act) w = w + band_point(B, qx, qy, c.x, c.y);
+                } else {
+                    // wave-uniform block, fetched by broadcast vector loads issued
+                    // together (faster here than scalar loads or staging in LDS)
+                    float co[(P + 1) * (P + 1)];
+                    const float UD_GLOBAL* cc = cf + off;
 #pragma unroll
-                for (int q = 0; q < (P + 1) * (P + 1); q++) co[q] = cc[q];
-                if (act) w = w + fixed_series<P>(co, (qx - c.x) * B.is2h, (qy - c.y) * B.is2h);
+                    for (int q = 0; q < (P + 1) * (P + 1); q++) co[q] = cc[q];
+                    if (act) w = w + fixed_series<P>(co, (qx - c.x) * B.is2h, (qy - c.y) * B.is2h);
+                }
+            };
+            // An internal entry without an expansion just below the top is taken
+            // along: its record loads with the top's record (two internal entries)
+            // or while the top's term is evaluated (a leaf or expansion on top,
+            // whose term is added first).  The lower entry's children are pushed
+            // below the upper entry's: the stack a pop of each in turn leaves.
+            const bool topInt = nd < L0 && off < 0;
+            const bool lowInt = sp > 0 && nd2 < L0 && off2 < 0;
+            if (!topInt && !lowInt) {
+                add_top();
                 continue;
             }
-            // an internal node takes the entry below it along when that one is an
-            // internal node without an expansion too: both records load together,
-            // and the lower entry's children are pushed below the upper entry's,
-            // which is the stack a pop of each in turn leaves
-            const bool pair = sp > 0 && nd2 < L0 && off2 < 0;
-            if (pair) sp--;
-            const FmmRec r = frec[nd];
-            const FmmRec r2 = frec[pair ? nd2 : nd];
-            const bool act2 = pair && ((mask2 >> lane) & 1ull);
-            const unsigned long long ml = __ballot(act && box4_meets_disc(r.lbox, qx, qy, B.radius));
-            const unsigned long long mr = __ballot(act && box4_meets_disc(r.rbox, qx, qy, B.radius));
+            if (lowInt) sp--;
+            const bool two = topInt && lowInt;
+            const int ndA = topInt ? nd : nd2;               // the upper internal entry
+            const FmmRec r = frec[ndA];
+            const FmmRec r2 = frec[two ? nd2 : ndA];
+            if (!topInt) add_top();
+            const bool actA = topInt ? act : (bool)((mask2 >> lane) & 1ull);
+            const bool act2 = two && ((mask2 >> lane) & 1ull);
+            const unsigned long long ml = __ballot(actA && box4_meets_disc(r.lbox, qx, qy, B.radius));
+            const unsigned long long mr = __ballot(actA && box4_meets_disc(r.rbox, qx, qy, B.radius));
             const unsigned long long ml2 = __ballot(act2 && box4_meets_disc(r2.lbox, qx, qy, B.radius));
             const unsigned long long mr2 = __ballot(act2 && box4_meets_disc(r2.rbox, qx, qy, B.radius));
             if (sp + 4 > kFmmStack) break;                   // unreachable for trees of depth <= 64
