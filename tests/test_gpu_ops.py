@@ -100,11 +100,14 @@ def test_filter_forward_backward(dev, k):
 
 
 @pytest.mark.parametrize("affine", [True, False])
-@pytest.mark.parametrize("vb", [1, 3])
-def test_transform(dev, affine, vb):
+@pytest.mark.parametrize("bn,vb", [(1, 1), (3, 1), (3, 3)])
+def test_transform(dev, affine, bn, vb):
+    """Forward and adjoint against the oracle; the vector gradient is written
+    without a prior fill when every element gets one value (stale NaN memory is
+    handed to the allocation first to show it)."""
     from cudaExtensions import transform
     rng = np.random.default_rng(1)
-    bn, N = 3, 5000
+    N = 5000
     M = rng.normal(size=(bn, 4, 4)).astype(np.float32)
     v = rng.normal(size=(vb, N, 3)).astype(np.float32)
     g = rng.normal(size=(bn, N, 3)).astype(np.float32)
@@ -112,7 +115,10 @@ def test_transform(dev, affine, vb):
     bw = transform.affineBackward if affine else transform.linearBackward
     out = fw(t(M, dev), t(v, dev))
     assert_close(out.cpu().numpy(), O.transform_forward(affine, M, v), RTOL, "transform forward")
-    gM, gV = bw(t(g, dev), t(M, dev), t(v, dev))
+    gd, Md, vd = t(g, dev), t(M, dev), t(v, dev)
+    junk = torch.full((vb, N, 3), float("nan"), device=dev)
+    del junk
+    gM, gV = bw(gd, Md, vd)
     rM, rV = O.transform_backward(affine, g, M, v)
     assert_close(gM.cpu().numpy(), rM, RTOL, "transform grad M")
     assert_close(gV.cpu().numpy(), rV, RTOL, "transform grad v")

@@ -28,7 +28,8 @@ def _backward(affine, gradIn, transformation, vector):
     bn = M.shape[0]
     vb, N, _ = v.shape
     gM = torch.zeros(M.shape, dtype=torch.float32, device=dev)
-    gV = torch.zeros(v.shape, dtype=torch.float32, device=dev)
+    # every element is written unless one shared vector set collects several views' sums
+    gV = (torch.empty if (vb > 1 or bn == 1) else torch.zeros)(v.shape, dtype=torch.float32, device=dev)
     rc = _lib.lib().ude_transform_backward(1 if affine else 0, bn, vb, N, _lib.ptr(g), _lib.ptr(M), _lib.ptr(v),
                                            _lib.ptr(gM), _lib.ptr(gV), _lib.stream(dev))
     _lib.check(rc, "transform.backward")
