@@ -202,6 +202,13 @@ int ude_filter_backward(int bn, int N, int W, int H, int k, double f, const int3
 int ude_transform_forward(int affine, int bn, int vb, int N, const float* M, const float* v, float* out,
                           ude_stream_t stream);
 
+/* transform.linearForwardFacing -- no counterpart in the reference's extension:
+ * linearForward of normals plus, in the same pass, the camera-facing copy the
+ * reference's renderer makes with two torch.where calls (render.py:200-203):
+ * facing = -out where out.z > 0, else out. */
+int ude_transform_forward_facing(int bn, int vb, int N, const float* M, const float* v, float* out, float* facing,
+                                 ude_stream_t stream);
+
 /* transform.{affine,linear}Backward -- replaces cudaTransformBackward
  * (transform.cpp:62-85, transformCuda.cu:119-156).  gM (bn,4,4) and, when
  * vb == 1 < bn, gV (1,N,3) are accumulated into (caller zeroed). */

@@ -115,6 +115,10 @@ def test_transform(dev, affine, bn, vb):
     bw = transform.affineBackward if affine else transform.linearBackward
     out = fw(t(M, dev), t(v, dev))
     assert_close(out.cpu().numpy(), O.transform_forward(affine, M, v), RTOL, "transform forward")
+    if not affine:      # the fused camera-facing copy equals the renderer's torch.where, bit for bit
+        same, facing = transform.linearForwardFacing(t(M, dev), t(v, dev))
+        assert torch.equal(same, out)
+        assert torch.equal(facing, torch.where(out[..., 2:3] > 0, -out, out))
     gd, Md, vd = t(g, dev), t(M, dev), t(v, dev)
     junk = torch.full((vb, N, 3), float("nan"), device=dev)
     del junk

@@ -736,8 +736,12 @@ __global__ __launch_bounds__(1024) void k_filter_block(long npix, int N, int W, 
 // ---------------------------------------------------------------------------
 // Transform (transformCuda.cu:11-117).  affine = 1 adds the translation column.
 // ---------------------------------------------------------------------------
+// facing (optional, linear transforms of normals): the transformed vector
+// negated where its z component is positive, so it faces the camera
+// (render.py:200-203)
 __global__ void k_transform_forward(int affine, int bn, int vb, int N, const float* __restrict__ M,
-                                    const float* __restrict__ v, float* __restrict__ out)
+                                    const float* __restrict__ v, float* __restrict__ out,
+                                    float* __restrict__ facing)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)bn * N) return;
@@ -745,10 +749,16 @@ __global__ void k_transform_forward(int affine, int bn, int vb, int N, const flo
     const float* m = M + 16 * b;
     long vo = 3 * ((vb > 1 ? b * N : 0) + x);
     float a0 = v[vo], a1 = v[vo + 1], a2 = v[vo + 2];
+    float o[3];
     for (int r = 0; r < 3; r++) {
         float s = m[4 * r] * a0 + m[4 * r + 1] * a1 + m[4 * r + 2] * a2;
         if (affine) s += m[4 * r + 3];
         out[3 * t + r] = s;
+        o[r] = s;
+    }
+    if (facing) {
+        const bool flip = o[2] > 0.0f;
+        for (int r = 0; r < 3; r++) facing[3 * t + r] = flip ? -o[r] : o[r];
     }
 }
 
@@ -1150,7 +1160,16 @@ int ude_transform_forward(int affine, int bn, int vb, int N, const float* M, con
 {
     if (bn <= 0 || N < 0 || (vb != 1 && vb != bn)) return UDE_ERR_ARG;
     if (N == 0) return UDE_OK;
-    k_transform_forward<<<grid1((long)bn * N, 256), 256, 0, st>>>(affine, bn, vb, N, M, v, out);
+    k_transform_forward<<<grid1((long)bn * N, 256), 256, 0, st>>>(affine, bn, vb, N, M, v, out, nullptr);
+    return launch_status();
+}
+
+int ude_transform_forward_facing(int bn, int vb, int N, const float* M, const float* v, float* out, float* facing,
+                                 hipStream_t st)
+{
+    if (bn <= 0 || N < 0 || (vb != 1 && vb != bn) || !facing) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    k_transform_forward<<<grid1((long)bn * N, 256), 256, 0, st>>>(0, bn, vb, N, M, v, out, facing);
     return launch_status();
 }
 

@@ -50,6 +50,7 @@ _SIGS = {
     "ude_filter_forward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
     "ude_filter_backward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
     "ude_transform_forward": (c_int, [c_int] * 4 + [c_ptr] * 4),
+    "ude_transform_forward_facing": (c_int, [c_int] * 3 + [c_ptr] * 5),
     "ude_transform_backward": (c_int, [c_int] * 4 + [c_ptr] * 6),
     "ude_interp_forward": (c_int, [c_int] * 7 + [c_ptr] * 5),
     "ude_interp_backward": (c_int, [c_int] * 7 + [c_ptr] * 7),

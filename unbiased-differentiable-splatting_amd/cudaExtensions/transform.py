@@ -50,3 +50,20 @@ def linearForward(transformation, vector):
 
 def linearBackward(gradIn, transformation, vector):
     return _backward(False, gradIn, transformation, vector)
+
+
+def linearForwardFacing(transformation, vector):
+    """(linearForward(transformation, vector), its camera-facing copy) in one
+    pass: the copy is negated where its z component is positive, as the
+    reference's renderer does for camera-space normals (render.py:200-203)."""
+    dev = _lib.require_gpu(transformation, vector)
+    M = _lib.f32(transformation, "transformation")
+    v = _lib.f32(vector, "vector")
+    bn = M.shape[0]
+    vb, N, _ = v.shape
+    out = torch.empty((bn, N, 3), dtype=torch.float32, device=dev)
+    facing = torch.empty_like(out)
+    rc = _lib.lib().ude_transform_forward_facing(bn, vb, N, _lib.ptr(M), _lib.ptr(v), _lib.ptr(out),
+                                                 _lib.ptr(facing), _lib.stream(dev))
+    _lib.check(rc, "transform.linearForwardFacing")
+    return out, facing

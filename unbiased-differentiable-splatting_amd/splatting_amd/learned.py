@@ -145,8 +145,7 @@ class PoissonFrame:
     def camera(self):
         tr = self._transform
         P = tr.affineForward(self.M, self.Pw)
-        Nc = tr.linearForward(self.M, self.Nw)
-        nb = torch.where(Nc[..., 2:3] > 0, -Nc, Nc).contiguous()
+        Nc, nb = tr.linearForwardFacing(self.M, self.Nw)
         return P, Nc, nb
 
     def image(self, seed, tile_scale=None, rate=None, kcap=None, seed_tensor=None):
