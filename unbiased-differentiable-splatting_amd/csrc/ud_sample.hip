@@ -874,10 +874,12 @@ __global__ __launch_bounds__(256) void k_bottom_up_global(const ViewState* __res
     while (p >= 0) {
         const NodeAux a = aux[p];
         const NodeLnk L = lnk[p];
-        __threadfence();
+        // release this thread's node record before the arrival count, acquire the
+        // sibling's after it (the second arrival reads the first's increment)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         int old = atomicAdd(&flags[p], 1);
         if (old == 0) return;
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         merge_up(p, L, a, nd, g, cnt, h, prec, geo, lnk);
         p = L.parent;
     }
