@@ -1994,19 +1994,28 @@ __global__ __launch_bounds__(256) void k_wide_hist(ViewState* vs, const int32_t*
 }
 
 // wide band: h_1 = max(h, 99.9th percentile of the histogram, rounded up to 1/16 px)
-__global__ void k_wide_band(ViewState* vs, VStride vsd)
+// (one workgroup of 256 threads, a thread per bin: the chosen bin is the first
+// whose inclusive count reaches the target, 0 when the histogram is empty)
+__global__ __launch_bounds__(256) void k_wide_band(ViewState* vs, VStride vsd)
 {
     vs = vw(vs, vsd.ws);
-    if (threadIdx.x != 0) return;
-    long total = 0;
-    for (int b = 0; b < 256; b++) total += vs->hhist[b];
-    long target = total - total / 1000, cum = 0;
-    int qb = 0;
-    for (int b = 0; b < 256; b++) {
-        cum += vs->hhist[b];
-        if (cum >= target) { qb = b; break; }
-    }
-    const float hw = fmaxf(vs->h, (float)(qb + 1) * 0.0625f);
+    __shared__ int wsum[4];
+    __shared__ int qbs;
+    const int t = threadIdx.x, wid = t >> 6;
+    const int c = vs->hhist[t];
+    const int inc = wave_incl_scan(c);
+    if ((t & 63) == 63) wsum[wid] = inc;
+    if (t == 0) qbs = 0;
+    __syncthreads();
+    int base = 0;
+    for (int q = 0; q < wid; q++) base += wsum[q];
+    const long cum = (long)base + inc;
+    const long total = (long)wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const long target = total - total / 1000;
+    if (cum >= target && cum - c < target) qbs = t;
+    __syncthreads();
+    if (t != 0) return;
+    const float hw = fmaxf(vs->h, (float)(qbs + 1) * 0.0625f);
     vs->band[1] = make_band(hw, (double)vs->prec);
 }
 
@@ -2610,7 +2619,7 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         if (rc != UDE_OK) return rc;
         k_wide_hist<<<dim3(grid1(N, 256) < 256u ? grid1(N, 256) : 256u, span), 256, 0, st>>>(w.vs, w.idsA, w.hraw,
                                                                                           vsd);
-        k_wide_band<<<dim3(1, span), 64, 0, st>>>(w.vs, vsd);
+        k_wide_band<<<dim3(1, span), 256, 0, st>>>(w.vs, vsd);
         fixed_expansions(N, 2, p, w, st, span, vsd);
         k_walk_records<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.fixOff[0],
                                                                        w.fixOff[1], w.pnode, w.frec, vsd);
