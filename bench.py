@@ -35,7 +35,7 @@ RATE, KCAP = 10.0, 40
 # HBM bytes per launch, measured offline (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # kernel, each scaled by the same counter on a 1 GiB device copy, bench.py --calibrate-copy);
 # k_sample is its two kernels, the walk with the capacity descents and the keyed draws
-TRAFFIC_BYTES = {"k_sample": 3.298e9, "k_fmm_eval": 2.349e8, "k_poisson": 5.255e8}
+TRAFFIC_BYTES = {"k_sample": 3.299e9, "k_fmm_eval": 2.349e8, "k_poisson": 5.239e8}
 
 
 def build_problem(dev, n_override=None):
