@@ -1,21 +1,23 @@
 """Benchmark: fwd+bwd frames/s of Poisson-rate splatting at equal gradient variance.
 
 Workload: 1M surfels on 64 random ellipsoids, one 512x512 view (focal 614.4),
-precision 0.01, smoothing 0, albedo interpolation without occlusion, L2 loss
-(sum) against a target rendered from centres perturbed by N(0, 0.01).  One
-step = affine + linear transform, Poisson-rate sampling with inclusion
-probabilities (LBVH, two fixed-order Gauss-transform densities, thinning at
-rate 10, about 8 samples per pixel, 1/pi weights), EWA weights, interpolation,
-loss, and the raw adjoints of all of them.  The per-tile learned rates replace
-the plain ones only when they reach the plain rates' gradient variance at a
-higher frame rate (the plain rates are their fallback); the chosen path is
-timed.  Next to it, the reference's own method (sequential Poisson sampling of
-k = 40 splats per pixel, z-sort / compositing, through the drop-in renderer) is
-timed on the same scene.
+precision 0.01, smoothing 0, L2 loss (sum) against a target rendered from
+centres perturbed by N(0, 0.01).  One step = affine + linear transform,
+Poisson-rate sampling with inclusion probabilities (LBVH, two fixed-order Gauss
+transform densities, thinning at rate 10, 1/pi weights), EWA weights, per-pixel
+z-sort and front-to-back alpha compositing of alpha = min(w, 1) / pi (an
+unbiased estimate of the image composited over every splat), albedo
+interpolation ("None" shading), loss, and the raw adjoints of all of them.  The
+per-tile learned rates replace the plain ones only when they reach the plain
+rates' gradient variance at a higher frame rate (the plain rates are their
+fallback); the chosen path is timed.  Next to it, the reference's own method
+(sequential Poisson sampling of k = 40 splats per pixel, z-sort / compositing,
+through the drop-in renderer) is timed on the same scene.
 
 Prints one JSON line (rank 0, N = 1 GPU).  Kernel and per-op device times are
 taken with HIP events on the stream the work launches on (torch's current
-stream).
+stream).  HBM traffic per launch comes from the rocprofv3 counter summaries
+committed under profiles/.
 """
 import argparse
 import json
@@ -32,10 +34,27 @@ for _p in (PKG, ROOT):
 METRIC = "fwd+bwd frames/s at 1M splats 512² (learned vs plain rates, equal grad variance)"
 HBM_PEAK_GBS = 8000.0
 RATE, KCAP = 10.0, 40
-# HBM bytes per launch, measured offline (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
-# kernel, each scaled by the same counter on a 1 GiB device copy, bench.py --calibrate-copy);
-# k_sample is its two kernels, the walk with the capacity descents and the keyed draws
-TRAFFIC_BYTES = {"k_sample": 3.299e9, "k_fmm_eval": 2.349e8, "k_poisson": 5.239e8}
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(summary):
+    """HBM bytes per launch from a counter summary under profiles/ (its
+    'traffic_bytes_per_launch:' line: rocprofv3 FETCH_SIZE + WRITE_SIZE of the
+    kernel, each scaled by the same counter on a 1 GiB device copy, bench.py
+    --calibrate-copy), or None when the summary is missing."""
+    try:
+        with open(os.path.join(PROFILES, summary)) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+TRAFFIC_SUMMARY = {"k_sample": "hbm_traffic_k_sample_1M_512x512_k40.md",
+                   "k_fmm_eval": "hbm_traffic_k_fmm_eval_1M_512x512.md",
+                   "k_poisson": "hbm_traffic_k_poisson_1M_512x512.md"}
 
 
 def build_problem(dev, n_override=None):
@@ -130,7 +149,7 @@ def cpu_baseline(sc, rows=None, k=40, threads=None):
     import numpy as np
     from oracle import oracle as O
     threads = threads or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+    O.set_threads(threads)
     p, n = sc.camera_space()
     nb = np.where(n[:, 2:3] > 0, -n, n).astype(np.float32)
     W, H, f = sc.width, sc.height, sc.focal
@@ -158,35 +177,87 @@ def cpu_baseline(sc, rows=None, k=40, threads=None):
                       % (sc.n, t_tree, rows, W, t_sample + t_ops, W / float(rows))}
 
 
-def cpu_baseline_poisson(sc, rate=RATE, kcap=KCAP):
-    """Oracle (C, OpenMP) on the whole Poisson-rate frame: tree build, both
-    fixed-order densities and the thinning pass over every pixel, then EWA
-    weights, interpolation and their FP64 adjoints."""
+def cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _composited_frame_on_host(sc, rate, kcap, threads):
+    """The headline frame by the oracle (C, OpenMP, `threads` threads): Poisson-rate
+    sampling of every pixel, then the composited estimate, L2 gradient and its FP64
+    adjoint.  Returns (seconds for sampling, seconds for the estimate fwd + bwd)."""
     import numpy as np
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+    O.set_threads(threads)
     p, n = sc.camera_space()
     nb = np.where(n[:, 2:3] > 0, -n, n).astype(np.float32)
     W, H, f = sc.width, sc.height, sc.focal
     t0 = time.time()
     ids, ipi, _, _ = O.poisson_view(W, H, f, p, nb, sc.stddevs, kcap, rate, 0.0, 0.01, seed=3)
     t_sample = time.time() - t0
-    idx4 = ids[None]
-    p4, n4, s4, a4 = p[None].astype(np.float64), n[None].astype(np.float64), sc.stddevs[None].astype(np.float64), \
-        sc.albedo[None].astype(np.float64)
     t1 = time.time()
-    w = O.splat_forward(W, H, f, idx4, p4, n4, s4)
-    wt = w * ipi[None]
-    img = O.interp_forward(idx4, wt, a4)
-    g = 2.0 * img
-    gW, _ = O.interp_backward(g, idx4, wt, a4)
-    O.splat_backward(W, H, f, gW * ipi[None], idx4, p4, n4, s4)
-    t_ops = time.time() - t1
-    frame = t_sample + t_ops
-    return {"value": 1.0 / frame, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "the whole frame: oracle Poisson-rate sampling of all %d splats at rate %g over all %dx%d "
-                      "pixels (%.2fs) + EWA/interpolation fwd+bwd in FP64 (%.2fs)" % (sc.n, rate, W, H, t_sample, t_ops)}
+    img = O.htc(W, H, f, ids[None], ipi[None].astype(np.float64), p[None], n[None], sc.stddevs[None],
+                sc.albedo[None])
+    O.htc(W, H, f, ids[None], ipi[None].astype(np.float64), p[None], n[None], sc.stddevs[None], sc.albedo[None],
+          gimg=2.0 * img)
+    return t_sample, time.time() - t1
+
+
+def cpu_baseline_poisson(sc, rate=RATE, kcap=KCAP, threads=None):
+    """Reference semantics on the host cores: the oracle's whole composited
+    Poisson-rate frame with the box's thread share and with one thread, plus the
+    FP64 direct Gaussian sums (the density the Hermite FGT approximates) at 4,096
+    random pixels over every culled splat."""
+    import math
+    import numpy as np
+    from oracle import oracle as O
+    threads = threads or min(16, os.cpu_count() or 1)
+    ts, te = _composited_frame_on_host(sc, rate, kcap, threads)
+    ts1, te1 = _composited_frame_on_host(sc, rate, kcap, 1)
+    # direct sums: the tree's bandwidth h and culled screen positions (its leaves)
+    p, n = sc.camera_space()
+    nb = np.where(n[:, 2:3] > 0, -n, n).astype(np.float32)
+    O.set_threads(threads)
+    T = O.Tree(sc.width, sc.height, sc.focal, p, nb, sc.stddevs, 0.0, 0.01)
+    h = float(T.h)
+    leaves = np.ascontiguousarray(T.cen[T.nc - 1:], np.float32)
+    rng = np.random.default_rng(4096)
+    q = np.stack([rng.integers(0, sc.width, 4096), rng.integers(0, sc.height, 4096)], 1).astype(np.float64)
+    c = 1.0 / (2.0 * math.pi * math.sqrt(h))
+    t0 = time.time()
+    O.direct_sums(leaves, h, c, q)
+    t_ds = time.time() - t0
+    O.set_threads(1)
+    t0 = time.time()
+    O.direct_sums(leaves, h, c, q[:256])
+    t_ds1 = (time.time() - t0) * 16.0
+    O.set_threads(threads)
+    return {"value": 1.0 / (ts + te), "unit": "frames/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": "the whole frame on %d threads: oracle Poisson-rate sampling of all %d splats at rate %g over "
+                      "all %dx%d pixels (%.2fs) + the composited estimate and its FP64 adjoint (%.2fs)"
+                      % (threads, sc.n, rate, sc.width, sc.height, ts, te),
+            "single_thread": {"value": 1.0 / (ts1 + te1), "unit": "frames/s", "cores": 1,
+                              "sample": "the same whole frame on 1 thread (%.2fs + %.2fs)" % (ts1, te1)},
+            "direct_sums": {"pixels": 4096, "splats": int(leaves.shape[0]),
+                            "gaussians": 4096 * int(leaves.shape[0]), "seconds": t_ds, "cores": threads,
+                            "single_thread_seconds": t_ds1,
+                            "sample": "FP64 exp sums at 4096 random pixels on %d threads; the 1-thread time is 256 "
+                                      "pixels scaled x16" % threads}}
 
 
 def timed_steps(fn, steps, kernel=None):
@@ -223,7 +294,8 @@ def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
         counter[0] += 1
         fr.step(1000 + counter[0])
 
-    rec = {"estimator": "Horvitz-Thompson (1/pi weights), linear splatting, L2 loss, raw adjoints",
+    rec = {"estimator": "composited Horvitz-Thompson (alpha = min(w, 1) / pi, z-sorted front-to-back compositing), "
+                        "L2 loss, raw adjoints",
            "plain_rate": RATE, "kcap": KCAP}
     if not use_learned or not os.path.exists(L.DEFAULT_WEIGHTS):
         rec["chosen"] = "plain (%s)" % ("learned rates not compared" if not use_learned else "no trained rate network")
@@ -290,7 +362,8 @@ def reference_path(dev, args, sc_n):
             "fps": 1000.0 / ms, "ms_per_step": ms,
             "roofline": {"bound": "hbm", "kernel": "k_sample (k_sample_walk + k_sample_draw)", "achieved": achieved,
                          "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES["k_sample"],
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(TRAFFIC_SUMMARY["k_sample"]),
                          "kernel_ms": ks_avg, "kernel_launches": ks_n, "algorithmic_bytes": nbytes["sample"]},
             "ops_ms": {k_: round(v, 4) for k_, v in sorted(ops.items())}}
 
@@ -385,8 +458,8 @@ def main():
     from splatting_amd import learned as L
     print("bench: building workload", file=sys.stderr, flush=True)
     sc = scenes.config(1, args.splats)
-    target = L.make_target(sc, dev, RATE, KCAP)
-    fr = L.PoissonFrame(sc, dev, rate=RATE, kcap=KCAP, target=target)
+    target = L.make_target(sc, dev, RATE, KCAP, composite=True)
+    fr = L.PoissonFrame(sc, dev, rate=RATE, kcap=KCAP, target=target, composite=True)
     print("bench: plain vs learned rates", file=sys.stderr, flush=True)
     step, chosen, rates = poisson_paths(dev, fr, use_learned=not args.no_rates)
     print("bench: warmup", file=sys.stderr, flush=True)
@@ -406,6 +479,8 @@ def main():
     #     4 B id per leaf, two 4 B densities per pixel and 8 B (index, 1/pi) per slot.
     n_vis = int((L.splat_tiles(fr) >= 0).sum().item())
     P = sc.width * sc.height
+    spp = float((fr.last_ids >= 0).sum().item()) / (P * fr.V)
+    fr.check_errors()
     kern = {
         "k_fmm_eval": {"kernel_ms": fe_ms / max(fe_n, 1), "kernel_launches": fe_n,
                        "algorithmic_bytes": 2 * (80 * (n_vis - 1) + 4 * P)},
@@ -414,7 +489,7 @@ def main():
     }
     for name, k in kern.items():
         k["achieved"] = k["algorithmic_bytes"] / (k["kernel_ms"] * 1e-3) / 1e9
-        k["traffic"] = TRAFFIC_BYTES[name]
+        k["traffic"] = measured_traffic(TRAFFIC_SUMMARY[name])
     dom = max(kern, key=lambda nm: kern[nm]["kernel_ms"])
     other = [nm for nm in kern if nm != dom][0]
     achieved = kern[dom]["achieved"]
@@ -423,10 +498,13 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "1M surfels on 64 ellipsoids, one 512x512 view, Poisson-rate sampling at rate %g "
-                               "(1/pi weights, about 8 samples per pixel), %s rates at the plain rates' gradient "
-                               "variance" % (RATE, chosen),
+                               "(%.2f samples per pixel, 1/pi weights), per-pixel z-sort and front-to-back alpha "
+                               "compositing of alpha = min(w, 1) / pi (unbiased for the image composited over every "
+                               "splat), albedo interpolation, L2 loss, fwd+bwd; %s rates at the plain rates' "
+                               "gradient variance" % (RATE, spp, chosen),
                    "splats": sc.n, "splats_in_view": n_vis, "width": sc.width, "height": sc.height,
-                   "rate": RATE, "kcap": KCAP, "rates": chosen, "parallelism": "none (1 GPU)"},
+                   "rate": RATE, "kcap": KCAP, "samples_per_pixel": spp, "compositing": True, "rates": chosen,
+                   "parallelism": "none (1 GPU)"},
         "roofline": dict({"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS},
                          **{k_: v for k_, v in kern[dom].items() if k_ != "achieved"},

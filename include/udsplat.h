@@ -255,6 +255,119 @@ int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothi
                     int sdBatch, int ab, const float* attr, float* gPts, float* gNrm, float* gSd, float* gAttr,
                     int prepared, void* ws, size_t wsBytes, ude_stream_t stream);
 
+/* ---- float64 and half variants ------------------------------------------ */
+
+/* The reference dispatches these ops on float and double (AT_DISPATCH_FLOATING_TYPES:
+ * splattingCuda.cu:84,214; transformCuda.cu:53,135; interpolationCuda.cu:68,153;
+ * shadingCuda.cu:63,153,225,315), filtering also on half (zSortingCuda.cu:100,220).
+ * Each _f64 entry point is its float counterpart above with every real tensor in
+ * double (FP64 arithmetic and FP64 atomics); the _f16 filtering entry points take
+ * half tensors and compute in float.  Arguments and outputs as the float ones. */
+size_t ude_splat_workspace_bytes_f64(int bn, int N);
+int ude_splat_forward_f64(int bn, int N, int W, int H, int k, double f, double smoothing,
+                          const int32_t* idx, const double* pts, const double* nrm, const double* sd, int sdBatch,
+                          double* out, void* ws, size_t wsBytes, ude_stream_t stream);
+int ude_splat_backward_f64(int bn, int N, int W, int H, int k, double f, double smoothing,
+                           const double* gradIn, const int32_t* idx, const double* pts, const double* nrm,
+                           const double* sd, int sdBatch, double* gPts, double* gNrm, double* gSd,
+                           void* ws, size_t wsBytes, ude_stream_t stream);
+int ude_filter_forward_f64(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const double* wts,
+                           const double* pts, const double* nrm, int pb, double* out, ude_stream_t stream);
+int ude_filter_backward_f64(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const double* wts,
+                            const double* gin, const double* pts, const double* nrm, int pb, double* out,
+                            ude_stream_t stream);
+/* half tensors (IEEE binary16) passed as void* */
+int ude_filter_forward_f16(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const void* wts,
+                           const void* pts, const void* nrm, int pb, void* out, ude_stream_t stream);
+int ude_filter_backward_f16(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const void* wts,
+                            const void* gin, const void* pts, const void* nrm, int pb, void* out,
+                            ude_stream_t stream);
+int ude_transform_forward_f64(int affine, int bn, int vb, int N, const double* M, const double* v, double* out,
+                              ude_stream_t stream);
+int ude_transform_backward_f64(int affine, int bn, int vb, int N, const double* g, const double* M,
+                               const double* v, double* gM, double* gV, ude_stream_t stream);
+int ude_interp_forward_f64(int bn, int W, int H, int k, int ab, int N, int C, const int32_t* idx,
+                           const double* wts, const double* attr, double* out, ude_stream_t stream);
+int ude_interp_backward_f64(int bn, int W, int H, int k, int ab, int N, int C, const double* g,
+                            const int32_t* idx, const double* wts, const double* attr, double* gW, double* gA,
+                            ude_stream_t stream);
+int ude_sh_forward_f64(int bn, int W, int H, int cb, const double* coef, const double* nrm, const double* diff,
+                       double* out, ude_stream_t stream);
+int ude_sh_backward_f64(int bn, int W, int H, int cb, const double* g, const double* coef, const double* nrm,
+                        const double* diff, double* gCoef, double* gN, double* gD, ude_stream_t stream);
+int ude_lambert_forward_f64(int bn, int W, int H, int L, int ib, const double* dirs, const double* ints,
+                            const double* nrm, const double* diff, double* out, ude_stream_t stream);
+int ude_lambert_backward_f64(int bn, int W, int H, int L, int ib, const double* g, const double* dirs,
+                             const double* ints, const double* nrm, const double* diff, double* gDir,
+                             double* gInt, double* gN, double* gD, ude_stream_t stream);
+
+/* ---- pixelgather ------------------------------------------------------- */
+
+/* pixelgather.scatter -- replaces scatterCuda (splatting/extensions/pixelGather.cpp:10-13,
+ * pixelGatherCuda.cu:7-74): out (bn, W, H, k, C)[b, i, j, s, c] = values[b, idx, c]
+ * for idx = indices[b, i, j, s] >= 0, else 0 (every element written).  values
+ * (bn, N, C).  float, double and (as void*) half, as the reference dispatches. */
+int ude_pixel_scatter(int bn, int W, int H, int k, int N, int C, const int32_t* idx, const float* values,
+                      float* out, ude_stream_t stream);
+int ude_pixel_scatter_f64(int bn, int W, int H, int k, int N, int C, const int32_t* idx, const double* values,
+                          double* out, ude_stream_t stream);
+int ude_pixel_scatter_f16(int bn, int W, int H, int k, int N, int C, const int32_t* idx, const void* values,
+                          void* out, ude_stream_t stream);
+
+/* pixelgather.gather -- replaces gatherCuda (pixelGather.cpp:15-18,
+ * pixelGatherCuda.cu:76-152): out (bn, N, C)[b, idx, c] += values[b, i, j, s, c]
+ * over the slots with idx = indices[b, i, j, s] >= 0 (caller zeroed; device
+ * atomics, so the summation order is not fixed).  float and double. */
+int ude_pixel_gather(int bn, int W, int H, int k, int N, int C, const int32_t* idx, const float* values, float* out,
+                     ude_stream_t stream);
+int ude_pixel_gather_f64(int bn, int W, int H, int k, int N, int C, const int32_t* idx, const double* values,
+                         double* out, ude_stream_t stream);
+
+/* ---- nearestsamples ------------------------------------------------------ */
+
+/* Workspace of ude_nearest_samples: N * K 64-bit words. */
+size_t ude_nearest_workspace_bytes(int N, int K);
+
+/* nearestsamples.nearest -- replaces cudaFastNearestSamples (nearestSamples.cpp:30-75,
+ * nearestSamplesCuda.cu:290-564).  idx / wts (bn, W, H, k); pts (1, N, 3) (the
+ * reference also assumes one point set for all views).  Each pixel's surface
+ * splat is its first slot of largest weight; out (N, K) int32 holds, for every
+ * splat, the K nearest distinct splats sampled at the pixels it is the surface
+ * of, by float32 squared distance (ties by index), the splat itself excluded, -1
+ * padded.  The rows do not depend on the order the pixels run in.  float and
+ * double (the double variant orders by the squared distance rounded to float). */
+int ude_nearest_samples(int bn, int W, int H, int k, int N, int K, const int32_t* idx, const float* wts,
+                        const float* pts, int32_t* out, void* ws, size_t wsBytes, ude_stream_t stream);
+int ude_nearest_samples_f64(int bn, int W, int H, int k, int N, int K, const int32_t* idx, const double* wts,
+                            const double* pts, int32_t* out, void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* ---- composited Horvitz-Thompson image estimate ------------------------- */
+
+/* No reference counterpart: the Poisson-rate counterpart of splatting.forward ->
+ * filtering.forward -> interpolation.forward (splattingCuda.cu:18-101,
+ * zSortingCuda.cu:11-123, interpolationCuda.cu:5-46) and of their backward ops
+ * (splattingCuda.cu:114-234, zSortingCuda.cu:126-243, interpolationCuda.cu:83-167).
+ * Per pixel, with alpha_s = min(w_s, 1) ipi_s and the samples ordered by the
+ * ray depth filtering uses (alphacompositing.cuh:6-54, stable, invalid depth
+ * 10000):  image = sum_s alpha_s attr_s prod_{t in front of s} (1 - alpha_t).
+ * With independent inclusions (samplePoisson) it is an unbiased estimate of the
+ * image composited over all splats of the sampling support.  Inputs as for
+ * ude_ht_forward; k <= 112. */
+size_t ude_htc_workspace_bytes(int bn, int N, int W, int H, int k);
+
+/* out: (bn, W, H, 3), the composited estimate. */
+int ude_htc_forward(int bn, int N, int W, int H, int k, double f, double smoothing, const int32_t* idx,
+                    const float* ipi, const float* pts, const float* nrm, const float* sd, int sdBatch, int ab,
+                    const float* attr, float* out, void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* Exact adjoint of ude_htc_forward (the depth order held fixed, as the
+ * reference's filtering backward holds it): outputs as ude_ht_backward.
+ * prepared != 0: ws is the workspace ude_htc_forward used for the same splats. */
+int ude_htc_backward(int bn, int N, int W, int H, int k, double f, double smoothing, const float* gradImage,
+                     const int32_t* idx, const float* ipi, const float* pts, const float* nrm, const float* sd,
+                     int sdBatch, int ab, const float* attr, float* gPts, float* gNrm, float* gSd, float* gAttr,
+                     int prepared, void* ws, size_t wsBytes, ude_stream_t stream);
+
 /* ---- L2 image loss -------------------------------------------------------- */
 
 /* No reference counterpart (its scripts use torch ops): grad = 2 (image - target)

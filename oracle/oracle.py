@@ -48,10 +48,17 @@ def lib():
         _lib.o_set_fixed_degree.argtypes = [ctypes.c_int]
         _lib.o_poisson_wide_h.restype = ctypes.c_float
         _lib.o_poisson_wide_h.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3
+        _lib.o_set_abs_terms.argtypes = [ctypes.c_int]
         _lib.o_sample_view_rows.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 +
                                            [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_int] * 2 +
                                            [ctypes.c_void_p])
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (the runtime reads
+    OMP_NUM_THREADS only once, so later changes go through this call)."""
+    lib().omp_set_num_threads(int(n))
 
 
 def _p(a):
@@ -93,6 +100,20 @@ def uniform(seed, view, pixel, launch, ordinal):
 
 def num_launches(k):
     return lib().o_num_launches_export(int(k))
+
+
+class abs_terms:
+    """Inside this block the per-slot operators return, per output element, the
+    sum of the absolute values of the terms they accumulate instead of the sum
+    (the magnitude an FP32 accumulation of the same terms is measured against)."""
+
+    def __enter__(self):
+        lib().o_set_abs_terms(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().o_set_abs_terms(0)
+        return False
 
 
 # ---------------------------------------------------------------- per-slot ops (FP64)
@@ -150,6 +171,95 @@ def filter_backward(idx, w, gin, W, H, f, pts, nrm):
                                  ctypes.c_int(k), ctypes.c_double(f), _p(idx), _p(w), _p(gin), _p(pts), _p(nrm),
                                  ctypes.c_int(pts.shape[0]), _p(out))
     assert rc == 0
+    return out
+
+
+def htc(W, H, f, idx, ipi, pts, nrm, sd, attr, gimg=None, smoothing=0.0):
+    """Composited Horvitz-Thompson estimate (FP64) and, with gimg (bn, W, H, 3), its
+    exact adjoint.  pts / nrm float32 (bn, N, 3) camera space, sd (1|bn, N, 2),
+    attr (1|bn, N, 3).  Returns image, or (image, (gP, gN, gS, gA))."""
+    idx = _c(idx, np.int32); ipi = _c(ipi, np.float64)
+    pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float64); attr = _c(attr, np.float64)
+    bn, _, _, k = idx.shape
+    N = pts.shape[1]
+    out = np.zeros((bn, W, H, 3), np.float64)
+    gP = np.zeros(pts.shape, np.float64); gN = np.zeros(nrm.shape, np.float64)
+    gS = np.zeros(sd.shape, np.float64); gA = np.zeros(attr.shape, np.float64)
+    g = None if gimg is None else _c(gimg, np.float64)
+    rc = lib().o_htc(ctypes.c_int(bn), ctypes.c_int(N), ctypes.c_int(W), ctypes.c_int(H), ctypes.c_int(k),
+                     ctypes.c_double(f), ctypes.c_double(smoothing), _p(idx), _p(ipi), _p(pts), _p(nrm), _p(sd),
+                     ctypes.c_int(sd.shape[0]), ctypes.c_int(attr.shape[0]), _p(attr),
+                     None if g is None else _p(g), _p(out), _p(gP), _p(gN), _p(gS), _p(gA))
+    assert rc == 0
+    return out if gimg is None else (out, (gP, gN, gS, gA))
+
+
+def pixel_scatter(idx, values):
+    """pixelgather.scatter (pixelGatherCuda.cu:7-35): out[b, i, j, s, :] =
+    values[b, idx[b, i, j, s], :] where idx >= 0, else 0."""
+    idx = np.asarray(idx)
+    values = np.asarray(values)
+    bn = idx.shape[0]
+    out = np.zeros(idx.shape + (values.shape[2],), values.dtype)
+    for b in range(bn):
+        m = idx[b] >= 0
+        out[b][m] = values[b][idx[b][m]]
+    return out
+
+
+def pixel_gather(idx, values, N):
+    """pixelgather.gather (pixelGatherCuda.cu:76-109): out[b, n, :] = sum of
+    values[b, i, j, s, :] over the slots with idx[b, i, j, s] == n, FP64."""
+    idx = np.asarray(idx)
+    values = np.asarray(values, np.float64)
+    bn = idx.shape[0]
+    out = np.zeros((bn, N, values.shape[4]), np.float64)
+    for b in range(bn):
+        m = idx[b] >= 0
+        np.add.at(out[b], idx[b][m], values[b][m])
+    return out
+
+
+def nearest_samples(idx, w, pts, K):
+    """nearestsamples.nearest (nearestSamplesCuda.cu:290-404): per pixel the surface
+    splat is the first slot of largest weight; every other splat sampled there is a
+    neighbour candidate of it at float32 squared distance ((dx dx + dy dy) + dz dz);
+    per splat the K nearest distinct candidates by (distance, index), -1 padded.
+    Returns (1, N, K) int32."""
+    idx = np.asarray(idx)
+    w = np.asarray(w)
+    P = np.asarray(pts)[0]
+    N = P.shape[0]
+    flat_i = idx.reshape(-1, idx.shape[-1])
+    flat_w = w.reshape(-1, w.shape[-1])
+    surf = flat_i[np.arange(flat_i.shape[0]), np.argmax(flat_w, axis=1)]
+    cand = {}
+    for s, row in zip(surf, flat_i):
+        if s < 0:
+            continue
+        for c in row:
+            if c >= 0 and c != s:
+                cand.setdefault(int(s), set()).add(int(c))
+    out = np.full((1, N, K), -1, np.int32)
+    for s, cs in cand.items():
+        cs = np.array(sorted(cs))
+        d = (P[s][None, :] - P[cs]).astype(np.float32)
+        d2 = d[:, 0] * d[:, 0]
+        d2 = (d2 + d[:, 1] * d[:, 1]).astype(np.float32)
+        d2 = (d2 + d[:, 2] * d[:, 2]).astype(np.float32)
+        order = np.lexsort((cs, d2))[:K]
+        out[0, s, :len(order)] = cs[order]
+    return out
+
+
+def direct_sums(screen, h, c, queries):
+    """FP64 direct Gaussian sums c sum_i exp(-|x - s_i|^2 / 2h^2) at each query x
+    over the screen points s (n, 2) float32 (OpenMP)."""
+    s = _c(screen, np.float32)
+    x = _c(queries, np.float64)
+    out = np.zeros(x.shape[0], np.float64)
+    lib().o_direct_sums(ctypes.c_long(s.shape[0]), _p(s), ctypes.c_double(h), ctypes.c_double(c),
+                        ctypes.c_long(x.shape[0]), _p(x), _p(out))
     return out
 
 

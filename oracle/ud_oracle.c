@@ -64,6 +64,15 @@
 
 #define O_PI 3.14159265358979323846
 
+/* Absolute-terms mode of the per-slot operators: with it on, every sum below
+ * that accumulates (over slots, pixels, lamps or matrix entries) adds |term|
+ * instead of term, so the output is the per-element magnitude sum_t |term_t|.
+ * Tests scale it by the FP32 unit roundoff to bound what a single-precision
+ * accumulation of the same terms can lose. */
+static int g_abs_terms = 0;
+void o_set_abs_terms(int on) { g_abs_terms = on; }
+#define AT(x) (g_abs_terms ? fabs(x) : (x))
+
 /* ------------------------------------------------------------------------ */
 /* Portable FP32 exp / log used by the sampler mirror (same algorithm as the  */
 /* device code, written independently).                                       */
@@ -186,10 +195,17 @@ static dual dabs(dual a) { return a.v < 0 ? dneg(a) : a; }
 static dual ddot3(const dual a[3], const dual b[3]) { return dadd(dadd(dmul(a[0], b[0]), dmul(a[1], b[1])), dmul(a[2], b[2])); }
 
 /* EWA weight (unclamped) of one (pixel, splat) pair, ewaSplatting.cuh:69-91.
- * Also returns the screen-space covariance V (row-major 2x2) if cov != NULL. */
-static dual ewa_weight_dual(double px, double py, double W, double H, double f,
-                            const double p_in[3], const double n_in[3], const double sd_in[2],
-                            double smoothing, dual cov[4])
+ * Also returns the screen-space covariance V (row-major 2x2) if cov != NULL, and,
+ * if kappa != NULL, the float32 evaluation condition of the weight: with the
+ * offset d = q - s whitened as (u, v) = T d (T^T T = V^-1, upper triangular),
+ * rounding T and d to float32 moves u by about eps (|t00 dx| + |t01 dy| + |u|)
+ * and v by eps (|t11 dy| + |v|), and the weight exp(-(u^2 + v^2) / 2) by
+ * |u| du + |v| dv relative: kappa = 1 + (1 + |u|)(|t00 dx| + |t01 dy|) +
+ * (1 + |v|) |t11 dy| (times eps, the relative error of a float32 weight and of
+ * its derivatives). */
+static dual ewa_weight_dual_k(double px, double py, double W, double H, double f,
+                              const double p_in[3], const double n_in[3], const double sd_in[2],
+                              double smoothing, dual cov[4], double* kappa)
 {
     dual p[3], n[3], sd[2];
     for (int i = 0; i < 3; i++) { p[i] = dvar(p_in[i], i); n[i] = dvar(n_in[i], 3 + i); }
@@ -229,10 +245,25 @@ static dual ewa_weight_dual(double px, double py, double W, double H, double f,
     /* evaluteGaussian, ewaSplatting.cuh:56-67 */
     dual dx = dsub(dc(px), sx), dy = dsub(dc(py), sy);
     dual detV = dsub(dmul(V00, V11), dmul(V01, V01));
+    if (kappa) {
+        const double t00 = sqrt(V11.v / detV.v), t01 = -V01.v / sqrt(V11.v * detV.v), t11 = 1.0 / sqrt(V11.v);
+        const double u = t00 * dx.v + t01 * dy.v, v = t11 * dy.v;
+        *kappa = 1.0 + (1.0 + fabs(u)) * (fabs(t00 * dx.v) + fabs(t01 * dy.v)) + (1.0 + fabs(v)) * fabs(t11 * dy.v);
+    }
     dual e = ddiv(dadd(dsub(dmul(V11, dmul(dx, dx)), dscal(dmul(V01, dmul(dx, dy)), 2.0)), dmul(V00, dmul(dy, dy))), detV);
     dual norm = dmul(dscal(dsqrt(detV), 2.0 * O_PI), dabs(detA));
     return ddiv(dexp(dscal(e, -0.5)), norm);
 }
+
+static dual ewa_weight_dual(double px, double py, double W, double H, double f,
+                            const double p_in[3], const double n_in[3], const double sd_in[2],
+                            double smoothing, dual cov[4])
+{
+    return ewa_weight_dual_k(px, py, W, H, f, p_in, n_in, sd_in, smoothing, cov, NULL);
+}
+
+/* the abs-terms scale of a term carrying a float32 EWA weight: its condition */
+#define KW(kap) (g_abs_terms ? (kap) : 1.0)
 
 /* splatting.forward (splattingCuda.cu:18-101), FP64. idx (bn,W,H,k), pts/nrm (bn,N,3),
  * sd (sdBatch,N,2). out (bn,W,H,k) = min(w,1), 0 for empty slots. */
@@ -252,8 +283,9 @@ void o_splat_forward(int bn, int N, int W, int H, int k, double f, double smooth
             const double* p = pts + 3 * ((long)b * N + id);
             const double* n = nrm + 3 * ((long)b * N + id);
             const double* sdv = sd + 2 * ((sdBatch > 1 ? (long)b * N : 0) + id);
-            dual w = ewa_weight_dual(i, j, W, H, f, p, n, sdv, smoothing, NULL);
-            out[o] = w.v < 1.0 ? w.v : 1.0;
+            double kap = 1.0;
+            dual w = ewa_weight_dual_k(i, j, W, H, f, p, n, sdv, smoothing, NULL, &kap);
+            out[o] = (w.v < 1.0 ? w.v : 1.0) * KW(kap);
         }
     }
 }
@@ -275,12 +307,13 @@ void o_splat_backward(int bn, int N, int W, int H, int k, double f, double smoot
             if (id < 0) continue;
             long po = 3 * ((long)b * N + id);
             long so = 2 * ((sdBatch > 1 ? (long)b * N : 0) + id);
-            dual w = ewa_weight_dual(i, j, W, H, f, pts + po, nrm + po, sd + so, smoothing, NULL);
+            double kap = 1.0;
+            dual w = ewa_weight_dual_k(i, j, W, H, f, pts + po, nrm + po, sd + so, smoothing, NULL, &kap);
             if (!(w.v <= 1.0)) continue;
-            double g = gradIn[o];
-            for (int c = 0; c < 3; c++) { gPts[po + c] += g * w.d[c]; gNrm[po + c] += g * w.d[3 + c]; }
-            gSd[so] += g * w.d[6];
-            gSd[so + 1] += g * w.d[7];
+            double g = gradIn[o] * KW(kap);
+            for (int c = 0; c < 3; c++) { gPts[po + c] += AT(g * w.d[c]); gNrm[po + c] += AT(g * w.d[3 + c]); }
+            gSd[so] += AT(g * w.d[6]);
+            gSd[so + 1] += AT(g * w.d[7]);
         }
     }
 }
@@ -397,12 +430,125 @@ int o_filter_backward(int bn, int N, int W, int H, int k, double f, const int32_
             for (int c = a; c < k; c++) {
                 int id = order[c];
                 double den = (c == a) ? w[base + kd] : -(1.0 - w[base + kd]);
-                if (fabs(den) > 1e-4) g += gin[base + id] * blended[id] / den;
+                if (fabs(den) > 1e-4) g += AT(gin[base + id] * blended[id] / den);
             }
             gout[base + kd] = g;
         }
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Composited Horvitz-Thompson estimate (no reference counterpart: the        */
+/* Poisson-rate form of splatting -> filtering -> interpolation, splattingCuda.cu:18-66, */
+/* alphacompositing.cuh:6-68, interpolationCuda.cu:5-46) and its exact adjoint, */
+/* FP64.  Per pixel, samples up to the first -1; alpha_s = min(w_s, 1) ipi_s   */
+/* (w: the EWA weight of ewaSplatting.cuh:69-91); ranks by the FP32 ray depth  */
+/* of filtering (stable); image = sum_s alpha_s a_s prod_{t in front} (1 - alpha_t). */
+/* Adjoint in depth order: T_r the product in front of rank r,                 */
+/* S_r = sum_{i>r} alpha_i (g . a_i) prod_{r<t<i} (1 - alpha_t),               */
+/* d/dalpha_r = T_r (g . a_r - S_r), d/da_r = alpha_r T_r g, d/dw = ipi d/dalpha */
+/* where the unclamped w <= 1 (splattingCuda.cu:173), chained through the EWA  */
+/* weight's exact derivatives.  pts / nrm are the float32 camera-space inputs. */
+/* gimg == NULL: forward only.  Pixels in parallel (OpenMP); gradients added   */
+/* with FP64 atomics, so their order (not their value beyond FP64 rounding)    */
+/* depends on the schedule.                                                     */
+/* ------------------------------------------------------------------------ */
+#define O_ADD(dst, v) do { double v_ = (v); _Pragma("omp atomic") dst += v_; } while (0)
+int o_htc(int bn, int N, int W, int H, int k, double f, double smoothing, const int32_t* idx, const double* ipi,
+          const float* pts, const float* nrm, const double* sd, int sdBatch, int ab, const double* attr,
+          const double* gimg, double* out, double* gP, double* gN, double* gS, double* gA)
+{
+    if (k > 1024) return -1;
+    const float ff = (float)f;
+    #pragma omp parallel
+    {
+        int* ord = (int*)malloc(sizeof(int) * 1024);
+        int* ids = (int*)malloc(sizeof(int) * 1024);
+        double* al = (double*)malloc(sizeof(double) * 1024);
+        double* Tr = (double*)malloc(sizeof(double) * 1024);
+        double* kap = (double*)malloc(sizeof(double) * 1024);
+        float* dep = (float*)malloc(sizeof(float) * 1024);
+        dual* wd = (dual*)malloc(sizeof(dual) * 1024);
+        #pragma omp for schedule(dynamic, 64)
+        for (long pix = 0; pix < (long)bn * W * H; pix++) {
+            const long b = pix / ((long)W * H);
+            const long r = pix % ((long)W * H);
+            const int i = (int)(r / H), j = (int)(r % H);
+            const long base = (long)k * pix;
+            int n = 0;
+            for (int s = 0; s < k; s++) {
+                const int id = idx[base + s];
+                if (id < 0) break;
+                const long po = 3 * (b * N + id);
+                double p[3] = { pts[po], pts[po + 1], pts[po + 2] };
+                double q[3] = { nrm[po], nrm[po + 1], nrm[po + 2] };
+                const double* sdv = sd + 2 * ((sdBatch > 1 ? b * N : 0) + id);
+                wd[n] = ewa_weight_dual_k(i, j, W, H, f, p, q, sdv, smoothing, NULL, &kap[n]);
+                const double w = wd[n].v < 1.0 ? wd[n].v : 1.0;
+                al[n] = w * ipi[base + s];
+                dep[n] = o_depth_f32(i, j, (float)W, (float)H, ff, pts + po, nrm + po);
+                ids[n] = id;
+                /* stable insertion by depth (alphacompositing.cuh:38-54) */
+                int q2 = n;
+                while (q2 > 0 && dep[ord[q2 - 1]] > dep[n]) { ord[q2] = ord[q2 - 1]; q2--; }
+                ord[q2] = n;
+                n++;
+            }
+            double T = 1.0;
+            for (int c = 0; c < 3; c++) out[3 * pix + c] = 0.0;
+            for (int q = 0; q < n; q++) {
+                const int s = ord[q];
+                const double* a = attr + 3 * ((ab > 1 ? b * N : 0) + ids[s]);
+                Tr[q] = T;
+                for (int c = 0; c < 3; c++) out[3 * pix + c] += AT(al[s] * T * a[c]) * KW(kap[s]);
+                T *= AT(1.0 - al[s]);
+            }
+            if (!gimg) continue;
+            const double* G = gimg + 3 * pix;
+            double S = 0.0;
+            for (int q = n - 1; q >= 0; q--) {
+                const int s = ord[q];
+                const long ao = 3 * ((ab > 1 ? b * N : 0) + ids[s]);
+                const double* a = attr + ao;
+                const double ga = AT(G[0] * a[0]) + AT(G[1] * a[1]) + AT(G[2] * a[2]);
+                const double dal = g_abs_terms ? Tr[q] * (ga + S) * kap[s] : Tr[q] * (ga - S);
+                for (int c = 0; c < 3; c++) O_ADD(gA[ao + c], AT(al[s] * Tr[q] * G[c]) * KW(kap[s]));
+                S = AT(al[s] * ga) * KW(kap[s]) + AT(1.0 - al[s]) * S;
+                if (!(wd[s].v <= 1.0)) continue;
+                const double dw = dal * ipi[base + s];
+                const long po = 3 * (b * N + ids[s]);
+                const long so = 2 * ((sdBatch > 1 ? b * N : 0) + ids[s]);
+                for (int c = 0; c < 3; c++) {
+                    O_ADD(gP[po + c], AT(dw * wd[s].d[c]));
+                    O_ADD(gN[po + c], AT(dw * wd[s].d[3 + c]));
+                }
+                O_ADD(gS[so], AT(dw * wd[s].d[6]));
+                O_ADD(gS[so + 1], AT(dw * wd[s].d[7]));
+            }
+        }
+        free(ord); free(ids); free(al); free(Tr); free(kap); free(dep); free(wd);
+    }
+    return 0;
+}
+
+/* FP64 direct Gaussian sums out[q] = c sum_i exp(-|x_q - s_i|^2 / (2 h^2)) over n
+ * screen points s (float pairs) at m query points x (double pairs): the exact
+ * density the Hermite FGT approximates (hermitefmm.cuh:184-237), the CPU
+ * reference semantics of the densities. */
+void o_direct_sums(long n, const float* s, double h, double c, long m, const double* x, double* out)
+{
+    const double inv = 1.0 / (2.0 * h * h);
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (long q = 0; q < m; q++) {
+        const double qx = x[2 * q], qy = x[2 * q + 1];
+        double acc = 0.0;
+        for (long i = 0; i < n; i++) {
+            const double dx = qx - s[2 * i], dy = qy - s[2 * i + 1];
+            acc += exp(-(dx * dx + dy * dy) * inv);
+        }
+        out[q] = c * acc;
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -415,8 +561,8 @@ void o_transform_forward(int mode, int bn, int vb, int N, const double* M, const
             const double* m = M + 16 * b;
             const double* a = v + 3 * ((vb > 1 ? b * N : 0) + x);
             for (int r = 0; r < 3; r++) {
-                double s = m[4 * r] * a[0] + m[4 * r + 1] * a[1] + m[4 * r + 2] * a[2];
-                if (mode == 0) s += m[4 * r + 3];
+                double s = AT(m[4 * r] * a[0]) + AT(m[4 * r + 1] * a[1]) + AT(m[4 * r + 2] * a[2]);
+                if (mode == 0) s += AT(m[4 * r + 3]);
                 out[3 * (b * N + x) + r] = s;
             }
         }
@@ -432,11 +578,11 @@ void o_transform_backward(int mode, int bn, int vb, int N, const double* g, cons
             const double* a = v + vo;
             const double* gi = g + 3 * (b * N + x);
             for (int r = 0; r < 3; r++) {
-                for (int c = 0; c < 3; c++) gM[16 * b + 4 * r + c] += gi[r] * a[c];
-                if (mode == 0) gM[16 * b + 4 * r + 3] += gi[r];
+                for (int c = 0; c < 3; c++) gM[16 * b + 4 * r + c] += AT(gi[r] * a[c]);
+                if (mode == 0) gM[16 * b + 4 * r + 3] += AT(gi[r]);
             }
             for (int c = 0; c < 3; c++)
-                gV[vo + c] += m[c] * gi[0] + m[4 + c] * gi[1] + m[8 + c] * gi[2];
+                gV[vo + c] += AT(m[c] * gi[0]) + AT(m[4 + c] * gi[1]) + AT(m[8 + c] * gi[2]);
         }
 }
 
@@ -453,7 +599,7 @@ void o_interp_forward(int bn, int W, int H, int k, int ab, int N, int C, const i
             int id = idx[k * pix + s];
             if (id < 0) continue;
             const double* a = attr + (long)C * ((ab > 1 ? b * N : 0) + id);
-            for (int c = 0; c < C; c++) out[C * pix + c] += w[k * pix + s] * a[c];
+            for (int c = 0; c < C; c++) out[C * pix + c] += AT(w[k * pix + s] * a[c]);
         }
     }
 }
@@ -471,8 +617,8 @@ void o_interp_backward(int bn, int W, int H, int k, int ab, int N, int C, const 
             if (id < 0) continue;
             long ao = (long)C * ((ab > 1 ? b * N : 0) + id);
             for (int c = 0; c < C; c++) {
-                gA[ao + c] += g[C * pix + c] * w[o];
-                gW[o] += g[C * pix + c] * attr[ao + c];
+                gA[ao + c] += AT(g[C * pix + c] * w[o]);
+                gW[o] += AT(g[C * pix + c] * attr[ao + c]);
             }
         }
     }
@@ -484,10 +630,10 @@ void o_interp_backward(int bn, int W, int H, int k, int ab, int N, int C, const 
 static void o_sh_matrix(const double* c, double M[16])
 {
     const double c1 = 0.429043, c2 = 0.511664, c3 = 0.743125, c4 = 0.886227, c5 = 0.247708;
-    M[0] = c1 * c[8];  M[1] = c1 * c[4];  M[2] = c1 * c[7];  M[3] = c2 * c[3];
-    M[4] = c1 * c[4];  M[5] = -c1 * c[8]; M[6] = c1 * c[5];  M[7] = c2 * c[1];
-    M[8] = c1 * c[7];  M[9] = c1 * c[5];  M[10] = c3 * c[6]; M[11] = c2 * c[2];
-    M[12] = c2 * c[3]; M[13] = c2 * c[1]; M[14] = c2 * c[2]; M[15] = c4 * c[0] - c5 * c[6];
+    M[0] = AT(c1 * c[8]);  M[1] = AT(c1 * c[4]);  M[2] = AT(c1 * c[7]);  M[3] = AT(c2 * c[3]);
+    M[4] = AT(c1 * c[4]);  M[5] = AT(-c1 * c[8]); M[6] = AT(c1 * c[5]);  M[7] = AT(c2 * c[1]);
+    M[8] = AT(c1 * c[7]);  M[9] = AT(c1 * c[5]);  M[10] = AT(c3 * c[6]); M[11] = AT(c2 * c[2]);
+    M[12] = AT(c2 * c[3]); M[13] = AT(c2 * c[1]); M[14] = AT(c2 * c[2]); M[15] = AT(c4 * c[0]) + AT(-c5 * c[6]);
 }
 
 static void o_sh_n4(const double* n, double n4[4], double* norm_out)
@@ -510,8 +656,8 @@ void o_sh_forward(int bn, int W, int H, int cb, const double* coef, const double
             o_sh_matrix(coef + 9 * (cb > 1 ? ch + 3 * b : ch), M);
             double res = 0.0;
             for (int r = 0; r < 4; r++)
-                for (int c = 0; c < 4; c++) res += n4[r] * M[4 * r + c] * n4[c];
-            out[3 * pix + ch] = diff[3 * pix + ch] * res / O_PI;
+                for (int c = 0; c < 4; c++) res += AT(n4[r] * M[4 * r + c] * n4[c]);
+            out[3 * pix + ch] = AT(diff[3 * pix + ch]) * res / O_PI;
         }
     }
 }
@@ -531,7 +677,7 @@ void o_sh_backward(int bn, int W, int H, int cb, const double* g, const double* 
         if (nn > 1e-5) {
             double inv = 1.0 / nn, inv3 = inv * inv * inv;
             for (int r = 0; r < 3; r++)
-                for (int c = 0; c < 3; c++) Gn[3 * r + c] = (r == c ? inv : 0.0) - n[r] * n[c] * inv3;
+                for (int c = 0; c < 3; c++) Gn[3 * r + c] = (r == c ? inv : 0.0) + AT(-n[r] * n[c] * inv3);
         }
         for (int ch = 0; ch < 3; ch++) {
             long co = 9 * (cb > 1 ? ch + 3 * b : ch);
@@ -539,13 +685,14 @@ void o_sh_backward(int bn, int W, int H, int cb, const double* g, const double* 
             o_sh_matrix(coef + co, M);
             double res = 0.0, gN4[4];
             for (int r = 0; r < 4; r++)
-                for (int c = 0; c < 4; c++) res += n4[r] * M[4 * r + c] * n4[c];
+                for (int c = 0; c < 4; c++) res += AT(n4[r] * M[4 * r + c] * n4[c]);
             for (int r = 0; r < 4; r++) {
                 gN4[r] = 0.0;
-                for (int c = 0; c < 4; c++) gN4[r] += (M[4 * r + c] + M[4 * c + r]) * n4[c];
+                for (int c = 0; c < 4; c++) gN4[r] += AT(M[4 * r + c] * n4[c]) + AT(M[4 * c + r] * n4[c]);
             }
             double gnv[3];
-            for (int r = 0; r < 3; r++) gnv[r] = Gn[3 * r] * gN4[0] + Gn[3 * r + 1] * gN4[1] + Gn[3 * r + 2] * gN4[2];
+            for (int r = 0; r < 3; r++)
+                gnv[r] = AT(Gn[3 * r] * gN4[0]) + AT(Gn[3 * r + 1] * gN4[1]) + AT(Gn[3 * r + 2] * gN4[2]);
             double gM[16];
             for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) gM[4 * r + c] = n4[r] * n4[c];
             double gc[9];
@@ -555,13 +702,13 @@ void o_sh_backward(int bn, int W, int H, int cb, const double* g, const double* 
             gc[3] = c2 * (gM[12] + gM[3]);
             gc[4] = c1 * (gM[4] + gM[1]);
             gc[5] = c1 * (gM[9] + gM[6]);
-            gc[6] = c3 * gM[10] - c5 * gM[15];
+            gc[6] = AT(c3 * gM[10]) + AT(-c5 * gM[15]);
             gc[7] = c1 * (gM[8] + gM[2]);
-            gc[8] = c1 * (gM[0] - gM[5]);
+            gc[8] = AT(c1 * gM[0]) + AT(-c1 * gM[5]);
             double gi = g[3 * pix + ch], di = diff[3 * pix + ch];
-            gD[3 * pix + ch] = gi * res / O_PI;
-            gN[3 * pix + ch] = gi * di * gnv[ch] / O_PI;
-            for (int c = 0; c < 9; c++) gCoef[co + c] += gi * di * gc[c] / O_PI;
+            gD[3 * pix + ch] = AT(gi) * res / O_PI;
+            gN[3 * pix + ch] = AT(gi * di) * gnv[ch] / O_PI;
+            for (int c = 0; c < 9; c++) gCoef[co + c] += AT(gi * di * gc[c] / O_PI);
         }
     }
 }
@@ -579,8 +726,9 @@ void o_lambert_forward(int bn, int W, int H, int L, int ib, const double* dirs, 
             const double* dir = dirs + 3 * (b * L + l);
             const double* I = ints + 3 * ((ib > 1 ? b * L : 0) + l);
             double c = dir[0] * n[0] + dir[1] * n[1] + dir[2] * n[2];
-            c = c > 0 ? c : 0;
-            for (int ch = 0; ch < 3; ch++) acc[ch] += d[ch] / O_PI * I[ch] * c;
+            if (!(c > 0)) continue;
+            if (g_abs_terms) c = fabs(dir[0] * n[0]) + fabs(dir[1] * n[1]) + fabs(dir[2] * n[2]);
+            for (int ch = 0; ch < 3; ch++) acc[ch] += AT(d[ch] / O_PI * I[ch] * c);
         }
         for (int ch = 0; ch < 3; ch++) out[3 * pix + ch] = acc[ch];
     }
@@ -602,14 +750,15 @@ void o_lambert_backward(int bn, int W, int H, int L, int ib, const double* g, co
             const double* I = ints + io;
             double c = dir[0] * n[0] + dir[1] * n[1] + dir[2] * n[2];
             if (!(c > 0)) continue;
+            if (g_abs_terms) c = fabs(dir[0] * n[0]) + fabs(dir[1] * n[1]) + fabs(dir[2] * n[2]);
             double dco[3] = { d[0] * I[0], d[1] * I[1], d[2] * I[2] };
-            double gdc = gi[0] * dco[0] + gi[1] * dco[1] + gi[2] * dco[2];
+            double gdc = AT(gi[0] * dco[0]) + AT(gi[1] * dco[1]) + AT(gi[2] * dco[2]);
             double sc = c / O_PI;
             for (int ch = 0; ch < 3; ch++) {
-                gDir[3 * (b * L + l) + ch] += gdc * n[ch] / O_PI;
-                gInt[io + ch] += gi[ch] * d[ch] * sc;
-                gN[3 * pix + ch] += gdc * dir[ch] / O_PI;
-                gD[3 * pix + ch] += gi[ch] * I[ch] * sc;
+                gDir[3 * (b * L + l) + ch] += AT(gdc * n[ch] / O_PI);
+                gInt[io + ch] += AT(gi[ch] * d[ch] * sc);
+                gN[3 * pix + ch] += AT(gdc * dir[ch] / O_PI);
+                gD[3 * pix + ch] += AT(gi[ch] * I[ch] * sc);
             }
         }
     }

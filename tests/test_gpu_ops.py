@@ -1,15 +1,18 @@
 """GPU parity of the per-slot operators against the FP64 oracle.
 
-Tolerance (FP32 kernels vs FP64 oracle): max |gpu - oracle| <= 1e-4 * max |oracle|
-per output tensor.  Inputs are seeded sphere scenes with std-devs scaled up so
-each splat covers several pixels, and slot tensors mixing nearby splats,
-random splats and empty (-1) slots.
+Tolerance (FP32 kernels vs FP64 oracle), per element:
+|gpu - oracle| <= 1e-4 |oracle| + 64 * 2^-24 * mag (+ 1e-30), where mag is the
+sum of |terms| the output accumulates (the oracle's abs-terms mode): the
+relative bound holds wherever the terms do not cancel, and the floor is what a
+float32 sum of the same terms can lose where they do.  Inputs are seeded sphere
+scenes with std-devs scaled up so each splat covers several pixels, and slot
+tensors mixing nearby splats, random splats and empty (-1) slots.
 """
 import numpy as np
 import pytest
 import torch
 
-from helpers import scene_inputs, near_slots, random_slots, assert_close
+from helpers import scene_inputs, near_slots, random_slots, assert_close, magnitude
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -35,8 +38,9 @@ def test_splat_forward(dev, case):
     w = splatting.forward(sc.width, sc.height, sc.focal, t(idx, dev), t(p[None], dev), t(n[None], dev),
                           t(sd[None], dev), 0.0)
     ref = O.splat_forward(sc.width, sc.height, sc.focal, idx, p[None], n[None], sd[None])
+    mag = magnitude(O.splat_forward, sc.width, sc.height, sc.focal, idx, p[None], n[None], sd[None])
     assert (ref > 0).mean() > 0.05
-    assert_close(w.cpu().numpy(), ref, RTOL, "splat forward")
+    assert_close(w.cpu().numpy(), ref, RTOL, "splat forward", mag=mag)
     assert np.all(w.cpu().numpy()[idx < 0] == 0)
 
 
@@ -48,10 +52,12 @@ def test_splat_backward(dev, case, smoothing):
     g = rng.normal(size=idx.shape).astype(np.float32)
     gp, gn, gs = splatting.backward(sc.width, sc.height, sc.focal, t(g, dev), t(idx, dev), t(p[None], dev),
                                     t(n[None], dev), t(sd[None], dev), smoothing)
-    rp, rn, rs = O.splat_backward(sc.width, sc.height, sc.focal, g, idx, p[None], n[None], sd[None], smoothing)
-    assert_close(gp.cpu().numpy(), rp, RTOL, "grad points")
-    assert_close(gn.cpu().numpy(), rn, RTOL, "grad normals")
-    assert_close(gs.cpu().numpy(), rs, RTOL, "grad stddevs")
+    args = (sc.width, sc.height, sc.focal, g, idx, p[None], n[None], sd[None], smoothing)
+    rp, rn, rs = O.splat_backward(*args)
+    mp, mn, ms = magnitude(O.splat_backward, *args)
+    assert_close(gp.cpu().numpy(), rp, RTOL, "grad points", mag=mp)
+    assert_close(gn.cpu().numpy(), rn, RTOL, "grad normals", mag=mn)
+    assert_close(gs.cpu().numpy(), rs, RTOL, "grad stddevs", mag=ms)
 
 
 def test_splat_batched_views_and_broadcast_sd(dev):
@@ -65,12 +71,15 @@ def test_splat_batched_views_and_broadcast_sd(dev):
     g = rng.normal(size=idx.shape).astype(np.float32)
     w = splatting.forward(sc.width, sc.height, sc.focal, t(idx, dev), t(p2, dev), t(n2, dev), t(sd[None], dev), 0.0)
     ref = O.splat_forward(sc.width, sc.height, sc.focal, idx, p2, n2, sd[None])
-    assert_close(w.cpu().numpy(), ref, RTOL, "batched forward")
+    mag = magnitude(O.splat_forward, sc.width, sc.height, sc.focal, idx, p2, n2, sd[None])
+    assert_close(w.cpu().numpy(), ref, RTOL, "batched forward", mag=mag)
     gp, gn, gs = splatting.backward(sc.width, sc.height, sc.focal, t(g, dev), t(idx, dev), t(p2, dev), t(n2, dev),
                                     t(sd[None], dev), 0.0)
     rp, rn, rs = O.splat_backward(sc.width, sc.height, sc.focal, g, idx, p2, n2, sd[None])
-    assert_close(gp.cpu().numpy(), rp, RTOL, "batched grad points")
-    assert_close(gs.cpu().numpy(), rs, RTOL, "batched grad stddevs")
+    mp, mn, ms = magnitude(O.splat_backward, sc.width, sc.height, sc.focal, g, idx, p2, n2, sd[None])
+    assert_close(gp.cpu().numpy(), rp, RTOL, "batched grad points", mag=mp)
+    assert_close(gn.cpu().numpy(), rn, RTOL, "batched grad normals", mag=mn)
+    assert_close(gs.cpu().numpy(), rs, RTOL, "batched grad stddevs", mag=ms)
 
 
 def test_covariance(dev, case):
@@ -96,7 +105,8 @@ def test_filter_forward_backward(dev, k):
     gw = filtering.backward(t(idx, dev), t(w, dev), t(g, dev), sc.width, sc.height, sc.focal, t(p[None], dev),
                             t(n[None], dev))
     rg = O.filter_backward(idx, w, g, sc.width, sc.height, sc.focal, p[None], n[None])
-    assert_close(gw.cpu().numpy(), rg, 5 * RTOL, "filter backward")
+    mg = magnitude(O.filter_backward, idx, w, g, sc.width, sc.height, sc.focal, p[None], n[None])
+    assert_close(gw.cpu().numpy(), rg, RTOL, "filter backward", mag=mg)
 
 
 @pytest.mark.parametrize("affine", [True, False])
@@ -114,7 +124,8 @@ def test_transform(dev, affine, bn, vb):
     fw = transform.affineForward if affine else transform.linearForward
     bw = transform.affineBackward if affine else transform.linearBackward
     out = fw(t(M, dev), t(v, dev))
-    assert_close(out.cpu().numpy(), O.transform_forward(affine, M, v), RTOL, "transform forward")
+    assert_close(out.cpu().numpy(), O.transform_forward(affine, M, v), RTOL, "transform forward",
+                 mag=magnitude(O.transform_forward, affine, M, v))
     if not affine:      # the fused camera-facing copy equals the renderer's torch.where, bit for bit
         same, facing = transform.linearForwardFacing(t(M, dev), t(v, dev))
         assert torch.equal(same, out)
@@ -124,8 +135,9 @@ def test_transform(dev, affine, bn, vb):
     del junk
     gM, gV = bw(gd, Md, vd)
     rM, rV = O.transform_backward(affine, g, M, v)
-    assert_close(gM.cpu().numpy(), rM, RTOL, "transform grad M")
-    assert_close(gV.cpu().numpy(), rV, RTOL, "transform grad v")
+    mM, mV = magnitude(O.transform_backward, affine, g, M, v)
+    assert_close(gM.cpu().numpy(), rM, RTOL, "transform grad M", mag=mM)
+    assert_close(gV.cpu().numpy(), rV, RTOL, "transform grad v", mag=mV)
 
 
 @pytest.mark.parametrize("C,ab", [(3, 1), (16, 2)])
@@ -138,11 +150,13 @@ def test_interpolation(dev, C, ab):
     a = rng.normal(size=(ab, N, C)).astype(np.float32)
     g = rng.normal(size=(bn, W, H, C)).astype(np.float32)
     out = interpolation.forward(t(idx, dev), t(w, dev), t(a, dev))
-    assert_close(out.cpu().numpy(), O.interp_forward(idx, w, a), RTOL, "interp forward")
+    assert_close(out.cpu().numpy(), O.interp_forward(idx, w, a), RTOL, "interp forward",
+                 mag=magnitude(O.interp_forward, idx, w, a))
     gW, gA = interpolation.backward(t(g, dev), t(idx, dev), t(w, dev), t(a, dev))
     rW, rA = O.interp_backward(g, idx, w, a)
-    assert_close(gW.cpu().numpy(), rW, RTOL, "interp grad w")
-    assert_close(gA.cpu().numpy(), rA, RTOL, "interp grad attr")
+    mW, mA = magnitude(O.interp_backward, g, idx, w, a)
+    assert_close(gW.cpu().numpy(), rW, RTOL, "interp grad w", mag=mW)
+    assert_close(gA.cpu().numpy(), rA, RTOL, "interp grad attr", mag=mA)
 
 
 @pytest.mark.parametrize("cb", [1, 2])
@@ -156,12 +170,14 @@ def test_sh_shading(dev, cb):
     diff = rng.uniform(size=(bn, W, H, 3)).astype(np.float32)
     g = rng.normal(size=(bn, W, H, 3)).astype(np.float32)
     out = shading.forwardSh(t(coef, dev), t(nrm, dev), t(diff, dev))
-    assert_close(out.cpu().numpy(), O.sh_forward(coef, nrm, diff), RTOL, "sh forward")
+    assert_close(out.cpu().numpy(), O.sh_forward(coef, nrm, diff), RTOL, "sh forward",
+                 mag=magnitude(O.sh_forward, coef, nrm, diff))
     gc, gn, gd = shading.backwardSh(t(g, dev), t(coef, dev), t(nrm, dev), t(diff, dev))
     rc, rn, rd = O.sh_backward(g, coef, nrm, diff)
-    assert_close(gc.cpu().numpy(), rc, RTOL, "sh grad coef")
-    assert_close(gn.cpu().numpy(), rn, RTOL, "sh grad normals")
-    assert_close(gd.cpu().numpy(), rd, RTOL, "sh grad diffuse")
+    mc, mn, md = magnitude(O.sh_backward, g, coef, nrm, diff)
+    assert_close(gc.cpu().numpy(), rc, RTOL, "sh grad coef", mag=mc)
+    assert_close(gn.cpu().numpy(), rn, RTOL, "sh grad normals", mag=mn)
+    assert_close(gd.cpu().numpy(), rd, RTOL, "sh grad diffuse", mag=md)
 
 
 @pytest.mark.parametrize("ib", [1, 2])
@@ -175,8 +191,10 @@ def test_lambert_shading(dev, ib):
     diff = rng.uniform(size=(bn, W, H, 3)).astype(np.float32)
     g = rng.normal(size=(bn, W, H, 3)).astype(np.float32)
     out = shading.forward(t(dirs, dev), t(ints, dev), t(nrm, dev), t(diff, dev))
-    assert_close(out.cpu().numpy(), O.lambert_forward(dirs, ints, nrm, diff), RTOL, "lambert forward")
+    assert_close(out.cpu().numpy(), O.lambert_forward(dirs, ints, nrm, diff), RTOL, "lambert forward",
+                 mag=magnitude(O.lambert_forward, dirs, ints, nrm, diff))
     gdir, gint, gn, gd = shading.backward(t(g, dev), t(dirs, dev), t(ints, dev), t(nrm, dev), t(diff, dev))
     r = O.lambert_backward(g, dirs, ints, nrm, diff)
-    for a, b, nm in zip((gdir, gint, gn, gd), r, ("dir", "int", "normal", "diffuse")):
-        assert_close(a.cpu().numpy(), b, RTOL, "lambert grad " + nm)
+    m = magnitude(O.lambert_backward, g, dirs, ints, nrm, diff)
+    for a, b, mm, nm in zip((gdir, gint, gn, gd), r, m, ("dir", "int", "normal", "diffuse")):
+        assert_close(a.cpu().numpy(), b, RTOL, "lambert grad " + nm, mag=mm)

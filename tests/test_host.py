@@ -32,10 +32,46 @@ def test_ops_refuse_cpu_tensors():
         lambda: interpolation.backward(torch.zeros(1, 2, 2, 3), idx, w, p),
         lambda: transform.affineForward(M, p),
         lambda: transform.linearForward(M, p),
-        lambda: shading.forwardSh(torch.zeros(3, 9), torch.zeros(1, 2, 2, 3), torch.zeros(1, 2, 2, 3)),
+        lambda: shading.forwardSh(torch.zeros(1, 3, 9), torch.zeros(1, 2, 2, 3), torch.zeros(1, 2, 2, 3)),
     ]
     for c in calls:
         with pytest.raises(RuntimeError, match="GPU only"):
+            c()
+
+
+def test_ops_reject_shapes_before_launch():
+    """Shapes the kernels index by are checked on the host, before any device
+    work: e.g. (1, N, 3) points with two views would be read past their end."""
+    from cudaExtensions import sampling, splatting, filtering, interpolation, transform, shading, estimator
+    p = torch.zeros(2, 4, 3)
+    p1 = torch.zeros(1, 4, 3)
+    s = torch.ones(1, 4, 2)
+    idx = torch.zeros(2, 2, 2, 3, dtype=torch.int32)
+    w = torch.zeros(2, 2, 2, 3)
+    M = torch.eye(4)[None].repeat(2, 1, 1)
+    bad = [
+        lambda: splatting.forward(2, 2, 1.0, idx, p1, p1, s, 0.0),                 # broadcast points
+        lambda: splatting.forward(3, 2, 1.0, idx, p, p, s, 0.0),                   # width != indices
+        lambda: splatting.forward(2, 2, 1.0, idx, p, p, torch.ones(1, 5, 2), 0.0),   # N mismatch
+        lambda: splatting.backward(2, 2, 1.0, w[..., :2], idx, p, p, s, 0.0),       # gradIn shape
+        lambda: estimator.forward(2, 2, 1.0, idx, w, p1, p1, s, torch.zeros(1, 4, 3)),
+        lambda: estimator.forward(2, 2, 1.0, idx, w, p, p, s, torch.zeros(3, 4, 3)),
+        lambda: estimator.backward(2, 2, 1.0, torch.zeros(2, 2, 2, 4), idx, w, p, p, s, torch.zeros(1, 4, 3)),
+        lambda: filtering.forward(idx, w[..., :2], 2, 2, 1.0, p, p),
+        lambda: filtering.forward(idx, w, 2, 2, 1.0, torch.zeros(3, 4, 3), torch.zeros(3, 4, 3)),
+        lambda: interpolation.forward(idx, w, torch.zeros(3, 4, 3)),
+        lambda: interpolation.backward(torch.zeros(2, 2, 2, 5), idx, w, torch.zeros(1, 4, 3)),
+        lambda: transform.affineForward(M, torch.zeros(3, 4, 3)),
+        lambda: transform.affineForward(torch.eye(3)[None], p1),
+        lambda: transform.linearBackward(torch.zeros(2, 5, 3), M, p),
+        lambda: shading.forward(torch.zeros(1, 2, 3), torch.zeros(1, 2, 3), torch.zeros(2, 2, 2, 3),
+                                torch.zeros(2, 2, 2, 3)),
+        lambda: shading.forwardSh(torch.zeros(3, 9), torch.zeros(1, 2, 2, 3), torch.zeros(1, 2, 2, 3)),
+        lambda: sampling.sample(2, 2, 1.0, p, p1, None, s, 3, 0.0),
+        lambda: sampling.samplePoisson(2, 2, 1.0, p, p, None, torch.ones(2, 5, 2), 3, 0.0),
+    ]
+    for i, c in enumerate(bad):
+        with pytest.raises(ValueError):
             c()
 
 
@@ -82,6 +118,48 @@ def test_world2camera_is_rotation_of_quaternion():
                       [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
         assert np.allclose(M[b, :3, :3], R, atol=1e-12)
         assert np.allclose(M[b, :3, 3], pos[b]) and np.allclose(M[b, 3], [0, 0, 0, 1])
+
+
+def test_world2camera_scales_with_quaternion_norm():
+    """A non-unit quaternion gives |q|^2 times the rotation of q / |q| (the entries
+    are homogeneous quadratics, as the reference's x^2 - y^2 - z^2 + w^2 ...)."""
+    from splatting_amd.utils import world2Camera
+    rng = np.random.default_rng(1)
+    q = rng.normal(size=(4, 4)) * 1.7
+    pos = rng.normal(size=(4, 3))
+    M = world2Camera(torch.tensor(pos), torch.tensor(q)).numpy()[:, 0]
+    Mu = world2Camera(torch.tensor(pos), torch.tensor(q / np.linalg.norm(q, axis=1, keepdims=True))).numpy()[:, 0]
+    s2 = (q * q).sum(1)
+    for b in range(4):
+        assert np.allclose(M[b, :3, :3], s2[b] * Mu[b, :3, :3], atol=1e-12)
+        R = Mu[b, :3, :3]
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-12) and abs(np.linalg.det(R) - 1.0) < 1e-12
+
+
+def test_lambert_module_is_ambient_plus_clamped_cosines():
+    """Per splat: diffuse * (ambient + sum_l I_l clamp(d_l . n, 0, 1) / pi), with the
+    lamp intensities shared over the batch when their batch is 1 (shading.py:12-29)."""
+    import math
+    from splatting_amd.shading import Lambert
+    rng = np.random.default_rng(2)
+    bn, L, N = 2, 3, 50
+    d = rng.normal(size=(bn, L, 3))
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    I = rng.uniform(size=(1, L, 3))
+    n = rng.normal(size=(bn, N, 3))
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    a = rng.uniform(size=(bn, N, 3))
+    amb = rng.uniform(size=(bn, 1, 3))
+    out = Lambert()(*(torch.tensor(x) for x in (amb, d, I, n, a))).numpy()
+    ref = np.zeros((bn, N, 3))
+    for b in range(bn):
+        for i in range(N):
+            acc = amb[b, 0] * a[b, i]
+            for l in range(L):
+                c = min(max(float(d[b, l] @ n[b, i]), 0.0), 1.0)
+                acc = acc + a[b, i] / math.pi * I[0, l] * c
+            ref[b, i] = acc
+    assert np.allclose(out, ref, rtol=1e-12, atol=1e-14)
 
 
 def test_transform_forward_flip():

@@ -92,3 +92,16 @@ def test_fmm_density_bracket_other_orders(p):
     assert np.all(img >= dens - far - tol)
     with pytest.raises(ValueError):
         O.set_fixed_degree(3)
+
+
+def test_direct_sums_closed_form():
+    """o_direct_sums against numpy's FP64 sum of the same Gaussians."""
+    import math
+    rng = np.random.default_rng(3)
+    s = rng.uniform(0, 64, size=(3000, 2)).astype(np.float32)
+    q = rng.uniform(0, 64, size=(50, 2))
+    h, c = 1.7, 1.0 / (2.0 * math.pi * math.sqrt(1.7))
+    got = O.direct_sums(s, h, c, q)
+    d2 = ((q[:, None, :] - s[None, :, :].astype(np.float64)) ** 2).sum(-1)
+    ref = c * np.exp(-d2 / (2 * h * h)).sum(1)
+    assert np.allclose(got, ref, rtol=1e-13, atol=0)

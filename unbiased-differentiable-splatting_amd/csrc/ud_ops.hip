@@ -12,6 +12,8 @@
 #include "ud_device.h"
 #include "../../include/udsplat.h"
 
+#include <hip/hip_fp16.h>
+
 using namespace ud;
 
 namespace {
@@ -83,19 +85,23 @@ __device__ inline void ewa_geom(const T* p, const T* n, double W, double H, doub
 }
 
 
-struct SplatRec {
-    float sxi, syi, sxf, syf;   // screen point split into integer part and fraction
-    float t00, t01, t11;        // upper-triangular T with T^T T = V^-1 (computed in FP64)
-    float norm;                 // 1 / (2 pi sqrt(det V) |det A|)
+template <class T>
+struct SplatRecT {
+    T sxi, syi, sxf, syf;       // screen point split into integer part and fraction
+    T t00, t01, t11;            // upper-triangular T with T^T T = V^-1 (computed in FP64)
+    T norm;                     // 1 / (2 pi sqrt(det V) |det A|)
 };
+using SplatRec = SplatRecT<float>;
 // With u = t00 dx + t01 dy, v = t11 dy the exponent is u^2 + v^2: rounding T to
 // FP32 perturbs it by O(eps sqrt(e lambda_max) |d|), while rounding V^-1 itself
 // would cost O(eps cond(V) e) for nearly edge-on splats.
 
-// Per (view, splat) FP64 precompute of the screen Gaussian (ewaSplatting.cuh:24-67).
+// Per (view, splat) FP64 precompute of the screen Gaussian (ewaSplatting.cuh:24-67),
+// rounded to the op's real type T (float, or double for the FP64 op surface).
+template <class T>
 __global__ void k_splat_prepare(int bn, int N, int W, int H, double f, double smoothing,
-                                const float* __restrict__ pts, const float* __restrict__ nrm,
-                                const float* __restrict__ sd, int sdBatch, SplatRec* __restrict__ rec)
+                                const T* __restrict__ pts, const T* __restrict__ nrm,
+                                const T* __restrict__ sd, int sdBatch, SplatRecT<T>* __restrict__ rec)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)bn * N) return;
@@ -112,44 +118,49 @@ __global__ void k_splat_prepare(int bn, int N, int W, int H, double f, double sm
     double V01 = J01 * J11 * s1 * s1;
     double V11 = J11 * J11 * s1 * s1 + smoothing;
     double detV = V00 * V11 - V01 * V01;
-    SplatRec r;
+    SplatRecT<T> r;
     double fx = floor(sx), fy = floor(sy);
-    r.sxi = (float)fx; r.syi = (float)fy;
-    r.sxf = (float)(sx - fx); r.syf = (float)(sy - fy);
-    r.t00 = (float)sqrt(V11 / detV);
-    r.t01 = (float)(-V01 / sqrt(V11 * detV));
-    r.t11 = (float)(1.0 / sqrt(V11));
-    r.norm = (float)(1.0 / (2.0 * UD_PI * sqrt(detV) * fabs(detA)));
+    r.sxi = (T)fx; r.syi = (T)fy;
+    r.sxf = (T)(sx - fx); r.syf = (T)(sy - fy);
+    r.t00 = (T)sqrt(V11 / detV);
+    r.t01 = (T)(-V01 / sqrt(V11 * detV));
+    r.t11 = (T)(1.0 / sqrt(V11));
+    r.norm = (T)(1.0 / (2.0 * UD_PI * sqrt(detV) * fabs(detA)));
     rec[t] = r;
 }
 
-__device__ __forceinline__ float ewa_eval(const SplatRec& r, float px, float py, float& u, float& v)
+__device__ __forceinline__ float ewa_exp(float x) { return __expf(x); }
+__device__ __forceinline__ double ewa_exp(double x) { return exp(x); }
+
+template <class T>
+__device__ __forceinline__ T ewa_eval(const SplatRecT<T>& r, T px, T py, T& u, T& v)
 {
-    float dx = (px - r.sxi) - r.sxf;
-    float dy = (py - r.syi) - r.syf;
+    T dx = (px - r.sxi) - r.sxf;
+    T dy = (py - r.syi) - r.syf;
     u = r.t00 * dx + r.t01 * dy;
     v = r.t11 * dy;
-    float e = u * u + v * v;
-    return r.norm * __expf(-0.5f * e);
+    T e = u * u + v * v;
+    return r.norm * ewa_exp(T(-0.5) * e);
 }
 
 // splatting.forward: one thread per slot (splattingCuda.cu:18-66)
+template <class T>
 __global__ void k_splat_forward(long nslots, int N, int W, int H, int k,
-                                const int32_t* __restrict__ idx, const SplatRec* __restrict__ rec,
-                                float* __restrict__ out)
+                                const int32_t* __restrict__ idx, const SplatRecT<T>* __restrict__ rec,
+                                T* __restrict__ out)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nslots) return;
     int id = idx[t];
-    if (id < 0) { out[t] = 0.0f; return; }
+    if (id < 0) { out[t] = T(0); return; }
     long pix = t / k;
     long b = pix / ((long)W * H);
     int r = (int)(pix - b * (long)W * H);
     int i = r / H, j = r - (r / H) * H;
-    SplatRec s = rec[b * N + id];
-    float dx, dy;
-    float w = ewa_eval(s, (float)i, (float)j, dx, dy);
-    out[t] = fminf(w, 1.0f);
+    SplatRecT<T> s = rec[b * N + id];
+    T dx, dy;
+    T w = ewa_eval(s, (T)i, (T)j, dx, dy);
+    out[t] = fmin(w, T(1));
 }
 
 // floats per (view, splat) record of the backward passes: splatting.backward's
@@ -161,21 +172,22 @@ constexpr int kHtRec = 16;
 // coordinates z = T d = (u, v):
 //   m0 = sum g w, (m1u, m1v) = sum g w z, (m2uu, m2uv, m2vv) = sum g w z z^T
 // for slots whose unclamped weight is <= 1 (splattingCuda.cu:173).
+template <class T>
 __global__ __launch_bounds__(256) void k_splat_moments(long nslots, int N, int W, int H, int k,
                                                        const int32_t* __restrict__ idx,
-                                                       const float* __restrict__ gin,
-                                                       const SplatRec* __restrict__ rec, float* __restrict__ mom)
+                                                       const T* __restrict__ gin,
+                                                       const SplatRecT<T>* __restrict__ rec, T* __restrict__ mom)
 {
     // each lane computes its slot's six sums; the wavefront then adds them with six
     // consecutive lanes per record, so one atomic instruction covers ten records'
     // lines instead of 64 (L2 serves the dwords of one line in one instruction
     // together)
-    __shared__ float sv[4][64][6];
+    __shared__ T sv[4][64][6];
     __shared__ long sr[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long si = -1;
-    float v[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+    T v[6] = { T(0), T(0), T(0), T(0), T(0), T(0) };
     const int id = t < nslots ? idx[t] : -1;
     if (id >= 0) {
         long pix = t / k;
@@ -183,12 +195,12 @@ __global__ __launch_bounds__(256) void k_splat_moments(long nslots, int N, int W
         int r = (int)(pix - b * (long)W * H);
         int i = r / H, j = r - (r / H) * H;
         const long s_ = b * N + id;
-        SplatRec s = rec[s_];
-        float dx, dy;
-        float w = ewa_eval(s, (float)i, (float)j, dx, dy);
-        if (w <= 1.0f) {
-            float gw = gin[t] * w;
-            if (gw != 0.0f) {
+        SplatRecT<T> s = rec[s_];
+        T dx, dy;
+        T w = ewa_eval(s, (T)i, (T)j, dx, dy);
+        if (w <= T(1)) {
+            T gw = gin[t] * w;
+            if (gw != T(0)) {
                 si = s_;
                 v[0] = gw;
                 v[1] = gw * dx;
@@ -207,8 +219,8 @@ __global__ __launch_bounds__(256) void k_splat_moments(long nslots, int N, int W
         const int slot = q + sub;
         if (lane < 60 && slot < 64) {
             const long s_ = sr[wv][slot];
-            const float x = sv[wv][slot][c];
-            if (s_ >= 0 && x != 0.0f) atomicAdd(mom + kMomRec * s_ + c, x);
+            const T x = sv[wv][slot][c];
+            if (s_ >= 0 && x != T(0)) atomicAdd(mom + kMomRec * s_ + c, x);
         }
     }
 }
@@ -218,30 +230,31 @@ __global__ __launch_bounds__(256) void k_splat_moments(long nslots, int N, int W
 // holds the six moments at [0, 6); records of kHtRec floats (the fused
 // estimator's) also hold the attribute gradient of the (view, splat) at
 // [8, 11), written to gAttr (ab == bn) or added into it (ab == 1 < bn).
+template <class T>
 __global__ __launch_bounds__(128) void k_splat_finalize(int bn, int N, int W, int H, double f, double smoothing,
-                                                        const float* __restrict__ pts, const float* __restrict__ nrm,
-                                                        const float* __restrict__ sd, int sdBatch,
-                                                        const SplatRec* __restrict__ rec,
-                                                        const float* __restrict__ mom, int recLen, int ab,
-                                                        float* __restrict__ gAttr, float* __restrict__ gPts,
-                                                        float* __restrict__ gNrm, float* __restrict__ gSd)
+                                                        const T* __restrict__ pts, const T* __restrict__ nrm,
+                                                        const T* __restrict__ sd, int sdBatch,
+                                                        const SplatRecT<T>* __restrict__ rec,
+                                                        const T* __restrict__ mom, int recLen, int ab,
+                                                        T* __restrict__ gAttr, T* __restrict__ gPts,
+                                                        T* __restrict__ gNrm, T* __restrict__ gSd)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)bn * N) return;
     long b = t / N, id = t % N;
     long so = 2 * ((sdBatch > 1 ? b * N : 0) + id);
-    const float* m = mom + (long)recLen * t;
+    const T* m = mom + (long)recLen * t;
     if (gAttr) {
-        float* ga = gAttr + 3 * ((ab > 1 ? b * N : 0) + id);
+        T* ga = gAttr + 3 * ((ab > 1 ? b * N : 0) + id);
         for (int c = 0; c < 3; c++) {
             if (ab == bn) ga[c] = m[8 + c];
-            else if (m[8 + c] != 0.0f) atomicAdd(ga + c, m[8 + c]);
+            else if (m[8 + c] != T(0)) atomicAdd(ga + c, m[8 + c]);
         }
     }
     double m0 = m[0], m1x = m[1], m1y = m[2], m2xx = m[3], m2xy = m[4], m2yy = m[5];
     if (m0 == 0.0 && m1x == 0.0 && m1y == 0.0 && m2xx == 0.0 && m2xy == 0.0 && m2yy == 0.0) {
-        for (int c = 0; c < 3; c++) { gPts[3 * t + c] = 0.0f; gNrm[3 * t + c] = 0.0f; }
-        if (sdBatch == bn) { gSd[so] = 0.0f; gSd[so + 1] = 0.0f; }
+        for (int c = 0; c < 3; c++) { gPts[3 * t + c] = T(0); gNrm[3 * t + c] = T(0); }
+        if (sdBatch == bn) { gSd[so] = T(0); gSd[so + 1] = T(0); }
         return;
     }
     D6 p[3], n[3];
@@ -263,7 +276,7 @@ __global__ __launch_bounds__(128) void k_splat_finalize(int bn, int N, int W, in
     double detV = V00 * V11 - V01 * V01;
     (void)detV;
     // T (as rounded in the slot pass) maps d to the whitened z = T d
-    const SplatRec& R = rec[t];
+    const SplatRecT<T>& R = rec[t];
     double T00 = R.t00, T01 = R.t01, T11 = R.t11;
     // G_s = V^-1 sum g w d = T^T (sum g w z)
     double Gsx = T00 * m1x;
@@ -299,11 +312,11 @@ __global__ __launch_bounds__(128) void k_splat_finalize(int bn, int N, int W, in
     for (int c = 0; c < 3; c++) {
         double gp = GA00 * A00.d[c] + GA01 * A01.d[c] + GA11 * A11.d[c] + Gsx * sx.d[c] + Gsy * sy.d[c];
         double gn = GA00 * A00.d[3 + c] + GA01 * A01.d[3 + c] + GA11 * A11.d[3 + c];
-        gPts[3 * t + c] = (float)gp;
-        gNrm[3 * t + c] = (float)gn;
+        gPts[3 * t + c] = (T)gp;
+        gNrm[3 * t + c] = (T)gn;
     }
-    if (sdBatch == bn) { gSd[so] = (float)gs0; gSd[so + 1] = (float)gs1; }     // one writer per std-dev
-    else { atomicAdd(gSd + so, (float)gs0); atomicAdd(gSd + so + 1, (float)gs1); }
+    if (sdBatch == bn) { gSd[so] = (T)gs0; gSd[so + 1] = (T)gs1; }     // one writer per std-dev
+    else { atomicAdd(gSd + so, (T)gs0); atomicAdd(gSd + so + 1, (T)gs1); }
 }
 
 // ---------------------------------------------------------------------------
@@ -393,20 +406,29 @@ __global__ __launch_bounds__(256) void k_ht_forward(long npix, int N, int W, int
 // The adjoint of one slot: interpolation.backward's attribute gradient g wt and
 // weight gradient gW, times 1 / pi, into splatting.backward's moments.  v[0, 6)
 // are the weight moments (zero for a slot whose unclamped weight exceeds 1),
-// v[6, 9) the attribute gradient.
+// v[6, 9) the attribute gradient.  With compositing factors cf (the composited
+// estimate), the attribute gradient is g wt T and the gradient of the slot's
+// alpha = wt is cf.y = T (g . a - S) instead of g . a (k_htc's weight pass).
 __device__ __forceinline__ void ht_slot_adjoint(const SplatRec& s, float px, float py, float g0, float g1, float g2,
-                                                const float* a, float q, float v[9])
+                                                const float* a, float q, const float2* cf, float v[9])
 {
     float u, w;
     const float we = ewa_eval(s, px, py, u, w);
-    const float wt = fminf(we, 1.0f) * q;
+    float wt = fminf(we, 1.0f) * q;
+    float acc;
+    if (cf) {
+        const float2 c = *cf;
+        wt = wt * c.x;
+        acc = c.y;
+    } else {
+        acc = 0.0f;
+        acc += g0 * a[0];
+        acc += g1 * a[1];
+        acc += g2 * a[2];
+    }
     v[6] = g0 * wt;
     v[7] = g1 * wt;
     v[8] = g2 * wt;
-    float acc = 0.0f;
-    acc += g0 * a[0];
-    acc += g1 * a[1];
-    acc += g2 * a[2];
     const float gw = (we <= 1.0f) ? (acc * q) * we : 0.0f;
     v[0] = gw;
     v[1] = gw * u;
@@ -440,6 +462,7 @@ __global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W,
                                                                    const float* __restrict__ ipi,
                                                                    const SplatRec* __restrict__ rec,
                                                                    const float* __restrict__ attr,
+                                                                   const float2* __restrict__ coef,
                                                                    float* __restrict__ mom)
 {
     __shared__ int key[kHtSlots];
@@ -460,9 +483,10 @@ __global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W,
         const float g0 = g[3 * pix], g1 = g[3 * pix + 1], g2 = g[3 * pix + 2];
         const float* av = attr + 3 * (ab > 1 ? b * N : 0);
         const SplatRec* rv = rec + b * N;
-        auto sample = [&](int id, const SplatRec& sr, const float* a, float q) {
+        const float2* cp = coef ? coef + pix * k : nullptr;
+        auto sample = [&](int id, int slot_, const SplatRec& sr, const float* a, float q) {
             float v[9];
-            ht_slot_adjoint(sr, px, py, g0, g1, g2, a, q, v);
+            ht_slot_adjoint(sr, px, py, g0, g1, g2, a, q, cp ? cp + slot_ : nullptr, v);
             int e = (int)(((uint32_t)id * 2654435761u) >> 21);            // 11 bits: kHtSlots entries
             int slot = -1;
             for (int t = 0; t < kHtProbes; t++) {
@@ -485,14 +509,14 @@ __global__ __launch_bounds__(kHtTile * kHtTile) void k_ht_backward(int N, int W,
                 const int n = slot_quad(ip, qp, c, rv, av, Q);
 #pragma unroll
                 for (int u = 0; u < 4; u++)
-                    if (u < n) sample(Q.id[u], Q.r[u], Q.a[u], Q.q[u]);
+                    if (u < n) sample(Q.id[u], 4 * c + u, Q.r[u], Q.a[u], Q.q[u]);
                 if (n < 4) break;
             }
         } else {
             for (int s = 0; s < k; s++) {
                 const int id = idx[pix * k + s];
                 if (id < 0) break;
-                sample(id, rv[id], av + 3 * id, ipi[pix * k + s]);
+                sample(id, s, rv[id], av + 3 * id, ipi[pix * k + s]);
             }
         }
     }
@@ -576,35 +600,58 @@ __global__ void k_covariance(int bn, int N, int W, int H, double f, double smoot
 
 // ---------------------------------------------------------------------------
 // Compositing.  Depth of slot s along the pixel ray (alphacompositing.cuh:6-36),
-// FP32 in a fixed operation order.
+// in a fixed operation order.  The storage type S of the weights and points is
+// half, float or double (the reference dispatches all three, zSortingCuda.cu:100,
+// 220); arithmetic runs in C = float for half and float, double for double.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float ray_depth(int i, int j, float Wf, float Hf, float ff, const float* p, const float* n)
+template <class S> struct CompOf { using type = float; };
+template <> struct CompOf<double> { using type = double; };
+__device__ __forceinline__ float ld_c(const float* p, long i) { return p[i]; }
+__device__ __forceinline__ double ld_c(const double* p, long i) { return p[i]; }
+__device__ __forceinline__ float ld_c(const __half* p, long i) { return __half2float(p[i]); }
+__device__ __forceinline__ void st_c(float* p, long i, float v) { p[i] = v; }
+__device__ __forceinline__ void st_c(double* p, long i, double v) { p[i] = v; }
+__device__ __forceinline__ void st_c(__half* p, long i, float v) { p[i] = __float2half(v); }
+
+template <class C>
+__device__ __forceinline__ C dot3c(const C* a, const C* b)
 {
-    float bx = (Wf - 2.0f * (float)i) / (2.0f * ff);
-    float by = -((Hf - 2.0f * (float)j) / (2.0f * ff));
-    float bz = 1.0f;
-    float nn = bx * bx;
+    C x = a[0] * b[0];
+    x = x + a[1] * b[1];
+    x = x + a[2] * b[2];
+    return x;
+}
+
+template <class C>
+__device__ __forceinline__ C ray_depth(int i, int j, C Wf, C Hf, C ff, const C* p, const C* n)
+{
+    C bx = (Wf - C(2) * (C)i) / (C(2) * ff);
+    C by = -((Hf - C(2) * (C)j) / (C(2) * ff));
+    C bz = C(1);
+    C nn = bx * bx;
     nn = nn + by * by;
     nn = nn + bz * bz;
-    float inv = 1.0f / sqrtf(nn);
-    float dir[3] = { -(bx * inv), -(by * inv), -(bz * inv) };
-    float num = dot3(p, n);
-    float den = dot3(dir, n);
-    float d = num / den;
-    return (d > 0.0f) ? d : 10000.0f;
+    C inv = C(1) / sqrt(nn);
+    C dir[3] = { -(bx * inv), -(by * inv), -(bz * inv) };
+    C num = dot3c(p, n);
+    C den = dot3c(dir, n);
+    C d = num / den;
+    return (d > C(0)) ? d : C(10000);
 }
 
 // One wavefront per pixel (k <= 64): lane s owns slot s.  The stable
 // insertion argsort of the reference becomes a rank per lane:
 //   rank_s = #{d_j < d_s} + #{j < s : d_j == d_s}.
-template <bool BACKWARD>
-__global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, int H, int k, float ff,
-                                                     const int32_t* __restrict__ idx, const float* __restrict__ wts,
-                                                     const float* __restrict__ gin,
-                                                     const float* __restrict__ pts, const float* __restrict__ nrm, int pb,
-                                                     float* __restrict__ out)
+template <bool BACKWARD, class S>
+__global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, int H, int k,
+                                                     typename CompOf<S>::type ff,
+                                                     const int32_t* __restrict__ idx, const S* __restrict__ wts,
+                                                     const S* __restrict__ gin,
+                                                     const S* __restrict__ pts, const S* __restrict__ nrm, int pb,
+                                                     S* __restrict__ out)
 {
-    __shared__ float sbuf[4][64];
+    using C = typename CompOf<S>::type;
+    __shared__ C sbuf[4][2][64];
     const int wv = threadIdx.x >> 6, s = threadIdx.x & 63;
     long pix = (long)blockIdx.x * 4 + wv;
     if (pix >= npix) return;   // whole wave exits together
@@ -614,75 +661,83 @@ __global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, in
     long base = pix * k;
     const bool act = s < k;
     int id = act ? idx[base + s] : -1;
-    float w = act ? wts[base + s] : 0.0f;
-    float d = 10000.0f;
+    C w = act ? ld_c(wts, base + s) : C(0);
+    C d = C(10000);
     if (id >= 0) {
         long po = 3 * ((pb > 1 ? b * N : 0) + id);
-        float p[3] = { pts[po], pts[po + 1], pts[po + 2] };
-        float n[3] = { nrm[po], nrm[po + 1], nrm[po + 2] };
-        d = ray_depth(i, j, (float)W, (float)H, ff, p, n);
+        C p[3] = { ld_c(pts, po), ld_c(pts, po + 1), ld_c(pts, po + 2) };
+        C n[3] = { ld_c(nrm, po), ld_c(nrm, po + 1), ld_c(nrm, po + 2) };
+        d = ray_depth<C>(i, j, (C)W, (C)H, ff, p, n);
     }
     int rank = 0;
     for (int q = 0; q < k; q++) {
-        float dq = __shfl(d, q, 64);
+        C dq = __shfl(d, q, 64);
         rank += (dq < d) || (dq == d && q < s);
     }
     // exclusive prefix product of (1 - w) in rank order
-    float* sb = sbuf[wv];
-    sb[s] = 1.0f;
+    C* sa = sbuf[wv][0];
+    C* sb = sbuf[wv][1];
+    sb[s] = C(1);
     __builtin_amdgcn_wave_barrier();
-    if (act) sb[rank] = 1.0f - w;
+    if (act) sb[rank] = C(1) - w;
     __builtin_amdgcn_wave_barrier();
-    float v = sb[s];
+    C v = sb[s];
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        float t = __shfl_up(v, o, 64);
+        C t = __shfl_up(v, o, 64);
         if (s >= o) v *= t;
     }
-    float excl = __shfl_up(v, 1, 64);
-    if (s == 0) excl = 1.0f;
-    float rem = __shfl(excl, rank, 64);
-    float blended = rem * w;
+    C excl = __shfl_up(v, 1, 64);
+    if (s == 0) excl = C(1);
+    C rem = __shfl(excl, rank, 64);     // transmittance in front of this slot
     if (!BACKWARD) {
-        if (act) out[base + s] = blended;
+        if (act) st_c(out, base + s, rem * w);
         return;
     }
-    // backward (alphacompositingGrad.cuh:7-26):
-    //   g_s = gb_s / w_s [|w_s| > eps] - (sum_{rank > rank_s} gb) / (1 - w_s) [|1 - w_s| > eps]
-    float gb = act ? gin[base + s] * blended : 0.0f;
+    // backward (alphacompositingGrad.cuh:7-26).  In depth order, with T_r the
+    // transmittance in front of rank r, the reference's sum
+    //   g_r = [|w_r| > eps] gin_r out_r / w_r - [|1 - w_r| > eps] sum_{i>r} gin_i out_i / (1 - w_r)
+    // is [|w_r| > eps] gin_r T_r - [|1 - w_r| > eps] T_r S_r with
+    //   S_r = sum_{i>r} gin_i w_i prod_{r<t<i} (1 - w_t),
+    // the suffix recurrence S_r = a_{r+1} + b_{r+1} S_{r+1} (a = gin w, b = 1 - w).
+    // Nothing is divided, so a small w or 1 - w amplifies no rounding.
+    C gi = act ? ld_c(gin, base + s) : C(0);
     __builtin_amdgcn_wave_barrier();
-    sb[s] = 0.0f;
+    sa[s] = C(0);
     __builtin_amdgcn_wave_barrier();
-    if (act) sb[rank] = gb;
+    if (act) sa[rank] = gi * w;             // sb[rank] still holds 1 - w, identity elsewhere
     __builtin_amdgcn_wave_barrier();
-    // suffix sums: inclusive scan over reversed lanes
-    float x = sb[63 - s];
+    // lane l holds the affine map x -> A + B x of rank 63 - l; an inclusive scan
+    // composes ranks 63 .. 63 - l, outer (smaller rank) applied last
+    C A = sa[63 - s], B = sb[63 - s];
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        float t = __shfl_up(x, o, 64);
-        if (s >= o) x += t;
+        C tA = __shfl_up(A, o, 64);
+        C tB = __shfl_up(B, o, 64);
+        if (s >= o) { A = A + B * tA; B = B * tB; }
     }
-    // x at lane l = sum_{q >= 63 - l} sb[q]; suffix strictly after rank r is lane 63 - (r + 1)
+    // S_r = U_{r+1}, held by lane 63 - (r + 1)
     int src = 62 - rank;
-    float after = __shfl(x, src < 0 ? 0 : src, 64);
-    if (rank == 63) after = 0.0f;
-    const float eps = 1e-4f;
-    float g = 0.0f;
-    if (fabsf(w) > eps) g += gb / w;
-    float den = -(1.0f - w);
-    if (fabsf(den) > eps) g += after / den;
-    if (act) out[base + s] = g;
+    C after = __shfl(A, src < 0 ? 0 : src, 64);
+    if (rank == 63) after = C(0);
+    const C eps = C(1e-4);
+    C g = C(0);
+    if (fabs(w) > eps) g += gi * rem;
+    if (fabs(C(1) - w) > eps) g -= rem * after;
+    if (act) st_c(out, base + s, g);
 }
 
 // General k (64 < k <= 1024): one workgroup per pixel, ranks from LDS, serial scans.
-template <bool BACKWARD>
-__global__ __launch_bounds__(1024) void k_filter_block(long npix, int N, int W, int H, int k, float ff,
-                                                       const int32_t* __restrict__ idx, const float* __restrict__ wts,
-                                                       const float* __restrict__ gin,
-                                                       const float* __restrict__ pts, const float* __restrict__ nrm, int pb,
-                                                       float* __restrict__ out)
+template <bool BACKWARD, class S>
+__global__ __launch_bounds__(1024) void k_filter_block(long npix, int N, int W, int H, int k,
+                                                       typename CompOf<S>::type ff,
+                                                       const int32_t* __restrict__ idx, const S* __restrict__ wts,
+                                                       const S* __restrict__ gin,
+                                                       const S* __restrict__ pts, const S* __restrict__ nrm, int pb,
+                                                       S* __restrict__ out)
 {
-    __shared__ float sd_[1024], sw[1024], sx[1024];
+    using C = typename CompOf<S>::type;
+    __shared__ C sd_[1024], sw[1024], sx[1024];
     __shared__ int srank[1024];
     long pix = blockIdx.x;
     long b = pix / ((long)W * H);
@@ -692,45 +747,187 @@ __global__ __launch_bounds__(1024) void k_filter_block(long npix, int N, int W, 
     int s = threadIdx.x;
     bool act = s < k;
     int id = act ? idx[base + s] : -1;
-    float w = act ? wts[base + s] : 0.0f;
-    float d = 10000.0f;
+    C w = act ? ld_c(wts, base + s) : C(0);
+    C d = C(10000);
     if (id >= 0) {
         long po = 3 * ((pb > 1 ? b * N : 0) + id);
-        float p[3] = { pts[po], pts[po + 1], pts[po + 2] };
-        float n[3] = { nrm[po], nrm[po + 1], nrm[po + 2] };
-        d = ray_depth(i, j, (float)W, (float)H, ff, p, n);
+        C p[3] = { ld_c(pts, po), ld_c(pts, po + 1), ld_c(pts, po + 2) };
+        C n[3] = { ld_c(nrm, po), ld_c(nrm, po + 1), ld_c(nrm, po + 2) };
+        d = ray_depth<C>(i, j, (C)W, (C)H, ff, p, n);
     }
     if (act) { sd_[s] = d; sw[s] = w; }
     __syncthreads();
     int rank = 0;
     if (act) {
-        for (int q = 0; q < k; q++) { float dq = sd_[q]; rank += (dq < d) || (dq == d && q < s); }
+        for (int q = 0; q < k; q++) { C dq = sd_[q]; rank += (dq < d) || (dq == d && q < s); }
         srank[rank] = s;
     }
     __syncthreads();
     if (s == 0) {   // sx[rank] = exclusive product of (1 - w) in rank order
-        float rem = 1.0f;
-        for (int q = 0; q < k; q++) { sx[q] = rem; rem *= (1.0f - sw[srank[q]]); }
+        C rem = C(1);
+        for (int q = 0; q < k; q++) { sx[q] = rem; rem *= (C(1) - sw[srank[q]]); }
     }
     __syncthreads();
-    float blended = act ? sx[rank] * w : 0.0f;
-    if (!BACKWARD) { if (act) out[base + s] = blended; return; }
-    float gb = act ? gin[base + s] * blended : 0.0f;
+    C rem = act ? sx[rank] : C(0);
+    if (!BACKWARD) { if (act) st_c(out, base + s, rem * w); return; }
+    // backward: the division-free form of the wave kernel, S in a serial suffix pass
+    C gi = act ? ld_c(gin, base + s) : C(0);
     __syncthreads();
-    if (act) sd_[rank] = gb;
+    if (act) sd_[rank] = gi * w;
     __syncthreads();
-    if (s == 0) {   // sx[rank] = sum of gb strictly after rank
-        float acc = 0.0f;
-        for (int q = k - 1; q >= 0; q--) { sx[q] = acc; acc += sd_[q]; }
+    if (s == 0) {   // sd_[rank] = S_rank = sum over later ranks, recurrence from the back
+        C acc = C(0);
+        for (int q = k - 1; q >= 0; q--) {
+            C a = sd_[q];
+            sd_[q] = acc;
+            acc = a + (C(1) - sw[srank[q]]) * acc;
+        }
     }
     __syncthreads();
     if (!act) return;
-    const float eps = 1e-4f;
-    float g = 0.0f;
-    if (fabsf(w) > eps) g += gb / w;
-    float den = -(1.0f - w);
-    if (fabsf(den) > eps) g += sx[rank] / den;
-    out[base + s] = g;
+    const C eps = C(1e-4);
+    C g = C(0);
+    if (fabs(w) > eps) g += gi * rem;
+    if (fabs(C(1) - w) > eps) g -= rem * sd_[rank];
+    st_c(out, base + s, g);
+}
+
+// ---------------------------------------------------------------------------
+// Composited Horvitz-Thompson estimate (no reference counterpart: the
+// Poisson-rate counterpart of splatting.forward -> filtering.forward ->
+// interpolation.forward).  With independent inclusions I_s ~ Bernoulli(pi_s)
+// and alpha_s = I_s min(w_s, 1) / pi_s, and a pixel's splats ordered by ray
+// depth as filtering orders them (alphacompositing.cuh:6-54),
+//     image = sum_s alpha_s a_s prod_{t in front of s} (1 - alpha_t)
+// has expectation sum_s w_s a_s prod_{t in front of s} (1 - w_t): the factors of
+// different splats are independent, each of mean (1 - w_t), so the estimate of
+// the image composited over every splat of the proposal's support is unbiased
+// (the reference's sampled set composited by filtering is not).  Its adjoint in
+// depth order, with T_r the product in front of rank r and
+// S_r = sum_{i > r} alpha_i (g . a_i) prod_{r < t < i} (1 - alpha_t):
+//     d/d alpha_r = T_r (g . a_r - S_r),   d/d a_r = alpha_r T_r g.
+// One lane per pixel over its samples (read up to the first -1); a sample's
+// depth, alpha and depth rank live in LDS arrays [entry][lane].
+// ---------------------------------------------------------------------------
+struct PlaneRec {
+    float nx, ny, nz, c;      // camera-space normal and c = p . n (ray depth c / (dir . n))
+};
+
+__global__ void k_plane_prepare(long n, const float* __restrict__ pts, const float* __restrict__ nrm,
+                                PlaneRec* __restrict__ out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float p[3] = { pts[3 * t], pts[3 * t + 1], pts[3 * t + 2] };
+    const float q[3] = { nrm[3 * t], nrm[3 * t + 1], nrm[3 * t + 2] };
+    PlaneRec r;
+    r.nx = q[0]; r.ny = q[1]; r.nz = q[2];
+    r.c = dot3(p, q);                          // ray_depth's numerator, same operation order
+    out[t] = r;
+}
+
+// the unit direction towards the camera through pixel (i, j), ray_depth's operation order
+__device__ __forceinline__ void pixel_ray(int i, int j, float Wf, float Hf, float ff, float dir[3])
+{
+    float bx = (Wf - 2.0f * (float)i) / (2.0f * ff);
+    float by = -((Hf - 2.0f * (float)j) / (2.0f * ff));
+    float bz = 1.0f;
+    float nn = bx * bx;
+    nn = nn + by * by;
+    nn = nn + bz * bz;
+    float inv = 1.0f / sqrtf(nn);
+    dir[0] = -(bx * inv); dir[1] = -(by * inv); dir[2] = -(bz * inv);
+}
+
+__device__ __forceinline__ float plane_depth(const PlaneRec& P, const float dir[3])
+{
+    const float n[3] = { P.nx, P.ny, P.nz };
+    const float d = P.c / dot3(dir, n);
+    return (d > 0.0f) ? d : 10000.0f;
+}
+
+constexpr int kHtcMaxK = 112;          // per-lane LDS entries (9 B each): at most 63 KB per wavefront
+
+// BWD = false: the image.  BWD = true: the compositing factors (T_s, T_s (g . a_s - S_s))
+// of every sample slot, read by k_ht_backward.
+template <bool BWD>
+__global__ __launch_bounds__(64) void k_htc(long npix, int N, int W, int H, int k, int ab, float ff,
+                                            const int32_t* __restrict__ idx, const float* __restrict__ ipi,
+                                            const SplatRec* __restrict__ rec, const PlaneRec* __restrict__ pl,
+                                            const float* __restrict__ attr, const float* __restrict__ g,
+                                            float* __restrict__ out, float2* __restrict__ coef)
+{
+    extern __shared__ float lds[];
+    float* sdep = lds;                                  // [k][64] depth by slot; later T by rank
+    float* salp = lds + (long)k * 64;                   // [k][64] alpha by slot
+    uint8_t* sord = (uint8_t*)(lds + 2L * k * 64);      // [k][64] slot of each depth rank
+    const int lane = threadIdx.x;
+    const long pix = (long)xcd_remap(blockIdx.x, gridDim.x) * 64 + lane;
+    if (pix >= npix) return;
+    const long b = pix / ((long)W * H);
+    const int r = (int)(pix - b * (long)W * H);
+    const int i = r / H, j = r - (r / H) * H;
+    const SplatRec* rv = rec + b * N;
+    const PlaneRec* pv = pl + b * N;
+    const float* av = attr + 3 * (ab > 1 ? b * N : 0);
+    const float px = (float)i, py = (float)j;
+    float dir[3];
+    pixel_ray(i, j, (float)W, (float)H, ff, dir);
+    const int32_t* ip = idx + pix * k;
+    const float* qp = ipi + pix * k;
+    int n = 0;
+    for (; n < k; n++) {
+        const int id = ip[n];
+        if (id < 0) break;
+        float u, v;
+        const float alpha = fminf(ewa_eval(rv[id], px, py, u, v), 1.0f) * qp[n];
+        const float d = plane_depth(pv[id], dir);
+        sdep[n * 64 + lane] = d;
+        salp[n * 64 + lane] = alpha;
+        // stable insertion: behind every earlier slot of equal depth
+        int q = n;
+        while (q > 0) {
+            const int o = sord[(q - 1) * 64 + lane];
+            if (!(sdep[o * 64 + lane] > d)) break;
+            sord[q * 64 + lane] = (uint8_t)o;
+            q--;
+        }
+        sord[q * 64 + lane] = (uint8_t)n;
+    }
+    if (!BWD) {
+        float T = 1.0f, o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+        for (int q = 0; q < n; q++) {
+            const int s = sord[q * 64 + lane];
+            const float al = salp[s * 64 + lane];
+            const float* a = av + 3 * ip[s];
+            const float c = al * T;
+            o0 += c * a[0]; o1 += c * a[1]; o2 += c * a[2];
+            T = T * (1.0f - al);
+        }
+        out[3 * pix] = o0; out[3 * pix + 1] = o1; out[3 * pix + 2] = o2;
+        return;
+    }
+    const float g0 = g[3 * pix], g1 = g[3 * pix + 1], g2 = g[3 * pix + 2];
+    float T = 1.0f;
+    for (int q = 0; q < n; q++) {             // T in front of each rank, over the depths no longer needed
+        const int s = sord[q * 64 + lane];
+        sdep[q * 64 + lane] = T;
+        T = T * (1.0f - salp[s * 64 + lane]);
+    }
+    float S = 0.0f;
+    float2* cp = coef + pix * k;
+    for (int q = n - 1; q >= 0; q--) {
+        const int s = sord[q * 64 + lane];
+        const float al = salp[s * 64 + lane];
+        const float Tq = sdep[q * 64 + lane];
+        const float* a = av + 3 * ip[s];
+        float ga = 0.0f;
+        ga += g0 * a[0];
+        ga += g1 * a[1];
+        ga += g2 * a[2];
+        cp[s] = make_float2(Tq, Tq * (ga - S));
+        S = al * ga + (1.0f - al) * S;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -739,63 +936,67 @@ __global__ __launch_bounds__(1024) void k_filter_block(long npix, int N, int W, 
 // facing (optional, linear transforms of normals): the transformed vector
 // negated where its z component is positive, so it faces the camera
 // (render.py:200-203)
-__global__ void k_transform_forward(int affine, int bn, int vb, int N, const float* __restrict__ M,
-                                    const float* __restrict__ v, float* __restrict__ out,
-                                    float* __restrict__ facing)
+template <class T>
+__global__ void k_transform_forward(int affine, int bn, int vb, int N, const T* __restrict__ M,
+                                    const T* __restrict__ v, T* __restrict__ out, T* __restrict__ facing)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)bn * N) return;
     long b = t / N, x = t % N;
-    const float* m = M + 16 * b;
+    const T* m = M + 16 * b;
     long vo = 3 * ((vb > 1 ? b * N : 0) + x);
-    float a0 = v[vo], a1 = v[vo + 1], a2 = v[vo + 2];
-    float o[3];
+    T a0 = v[vo], a1 = v[vo + 1], a2 = v[vo + 2];
+    T o[3];
     for (int r = 0; r < 3; r++) {
-        float s = m[4 * r] * a0 + m[4 * r + 1] * a1 + m[4 * r + 2] * a2;
+        T s = m[4 * r] * a0 + m[4 * r + 1] * a1 + m[4 * r + 2] * a2;
         if (affine) s += m[4 * r + 3];
         out[3 * t + r] = s;
         o[r] = s;
     }
     if (facing) {
-        const bool flip = o[2] > 0.0f;
+        const bool flip = o[2] > T(0);
         for (int r = 0; r < 3; r++) facing[3 * t + r] = flip ? -o[r] : o[r];
     }
 }
 
+__device__ __forceinline__ float wave_sum_t(float v) { return wave_sum(v); }
+__device__ __forceinline__ double wave_sum_t(double v) { return wave_sum_d(v); }
+
 // gradient: dM[r][c] = sum_n g_r v_c (+ g_r in column 3), dv = M33^T g.
 // The 12 matrix partial sums are reduced per workgroup, then one atomic each.
-__global__ __launch_bounds__(256) void k_transform_backward(int affine, int bn, int vb, int N, const float* __restrict__ g,
-                                                            const float* __restrict__ M, const float* __restrict__ v,
-                                                            float* __restrict__ gM, float* __restrict__ gV)
+template <class T>
+__global__ __launch_bounds__(256) void k_transform_backward(int affine, int bn, int vb, int N, const T* __restrict__ g,
+                                                            const T* __restrict__ M, const T* __restrict__ v,
+                                                            T* __restrict__ gM, T* __restrict__ gV)
 {
-    __shared__ float red[4][12];
+    __shared__ T red[4][12];
     int b = blockIdx.y;
-    float acc[12];
-    for (int q = 0; q < 12; q++) acc[q] = 0.0f;
-    const float* m = M + 16 * b;
+    T acc[12];
+    for (int q = 0; q < 12; q++) acc[q] = T(0);
+    const T* m = M + 16 * b;
     // grid-stride over the points: few workgroups per view, so the 12 matrix
     // sums see few atomics (they all land on the same 12 words)
     for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += (long)gridDim.x * blockDim.x) {
         long vo = 3 * ((vb > 1 ? (long)b * N : 0) + x);
         long go = 3 * ((long)b * N + x);
-        float g0 = g[go], g1 = g[go + 1], g2 = g[go + 2];
-        float a[3] = { v[vo], v[vo + 1], v[vo + 2] };
-        float gg[3] = { g0, g1, g2 };
+        T g0 = g[go], g1 = g[go + 1], g2 = g[go + 2];
+        T a[3] = { v[vo], v[vo + 1], v[vo + 2] };
+        T gg[3] = { g0, g1, g2 };
         for (int r = 0; r < 3; r++) {
             for (int c = 0; c < 3; c++) acc[4 * r + c] += gg[r] * a[c];
             if (affine) acc[4 * r + 3] += gg[r];
         }
-        float gv[3];
+        T gv[3];
         for (int c = 0; c < 3; c++) gv[c] = m[c] * g0 + m[4 + c] * g1 + m[8 + c] * g2;
         if (vb > 1 || bn == 1) { for (int c = 0; c < 3; c++) gV[vo + c] = gv[c]; }
         else { for (int c = 0; c < 3; c++) atomicAdd(gV + vo + c, gv[c]); }
     }
-    for (int q = 0; q < 12; q++) acc[q] = wave_sum(acc[q]);
+    for (int q = 0; q < 12; q++) acc[q] = wave_sum_t(acc[q]);
     int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) for (int q = 0; q < 12; q++) red[wv][q] = acc[q];
     __syncthreads();
     if (threadIdx.x < 12) {
-        float s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        T s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
         atomicAdd(gM + 16 * b + threadIdx.x, s);
     }
 }
@@ -803,76 +1004,81 @@ __global__ __launch_bounds__(256) void k_transform_backward(int affine, int bn, 
 // ---------------------------------------------------------------------------
 // Interpolation (interpolationCuda.cu:5-129).
 // ---------------------------------------------------------------------------
+template <class T>
 __global__ void k_interp_forward(long npix, int W, int H, int k, int ab, int N, int C, int vec4,
-                                 const int32_t* __restrict__ idx, const float* __restrict__ wts,
-                                 const float* __restrict__ attr, float* __restrict__ out)
+                                 const int32_t* __restrict__ idx, const T* __restrict__ wts,
+                                 const T* __restrict__ attr, T* __restrict__ out)
 {
     long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     long b = pix / ((long)W * H);
     const long abase = ab > 1 ? b * N : 0;
     if (C == 3) {
-        float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
-        auto add = [&](int id, float w) {
+        T o0 = T(0), o1 = T(0), o2 = T(0);
+        auto add = [&](int id, T w) {
             if (id < 0) return;
-            const float* a = attr + 3 * (abase + id);
+            const T* a = attr + 3 * (abase + id);
             o0 += w * a[0]; o1 += w * a[1]; o2 += w * a[2];
         };
-        if (vec4) {
-            // a pixel's k slots are contiguous and 16-byte aligned: read them four at a time
-            const int4* ip = (const int4*)(idx + pix * k);
-            const float4* wp = (const float4*)(wts + pix * k);
-            for (int s = 0; s < (k >> 2); s++) {
-                const int4 i4 = ip[s];
-                const float4 w4 = wp[s];
-                add(i4.x, w4.x); add(i4.y, w4.y); add(i4.z, w4.z); add(i4.w, w4.w);
+        if constexpr (sizeof(T) == 4) {
+            if (vec4) {
+                // a pixel's k slots are contiguous and 16-byte aligned: read them four at a time
+                const int4* ip = (const int4*)(idx + pix * k);
+                const float4* wp = (const float4*)(wts + pix * k);
+                for (int s = 0; s < (k >> 2); s++) {
+                    const int4 i4 = ip[s];
+                    const float4 w4 = wp[s];
+                    add(i4.x, w4.x); add(i4.y, w4.y); add(i4.z, w4.z); add(i4.w, w4.w);
+                }
+                out[3 * pix] = o0; out[3 * pix + 1] = o1; out[3 * pix + 2] = o2;
+                return;
             }
-        } else {
-            for (int s = 0; s < k; s++) add(idx[pix * k + s], wts[pix * k + s]);
         }
+        for (int s = 0; s < k; s++) add(idx[pix * k + s], wts[pix * k + s]);
         out[3 * pix] = o0; out[3 * pix + 1] = o1; out[3 * pix + 2] = o2;
         return;
     }
-    for (int c = 0; c < C; c++) out[(long)C * pix + c] = 0.0f;
+    for (int c = 0; c < C; c++) out[(long)C * pix + c] = T(0);
     for (int s = 0; s < k; s++) {
         int id = idx[pix * k + s];
         if (id < 0) continue;
-        float w = wts[pix * k + s];
-        const float* a = attr + (long)C * (abase + id);
+        T w = wts[pix * k + s];
+        const T* a = attr + (long)C * (abase + id);
         for (int c = 0; c < C; c++) out[(long)C * pix + c] += w * a[c];
     }
 }
 
+template <class T>
 __global__ __launch_bounds__(256) void k_interp_backward(long nslots, int W, int H, int k, int ab, int N, int C,
-                                                         const float* __restrict__ g,
+                                                         const T* __restrict__ g,
                                                          const int32_t* __restrict__ idx,
-                                                         const float* __restrict__ wts,
-                                                         const float* __restrict__ attr, float* __restrict__ gW,
-                                                         float* __restrict__ gA)
+                                                         const T* __restrict__ wts,
+                                                         const T* __restrict__ attr, T* __restrict__ gW,
+                                                         T* __restrict__ gA)
 {
     // three-channel attributes: the wavefront adds the slots' g w with three
     // consecutive lanes per attribute row (21 rows per atomic instruction)
-    __shared__ float sv[4][64][3];
+    __shared__ T sv[4][64][3];
     __shared__ long sa[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long ao = -1;
-    float v3[3] = { 0.0f, 0.0f, 0.0f };
+    T v3[3] = { T(0), T(0), T(0) };
     if (t < nslots) {
         const int id = idx[t];
-        if (id < 0) gW[t] = 0.0f;
+        if (id < 0) gW[t] = T(0);
         else {
             long pix = t / k;
             long b = pix / ((long)W * H);
             ao = (long)C * ((ab > 1 ? b * N : 0) + id);
-            float w = wts[t];
-            float acc = 0.0f;
+            T w = wts[t];
+            T acc = T(0);
             for (int c = 0; c < C; c++) {
-                float gc = g[(long)C * pix + c];
+                T gc = g[(long)C * pix + c];
                 acc += gc * attr[ao + c];
-                const float v = gc * w;
+                const T v = gc * w;
                 if (C == 3) v3[c] = v;
-                else if (v != 0.0f) atomicAdd(gA + ao + c, v);      // zero terms (occluded slots) add nothing
+                else if (v != T(0)) atomicAdd(gA + ao + c, v);      // zero terms (occluded slots) add nothing
             }
             gW[t] = acc;
         }
@@ -886,8 +1092,8 @@ __global__ __launch_bounds__(256) void k_interp_backward(long nslots, int W, int
         const int slot = q + sub;
         if (lane < 63 && slot < 64) {
             const long a = sa[wv][slot];
-            const float x = sv[wv][slot][c];
-            if (a >= 0 && x != 0.0f) atomicAdd(gA + a + c, x);
+            const T x = sv[wv][slot][c];
+            if (a >= 0 && x != T(0)) atomicAdd(gA + a + c, x);
         }
     }
 }
@@ -895,91 +1101,95 @@ __global__ __launch_bounds__(256) void k_interp_backward(long nslots, int W, int
 // ---------------------------------------------------------------------------
 // Shading (shadingCuda.cu, diffuseShading*.cuh).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void sh_matrix(const float* c, float M[16])
+template <class T>
+__device__ __forceinline__ void sh_matrix(const T* c, T M[16])
 {
-    const float c1 = 0.429043f, c2 = 0.511664f, c3 = 0.743125f, c4 = 0.886227f, c5 = 0.247708f;
+    const T c1 = T(0.429043), c2 = T(0.511664), c3 = T(0.743125), c4 = T(0.886227), c5 = T(0.247708);
     M[0] = c1 * c[8];  M[1] = c1 * c[4];  M[2] = c1 * c[7];  M[3] = c2 * c[3];
     M[4] = c1 * c[4];  M[5] = -c1 * c[8]; M[6] = c1 * c[5];  M[7] = c2 * c[1];
     M[8] = c1 * c[7];  M[9] = c1 * c[5];  M[10] = c3 * c[6]; M[11] = c2 * c[2];
     M[12] = c2 * c[3]; M[13] = c2 * c[1]; M[14] = c2 * c[2]; M[15] = c4 * c[0] - c5 * c[6];
 }
 
-__device__ __forceinline__ float sh_n4(const float* n, float n4[4])
+template <class T>
+__device__ __forceinline__ T sh_n4(const T* n, T n4[4])
 {
-    float nn = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    if (nn > 1e-5f) { float inv = 1.0f / nn; n4[0] = n[0] * inv; n4[1] = n[1] * inv; n4[2] = n[2] * inv; n4[3] = 1.0f; }
-    else { n4[0] = n4[1] = n4[2] = 0.0f; n4[3] = 1.0f; }
+    T nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (nn > T(1e-5)) { T inv = T(1) / nn; n4[0] = n[0] * inv; n4[1] = n[1] * inv; n4[2] = n[2] * inv; n4[3] = T(1); }
+    else { n4[0] = n4[1] = n4[2] = T(0); n4[3] = T(1); }
     return nn;
 }
 
-__global__ void k_sh_forward(long npix, int W, int H, int cb, const float* __restrict__ coef,
-                             const float* __restrict__ nrm, const float* __restrict__ diff, float* __restrict__ out)
+template <class T>
+__global__ void k_sh_forward(long npix, int W, int H, int cb, const T* __restrict__ coef,
+                             const T* __restrict__ nrm, const T* __restrict__ diff, T* __restrict__ out)
 {
     long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     long b = pix / ((long)W * H);
-    float n4[4];
+    T n4[4];
     sh_n4(nrm + 3 * pix, n4);
     for (int ch = 0; ch < 3; ch++) {
-        float M[16];
+        T M[16];
         sh_matrix(coef + 9 * (cb > 1 ? ch + 3 * b : ch), M);
-        float res = 0.0f;
+        T res = T(0);
         for (int rr = 0; rr < 4; rr++) {
-            float y = n4[0] * M[rr] + n4[1] * M[4 + rr] + n4[2] * M[8 + rr] + n4[3] * M[12 + rr];
+            T y = n4[0] * M[rr] + n4[1] * M[4 + rr] + n4[2] * M[8 + rr] + n4[3] * M[12 + rr];
             res += y * n4[rr];
         }
-        out[3 * pix + ch] = diff[3 * pix + ch] * res / (float)UD_PI;
+        out[3 * pix + ch] = diff[3 * pix + ch] * res / T(UD_PI);
     }
 }
 
 // backwardSh keeps the reference's per-component normal gradient
 // (shadingCuda.cu:283 pairs colour channel i with normal component i).
-__global__ __launch_bounds__(256) void k_sh_backward(long npix, int W, int H, int cb, const float* __restrict__ g,
-                                                     const float* __restrict__ coef, const float* __restrict__ nrm,
-                                                     const float* __restrict__ diff, float* __restrict__ gCoef,
-                                                     float* __restrict__ gN, float* __restrict__ gD)
+template <class T>
+__global__ __launch_bounds__(256) void k_sh_backward(long npix, int W, int H, int cb, const T* __restrict__ g,
+                                                     const T* __restrict__ coef, const T* __restrict__ nrm,
+                                                     const T* __restrict__ diff, T* __restrict__ gCoef,
+                                                     T* __restrict__ gN, T* __restrict__ gD)
 {
-    __shared__ float red[4][27];
-    const float c1 = 0.429043f, c2 = 0.511664f, c3 = 0.743125f, c4 = 0.886227f, c5 = 0.247708f;
+    __shared__ T red[4][27];
+    const T c1 = T(0.429043), c2 = T(0.511664), c3 = T(0.743125), c4 = T(0.886227), c5 = T(0.247708);
     long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
     // one workgroup never spans two views when W*H % 256 == 0; otherwise fall back to per-lane atomics
     const long WH = (long)W * H;
     long b0 = ((long)blockIdx.x * blockDim.x) / WH;
     long bl = ((long)blockIdx.x * blockDim.x + blockDim.x - 1) / WH;
     bool uniformView = (b0 == bl) || cb == 1;
-    float acc[27];
-    for (int q = 0; q < 27; q++) acc[q] = 0.0f;
+    T acc[27];
+    for (int q = 0; q < 27; q++) acc[q] = T(0);
     long b = pix < npix ? pix / WH : b0;
     if (pix < npix) {
-        const float* n = nrm + 3 * pix;
-        float n4[4];
-        float nn = sh_n4(n, n4);
-        float Gn[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (nn > 1e-5f) {
-            float inv = 1.0f / nn, inv3 = inv * inv * inv;
+        const T* n = nrm + 3 * pix;
+        T n4[4];
+        T nn = sh_n4(n, n4);
+        T Gn[9] = { T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0) };
+        if (nn > T(1e-5)) {
+            T inv = T(1) / nn, inv3 = inv * inv * inv;
             for (int r = 0; r < 3; r++)
-                for (int c = 0; c < 3; c++) Gn[3 * r + c] = (r == c ? inv : 0.0f) - n[r] * n[c] * inv3;
+                for (int c = 0; c < 3; c++) Gn[3 * r + c] = (r == c ? inv : T(0)) - n[r] * n[c] * inv3;
         }
         for (int ch = 0; ch < 3; ch++) {
-            float M[16];
+            T M[16];
             sh_matrix(coef + 9 * (cb > 1 ? ch + 3 * b : ch), M);
-            float res = 0.0f, gN4[4];
+            T res = T(0), gN4[4];
             for (int rr = 0; rr < 4; rr++) {
-                float y = n4[0] * M[rr] + n4[1] * M[4 + rr] + n4[2] * M[8 + rr] + n4[3] * M[12 + rr];
+                T y = n4[0] * M[rr] + n4[1] * M[4 + rr] + n4[2] * M[8 + rr] + n4[3] * M[12 + rr];
                 res += y * n4[rr];
             }
             for (int rr = 0; rr < 4; rr++) {
-                gN4[rr] = 0.0f;
+                gN4[rr] = T(0);
                 for (int c = 0; c < 4; c++) gN4[rr] += (M[4 * rr + c] + M[4 * c + rr]) * n4[c];
             }
-            float gnv = Gn[3 * ch] * gN4[0] + Gn[3 * ch + 1] * gN4[1] + Gn[3 * ch + 2] * gN4[2];
-            float gi = g[3 * pix + ch], di = diff[3 * pix + ch];
-            gD[3 * pix + ch] = gi * res / (float)UD_PI;
-            gN[3 * pix + ch] = gi * di * gnv / (float)UD_PI;
-            float sc = gi * di / (float)UD_PI;
-            float gM[16];
+            T gnv = Gn[3 * ch] * gN4[0] + Gn[3 * ch + 1] * gN4[1] + Gn[3 * ch + 2] * gN4[2];
+            T gi = g[3 * pix + ch], di = diff[3 * pix + ch];
+            gD[3 * pix + ch] = gi * res / T(UD_PI);
+            gN[3 * pix + ch] = gi * di * gnv / T(UD_PI);
+            T sc = gi * di / T(UD_PI);
+            T gM[16];
             for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) gM[4 * r + c] = n4[r] * n4[c];
-            float gc[9];
+            T gc[9];
             gc[0] = c4 * gM[15];
             gc[1] = c2 * (gM[7] + gM[13]);
             gc[2] = c2 * (gM[14] + gM[11]);
@@ -998,63 +1208,65 @@ __global__ __launch_bounds__(256) void k_sh_backward(long npix, int W, int H, in
                 for (int c = 0; c < 9; c++) atomicAdd(gCoef + 9 * (cb > 1 ? ch + 3 * b : ch) + c, acc[9 * ch + c]);
         return;
     }
-    for (int q = 0; q < 27; q++) acc[q] = wave_sum(acc[q]);
+    for (int q = 0; q < 27; q++) acc[q] = wave_sum_t(acc[q]);
     int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) for (int q = 0; q < 27; q++) red[wv][q] = acc[q];
     __syncthreads();
     if (threadIdx.x < 27) {
         int q = threadIdx.x, ch = q / 9, c = q % 9;
-        float s = red[0][q] + red[1][q] + red[2][q] + red[3][q];
+        T s = red[0][q] + red[1][q] + red[2][q] + red[3][q];
         atomicAdd(gCoef + 9 * (cb > 1 ? ch + 3 * b0 : ch) + c, s);
     }
 }
 
-__global__ void k_lambert_forward(long npix, int W, int H, int L, int ib, const float* __restrict__ dirs,
-                                  const float* __restrict__ ints, const float* __restrict__ nrm,
-                                  const float* __restrict__ diff, float* __restrict__ out)
+template <class T>
+__global__ void k_lambert_forward(long npix, int W, int H, int L, int ib, const T* __restrict__ dirs,
+                                  const T* __restrict__ ints, const T* __restrict__ nrm,
+                                  const T* __restrict__ diff, T* __restrict__ out)
 {
     long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     long b = pix / ((long)W * H);
-    const float* n = nrm + 3 * pix;
-    const float* d = diff + 3 * pix;
-    float acc[3] = { 0.0f, 0.0f, 0.0f };
+    const T* n = nrm + 3 * pix;
+    const T* d = diff + 3 * pix;
+    T acc[3] = { T(0), T(0), T(0) };
     for (int l = 0; l < L; l++) {
-        const float* dir = dirs + 3 * (b * L + l);
-        const float* I = ints + 3 * ((ib > 1 ? b * L : 0) + l);
-        float c = dir[0] * n[0] + dir[1] * n[1] + dir[2] * n[2];
-        c = fmaxf(0.0f, c);
-        for (int ch = 0; ch < 3; ch++) acc[ch] += d[ch] / (float)UD_PI * I[ch] * c;
+        const T* dir = dirs + 3 * (b * L + l);
+        const T* I = ints + 3 * ((ib > 1 ? b * L : 0) + l);
+        T c = dir[0] * n[0] + dir[1] * n[1] + dir[2] * n[2];
+        c = fmax(T(0), c);
+        for (int ch = 0; ch < 3; ch++) acc[ch] += d[ch] / T(UD_PI) * I[ch] * c;
     }
     for (int ch = 0; ch < 3; ch++) out[3 * pix + ch] = acc[ch];
 }
 
-__global__ void k_lambert_backward(long npix, int W, int H, int L, int ib, const float* __restrict__ g,
-                                   const float* __restrict__ dirs, const float* __restrict__ ints,
-                                   const float* __restrict__ nrm, const float* __restrict__ diff,
-                                   float* __restrict__ gDir, float* __restrict__ gInt,
-                                   float* __restrict__ gN, float* __restrict__ gD)
+template <class T>
+__global__ void k_lambert_backward(long npix, int W, int H, int L, int ib, const T* __restrict__ g,
+                                   const T* __restrict__ dirs, const T* __restrict__ ints,
+                                   const T* __restrict__ nrm, const T* __restrict__ diff,
+                                   T* __restrict__ gDir, T* __restrict__ gInt,
+                                   T* __restrict__ gN, T* __restrict__ gD)
 {
     long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= npix) return;
     long b = pix / ((long)W * H);
-    const float* n = nrm + 3 * pix;
-    const float* d = diff + 3 * pix;
-    const float* gi = g + 3 * pix;
-    float gn[3] = { 0, 0, 0 }, gd[3] = { 0, 0, 0 };
+    const T* n = nrm + 3 * pix;
+    const T* d = diff + 3 * pix;
+    const T* gi = g + 3 * pix;
+    T gn[3] = { T(0), T(0), T(0) }, gd[3] = { T(0), T(0), T(0) };
     for (int l = 0; l < L; l++) {
-        const float* dir = dirs + 3 * (b * L + l);
+        const T* dir = dirs + 3 * (b * L + l);
         long io = 3 * ((ib > 1 ? b * L : 0) + l);
-        const float* I = ints + io;
-        float c = dir[0] * n[0] + dir[1] * n[1] + dir[2] * n[2];
-        if (!(c > 0.0f)) continue;
-        float dco[3] = { d[0] * I[0], d[1] * I[1], d[2] * I[2] };
-        float gdc = gi[0] * dco[0] + gi[1] * dco[1] + gi[2] * dco[2];
-        float sc = c / (float)UD_PI;
+        const T* I = ints + io;
+        T c = dir[0] * n[0] + dir[1] * n[1] + dir[2] * n[2];
+        if (!(c > T(0))) continue;
+        T dco[3] = { d[0] * I[0], d[1] * I[1], d[2] * I[2] };
+        T gdc = gi[0] * dco[0] + gi[1] * dco[1] + gi[2] * dco[2];
+        T sc = c / T(UD_PI);
         for (int ch = 0; ch < 3; ch++) {
-            atomicAdd(gDir + 3 * (b * L + l) + ch, gdc * n[ch] / (float)UD_PI);
+            atomicAdd(gDir + 3 * (b * L + l) + ch, gdc * n[ch] / T(UD_PI));
             atomicAdd(gInt + io + ch, gi[ch] * d[ch] * sc);
-            gn[ch] += gdc * dir[ch] / (float)UD_PI;
+            gn[ch] += gdc * dir[ch] / T(UD_PI);
             gd[ch] += gi[ch] * I[ch] * sc;
         }
     }
@@ -1062,6 +1274,13 @@ __global__ void k_lambert_backward(long npix, int W, int H, int L, int ib, const
 }
 
 __global__ void k_fill_f32(float* __restrict__ p, long n, float v)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+template <class T>
+__global__ void k_fill(T* __restrict__ p, long n, T v)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -1075,6 +1294,154 @@ inline int launch_status()
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
 }
 
+// ---------------------------------------------------------------------------
+// Host bodies of the per-slot ops, shared by the float entry points and the
+// float64 (and, for filtering, half) ones the reference's dispatch also accepts.
+// ---------------------------------------------------------------------------
+template <class T>
+size_t splat_ws_bytes(int bn, int N)
+{
+    return (size_t)bn * N * (sizeof(SplatRecT<T>) + kMomRec * sizeof(T)) + 256;
+}
+
+template <class T>
+int splat_forward_t(int bn, int N, int W, int H, int k, double f, double smoothing, const int32_t* idx,
+                    const T* pts, const T* nrm, const T* sd, int sdBatch, T* out, void* ws, size_t wsBytes,
+                    hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
+    long nslots = (long)bn * W * H * k;
+    if (N == 0) {
+        k_fill<T><<<grid1(nslots, 256), 256, 0, st>>>(out, nslots, T(0));
+        return launch_status();
+    }
+    if (wsBytes < splat_ws_bytes<T>(bn, N)) return UDE_ERR_WORKSPACE;
+    SplatRecT<T>* rec = (SplatRecT<T>*)ws;
+    k_splat_prepare<T><<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
+                                                                 rec);
+    k_splat_forward<T><<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, rec, out);
+    return launch_status();
+}
+
+template <class T>
+int splat_backward_t(int bn, int N, int W, int H, int k, double f, double smoothing, const T* gradIn,
+                     const int32_t* idx, const T* pts, const T* nrm, const T* sd, int sdBatch, T* gPts, T* gNrm,
+                     T* gSd, void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    if (wsBytes < splat_ws_bytes<T>(bn, N)) return UDE_ERR_WORKSPACE;
+    long nslots = (long)bn * W * H * k;
+    SplatRecT<T>* rec = (SplatRecT<T>*)ws;
+    T* mom = (T*)(rec + (long)bn * N);
+    k_splat_prepare<T><<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
+                                                                 rec);
+    k_fill<T><<<grid1((long)kMomRec * bn * N, 256), 256, 0, st>>>(mom, (long)kMomRec * bn * N, T(0));
+    k_splat_moments<T><<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, gradIn, rec, mom);
+    k_splat_finalize<T><<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
+                                                                  rec, mom, kMomRec, 1, nullptr, gPts, gNrm, gSd);
+    return launch_status();
+}
+
+template <bool BWD, class S>
+int filter_t(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const S* wts, const S* gin,
+             const S* pts, const S* nrm, int pb, S* out, hipStream_t st)
+{
+    using C = typename CompOf<S>::type;
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || k > 1024 || (pb != 1 && pb != bn)) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    if (k <= 64)
+        k_filter_wave<BWD, S><<<grid1(npix, 4), 256, 0, st>>>(npix, N, W, H, k, (C)f, idx, wts, gin, pts, nrm, pb,
+                                                               out);
+    else
+        k_filter_block<BWD, S><<<(unsigned)npix, ((k + 63) / 64) * 64, 0, st>>>(npix, N, W, H, k, (C)f, idx, wts,
+                                                                                 gin, pts, nrm, pb, out);
+    return launch_status();
+}
+
+template <class T>
+int transform_forward_t(int affine, int bn, int vb, int N, const T* M, const T* v, T* out, T* facing,
+                        hipStream_t st)
+{
+    if (bn <= 0 || N < 0 || (vb != 1 && vb != bn)) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    k_transform_forward<T><<<grid1((long)bn * N, 256), 256, 0, st>>>(affine, bn, vb, N, M, v, out, facing);
+    return launch_status();
+}
+
+template <class T>
+int transform_backward_t(int affine, int bn, int vb, int N, const T* g, const T* M, const T* v, T* gM, T* gV,
+                         hipStream_t st)
+{
+    if (bn <= 0 || N < 0 || (vb != 1 && vb != bn)) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    const unsigned blocks = grid1(N, 256) < 512u ? grid1(N, 256) : 512u;
+    dim3 grid(blocks, bn);
+    k_transform_backward<T><<<grid, 256, 0, st>>>(affine, bn, vb, N, g, M, v, gM, gV);
+    return launch_status();
+}
+
+template <class T>
+int interp_forward_t(int bn, int W, int H, int k, int ab, int N, int C, const int32_t* idx, const T* wts,
+                     const T* attr, T* out, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0 || (ab != 1 && ab != bn)) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)wts % 16 == 0);
+    k_interp_forward<T><<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, k, ab, N, C, vec4, idx, wts, attr, out);
+    return launch_status();
+}
+
+template <class T>
+int interp_backward_t(int bn, int W, int H, int k, int ab, int N, int C, const T* g, const int32_t* idx,
+                      const T* wts, const T* attr, T* gW, T* gA, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0 || (ab != 1 && ab != bn)) return UDE_ERR_ARG;
+    long nslots = (long)bn * W * H * k;
+    k_interp_backward<T><<<grid1(nslots, 256), 256, 0, st>>>(nslots, W, H, k, ab, N, C, g, idx, wts, attr, gW, gA);
+    return launch_status();
+}
+
+template <class T>
+int sh_forward_t(int bn, int W, int H, int cb, const T* coef, const T* nrm, const T* diff, T* out, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || (cb != 1 && cb != bn)) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_sh_forward<T><<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, cb, coef, nrm, diff, out);
+    return launch_status();
+}
+
+template <class T>
+int sh_backward_t(int bn, int W, int H, int cb, const T* g, const T* coef, const T* nrm, const T* diff, T* gCoef,
+                  T* gN, T* gD, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || (cb != 1 && cb != bn)) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_sh_backward<T><<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, cb, g, coef, nrm, diff, gCoef, gN, gD);
+    return launch_status();
+}
+
+template <class T>
+int lambert_forward_t(int bn, int W, int H, int L, int ib, const T* dirs, const T* ints, const T* nrm,
+                      const T* diff, T* out, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || L < 0 || (ib != 1 && ib != bn)) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_lambert_forward<T><<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, L, ib, dirs, ints, nrm, diff, out);
+    return launch_status();
+}
+
+template <class T>
+int lambert_backward_t(int bn, int W, int H, int L, int ib, const T* g, const T* dirs, const T* ints, const T* nrm,
+                       const T* diff, T* gDir, T* gInt, T* gN, T* gD, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || L < 0 || (ib != 1 && ib != bn)) return UDE_ERR_ARG;
+    long npix = (long)bn * W * H;
+    k_lambert_backward<T><<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, L, ib, g, dirs, ints, nrm, diff, gDir, gInt,
+                                                            gN, gD);
+    return launch_status();
+}
+
 } // namespace
 
 // ===========================================================================
@@ -1082,26 +1449,21 @@ inline int launch_status()
 // ===========================================================================
 extern "C" {
 
-size_t ude_splat_workspace_bytes(int bn, int N)
-{
-    return (size_t)bn * N * (sizeof(SplatRec) + kMomRec * sizeof(float)) + 256;
-}
+size_t ude_splat_workspace_bytes(int bn, int N) { return splat_ws_bytes<float>(bn, N); }
+size_t ude_splat_workspace_bytes_f64(int bn, int N) { return splat_ws_bytes<double>(bn, N); }
 
 int ude_splat_forward(int bn, int N, int W, int H, int k, double f, double smoothing,
                       const int32_t* idx, const float* pts, const float* nrm, const float* sd, int sdBatch,
                       float* out, void* ws, size_t wsBytes, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
-    long nslots = (long)bn * W * H * k;
-    if (N == 0) {
-        k_fill_f32<<<grid1(nslots, 256), 256, 0, st>>>(out, nslots, 0.0f);
-        return launch_status();
-    }
-    if (wsBytes < ude_splat_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
-    SplatRec* rec = (SplatRec*)ws;
-    k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
-    k_splat_forward<<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, rec, out);
-    return launch_status();
+    return splat_forward_t<float>(bn, N, W, H, k, f, smoothing, idx, pts, nrm, sd, sdBatch, out, ws, wsBytes, st);
+}
+
+int ude_splat_forward_f64(int bn, int N, int W, int H, int k, double f, double smoothing,
+                          const int32_t* idx, const double* pts, const double* nrm, const double* sd, int sdBatch,
+                          double* out, void* ws, size_t wsBytes, hipStream_t st)
+{
+    return splat_forward_t<double>(bn, N, W, H, k, f, smoothing, idx, pts, nrm, sd, sdBatch, out, ws, wsBytes, st);
 }
 
 int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoothing,
@@ -1109,18 +1471,17 @@ int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoo
                        const float* sd, int sdBatch, float* gPts, float* gNrm, float* gSd,
                        void* ws, size_t wsBytes, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
-    if (N == 0) return UDE_OK;
-    if (wsBytes < ude_splat_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
-    long nslots = (long)bn * W * H * k;
-    SplatRec* rec = (SplatRec*)ws;
-    float* mom = (float*)(rec + (long)bn * N);
-    k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
-    k_fill_f32<<<grid1((long)kMomRec * bn * N, 256), 256, 0, st>>>(mom, (long)kMomRec * bn * N, 0.0f);
-    k_splat_moments<<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, gradIn, rec, mom);
-    k_splat_finalize<<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
-                                                              rec, mom, kMomRec, 1, nullptr, gPts, gNrm, gSd);
-    return launch_status();
+    return splat_backward_t<float>(bn, N, W, H, k, f, smoothing, gradIn, idx, pts, nrm, sd, sdBatch, gPts, gNrm,
+                                   gSd, ws, wsBytes, st);
+}
+
+int ude_splat_backward_f64(int bn, int N, int W, int H, int k, double f, double smoothing,
+                           const double* gradIn, const int32_t* idx, const double* pts, const double* nrm,
+                           const double* sd, int sdBatch, double* gPts, double* gNrm, double* gSd,
+                           void* ws, size_t wsBytes, hipStream_t st)
+{
+    return splat_backward_t<double>(bn, N, W, H, k, f, smoothing, gradIn, idx, pts, nrm, sd, sdBatch, gPts, gNrm,
+                                    gSd, ws, wsBytes, st);
 }
 
 int ude_covariance(int bn, int N, int W, int H, double f, double smoothing, const float* pts,
@@ -1135,72 +1496,95 @@ int ude_covariance(int bn, int N, int W, int H, double f, double smoothing, cons
 int ude_filter_forward(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
                        const float* pts, const float* nrm, int pb, float* out, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || k > 1024) return UDE_ERR_ARG;
-    long npix = (long)bn * W * H;
-    if (k <= 64)
-        k_filter_wave<false><<<grid1(npix, 4), 256, 0, st>>>(npix, N, W, H, k, (float)f, idx, wts, nullptr, pts, nrm, pb, out);
-    else
-        k_filter_block<false><<<(unsigned)npix, ((k + 63) / 64) * 64, 0, st>>>(npix, N, W, H, k, (float)f, idx, wts, nullptr, pts, nrm, pb, out);
-    return launch_status();
+    return filter_t<false, float>(bn, N, W, H, k, f, idx, wts, nullptr, pts, nrm, pb, out, st);
+}
+
+int ude_filter_forward_f64(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const double* wts,
+                           const double* pts, const double* nrm, int pb, double* out, hipStream_t st)
+{
+    return filter_t<false, double>(bn, N, W, H, k, f, idx, wts, nullptr, pts, nrm, pb, out, st);
+}
+
+int ude_filter_forward_f16(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const void* wts,
+                           const void* pts, const void* nrm, int pb, void* out, hipStream_t st)
+{
+    return filter_t<false, __half>(bn, N, W, H, k, f, idx, (const __half*)wts, nullptr, (const __half*)pts,
+                                   (const __half*)nrm, pb, (__half*)out, st);
 }
 
 int ude_filter_backward(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
                         const float* gin, const float* pts, const float* nrm, int pb, float* out, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || k > 1024) return UDE_ERR_ARG;
-    long npix = (long)bn * W * H;
-    if (k <= 64)
-        k_filter_wave<true><<<grid1(npix, 4), 256, 0, st>>>(npix, N, W, H, k, (float)f, idx, wts, gin, pts, nrm, pb, out);
-    else
-        k_filter_block<true><<<(unsigned)npix, ((k + 63) / 64) * 64, 0, st>>>(npix, N, W, H, k, (float)f, idx, wts, gin, pts, nrm, pb, out);
-    return launch_status();
+    return filter_t<true, float>(bn, N, W, H, k, f, idx, wts, gin, pts, nrm, pb, out, st);
+}
+
+int ude_filter_backward_f64(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const double* wts,
+                            const double* gin, const double* pts, const double* nrm, int pb, double* out,
+                            hipStream_t st)
+{
+    return filter_t<true, double>(bn, N, W, H, k, f, idx, wts, gin, pts, nrm, pb, out, st);
+}
+
+int ude_filter_backward_f16(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const void* wts,
+                            const void* gin, const void* pts, const void* nrm, int pb, void* out, hipStream_t st)
+{
+    return filter_t<true, __half>(bn, N, W, H, k, f, idx, (const __half*)wts, (const __half*)gin,
+                                  (const __half*)pts, (const __half*)nrm, pb, (__half*)out, st);
 }
 
 int ude_transform_forward(int affine, int bn, int vb, int N, const float* M, const float* v, float* out, hipStream_t st)
 {
-    if (bn <= 0 || N < 0 || (vb != 1 && vb != bn)) return UDE_ERR_ARG;
-    if (N == 0) return UDE_OK;
-    k_transform_forward<<<grid1((long)bn * N, 256), 256, 0, st>>>(affine, bn, vb, N, M, v, out, nullptr);
-    return launch_status();
+    return transform_forward_t<float>(affine, bn, vb, N, M, v, out, nullptr, st);
+}
+
+int ude_transform_forward_f64(int affine, int bn, int vb, int N, const double* M, const double* v, double* out,
+                              hipStream_t st)
+{
+    return transform_forward_t<double>(affine, bn, vb, N, M, v, out, nullptr, st);
 }
 
 int ude_transform_forward_facing(int bn, int vb, int N, const float* M, const float* v, float* out, float* facing,
                                  hipStream_t st)
 {
-    if (bn <= 0 || N < 0 || (vb != 1 && vb != bn) || !facing) return UDE_ERR_ARG;
-    if (N == 0) return UDE_OK;
-    k_transform_forward<<<grid1((long)bn * N, 256), 256, 0, st>>>(0, bn, vb, N, M, v, out, facing);
-    return launch_status();
+    if (!facing) return UDE_ERR_ARG;
+    return transform_forward_t<float>(0, bn, vb, N, M, v, out, facing, st);
 }
 
 int ude_transform_backward(int affine, int bn, int vb, int N, const float* g, const float* M, const float* v,
                            float* gM, float* gV, hipStream_t st)
 {
-    if (bn <= 0 || N < 0 || (vb != 1 && vb != bn)) return UDE_ERR_ARG;
-    if (N == 0) return UDE_OK;
-    const unsigned blocks = grid1(N, 256) < 512u ? grid1(N, 256) : 512u;
-    dim3 grid(blocks, bn);
-    k_transform_backward<<<grid, 256, 0, st>>>(affine, bn, vb, N, g, M, v, gM, gV);
-    return launch_status();
+    return transform_backward_t<float>(affine, bn, vb, N, g, M, v, gM, gV, st);
+}
+
+int ude_transform_backward_f64(int affine, int bn, int vb, int N, const double* g, const double* M,
+                               const double* v, double* gM, double* gV, hipStream_t st)
+{
+    return transform_backward_t<double>(affine, bn, vb, N, g, M, v, gM, gV, st);
 }
 
 int ude_interp_forward(int bn, int W, int H, int k, int ab, int N, int C, const int32_t* idx, const float* wts,
                        const float* attr, float* out, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0) return UDE_ERR_ARG;
-    long npix = (long)bn * W * H;
-    const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)wts % 16 == 0);
-    k_interp_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, k, ab, N, C, vec4, idx, wts, attr, out);
-    return launch_status();
+    return interp_forward_t<float>(bn, W, H, k, ab, N, C, idx, wts, attr, out, st);
+}
+
+int ude_interp_forward_f64(int bn, int W, int H, int k, int ab, int N, int C, const int32_t* idx, const double* wts,
+                           const double* attr, double* out, hipStream_t st)
+{
+    return interp_forward_t<double>(bn, W, H, k, ab, N, C, idx, wts, attr, out, st);
 }
 
 int ude_interp_backward(int bn, int W, int H, int k, int ab, int N, int C, const float* g, const int32_t* idx,
                         const float* wts, const float* attr, float* gW, float* gA, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0) return UDE_ERR_ARG;
-    long nslots = (long)bn * W * H * k;
-    k_interp_backward<<<grid1(nslots, 256), 256, 0, st>>>(nslots, W, H, k, ab, N, C, g, idx, wts, attr, gW, gA);
-    return launch_status();
+    return interp_backward_t<float>(bn, W, H, k, ab, N, C, g, idx, wts, attr, gW, gA, st);
+}
+
+int ude_interp_backward_f64(int bn, int W, int H, int k, int ab, int N, int C, const double* g,
+                            const int32_t* idx, const double* wts, const double* attr, double* gW, double* gA,
+                            hipStream_t st)
+{
+    return interp_backward_t<double>(bn, W, H, k, ab, N, C, g, idx, wts, attr, gW, gA, st);
 }
 
 size_t ude_ht_workspace_bytes(int bn, int N)
@@ -1227,7 +1611,7 @@ int ude_ht_forward(int bn, int N, int W, int H, int k, double f, double smoothin
     }
     if (wsBytes < ude_ht_workspace_bytes(bn, N)) return UDE_ERR_WORKSPACE;
     SplatRec* rec = (SplatRec*)ws;
-    k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+    k_splat_prepare<float><<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
     const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)ipi % 16 == 0);
     k_ht_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, N, W, H, k, ab, vec4, idx, ipi, rec, attr, out);
     return launch_status();
@@ -1244,12 +1628,81 @@ int ude_ht_backward(int bn, int N, int W, int H, int k, double f, double smoothi
     SplatRec* rec = (SplatRec*)ws;
     float* mom = ht_records(ws, bn, N);
     if (!prepared)
-        k_splat_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+        k_splat_prepare<float><<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
     k_fill_f32<<<grid1((long)kHtRec * bn * N, 256), 256, 0, st>>>(mom, (long)kHtRec * bn * N, 0.0f);
     const dim3 tiles((unsigned)(((W + kHtTile - 1) / kHtTile) * ((H + kHtTile - 1) / kHtTile)), (unsigned)bn);
     const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)ipi % 16 == 0);
-    k_ht_backward<<<tiles, kHtTile * kHtTile, 0, st>>>(N, W, H, k, ab, vec4, gradImage, idx, ipi, rec, attr, mom);
-    k_splat_finalize<<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
+    k_ht_backward<<<tiles, kHtTile * kHtTile, 0, st>>>(N, W, H, k, ab, vec4, gradImage, idx, ipi, rec, attr, nullptr,
+                                                        mom);
+    k_splat_finalize<float><<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
+                                                              rec, mom, kHtRec, ab, gAttr, gPts, gNrm, gSd);
+    return launch_status();
+}
+
+// Composited estimate: the fused estimator's workspace, then the plane records of
+// the depth order and the per-slot compositing factors of the backward
+size_t ude_htc_workspace_bytes(int bn, int N, int W, int H, int k)
+{
+    return ude_ht_workspace_bytes(bn, N) + (size_t)bn * N * sizeof(PlaneRec) +
+           (size_t)bn * W * H * k * sizeof(float2) + 256;
+}
+
+static PlaneRec* htc_planes(void* ws, int bn, int N)
+{
+    return (PlaneRec*)(ht_records(ws, bn, N) + (long)kHtRec * bn * N);
+}
+
+static int htc_check(int bn, int N, int W, int H, int k, int ab)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || k > kHtcMaxK || N < 0 || (ab != 1 && ab != bn)) return UDE_ERR_ARG;
+    return UDE_OK;
+}
+
+int ude_htc_forward(int bn, int N, int W, int H, int k, double f, double smoothing, const int32_t* idx,
+                    const float* ipi, const float* pts, const float* nrm, const float* sd, int sdBatch, int ab,
+                    const float* attr, float* out, void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (htc_check(bn, N, W, H, k, ab) != UDE_OK) return UDE_ERR_ARG;
+    const long npix = (long)bn * W * H;
+    if (N == 0) {
+        k_fill_f32<<<grid1(3 * npix, 256), 256, 0, st>>>(out, 3 * npix, 0.0f);
+        return launch_status();
+    }
+    if (wsBytes < ude_htc_workspace_bytes(bn, N, W, H, k)) return UDE_ERR_WORKSPACE;
+    SplatRec* rec = (SplatRec*)ws;
+    PlaneRec* pl = htc_planes(ws, bn, N);
+    k_splat_prepare<float><<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+    k_plane_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>((long)bn * N, pts, nrm, pl);
+    k_htc<false><<<grid1(npix, 64), 64, (size_t)k * 64 * 9, st>>>(npix, N, W, H, k, ab, (float)f, idx, ipi, rec, pl,
+                                                                  attr, nullptr, out, nullptr);
+    return launch_status();
+}
+
+int ude_htc_backward(int bn, int N, int W, int H, int k, double f, double smoothing, const float* gradImage,
+                     const int32_t* idx, const float* ipi, const float* pts, const float* nrm, const float* sd,
+                     int sdBatch, int ab, const float* attr, float* gPts, float* gNrm, float* gSd, float* gAttr,
+                     int prepared, void* ws, size_t wsBytes, hipStream_t st)
+{
+    if (htc_check(bn, N, W, H, k, ab) != UDE_OK) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    if (wsBytes < ude_htc_workspace_bytes(bn, N, W, H, k)) return UDE_ERR_WORKSPACE;
+    const long npix = (long)bn * W * H;
+    SplatRec* rec = (SplatRec*)ws;
+    float* mom = ht_records(ws, bn, N);
+    PlaneRec* pl = htc_planes(ws, bn, N);
+    float2* coef = (float2*)(pl + (long)bn * N);
+    if (!prepared) {
+        k_splat_prepare<float><<<grid1((long)bn * N, 256), 256, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch, rec);
+        k_plane_prepare<<<grid1((long)bn * N, 256), 256, 0, st>>>((long)bn * N, pts, nrm, pl);
+    }
+    k_htc<true><<<grid1(npix, 64), 64, (size_t)k * 64 * 9, st>>>(npix, N, W, H, k, ab, (float)f, idx, ipi, rec, pl,
+                                                                 attr, gradImage, nullptr, coef);
+    k_fill_f32<<<grid1((long)kHtRec * bn * N, 256), 256, 0, st>>>(mom, (long)kHtRec * bn * N, 0.0f);
+    const dim3 tiles((unsigned)(((W + kHtTile - 1) / kHtTile) * ((H + kHtTile - 1) / kHtTile)), (unsigned)bn);
+    const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)ipi % 16 == 0);
+    k_ht_backward<<<tiles, kHtTile * kHtTile, 0, st>>>(N, W, H, k, ab, vec4, gradImage, idx, ipi, rec, attr, coef,
+                                                        mom);
+    k_splat_finalize<float><<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
                                                               rec, mom, kHtRec, ab, gAttr, gPts, gNrm, gSd);
     return launch_status();
 }
@@ -1270,38 +1723,51 @@ int ude_l2_grad(long n, const float* image, const float* target, float* grad, do
 int ude_sh_forward(int bn, int W, int H, int cb, const float* coef, const float* nrm, const float* diff,
                    float* out, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0) return UDE_ERR_ARG;
-    long npix = (long)bn * W * H;
-    k_sh_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, cb, coef, nrm, diff, out);
-    return launch_status();
+    return sh_forward_t<float>(bn, W, H, cb, coef, nrm, diff, out, st);
+}
+
+int ude_sh_forward_f64(int bn, int W, int H, int cb, const double* coef, const double* nrm, const double* diff,
+                       double* out, hipStream_t st)
+{
+    return sh_forward_t<double>(bn, W, H, cb, coef, nrm, diff, out, st);
 }
 
 int ude_sh_backward(int bn, int W, int H, int cb, const float* g, const float* coef, const float* nrm,
                     const float* diff, float* gCoef, float* gN, float* gD, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0) return UDE_ERR_ARG;
-    long npix = (long)bn * W * H;
-    k_sh_backward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, cb, g, coef, nrm, diff, gCoef, gN, gD);
-    return launch_status();
+    return sh_backward_t<float>(bn, W, H, cb, g, coef, nrm, diff, gCoef, gN, gD, st);
+}
+
+int ude_sh_backward_f64(int bn, int W, int H, int cb, const double* g, const double* coef, const double* nrm,
+                        const double* diff, double* gCoef, double* gN, double* gD, hipStream_t st)
+{
+    return sh_backward_t<double>(bn, W, H, cb, g, coef, nrm, diff, gCoef, gN, gD, st);
 }
 
 int ude_lambert_forward(int bn, int W, int H, int L, int ib, const float* dirs, const float* ints,
                         const float* nrm, const float* diff, float* out, hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0 || L < 0) return UDE_ERR_ARG;
-    long npix = (long)bn * W * H;
-    k_lambert_forward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, L, ib, dirs, ints, nrm, diff, out);
-    return launch_status();
+    return lambert_forward_t<float>(bn, W, H, L, ib, dirs, ints, nrm, diff, out, st);
+}
+
+int ude_lambert_forward_f64(int bn, int W, int H, int L, int ib, const double* dirs, const double* ints,
+                            const double* nrm, const double* diff, double* out, hipStream_t st)
+{
+    return lambert_forward_t<double>(bn, W, H, L, ib, dirs, ints, nrm, diff, out, st);
 }
 
 int ude_lambert_backward(int bn, int W, int H, int L, int ib, const float* g, const float* dirs, const float* ints,
                          const float* nrm, const float* diff, float* gDir, float* gInt, float* gN, float* gD,
                          hipStream_t st)
 {
-    if (bn <= 0 || W <= 0 || H <= 0 || L < 0) return UDE_ERR_ARG;
-    long npix = (long)bn * W * H;
-    k_lambert_backward<<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, L, ib, g, dirs, ints, nrm, diff, gDir, gInt, gN, gD);
-    return launch_status();
+    return lambert_backward_t<float>(bn, W, H, L, ib, g, dirs, ints, nrm, diff, gDir, gInt, gN, gD, st);
+}
+
+int ude_lambert_backward_f64(int bn, int W, int H, int L, int ib, const double* g, const double* dirs,
+                             const double* ints, const double* nrm, const double* diff, double* gDir, double* gInt,
+                             double* gN, double* gD, hipStream_t st)
+{
+    return lambert_backward_t<double>(bn, W, H, L, ib, g, dirs, ints, nrm, diff, gDir, gInt, gN, gD, st);
 }
 
 } // extern "C"

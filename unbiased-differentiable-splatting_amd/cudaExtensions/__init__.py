@@ -8,7 +8,10 @@ Modules and functions mirror the reference bindings (setup.py:11-70):
     transform.affineForward / affineBackward / linearForward / linearBackward
     interpolation.forward / backward
     shading.forward / backward / forwardSh / backwardSh
+    pixelgather.scatter / gather
+    nearestsamples.nearest
 and, with no reference counterpart, the fused Horvitz-Thompson image estimate
 over sampling.samplePoisson's output (estimator.forward / backward).
 """
-from . import sampling, splatting, filtering, transform, interpolation, shading, estimator  # noqa: F401
+from . import sampling, splatting, filtering, transform, interpolation, shading, estimator, pixelgather, \
+    nearestsamples  # noqa: F401

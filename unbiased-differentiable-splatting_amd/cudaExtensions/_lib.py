@@ -44,6 +44,22 @@ _SIGS = {
     "ude_profile_sampler_begin": (c_int, [c_int]),
     "ude_profile_sampler_end": (c_int, [c_ptr, c_ptr]),
     "ude_covariance": (c_int, [c_int] * 4 + [c_dbl] * 2 + [c_ptr] * 3 + [c_int, c_ptr, c_ptr]),
+    "ude_splat_workspace_bytes_f64": (c_size, [c_int] * 2),
+    "ude_splat_forward_f64": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 4 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
+    "ude_splat_backward_f64": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4 + [c_size,
+                                                                                                          c_ptr]),
+    "ude_filter_forward_f64": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
+    "ude_filter_backward_f64": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
+    "ude_filter_forward_f16": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
+    "ude_filter_backward_f16": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
+    "ude_transform_forward_f64": (c_int, [c_int] * 4 + [c_ptr] * 4),
+    "ude_transform_backward_f64": (c_int, [c_int] * 4 + [c_ptr] * 6),
+    "ude_interp_forward_f64": (c_int, [c_int] * 7 + [c_ptr] * 5),
+    "ude_interp_backward_f64": (c_int, [c_int] * 7 + [c_ptr] * 7),
+    "ude_sh_forward_f64": (c_int, [c_int] * 4 + [c_ptr] * 5),
+    "ude_sh_backward_f64": (c_int, [c_int] * 4 + [c_ptr] * 8),
+    "ude_lambert_forward_f64": (c_int, [c_int] * 5 + [c_ptr] * 6),
+    "ude_lambert_backward_f64": (c_int, [c_int] * 5 + [c_ptr] * 10),
     "ude_splat_workspace_bytes": (c_size, [c_int] * 2),
     "ude_splat_forward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 4 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "ude_splat_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
@@ -58,7 +74,19 @@ _SIGS = {
     "ude_ht_forward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] * 2 + [c_ptr] * 3 + [c_size, c_ptr]),
     "ude_ht_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 6 + [c_int] * 2 + [c_ptr] * 5 + [c_int, c_ptr,
                                                                                                     c_size, c_ptr]),
+    "ude_htc_workspace_bytes": (c_size, [c_int] * 5),
+    "ude_htc_forward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] * 2 + [c_ptr] * 3 + [c_size, c_ptr]),
+    "ude_htc_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 6 + [c_int] * 2 + [c_ptr] * 5 + [c_int, c_ptr,
+                                                                                                     c_size, c_ptr]),
     "ude_l2_grad": (c_int, [ctypes.c_long] + [c_ptr] * 5),
+    "ude_pixel_scatter": (c_int, [c_int] * 6 + [c_ptr] * 4),
+    "ude_pixel_scatter_f64": (c_int, [c_int] * 6 + [c_ptr] * 4),
+    "ude_pixel_scatter_f16": (c_int, [c_int] * 6 + [c_ptr] * 4),
+    "ude_pixel_gather": (c_int, [c_int] * 6 + [c_ptr] * 4),
+    "ude_pixel_gather_f64": (c_int, [c_int] * 6 + [c_ptr] * 4),
+    "ude_nearest_workspace_bytes": (c_size, [c_int] * 2),
+    "ude_nearest_samples": (c_int, [c_int] * 6 + [c_ptr] * 5 + [c_size, c_ptr]),
+    "ude_nearest_samples_f64": (c_int, [c_int] * 6 + [c_ptr] * 5 + [c_size, c_ptr]),
     "ude_sh_forward": (c_int, [c_int] * 4 + [c_ptr] * 5),
     "ude_sh_backward": (c_int, [c_int] * 4 + [c_ptr] * 8),
     "ude_lambert_forward": (c_int, [c_int] * 5 + [c_ptr] * 6),
@@ -96,6 +124,30 @@ def stream(device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+def expect(t, name, *dims):
+    """ValueError unless ``t`` has len(dims) axes of the given sizes (None: any
+    size, a tuple: one of its sizes).  Every op checks the shapes its kernels
+    index by before it launches them, so a wrong shape raises here instead of
+    reading out of bounds on the device."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s: expected a torch.Tensor" % name)
+    ok = t.dim() == len(dims)
+    if ok:
+        for got, want in zip(t.shape, dims):
+            if want is None:
+                continue
+            if isinstance(want, tuple):
+                if int(got) not in want:
+                    ok = False
+            elif int(got) != int(want):
+                ok = False
+    if not ok:
+        fmt = ", ".join("*" if d is None else ("|".join(str(x) for x in sorted(set(d))) if isinstance(d, tuple)
+                                                 else str(d)) for d in dims)
+        raise ValueError("%s must have shape (%s), got %s" % (name, fmt, tuple(t.shape)))
+    return t
+
+
 def require_gpu(*tensors, dtype=torch.float32):
     dev = None
     for t in tensors:
@@ -108,6 +160,27 @@ def require_gpu(*tensors, dtype=torch.float32):
         elif t.device != dev:
             raise RuntimeError("all tensors must be on the same device")
     return dev
+
+
+SUFFIX = {torch.float32: "", torch.float64: "_f64", torch.float16: "_f16"}
+
+
+def reals(named, allowed=(torch.float32, torch.float64)):
+    """The common dtype of the (name, tensor) pairs, which must be one of
+    ``allowed`` (the reference dispatches float and double, filtering also
+    half), and the tensors made contiguous.  Returns (dtype, [tensors])."""
+    dt = named[0][1].dtype
+    if dt not in allowed:
+        raise TypeError("%s must be one of %s (got %s)" % (named[0][0], ", ".join(str(a) for a in allowed), dt))
+    for name, t in named[1:]:
+        if t.dtype != dt:
+            raise TypeError("%s must have the dtype of %s (%s), got %s" % (name, named[0][0], dt, t.dtype))
+    return dt, [t.contiguous() for _, t in named]
+
+
+def fn(name, dtype):
+    """The entry point of ``name`` for ``dtype`` (ude_x, ude_x_f64 or ude_x_f16)."""
+    return getattr(lib(), name + SUFFIX[dtype])
 
 
 def f32(t, name):

@@ -12,10 +12,33 @@ read.  The image equals
 
 bit for bit; the gradients equal the unfused backward ops' up to the order of
 their atomic sums.  No per-slot tensor is stored in either pass.
+
+With ``composite=True`` the samples of a pixel are alpha-composited front to back
+in the ray-depth order of filtering.forward (zSorting.cpp:24-48), each with alpha
+min(w, 1) / pi: the Poisson-rate counterpart of
+
+    interpolation.forward(indices, filtering.forward(indices, w / pi, ...), attribute)
+
+and, since inclusions are independent, an unbiased estimate of the image
+composited over every splat of the sampling support (prod (1 - alpha_t) has
+mean prod (1 - w_t)).  Its backward is the exact adjoint (no dropped terms).
 """
 import torch
 
 from . import _lib
+
+
+def check_shapes(width, height, indices, invPi, points, normals, stdDevs, attribute):
+    """indices / invPi (bn, width, height, k), points / normals (bn, N, 3), stdDevs
+    (1 | bn, N, 2), attribute (1 | bn, N, 3): the shapes the kernels index by."""
+    _lib.expect(indices, "indices", None, int(width), int(height), None)
+    _lib.expect(invPi, "invPi", *indices.shape)
+    bn = indices.shape[0]
+    _lib.expect(stdDevs, "stdDevs", (1, bn), None, 2)
+    N = stdDevs.shape[1]
+    _lib.expect(points, "points", bn, N, 3)
+    _lib.expect(normals, "normals", bn, N, 3)
+    _lib.expect(attribute, "attribute", (1, bn), N, 3)
 
 
 def _inputs(indices, invPi, points, normals, stdDevs, attribute):
@@ -32,15 +55,23 @@ def _inputs(indices, invPi, points, normals, stdDevs, attribute):
     return indices, invPi, points, normals, stdDevs, attribute
 
 
-def workspace(indices, stdDevs):
+MAX_COMPOSITE_SAMPLES = 112     # slots per pixel of the composited estimate
+
+
+def _ws_bytes(shape, N, composite):
+    bn, W, H, k = shape
+    L = _lib.lib()
+    return L.ude_htc_workspace_bytes(bn, N, W, H, k) if composite else L.ude_ht_workspace_bytes(bn, N)
+
+
+def workspace(indices, stdDevs, composite=False):
     """A workspace for forward and backward of these shapes; passing the same one to
     forward and then to backward(..., prepared=True) reuses the per-splat records."""
-    bn = indices.shape[0]
-    return _lib.workspace(_lib.lib().ude_ht_workspace_bytes(bn, stdDevs.shape[1]), indices.device)
+    return _lib.workspace(_ws_bytes(tuple(indices.shape), stdDevs.shape[1], composite), indices.device)
 
 
-def _ws(ws, bn, N, dev):
-    need = _lib.lib().ude_ht_workspace_bytes(bn, N)
+def _ws(ws, shape, N, dev, composite):
+    need = _ws_bytes(shape, N, composite)
     if ws is None:
         return _lib.workspace(need, dev)
     if ws.dtype != torch.uint8 or ws.numel() < need or ws.device != dev:
@@ -49,35 +80,43 @@ def _ws(ws, bn, N, dev):
 
 
 def forward(width, height, focallength, indices, invPi, points, normals, stdDevs, attribute, smoothing=0.0,
-            workspace=None):
-    """image (bn, W, H, 3) = sum over a pixel's samples of min(w, 1) / pi * attribute."""
+            workspace=None, composite=False):
+    """image (bn, W, H, 3) = sum over a pixel's samples of min(w, 1) / pi * attribute;
+    composite=True: front-to-back compositing of alpha = min(w, 1) / pi in ray-depth order."""
+    check_shapes(width, height, indices, invPi, points, normals, stdDevs, attribute)
     dev = _lib.require_gpu(indices, invPi, points, normals, stdDevs, attribute)
     indices, invPi, points, normals, stdDevs, attribute = _inputs(indices, invPi, points, normals, stdDevs,
                                                                   attribute)
     bn, W, H, k = indices.shape
+    if composite and k > MAX_COMPOSITE_SAMPLES:
+        raise ValueError("the composited estimate takes at most %d slots per pixel" % MAX_COMPOSITE_SAMPLES)
     N = stdDevs.shape[1]
     out = torch.empty((bn, W, H, 3), dtype=torch.float32, device=dev)
     L = _lib.lib()
-    ws = _ws(workspace, bn, N, dev)
-    rc = L.ude_ht_forward(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(indices), _lib.ptr(invPi),
-                          _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs), stdDevs.shape[0],
-                          attribute.shape[0], _lib.ptr(attribute), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
-                          _lib.stream(dev))
+    ws = _ws(workspace, tuple(indices.shape), N, dev, composite)
+    fn = L.ude_htc_forward if composite else L.ude_ht_forward
+    rc = fn(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(indices), _lib.ptr(invPi),
+            _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs), stdDevs.shape[0],
+            attribute.shape[0], _lib.ptr(attribute), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+            _lib.stream(dev))
     _lib.check(rc, "estimator.forward")
     return out
 
 
 def backward(width, height, focallength, gradImage, indices, invPi, points, normals, stdDevs, attribute,
-             smoothing=0.0, workspace=None, prepared=False):
+             smoothing=0.0, workspace=None, prepared=False, composite=False):
     """[gradPoints, gradNormals, gradStdDevs, gradAttribute] of sum(gradImage * image).
-    prepared=True: `workspace` is the one forward used on the same splats."""
+    prepared=True: `workspace` is the one forward used on the same splats.
+    composite: as in forward (the depth order held fixed)."""
+    check_shapes(width, height, indices, invPi, points, normals, stdDevs, attribute)
+    _lib.expect(gradImage, "gradImage", indices.shape[0], int(width), int(height), 3)
     dev = _lib.require_gpu(gradImage, indices, invPi, points, normals, stdDevs, attribute)
     gradImage = _lib.f32(gradImage, "gradImage")
     indices, invPi, points, normals, stdDevs, attribute = _inputs(indices, invPi, points, normals, stdDevs,
                                                                   attribute)
     bn, W, H, k = indices.shape
-    if tuple(gradImage.shape) != (bn, W, H, 3):
-        raise ValueError("gradImage must be %s, got %s" % ((bn, W, H, 3), tuple(gradImage.shape)))
+    if composite and k > MAX_COMPOSITE_SAMPLES:
+        raise ValueError("the composited estimate takes at most %d slots per pixel" % MAX_COMPOSITE_SAMPLES)
     N = stdDevs.shape[1]
     gp = torch.empty(points.shape, dtype=torch.float32, device=dev)
     gn = torch.empty(normals.shape, dtype=torch.float32, device=dev)
@@ -87,12 +126,13 @@ def backward(width, height, focallength, gradImage, indices, invPi, points, norm
     if prepared and workspace is None:
         raise ValueError("prepared=True needs the workspace forward used")
     L = _lib.lib()
-    ws = _ws(workspace, bn, N, dev)
-    rc = L.ude_ht_backward(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(gradImage),
-                           _lib.ptr(indices), _lib.ptr(invPi), _lib.ptr(points), _lib.ptr(normals),
-                           _lib.ptr(stdDevs), stdDevs.shape[0], attribute.shape[0], _lib.ptr(attribute),
-                           _lib.ptr(gp), _lib.ptr(gn), _lib.ptr(gs), _lib.ptr(ga), int(bool(prepared)), _lib.ptr(ws),
-                           ws.numel(), _lib.stream(dev))
+    ws = _ws(workspace, tuple(indices.shape), N, dev, composite)
+    fn = L.ude_htc_backward if composite else L.ude_ht_backward
+    rc = fn(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(gradImage),
+            _lib.ptr(indices), _lib.ptr(invPi), _lib.ptr(points), _lib.ptr(normals),
+            _lib.ptr(stdDevs), stdDevs.shape[0], attribute.shape[0], _lib.ptr(attribute),
+            _lib.ptr(gp), _lib.ptr(gn), _lib.ptr(gs), _lib.ptr(ga), int(bool(prepared)), _lib.ptr(ws),
+            ws.numel(), _lib.stream(dev))
     _lib.check(rc, "estimator.backward")
     return [gp, gn, gs, ga]
 

@@ -1,42 +1,55 @@
 """filtering -- per-pixel z-sort and front-to-back alpha compositing.
 
-Mirrors FMMKdTreeSampling/zSorting.cpp:24-48.
+Mirrors FMMKdTreeSampling/zSorting.cpp:24-48.  float32, float64 and float16
+weights / points are accepted, as the reference dispatches all three
+(zSortingCuda.cu:100, 220); half tensors are composited in float32.
 """
 import torch
 
 from . import _lib
 
 
+_ALLOWED = (torch.float32, torch.float64, torch.float16)
+
+
+def check_shapes(indices, weights, width, height, points, normals):
+    """indices / weights (bn, width, height, k), points / normals (1 | bn, N, 3)."""
+    _lib.expect(indices, "indices", None, int(width), int(height), None)
+    _lib.expect(weights, "weights", *indices.shape)
+    _lib.expect(points, "points", (1, indices.shape[0]), None, 3)
+    _lib.expect(normals, "normals", *points.shape)
+
+
 def forward(indices, weights, width, height, focallength, points, normals):
+    check_shapes(indices, weights, width, height, points, normals)
     dev = _lib.require_gpu(indices, weights, points, normals)
     indices = _lib.i32(indices, "indices")
-    weights = _lib.f32(weights, "weights")
-    points = _lib.f32(points, "points")
-    normals = _lib.f32(normals, "normals")
+    dt, (weights, points, normals) = _lib.reals([("weights", weights), ("points", points), ("normals", normals)],
+                                                _ALLOWED)
     bn, W, H, k = indices.shape
     if k > 1024:
         raise ValueError("filtering supports at most 1024 samples per pixel")
-    out = torch.empty(weights.shape, dtype=torch.float32, device=dev)
-    rc = _lib.lib().ude_filter_forward(bn, points.shape[1], int(width), int(height), k, float(focallength),
-                                       _lib.ptr(indices), _lib.ptr(weights), _lib.ptr(points), _lib.ptr(normals),
-                                       points.shape[0], _lib.ptr(out), _lib.stream(dev))
+    out = torch.empty(weights.shape, dtype=dt, device=dev)
+    rc = _lib.fn("ude_filter_forward", dt)(bn, points.shape[1], int(width), int(height), k, float(focallength),
+                                           _lib.ptr(indices), _lib.ptr(weights), _lib.ptr(points), _lib.ptr(normals),
+                                           points.shape[0], _lib.ptr(out), _lib.stream(dev))
     _lib.check(rc, "filtering.forward")
     return out
 
 
 def backward(indices, weights, gradIn, width, height, focallength, points, normals):
+    check_shapes(indices, weights, width, height, points, normals)
+    _lib.expect(gradIn, "gradIn", *indices.shape)
     dev = _lib.require_gpu(indices, weights, gradIn, points, normals)
     indices = _lib.i32(indices, "indices")
-    weights = _lib.f32(weights, "weights")
-    gradIn = _lib.f32(gradIn, "gradIn")
-    points = _lib.f32(points, "points")
-    normals = _lib.f32(normals, "normals")
+    dt, (weights, gradIn, points, normals) = _lib.reals([("weights", weights), ("gradIn", gradIn),
+                                                         ("points", points), ("normals", normals)], _ALLOWED)
     bn, W, H, k = indices.shape
     if k > 1024:
         raise ValueError("filtering supports at most 1024 samples per pixel")
-    out = torch.empty(weights.shape, dtype=torch.float32, device=dev)
-    rc = _lib.lib().ude_filter_backward(bn, points.shape[1], int(width), int(height), k, float(focallength),
-                                        _lib.ptr(indices), _lib.ptr(weights), _lib.ptr(gradIn), _lib.ptr(points),
-                                        _lib.ptr(normals), points.shape[0], _lib.ptr(out), _lib.stream(dev))
+    out = torch.empty(weights.shape, dtype=dt, device=dev)
+    rc = _lib.fn("ude_filter_backward", dt)(bn, points.shape[1], int(width), int(height), k, float(focallength),
+                                            _lib.ptr(indices), _lib.ptr(weights), _lib.ptr(gradIn), _lib.ptr(points),
+                                            _lib.ptr(normals), points.shape[0], _lib.ptr(out), _lib.stream(dev))
     _lib.check(rc, "filtering.backward")
     return out

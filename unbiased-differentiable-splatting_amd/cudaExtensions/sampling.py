@@ -24,10 +24,44 @@ def _next_seed():
     return s
 
 
+def _check_splats(points, normals, stdDevs):
+    """points / normals (bn, N, 3), stdDevs (1 | bn, N, 2) (samplingCuda.cu:304-320)."""
+    _lib.expect(points, "points", None, None, 3)
+    bn, N = points.shape[0], points.shape[1]
+    _lib.expect(normals, "normals", bn, N, 3)
+    _lib.expect(stdDevs, "stdDevs", (1, bn), N, 2)
+
+
+def _error_flag(errors, dev):
+    """The device word the sampler ORs its error bits into: ``errors`` (an int32
+    (1,) tensor the caller keeps, never cleared here) or a fresh zero."""
+    if errors is None:
+        return torch.zeros(1, dtype=torch.int32, device=dev)
+    _lib.require_gpu(errors)
+    if errors.dtype != torch.int32 or errors.numel() != 1 or not errors.is_contiguous() or errors.device != dev:
+        raise ValueError("errors must be a contiguous int32 tensor of one element on %s" % dev)
+    return errors
+
+
+def raise_errors(e):
+    """Raise for the error bits ``e`` of a sampling call (0: nothing)."""
+    e = int(e)
+    if e & 8:
+        raise RuntimeError("sampling.samplePoisson: a pixel's tree walk exceeded its node bound")
+    if e & 4:
+        raise RuntimeError("sampling.samplePoisson: a pixel drew more than numSamples splats (raise "
+                           "numSamples or lower the rate)")
+    if e != 0:
+        raise RuntimeError("sampling.sample: descent exceeded the 63-level trace bound")
+
+
 def sample(width, height, focallength, points, normals, colors, stdDevs, numSamples, smoothing,
-           precision=0.001, seed=None):
-    """Returns int32 indices (bn, width, height, numSamples); -1 marks an empty slot."""
+           precision=0.001, seed=None, errors=None):
+    """Returns int32 indices (bn, width, height, numSamples); -1 marks an empty slot.
+    ``errors``: optional int32 (1,) device tensor the call ORs its error bits into
+    (checked by the caller, e.g. with raise_errors(errors.item()))."""
     global last_error
+    _check_splats(points, normals, stdDevs)
     dev = _lib.require_gpu(points, normals, stdDevs)
     points = _lib.f32(points, "points")
     normals = _lib.f32(normals, "normals")
@@ -37,7 +71,7 @@ def sample(width, height, focallength, points, normals, colors, stdDevs, numSamp
     if k > 256:
         raise ValueError("numSamples > 256 is not supported")
     out = torch.empty((bn, W, H, k), dtype=torch.int32, device=dev)
-    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    err = _error_flag(errors, dev)
     L = _lib.lib()
     wsb = L.ude_sample_workspace_bytes(N, W, H, k)
     ws = _lib.workspace(wsb, dev)
@@ -53,6 +87,7 @@ def sample(width, height, focallength, points, normals, colors, stdDevs, numSamp
 
 def covariance(width, height, focallength, points, normals, stdDevs, smoothing):
     """Per-splat screen-space covariance (bn, N, 2, 2) (samplingCuda.cu:323-361)."""
+    _check_splats(points, normals, stdDevs)
     dev = _lib.require_gpu(points, normals, stdDevs)
     points = _lib.f32(points, "points")
     normals = _lib.f32(normals, "normals")
@@ -70,6 +105,7 @@ def approximateFMMWeights(width, height, focallength, points, normals, colors, s
     """Fixed-order (degree 6) Hermite FGT density image (bn, width, height):
     sum_i exp(-|q - s_i|^2 / 2h^2) / (2 pi sqrt(h)) at every pixel q
     (sampling.cpp:136-175).  ``colors`` is accepted and unused, as in the reference."""
+    _check_splats(points, normals, stdDevs)
     dev = _lib.require_gpu(points, normals, stdDevs)
     points = _lib.f32(points, "points")
     normals = _lib.f32(normals, "normals")
@@ -88,7 +124,7 @@ def approximateFMMWeights(width, height, focallength, points, normals, colors, s
 
 def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, numSamples, smoothing,
                   precision=0.001, seed=None, rate=None, tileScale=None, densities=None, seedTensor=None,
-                  batched=True):
+                  batched=True, errors=None):
     """Poisson-rate sampling with inclusion probabilities (no reference counterpart).
 
     Every visible splat i is included at pixel q independently with the
@@ -109,8 +145,13 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
     runs, so a captured graph draws fresh samples after the caller advances it.
     ``batched``: keep every view's tree resident and run each per-pixel pass as
     one launch over all views (bn times the workspace of one view), else view
-    by view."""
+    by view.  ``errors``: optional int32 (1,) device tensor the call ORs its
+    error bits into (bit 4: a pixel had more than numSamples inclusions, the
+    extra ones dropped; bit 8: a walk exceeded its node bound); it is never
+    cleared here, so one flag can collect a whole training run and be checked at
+    a point that synchronises anyway (raise_errors(errors.item()))."""
     global last_error
+    _check_splats(points, normals, stdDevs)
     dev = _lib.require_gpu(points, normals, stdDevs)
     points = _lib.f32(points, "points")
     normals = _lib.f32(normals, "normals")
@@ -126,7 +167,7 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
             raise ValueError("tileScale must have shape (bn, ceil(W/16), ceil(H/16))")
     ids = torch.empty((bn, W, H, k), dtype=torch.int32, device=dev)
     ipi = torch.empty((bn, W, H, k), dtype=torch.float32, device=dev)
-    err = torch.zeros(1, dtype=torch.int32, device=dev)
+    err = _error_flag(errors, dev)
     L = _lib.lib()
     ws = _lib.workspace(L.ude_poisson_workspace_bytes(bn if batched else 1, N, W, H), dev)
     s = _next_seed() if seed is None and seedTensor is None else int(seed or 0)
@@ -154,6 +195,7 @@ def tileFeatures(width, height, focallength, points, normals, stdDevs, smoothing
     """32 features per 16x16 pixel tile (bn, ceil(W/16), ceil(H/16), 32) for the rate
     network, from the splats, the densities returned by samplePoisson and
     (optionally) the image gradient (bn, W, H, 3)."""
+    _check_splats(points, normals, stdDevs)
     dev = _lib.require_gpu(points, normals, stdDevs, densities)
     points = _lib.f32(points, "points")
     normals = _lib.f32(normals, "normals")
@@ -182,14 +224,7 @@ def tileFeatures(width, height, focallength, points, normals, stdDevs, smoothing
 def check_errors():
     """Synchronising check of the last sample() call's device error flag."""
     if last_error is not None:
-        e = int(last_error.item())
-        if e & 8:
-            raise RuntimeError("sampling.samplePoisson: a pixel's tree walk exceeded its node bound")
-        if e & 4:
-            raise RuntimeError("sampling.samplePoisson: a pixel drew more than numSamples splats (raise "
-                               "numSamples or lower the rate)")
-        if e != 0:
-            raise RuntimeError("sampling.sample: descent exceeded the 63-level trace bound")
+        raise_errors(last_error.item())
 
 
 _PROFILED = {"sample": 0, "fmm_eval": 1, "poisson": 2}

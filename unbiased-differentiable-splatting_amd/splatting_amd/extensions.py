@@ -2,7 +2,8 @@
 
 Mirrors splatting/extensions/__init__.py (FmmSampling :104-117, AffineTransform /
 LinearTransform :124-164, EWASplatting :174-233, DepthFiltering :372-400,
-LinearInterpolation :445-484, DiffuseShading / SphericalHarmonicShading :524-581).
+PixelScatterGather :407-438, LinearInterpolation :445-484, DiffuseShading /
+SphericalHarmonicShading :524-581).
 
 ``EWASplatting`` has two gradient modes:
   * ``"reference"`` (default) reproduces the reference's post-processing: row-wise
@@ -13,7 +14,7 @@ LinearInterpolation :445-484, DiffuseShading / SphericalHarmonicShading :524-581
 """
 import torch as th
 
-from cudaExtensions import sampling, splatting, filtering, transform, interpolation, shading
+from cudaExtensions import sampling, splatting, filtering, transform, interpolation, shading, pixelgather
 
 from .utils import conditionalNormalization
 
@@ -95,6 +96,22 @@ class DepthFiltering(th.autograd.Function):
         gW = filtering.backward(indices, weights, gradIn.contiguous(), ctx.width, ctx.height, ctx.focallength,
                                 points, normals)
         return None, gW, None, None, None, None, None
+
+
+class PixelScatterGather(th.autograd.Function):
+    """values (bn, N, C) -> their rows per sampled slot (bn, W, H, k, C); the
+    backward sums the slots' gradients per splat (extensions/__init__.py:407-438)."""
+
+    @staticmethod
+    def forward(ctx, indices, values):
+        ctx.save_for_backward(indices)
+        ctx.numSources = values.shape[1]
+        return pixelgather.scatter(indices, values)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        indices, = ctx.saved_tensors
+        return None, pixelgather.gather(indices, grad_output.contiguous(), ctx.numSources)
 
 
 class LinearInterpolation(th.autograd.Function):

@@ -118,12 +118,14 @@ class PoissonFrame:
     (``scene.view`` (4, 4) or (V, 4, 4)): world positions / normals through the
     affine / linear transforms, Poisson-rate sampling of all views (optionally
     with per-tile rate multipliers), EWA weights times 1/pi, interpolated albedo
-    (shading "None", no occlusion, so the image estimate is unbiased), L2 loss
-    against ``target``, and the raw adjoints of every op.  Gradients are with
-    respect to world positions / normals, std-devs and albedo, summed over the
-    views."""
+    (shading "None"), L2 loss against ``target``, and the raw adjoints of every
+    op.  ``composite``: the samples of a pixel are z-sorted and alpha-composited
+    front to back (alpha = min(w, 1) / pi), an unbiased estimate of the image
+    composited over all splats; otherwise they are summed without occlusion.
+    Gradients are with respect to world positions / normals, std-devs and albedo,
+    summed over the views."""
 
-    def __init__(self, scene, device, rate=10.0, kcap=40, precision=0.01, target=None):
+    def __init__(self, scene, device, rate=10.0, kcap=40, precision=0.01, target=None, composite=False):
         from cudaExtensions import transform
         self.sc = scene
         self.dev = device
@@ -136,11 +138,21 @@ class PoissonFrame:
         self.S = t(scene.stddevs)[None].contiguous()
         self.A = t(scene.albedo)[None].contiguous()
         self.rate, self.kcap, self.prec = rate, kcap, precision
+        self.composite = bool(composite)
         self.target = target
         self.W, self.H, self.f = scene.width, scene.height, scene.focal
         self.dens = torch.zeros((self.V, 2, self.W, self.H), dtype=torch.float32, device=device)
         self._transform = transform
         self._ws = None              # fused-estimator workspace shared by forward and backward
+        self._ws_shape = None
+        # error bits of every draw of this frame object (ORed on the device, never
+        # cleared): a capacity overflow drops samples, so callers check it
+        self.errors = torch.zeros(1, dtype=torch.int32, device=device)
+
+    def check_errors(self):
+        """Raise if any draw so far overflowed its capacity or its walk bound (synchronises)."""
+        from cudaExtensions import sampling
+        sampling.raise_errors(self.errors.item())
 
     def camera(self):
         tr = self._transform
@@ -155,10 +167,12 @@ class PoissonFrame:
         P, Nc, nb = self.camera()
         ids, ipi = sampling.samplePoisson(self.W, self.H, self.f, P, nb, None, self.S, kcap or self.kcap, 0.0,
                                           self.prec, seed=seed, rate=rate or self.rate, tileScale=tile_scale,
-                                          densities=self.dens, seedTensor=seed_tensor)
-        if self._ws is None:
-            self._ws = estimator.workspace(ids, self.S)
-        img = estimator.forward(self.W, self.H, self.f, ids, ipi, P, Nc, self.S, self.A, workspace=self._ws)
+                                          densities=self.dens, seedTensor=seed_tensor, errors=self.errors)
+        if self._ws is None or self._ws_shape != tuple(ids.shape):
+            self._ws = estimator.workspace(ids, self.S, composite=self.composite)
+            self._ws_shape = tuple(ids.shape)
+        img = estimator.forward(self.W, self.H, self.f, ids, ipi, P, Nc, self.S, self.A, workspace=self._ws,
+                                composite=self.composite)
         self.last_ids = ids
         return img, (P, Nc, ids, ipi)
 
@@ -172,7 +186,7 @@ class PoissonFrame:
         if not with_loss:
             loss = None
         gP, gN, gS, gA = estimator.backward(self.W, self.H, self.f, g, ids, ipi, P, Nc, self.S, self.A,
-                                            workspace=self._ws, prepared=True)
+                                            workspace=self._ws, prepared=True, composite=self.composite)
         _, gPw = tr.affineBackward(gP, self.M, self.Pw)
         _, gNw = tr.linearBackward(gN, self.M, self.Nw)
         return loss, img, (gPw, gNw, gS, gA), g
@@ -264,13 +278,14 @@ def variance_targets(frame, seeds, low=0.25):
     return feats.reshape(-1, N_IN), y.float(), n.float()
 
 
-def gradient_variance(frame, seeds, tile_scale=None, rate=None, kcap=None, groups=1):
+def gradient_variance(frame, seeds, tile_scale=None, rate=None, kcap=None, groups=1, out_groups=None):
     """Mean over all parameters of the per-parameter variance of the sampled
     gradient across seeds (raw adjoints), and the mean gradient.  With groups > 1
     the seeds are split into that many consecutive groups and the median of the
     groups' values is returned: the estimator's heavy tails (a rare pair with a
     small inclusion probability and a large weight) can put one seed's gradient
-    far out, and the median of groups keeps such a seed from deciding the value."""
+    far out, and the median of groups keeps such a seed from deciding the value.
+    ``out_groups``: a list that receives the groups' values."""
     S = len(seeds)
     G = max(1, min(int(groups), S // 2))
     bounds = [S * q // G for q in range(G + 1)]
@@ -286,7 +301,20 @@ def gradient_variance(frame, seeds, tile_scale=None, rate=None, kcap=None, group
         mean = s1 / n
         vals.append(float(((s2 - n * mean * mean) / (n - 1)).clamp(min=0).mean().item()))
         total = s1 if total is None else total + s1
+    if out_groups is not None:
+        out_groups.extend(vals)
     return float(np.median(vals)), total / S
+
+
+def kappa_interval(plain_groups, learned_groups, resamples=4000, seed=0):
+    """95 % bootstrap interval of kappa = median(learned) / median(plain) over the
+    seed groups (groups resampled with replacement within each set)."""
+    rng = np.random.default_rng(seed)
+    p = np.asarray(plain_groups, np.float64)
+    q = np.asarray(learned_groups, np.float64)
+    r = [np.median(q[rng.integers(0, q.size, q.size)]) / np.median(p[rng.integers(0, p.size, p.size)])
+         for _ in range(resamples)]
+    return [float(np.percentile(r, 2.5)), float(np.percentile(r, 97.5))]
 
 
 def rate_features(frame, net, seed):
@@ -317,9 +345,11 @@ def compare_rates(frame, net, seeds, kcap_learned=None, groups=8):
     feats, counts, ts = rate_features(frame, net, base - 1)
     kcl = kcap_learned or frame.kcap * 2
     G = groups                             # median of seed groups (heavy tails)
-    v_plain, _ = gradient_variance(frame, [base + s for s in range(S)], groups=G)
-    v_rep, _ = gradient_variance(frame, [base + S + s for s in range(S)], groups=G)
-    v_learned, _ = gradient_variance(frame, [base + 2 * S + s for s in range(S)], tile_scale=ts, kcap=kcl, groups=G)
+    gp, gr, gl = [], [], []
+    v_plain, _ = gradient_variance(frame, [base + s for s in range(S)], groups=G, out_groups=gp)
+    v_rep, _ = gradient_variance(frame, [base + S + s for s in range(S)], groups=G, out_groups=gr)
+    v_learned, _ = gradient_variance(frame, [base + 2 * S + s for s in range(S)], tile_scale=ts, kcap=kcl, groups=G,
+                                     out_groups=gl)
     frame.image(base - 2)
     n_plain = float((frame.last_ids >= 0).float().sum(-1).mean().item())
     over_plain = max_count(frame.last_ids) >= frame.kcap
@@ -329,6 +359,7 @@ def compare_rates(frame, net, seeds, kcap_learned=None, groups=8):
     return {"plain_grad_variance": v_plain, "plain_grad_variance_repeat": v_rep,
             "measurement_noise": abs(v_rep / v_plain - 1.0),
             "learned_grad_variance_equal_budget": v_learned, "kappa": v_learned / v_plain,
+            "kappa_ci95": kappa_interval(gp + gr, gl),
             "samples_per_pixel_plain": n_plain, "samples_per_pixel_learned": n_learned,
             "kcap_plain": frame.kcap, "kcap_learned": kcl, "capacity_reached": over_plain or over_learned,
             "tile_scale_range": [float(ts.min().item()), float(ts.max().item())], "variance_seeds": S,
@@ -336,12 +367,12 @@ def compare_rates(frame, net, seeds, kcap_learned=None, groups=8):
             "tile_scale": ts}
 
 
-def make_target(scene, device, rate, kcap, precision=0.01, seeds=range(10_000, 10_016)):
+def make_target(scene, device, rate, kcap, precision=0.01, seeds=range(10_000, 10_016), composite=False):
     """Target image: the mean Poisson-rate estimate of the scene with perturbed centres."""
     from . import scenes
     pert = scenes.Scene(scenes.perturbed_target_positions(scene), scene.normals, scene.stddevs, scene.albedo,
                         scene.width, scene.height, scene.focal, scene.view)
-    fr = PoissonFrame(pert, device, rate=rate, kcap=kcap, precision=precision)
+    fr = PoissonFrame(pert, device, rate=rate, kcap=kcap, precision=precision, composite=composite)
     return mean_image(fr, list(seeds))
 
 

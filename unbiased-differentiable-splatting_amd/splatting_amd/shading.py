@@ -6,24 +6,18 @@ import torch as th
 from .extensions import SphericalHarmonicShading
 
 
-def _cosine(normal, light):
-    return th.clamp(th.sum(light.mul(normal), dim=-1, keepdim=True), 0.0, 1.0)
-
-
 class Lambert(th.nn.Module):
-    """Forward (per-splat) Lambert shading, shading.py:12-29."""
+    """Per-splat Lambert shading (shading.py:12-29): every splat's diffuse colour
+    times the ambient light plus, for each lamp l, its intensity I_l scaled by
+    clamp(d_l . n, 0, 1) / pi.  lampDirections (bn, L, 3), lampIntensities
+    (1 | bn, L, 3), normals / diffuse (bn, N, 3) -> (bn, N, 3)."""
 
     def forward(self, ambientLight, lampDirections, lampIntensities, normals, diffuse):
-        bn, l, _ = lampDirections.shape
-        lampDirections = lampDirections.view(bn, l, 1, 3)
-        normals = normals.unsqueeze(1)
-        ambientIrradiance = ambientLight * diffuse
-        bi, li, _ = lampIntensities.shape
-        radiance = lampIntensities.view(bi, li, 1, 3)
-        cos = _cosine(normals, lampDirections)
-        irradiance = radiance.mul(cos)
-        brdf = diffuse.unsqueeze(1) / math.pi
-        return ambientIrradiance + th.mul(irradiance, brdf).sum(dim=1)
+        # cosine of every (lamp, splat) pair: (bn, L, N)
+        cosines = th.matmul(lampDirections, normals.transpose(1, 2)).clamp(0.0, 1.0)
+        # sum_l I_l cos_l: (bn, N, 3)
+        direct = th.matmul(cosines.transpose(1, 2), lampIntensities.expand(cosines.shape[0], -1, -1))
+        return diffuse * (ambientLight + direct / math.pi)
 
 
 def _init_sh(batched):

@@ -1,47 +1,49 @@
-"""Camera and gradient helpers (reference: splatting/extensions/utils.py)."""
+"""Camera and gradient helpers with the conventions of the reference's
+splatting/extensions/utils.py (view matrix from a position and an (x, y, z, w)
+quaternion, the 'z' / '-z' forward axis, the row-wise gradient clip)."""
 import torch as th
 
 
 def normalize(x, eps=1e-4):
-    norm = x.norm(dim=-1, keepdim=True)
-    return x / (norm + eps)
+    """x / (|x| + eps) along the last axis (utils.py:7-9)."""
+    return x / x.norm(dim=-1, keepdim=True).add(eps)
+
+
+def _skew(v):
+    """(..., 3) -> (..., 3, 3) cross-product matrices [v]_x, [v]_x u = v x u."""
+    x, y, z = v.unbind(-1)
+    o = th.zeros_like(x)
+    return th.stack([th.stack([o, -z, y], -1), th.stack([z, o, -x], -1), th.stack([-y, x, o], -1)], -2)
 
 
 def world2Camera(camPositions, camQuaternion):
-    """(bn, 1, 4, 4) view matrices from positions and (x, y, z, w) quaternions
-    (utils.py:11-49, Euler-Rodrigues rotation)."""
-    bn = camPositions.shape[0]
-    M = th.zeros((bn, 1, 4, 4), dtype=camPositions.dtype, device=camPositions.device)
-    x, y, z, w = camQuaternion.split(1, dim=-1)
-    x2, y2, z2, w2 = x * x, y * y, z * z, w * w
-    xy, zw, xz, yw, yz, xw = x * y, z * w, x * z, y * w, y * z, x * w
-    M[..., 0, 0] = x2 - y2 - z2 + w2
-    M[..., 1, 0] = 2 * (xy + zw)
-    M[..., 2, 0] = 2 * (xz - yw)
-    M[..., 0, 1] = 2 * (xy - zw)
-    M[..., 1, 1] = -x2 + y2 - z2 + w2
-    M[..., 2, 1] = 2 * (yz + xw)
-    M[..., 0, 2] = 2 * (xz + yw)
-    M[..., 1, 2] = 2 * (yz - xw)
-    M[..., 2, 2] = -x2 - y2 + z2 + w2
-    M[..., :3, 3] = camPositions.unsqueeze(1)
-    M[..., 3, 3] = 1.0
-    return M
+    """(bn, 1, 4, 4) view matrices [R | t; 0 0 0 1] (utils.py:11-49).
+
+    R is the homogeneous quadratic form of the quaternion q = (v, w), v = (x, y, z):
+    R = (w^2 - |v|^2) I + 2 v v^T + 2 w [v]_x, which is the rotation of q / |q|
+    scaled by |q|^2 (the reference's Euler-Rodrigues entries, e.g.
+    R_00 = x^2 - y^2 - z^2 + w^2); t is the camera position."""
+    v = camQuaternion[..., :3]
+    w = camQuaternion[..., 3:4]
+    eye = th.eye(3, dtype=camQuaternion.dtype, device=camQuaternion.device)
+    R = ((w * w - (v * v).sum(-1, keepdim=True))[..., None] * eye + 2.0 * v[..., :, None] * v[..., None, :]
+         + 2.0 * w[..., None] * _skew(v))
+    top = th.cat([R, camPositions[..., :, None]], dim=-1)                       # (bn, 3, 4)
+    last = th.zeros_like(top[..., :1, :])
+    last[..., 0, 3] = 1.0
+    return th.cat([top, last], dim=-2).unsqueeze(1)
 
 
 def transformForward(viewMatrix, forward):
-    """'z': flip y and mirror z (utils.py:51-60); '-z': identity."""
-    if forward == "z":
-        mirror = th.eye(4, dtype=viewMatrix.dtype, device=viewMatrix.device).view(1, 1, 4, 4).clone()
-        mirror[..., 2, 2] = -1.0
-        flip = th.eye(4, dtype=viewMatrix.dtype, device=viewMatrix.device).view(1, 1, 4, 4).clone()
-        flip[..., 1, 1] = -1.0
-        viewMatrix = flip @ mirror @ viewMatrix
-    return viewMatrix
+    """forward == 'z': the camera looks along +z, so the view's y and z rows change
+    sign (the reference's flip @ mirror, utils.py:51-60); '-z' keeps the matrix."""
+    if forward != "z":
+        return viewMatrix
+    sign = th.tensor([1.0, -1.0, -1.0, 1.0], dtype=viewMatrix.dtype, device=viewMatrix.device)
+    return viewMatrix * sign[:, None]
 
 
 def conditionalNormalization(dx, radius=1.0, eps=1e-4):
-    """Row-wise clip to norm <= 1 (utils.py:147-150)."""
-    norm = dx.norm(dim=-1, keepdim=True)
-    scale = th.clamp(norm + eps, 1.0)
-    return dx.div(scale)
+    """Rows of norm above 1 - eps scaled to norm |dx| / (|dx| + eps), others kept
+    (utils.py:147-150)."""
+    return dx / dx.norm(dim=-1, keepdim=True).add(eps).clamp(min=1.0)
