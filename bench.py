@@ -1,18 +1,20 @@
-"""Benchmark: fwd+bwd frames/s of Poisson-rate splatting at equal gradient variance.
+"""Benchmark: fwd+bwd frames/s at 1M splats, 512x512.
 
-Workload: 1M surfels on 64 random ellipsoids, one 512x512 view (focal 614.4),
-precision 0.01, smoothing 0, L2 loss (sum) against a target rendered from
-centres perturbed by N(0, 0.01).  One step = affine + linear transform,
-Poisson-rate sampling with inclusion probabilities (LBVH, two fixed-order Gauss
-transform densities, thinning at rate 10, 1/pi weights), EWA weights, per-pixel
-z-sort and front-to-back alpha compositing of alpha = min(w, 1) / pi (an
-unbiased estimate of the image composited over every splat), albedo
-interpolation ("None" shading), loss, and the raw adjoints of all of them.  The
-per-tile learned rates replace the plain ones only when they reach the plain
-rates' gradient variance at a higher frame rate (the plain rates are their
-fallback); the chosen path is timed.  Next to it, the reference's own method
-(sequential Poisson sampling of k = 40 splats per pixel, z-sort / compositing,
-through the drop-in renderer) is timed on the same scene.
+Headline (`value`): the reference's frame on the config-1 workload -- 1M surfels
+on 64 random ellipsoids, one 512x512 view (focal 614.4), precision 0.01,
+smoothing 0 -- through the drop-in renderer: affine + linear transform,
+sequential Poisson sampling of k = 40 splats per pixel (LBVH, Hermite FGT,
+capacity descents, keyed draws), EWA weights, per-pixel z-sort and
+front-to-back alpha compositing, albedo interpolation ("None" shading), L2 loss
+(sum) against a target rendered from centres perturbed by N(0, 0.01), and the
+backward of every op.  Its roofline line is the sampler pair
+(k_sample_walk + k_sample_draw), the frame's dominant kernels.
+
+Extras on the same scene: the Poisson-rate path of the metric's comparison --
+Poisson-rate sampling with inclusion probabilities (rate 10, 1/pi weights),
+EWA weights, albedo interpolation without occlusion (the Horvitz-Thompson
+estimate, unbiased), L2, fwd+bwd -- with the per-tile learned rates against the
+plain ones at equal gradient variance; and the batched-view box room.
 
 Prints one JSON line (rank 0, N = 1 GPU).  Kernel and per-op device times are
 taken with HIP events on the stream the work launches on (torch's current
@@ -142,18 +144,17 @@ def algorithmic_bytes(N, P, k):
     }
 
 
-def cpu_baseline(sc, rows=None, k=40, threads=None):
-    """Oracle (C, OpenMP) on the same frame: full tree build, then sampling +
-    EWA/compositing/interpolation fwd+bwd on `rows` pixel rows (default: all;
-    fewer rows are scaled to the full image)."""
+def _reference_frame_on_host(sc, rows, k, threads):
+    """The reference's frame by the oracle (C, OpenMP, `threads` threads): the full
+    tree build, then sampling and EWA / compositing / interpolation fwd+bwd (FP64)
+    on `rows` pixel rows through the image centre.  Returns (tree seconds, seconds
+    of the per-row part)."""
     import numpy as np
     from oracle import oracle as O
-    threads = threads or min(16, os.cpu_count() or 1)
     O.set_threads(threads)
     p, n = sc.camera_space()
     nb = np.where(n[:, 2:3] > 0, -n, n).astype(np.float32)
     W, H, f = sc.width, sc.height, sc.focal
-    rows = rows or W
     i0 = W // 2 - rows // 2
     t0 = time.time()
     idx, t_tree = O.sample_view_rows(W, H, f, p, nb, sc.stddevs, k, i0, i0 + rows, 0.0, 0.01, seed=3)
@@ -169,12 +170,7 @@ def cpu_baseline(sc, rows=None, k=40, threads=None):
     gW, gA = O.interp_backward(g, idx4, dw, a4)
     gw = O.filter_backward(idx4, w, gW, W, H, f, p[None], n[None])
     O.splat_backward(W, H, f, gw, idx4, p4, n4, s4)
-    t_ops = time.time() - t1
-    frame = t_tree + (W / float(rows)) * (t_sample + t_ops)
-    return {"value": 1.0 / frame, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "oracle tree build over all %d splats (%.2fs) + sampling and EWA/compositing/"
-                      "interpolation fwd+bwd on %d of %d pixel rows (%.2fs), per-row part scaled x%g"
-                      % (sc.n, t_tree, rows, W, t_sample + t_ops, W / float(rows))}
+    return t_tree, t_sample + (time.time() - t1)
 
 
 def cpu_model():
@@ -196,39 +192,21 @@ def cpu_model():
     return "unknown"
 
 
-def _composited_frame_on_host(sc, rate, kcap, threads):
-    """The headline frame by the oracle (C, OpenMP, `threads` threads): Poisson-rate
-    sampling of every pixel, then the composited estimate, L2 gradient and its FP64
-    adjoint.  Returns (seconds for sampling, seconds for the estimate fwd + bwd)."""
-    import numpy as np
-    from oracle import oracle as O
-    O.set_threads(threads)
-    p, n = sc.camera_space()
-    nb = np.where(n[:, 2:3] > 0, -n, n).astype(np.float32)
-    W, H, f = sc.width, sc.height, sc.focal
-    t0 = time.time()
-    ids, ipi, _, _ = O.poisson_view(W, H, f, p, nb, sc.stddevs, kcap, rate, 0.0, 0.01, seed=3)
-    t_sample = time.time() - t0
-    t1 = time.time()
-    img = O.htc(W, H, f, ids[None], ipi[None].astype(np.float64), p[None], n[None], sc.stddevs[None],
-                sc.albedo[None])
-    O.htc(W, H, f, ids[None], ipi[None].astype(np.float64), p[None], n[None], sc.stddevs[None], sc.albedo[None],
-          gimg=2.0 * img)
-    return t_sample, time.time() - t1
-
-
-def cpu_baseline_poisson(sc, rate=RATE, kcap=KCAP, threads=None):
-    """Reference semantics on the host cores: the oracle's whole composited
-    Poisson-rate frame with the box's thread share and with one thread, plus the
-    FP64 direct Gaussian sums (the density the Hermite FGT approximates) at 4,096
-    random pixels over every culled splat."""
+def cpu_baseline(sc, k=40, threads=None):
+    """Reference semantics on the host cores: the oracle's whole frame (all 512
+    rows) on the box's thread share, the same frame on one thread (16 rows, the
+    per-row part scaled to the image), and the FP64 direct Gaussian sums (the
+    density the Hermite FGT approximates) at 4,096 random pixels over every culled
+    splat."""
     import math
     import numpy as np
     from oracle import oracle as O
     threads = threads or min(16, os.cpu_count() or 1)
-    ts, te = _composited_frame_on_host(sc, rate, kcap, threads)
-    ts1, te1 = _composited_frame_on_host(sc, rate, kcap, 1)
-    # direct sums: the tree's bandwidth h and culled screen positions (its leaves)
+    W = sc.width
+    tt, tr = _reference_frame_on_host(sc, W, k, threads)
+    rows1 = 16
+    tt1, tr1 = _reference_frame_on_host(sc, rows1, k, 1)
+    frame1 = tt1 + tr1 * W / float(rows1)
     p, n = sc.camera_space()
     nb = np.where(n[:, 2:3] > 0, -n, n).astype(np.float32)
     O.set_threads(threads)
@@ -246,13 +224,14 @@ def cpu_baseline_poisson(sc, rate=RATE, kcap=KCAP, threads=None):
     O.direct_sums(leaves, h, c, q[:256])
     t_ds1 = (time.time() - t0) * 16.0
     O.set_threads(threads)
-    return {"value": 1.0 / (ts + te), "unit": "frames/s", "cores": threads, "kind": "port",
-            "cpu_model": cpu_model(),
-            "sample": "the whole frame on %d threads: oracle Poisson-rate sampling of all %d splats at rate %g over "
-                      "all %dx%d pixels (%.2fs) + the composited estimate and its FP64 adjoint (%.2fs)"
-                      % (threads, sc.n, rate, sc.width, sc.height, ts, te),
-            "single_thread": {"value": 1.0 / (ts1 + te1), "unit": "frames/s", "cores": 1,
-                              "sample": "the same whole frame on 1 thread (%.2fs + %.2fs)" % (ts1, te1)},
+    return {"value": 1.0 / (tt + tr), "unit": "frames/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "sample": "the whole frame on %d threads: oracle tree build over all %d splats (%.2fs), then sequential "
+                      "Poisson sampling (k = %d) and EWA / compositing / interpolation fwd+bwd in FP64 over all "
+                      "%dx%d pixels (%.2fs)" % (threads, sc.n, tt, k, sc.width, sc.height, tr),
+            "single_thread": {"value": 1.0 / frame1, "unit": "frames/s", "cores": 1,
+                              "sample": "the same frame on 1 thread: tree build (%.2fs) + %d of %d pixel rows "
+                                        "(%.2fs) scaled x%g" % (tt1, rows1, W, tr1, W / float(rows1))},
             "direct_sums": {"pixels": 4096, "splats": int(leaves.shape[0]),
                             "gaussians": 4096 * int(leaves.shape[0]), "seconds": t_ds, "cores": threads,
                             "single_thread_seconds": t_ds1,
@@ -294,8 +273,7 @@ def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
         counter[0] += 1
         fr.step(1000 + counter[0])
 
-    rec = {"estimator": "composited Horvitz-Thompson (alpha = min(w, 1) / pi, z-sorted front-to-back compositing), "
-                        "L2 loss, raw adjoints",
+    rec = {"estimator": "Horvitz-Thompson (sum of min(w, 1) / pi times albedo, no occlusion), L2 loss, raw adjoints",
            "plain_rate": RATE, "kcap": KCAP}
     if not use_learned or not os.path.exists(L.DEFAULT_WEIGHTS):
         rec["chosen"] = "plain (%s)" % ("learned rates not compared" if not use_learned else "no trained rate network")
@@ -425,6 +403,48 @@ def batched_views(dev, steps, warmup, n_splats=2_000_000, views=16):
             "memory_gb": torch.cuda.max_memory_allocated() / 1e9}
 
 
+def poisson_extra(dev, args, sc):
+    """The Poisson-rate path of the metric's comparison on the same scene: plain
+    against learned rates at equal gradient variance, the chosen step timed, and
+    the rooflines of its two per-pixel passes."""
+    from splatting_amd import learned as L
+    target = L.make_target(sc, dev, RATE, KCAP)
+    fr = L.PoissonFrame(sc, dev, rate=RATE, kcap=KCAP, target=target)
+    print("bench: plain vs learned rates", file=sys.stderr, flush=True)
+    step, chosen, rates = poisson_paths(dev, fr, use_learned=not args.no_rates)
+    for _ in range(args.warmup):
+        step()
+    print("bench: Poisson-rate frames (%s rates)" % chosen, file=sys.stderr, flush=True)
+    ms, fe_ms, fe_n = timed_steps(step, args.steps, "fmm_eval")
+    _, po_ms, po_n = timed_steps(step, max(args.steps // 2, 1), "poisson")
+    # Algorithmic bytes per launch, n = splats in view, P = pixels:
+    #   k_fmm_eval (both bands): per band the 80 B children record of each of the n - 1
+    #     internal nodes and one 4 B density per pixel; the expansion coefficients it
+    #     reads (196 B per accepted node, a count only the device knows) are not counted,
+    #     so this is a lower bound;
+    #   k_poisson: the 32 B walk record of each of the 2n - 1 nodes, 8 B position and
+    #     4 B id per leaf, two 4 B densities per pixel and 8 B (index, 1/pi) per slot.
+    n_vis = int((L.splat_tiles(fr) >= 0).sum().item())
+    P = sc.width * sc.height
+    spp = float((fr.last_ids >= 0).sum().item()) / (P * fr.V)
+    fr.check_errors()
+    kern = {
+        "k_fmm_eval": {"kernel_ms": fe_ms / max(fe_n, 1), "kernel_launches": fe_n,
+                       "algorithmic_bytes": 2 * (80 * (n_vis - 1) + 4 * P)},
+        "k_poisson": {"kernel_ms": po_ms / max(po_n, 1), "kernel_launches": po_n,
+                      "algorithmic_bytes": 32 * (2 * n_vis - 1) + 12 * n_vis + 8 * P + 8 * P * KCAP},
+    }
+    for name, k in kern.items():
+        k["achieved"] = k["algorithmic_bytes"] / (k["kernel_ms"] * 1e-3) / 1e9
+        k["frac"] = k["achieved"] / HBM_PEAK_GBS
+        k["traffic"] = measured_traffic(TRAFFIC_SUMMARY[name])
+    return {"workload": "the same scene, Poisson-rate sampling at rate %g (%.2f samples per pixel, 1/pi weights), EWA "
+                        "weights, albedo interpolation without occlusion (Horvitz-Thompson, unbiased), L2, fwd+bwd; "
+                        "%s rates at the plain rates' gradient variance" % (RATE, spp, chosen),
+            "fps": 1000.0 / ms, "ms_per_step": ms, "samples_per_pixel": spp, "splats_in_view": n_vis,
+            "compositing": False, "rates": rates, "kernels": kern}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -433,7 +453,7 @@ def main():
     ap.add_argument("--splats", type=int, default=None, help="override the splat count (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rates", action="store_true", help="time the plain rates without the learned comparison")
-    ap.add_argument("--no-reference", action="store_true", help="skip the reference-method frame")
+    ap.add_argument("--no-poisson", action="store_true", help="skip the Poisson-rate path")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-view (box room) workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of known size (1 GiB read + 1 GiB written per launch), "
@@ -455,76 +475,30 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 3}))
         return
     from splatting_amd import scenes
-    from splatting_amd import learned as L
-    print("bench: building workload", file=sys.stderr, flush=True)
+    print("bench: reference frame", file=sys.stderr, flush=True)
     sc = scenes.config(1, args.splats)
-    target = L.make_target(sc, dev, RATE, KCAP, composite=True)
-    fr = L.PoissonFrame(sc, dev, rate=RATE, kcap=KCAP, target=target, composite=True)
-    print("bench: plain vs learned rates", file=sys.stderr, flush=True)
-    step, chosen, rates = poisson_paths(dev, fr, use_learned=not args.no_rates)
-    print("bench: warmup", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        step()
-    print("bench: timed steps (%s rates)" % chosen, file=sys.stderr, flush=True)
-    ms_per_step, fe_ms, fe_n = timed_steps(step, args.steps, "fmm_eval")
-    _, po_ms, po_n = timed_steps(step, max(args.steps // 2, 1), "poisson")
-    fps = 1000.0 / ms_per_step
-    # The two per-pixel passes dominate the frame; the roofline line is the slower one.
-    # Algorithmic bytes per launch, n = splats in view, P = pixels:
-    #   k_fmm_eval (both bands): per band the 80 B children record of each of the n - 1
-    #     internal nodes and one 4 B density per pixel; the expansion coefficients it
-    #     reads (196 B per accepted node, a count only the device knows) are not counted,
-    #     so this is a lower bound;
-    #   k_poisson: the 32 B walk record of each of the 2n - 1 nodes, 8 B position and
-    #     4 B id per leaf, two 4 B densities per pixel and 8 B (index, 1/pi) per slot.
-    n_vis = int((L.splat_tiles(fr) >= 0).sum().item())
-    P = sc.width * sc.height
-    spp = float((fr.last_ids >= 0).sum().item()) / (P * fr.V)
-    fr.check_errors()
-    kern = {
-        "k_fmm_eval": {"kernel_ms": fe_ms / max(fe_n, 1), "kernel_launches": fe_n,
-                       "algorithmic_bytes": 2 * (80 * (n_vis - 1) + 4 * P)},
-        "k_poisson": {"kernel_ms": po_ms / max(po_n, 1), "kernel_launches": po_n,
-                      "algorithmic_bytes": 32 * (2 * n_vis - 1) + 12 * n_vis + 8 * P + 8 * P * KCAP},
-    }
-    for name, k in kern.items():
-        k["achieved"] = k["algorithmic_bytes"] / (k["kernel_ms"] * 1e-3) / 1e9
-        k["traffic"] = measured_traffic(TRAFFIC_SUMMARY[name])
-    dom = max(kern, key=lambda nm: kern[nm]["kernel_ms"])
-    other = [nm for nm in kern if nm != dom][0]
-    achieved = kern[dom]["achieved"]
+    ref = reference_path(dev, args, sc.n)
+    roof = ref.pop("roofline")
     line = {
-        "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": ref["fps"], "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ref["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "1M surfels on 64 ellipsoids, one 512x512 view, Poisson-rate sampling at rate %g "
-                               "(%.2f samples per pixel, 1/pi weights), per-pixel z-sort and front-to-back alpha "
-                               "compositing of alpha = min(w, 1) / pi (unbiased for the image composited over every "
-                               "splat), albedo interpolation, L2 loss, fwd+bwd; %s rates at the plain rates' "
-                               "gradient variance" % (RATE, spp, chosen),
-                   "splats": sc.n, "splats_in_view": n_vis, "width": sc.width, "height": sc.height,
-                   "rate": RATE, "kcap": KCAP, "samples_per_pixel": spp, "compositing": True, "rates": chosen,
-                   "parallelism": "none (1 GPU)"},
-        "roofline": dict({"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK_GBS},
-                         **{k_: v for k_, v in kern[dom].items() if k_ != "achieved"},
-                         next_kernel=dict({"kernel": other, "frac": kern[other]["achieved"] / HBM_PEAK_GBS},
-                                          **kern[other])),
-        "rates": rates,
+        "config": {"workload": "1M surfels on 64 ellipsoids, one 512x512 view, the reference's frame: sequential "
+                               "Poisson sampling of k = 40 splats per pixel, EWA weights, per-pixel z-sort and "
+                               "front-to-back alpha compositing, albedo interpolation, L2 loss, fwd+bwd",
+                   "splats": sc.n, "width": sc.width, "height": sc.height, "samples_per_pixel": 40,
+                   "compositing": True, "parallelism": "none (1 GPU)"},
+        "roofline": roof,
+        "ops_ms": ref.pop("ops_ms"),
     }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(sc)
+    if not args.no_poisson:
+        line["poisson_path"] = poisson_extra(dev, args, sc)
     if not args.no_batched:
         print("bench: batched views", file=sys.stderr, flush=True)
         line["batched_views"] = batched_views(dev, max(args.steps // 2, 3), 2)
-    if not args.no_reference:
-        print("bench: reference method", file=sys.stderr, flush=True)
-        del fr
-        line["reference_path"] = reference_path(dev, args, sc.n)
-        if not args.no_cpu_baseline:
-            print("bench: cpu baseline of the reference method", file=sys.stderr, flush=True)
-            line["reference_path"]["cpu_baseline"] = cpu_baseline(sc)
-    if not args.no_cpu_baseline:
-        print("bench: cpu baseline", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline_poisson(sc)
     print(json.dumps(line))
 
 

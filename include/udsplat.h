@@ -78,7 +78,9 @@ size_t ude_poisson_workspace_bytes(int bn, int N, int W, int H);
  * splat ids (-1 padded) and 1 / pi_i (0 padded), so sum_s w_s a_s / pi_s is a
  * Horvitz-Thompson estimate of sum_i w_i a_i.  densOut (optional, (bn, 2, W, H))
  * receives D0 and D1.  A pixel with more than kcap inclusions sets bit 4 of
- * *errFlag; a walk longer than the tree (not expected) stops and sets bit 8.  seedDev (optional, device memory): the Philox key is read there
+ * *errFlag; a walk longer than the tree (not expected) stops and sets bit 8; a
+ * tile whose rate times multiplier is negative or not finite draws nothing and
+ * sets bit 16.  seedDev (optional, device memory): the Philox key is read there
  * at run time instead of from seed, so a captured HIP graph draws fresh
  * samples when the caller advances it between replays.  With a workspace of
  * ude_poisson_workspace_bytes(bn, ...) bytes all views' trees are built
@@ -194,6 +196,18 @@ int ude_filter_forward(int bn, int N, int W, int H, int k, double f, const int32
 int ude_filter_backward(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
                         const float* gin, const float* pts, const float* nrm, int pb, float* out,
                         ude_stream_t stream);
+
+/* filtering.backward(..., exact=True): the exact derivative of the composite.  The
+ * reference drops a slot's own term where |w| <= 1e-4 and its occlusion term where
+ * |1 - w| <= 1e-4 (alphacompositingGrad.cuh:7-26: it divides by w and by 1 - w);
+ * the adjoint here divides by neither, so this variant keeps every term.  Same
+ * arguments as ude_filter_backward.  Used by the raw (unclipped) gradient mode. */
+int ude_filter_backward_exact(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
+                              const float* gin, const float* pts, const float* nrm, int pb, float* out,
+                              ude_stream_t stream);
+int ude_filter_backward_exact_f64(int bn, int N, int W, int H, int k, double f, const int32_t* idx,
+                                  const double* wts, const double* gin, const double* pts, const double* nrm, int pb,
+                                  double* out, ude_stream_t stream);
 
 /* ---- transform ---------------------------------------------------------- */
 

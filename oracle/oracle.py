@@ -162,12 +162,13 @@ def filter_forward(idx, w, W, H, f, pts, nrm):
     return out
 
 
-def filter_backward(idx, w, gin, W, H, f, pts, nrm):
+def filter_backward(idx, w, gin, W, H, f, pts, nrm, exact=False):
+    """filtering.backward; exact=True: the exact derivative (no dropped terms)."""
     idx = _c(idx, np.int32); w = _c(w, np.float64); gin = _c(gin, np.float64)
     pts = _c(pts, np.float32); nrm = _c(nrm, np.float32)
     bn, _, _, k = idx.shape
     out = np.zeros(idx.shape, np.float64)
-    rc = lib().o_filter_backward(ctypes.c_int(bn), ctypes.c_int(pts.shape[1]), ctypes.c_int(W), ctypes.c_int(H),
+    rc = (lib().o_filter_backward_exact if exact else lib().o_filter_backward)(ctypes.c_int(bn), ctypes.c_int(pts.shape[1]), ctypes.c_int(W), ctypes.c_int(H),
                                  ctypes.c_int(k), ctypes.c_double(f), _p(idx), _p(w), _p(gin), _p(pts), _p(nrm),
                                  ctypes.c_int(pts.shape[0]), _p(out))
     assert rc == 0
@@ -425,6 +426,14 @@ def sample_view_rows(W, H, f, pts, nrm, sd, k, i0, i1, smoothing=0.0, precision=
     if rc != 0:
         raise RuntimeError("oracle sample_view_rows failed: %d" % rc)
     return out, tt.value
+
+
+def tie_pixels():
+    """Pixels of the last sample / sample_view(_rows) call whose sorted key list
+    met two equal keys (where the reference's insertion order differs from
+    sorting by (key, draw number))."""
+    lib().o_tie_pixels.restype = ctypes.c_long
+    return int(lib().o_tie_pixels())
 
 
 def sample(W, H, f, pts, nrm, sd, k, smoothing=0.0, precision=0.01, seed=0):

@@ -205,7 +205,7 @@ static dual ddot3(const dual a[3], const dual b[3]) { return dadd(dadd(dmul(a[0]
  * its derivatives). */
 static dual ewa_weight_dual_k(double px, double py, double W, double H, double f,
                               const double p_in[3], const double n_in[3], const double sd_in[2],
-                              double smoothing, dual cov[4], double* kappa)
+                              double smoothing, dual cov[4], double* kappa, double* uv)
 {
     dual p[3], n[3], sd[2];
     for (int i = 0; i < 3; i++) { p[i] = dvar(p_in[i], i); n[i] = dvar(n_in[i], 3 + i); }
@@ -252,14 +252,32 @@ static dual ewa_weight_dual_k(double px, double py, double W, double H, double f
     }
     dual e = ddiv(dadd(dsub(dmul(V11, dmul(dx, dx)), dscal(dmul(V01, dmul(dx, dy)), 2.0)), dmul(V00, dmul(dy, dy))), detV);
     dual norm = dmul(dscal(dsqrt(detV), 2.0 * O_PI), dabs(detA));
-    return ddiv(dexp(dscal(e, -0.5)), norm);
+    dual w = ddiv(dexp(dscal(e, -0.5)), norm);
+    if (uv) {                                 /* whitened offsets, u^2 + v^2 = e */
+        const double t00 = sqrt(V11.v / detV.v), t01 = -V01.v / sqrt(V11.v * detV.v), t11 = 1.0 / sqrt(V11.v);
+        uv[0] = t00 * dx.v + t01 * dy.v;
+        uv[1] = t11 * dy.v;
+    }
+    if (kappa && w.v > 0.0) {
+        /* plus the weight's relative condition in its inputs, c = sum_i |x_i dw/dx_i| / w,
+         * at 1/16: rounding anywhere in the chain (the plane intersection of an
+         * edge-on surfel, the inverse Jacobian) acts like relative perturbations of
+         * the inputs, and a floor of m roundoffs of kappa then admits m / 16 roundoffs
+         * per unit of c (measured: float32 chains lose under 1 unit on the edge-on
+         * surfels of the test scenes, float64 chains up to about 10) */
+        double c = 0.0;
+        for (int i = 0; i < 3; i++) c += fabs(p_in[i] * w.d[i]) + fabs(n_in[i] * w.d[3 + i]);
+        c += fabs(sd_in[0] * w.d[6]) + fabs(sd_in[1] * w.d[7]);
+        *kappa += c / (16.0 * w.v);
+    }
+    return w;
 }
 
 static dual ewa_weight_dual(double px, double py, double W, double H, double f,
                             const double p_in[3], const double n_in[3], const double sd_in[2],
                             double smoothing, dual cov[4])
 {
-    return ewa_weight_dual_k(px, py, W, H, f, p_in, n_in, sd_in, smoothing, cov, NULL);
+    return ewa_weight_dual_k(px, py, W, H, f, p_in, n_in, sd_in, smoothing, cov, NULL, NULL);
 }
 
 /* the abs-terms scale of a term carrying a float32 EWA weight: its condition */
@@ -284,10 +302,48 @@ void o_splat_forward(int bn, int N, int W, int H, int k, double f, double smooth
             const double* n = nrm + 3 * ((long)b * N + id);
             const double* sdv = sd + 2 * ((sdBatch > 1 ? (long)b * N : 0) + id);
             double kap = 1.0;
-            dual w = ewa_weight_dual_k(i, j, W, H, f, p, n, sdv, smoothing, NULL, &kap);
+            dual w = ewa_weight_dual_k(i, j, W, H, f, p, n, sdv, smoothing, NULL, &kap, NULL);
             out[o] = (w.v < 1.0 ? w.v : 1.0) * KW(kap);
         }
     }
+}
+
+/* Coefficients c[8][6] of d ln w / dx_q = sum_m c_qm m(u, v), m = (1, u, v, u^2, uv,
+ * v^2), of one splat (exact: ln w is -e/2 minus the log normaliser, e quadratic in
+ * the offset).  Solved from the exact derivatives at six offsets around the
+ * projected centre (a unisolvent set for quadratics). */
+static void o_logw_poly(int W, int H, double f, const double* p, const double* n, const double* sd,
+                        double smoothing, double* c)
+{
+    const double sx = -(p[0] * f) / p[2] + W / 2.0, sy = (p[1] * f) / p[2] + H / 2.0;
+    const double off[6][2] = { { 0, 0 }, { 0.5, 0 }, { -0.5, 0 }, { 0, 0.5 }, { 0, -0.5 }, { 0.5, 0.5 } };
+    double A[6][6], R[6][8];
+    for (int r = 0; r < 6; r++) {
+        double uv[2];
+        dual w = ewa_weight_dual_k(sx + off[r][0], sy + off[r][1], W, H, f, p, n, sd, smoothing, NULL, NULL, uv);
+        const double u = uv[0], v = uv[1];
+        A[r][0] = 1.0; A[r][1] = u; A[r][2] = v; A[r][3] = u * u; A[r][4] = u * v; A[r][5] = v * v;
+        for (int q = 0; q < 8; q++) R[r][q] = w.v > 0.0 ? w.d[q] / w.v : 0.0;
+    }
+    /* Gauss-Jordan with partial pivoting on [A | R] */
+    for (int col = 0; col < 6; col++) {
+        int piv = col;
+        for (int r = col + 1; r < 6; r++) if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+        if (piv != col) {
+            for (int t = 0; t < 6; t++) { double x = A[col][t]; A[col][t] = A[piv][t]; A[piv][t] = x; }
+            for (int t = 0; t < 8; t++) { double x = R[col][t]; R[col][t] = R[piv][t]; R[piv][t] = x; }
+        }
+        const double d = A[col][col];
+        if (d == 0.0) { for (int t = 0; t < 48; t++) c[t] = 0.0; return; }
+        for (int r = 0; r < 6; r++) {
+            if (r == col) continue;
+            const double fct = A[r][col] / d;
+            for (int t = 0; t < 6; t++) A[r][t] -= fct * A[col][t];
+            for (int t = 0; t < 8; t++) R[r][t] -= fct * R[col][t];
+        }
+    }
+    for (int q = 0; q < 8; q++)
+        for (int mm = 0; mm < 6; mm++) c[6 * q + mm] = R[mm][q] / A[mm][mm];
 }
 
 /* splatting.backward (splattingCuda.cu:114-234), FP64 exact derivatives, gated on
@@ -297,6 +353,20 @@ void o_splat_backward(int bn, int N, int W, int H, int k, double f, double smoot
                       const double* nrm, const double* sd, int sdBatch,
                       double* gPts, double* gNrm, double* gSd)
 {
+    /* abs-terms mode: the adjoint is a combination of sums over a splat's slots of
+     * g w m(u, v), m in {1, u, v, u^2, uv, v^2} (d ln w / dx_i is exactly the
+     * quadratic sum_m c_im m(u, v) in the whitened offset, with per-splat c), which
+     * is how the GPU accumulates it; those sums round by |g| w sum_m |c_im| |m|,
+     * more than |g dw/dx_i| where the polynomial cancels (a std-dev gradient's
+     * (u^2 - 1)).  c is recovered per splat from the exact derivatives at six
+     * offsets around the splat's centre. */
+    double* coef = NULL;
+    unsigned char* have = NULL;
+    if (g_abs_terms) {
+        size_t ns = (size_t)(bn > 0 ? bn : 1) * (size_t)(N > 0 ? N : 1);
+        coef = (double*)calloc(ns * 48, sizeof(double));
+        have = (unsigned char*)calloc(ns, 1);
+    }
     for (long pix = 0; pix < (long)bn * W * H; pix++) {
         int b = (int)(pix / ((long)W * H));
         long r = pix % ((long)W * H);
@@ -307,15 +377,38 @@ void o_splat_backward(int bn, int N, int W, int H, int k, double f, double smoot
             if (id < 0) continue;
             long po = 3 * ((long)b * N + id);
             long so = 2 * ((sdBatch > 1 ? (long)b * N : 0) + id);
-            double kap = 1.0;
-            dual w = ewa_weight_dual_k(i, j, W, H, f, pts + po, nrm + po, sd + so, smoothing, NULL, &kap);
+            double kap = 1.0, uv[2] = { 0.0, 0.0 };
+            dual w = ewa_weight_dual_k(i, j, W, H, f, pts + po, nrm + po, sd + so, smoothing, NULL, &kap, uv);
             if (!(w.v <= 1.0)) continue;
-            double g = gradIn[o] * KW(kap);
-            for (int c = 0; c < 3; c++) { gPts[po + c] += AT(g * w.d[c]); gNrm[po + c] += AT(g * w.d[3 + c]); }
-            gSd[so] += AT(g * w.d[6]);
-            gSd[so + 1] += AT(g * w.d[7]);
+            if (g_abs_terms) {
+                long sp = (long)b * N + id;
+                double* c = coef + 48 * sp;
+                if (!have[sp]) { o_logw_poly(W, H, f, pts + po, nrm + po, sd + so, smoothing, c); have[sp] = 1; }
+                const double u = uv[0], v = uv[1];
+                const double m[6] = { 1.0, fabs(u), fabs(v), u * u, fabs(u * v), v * v };
+                const double t = fabs(gradIn[o]) * KW(kap) * fabs(w.v);
+                double a[8];
+                for (int q = 0; q < 8; q++) {
+                    a[q] = 0.0;
+                    for (int mm = 0; mm < 6; mm++) a[q] += fabs(c[6 * q + mm]) * m[mm];
+                    a[q] = t * (a[q] > 0.0 ? a[q] : 0.0);
+                    /* never below the slot's own derivative term */
+                    const double own = fabs(gradIn[o]) * KW(kap) * fabs(w.d[q]);
+                    if (own > a[q]) a[q] = own;
+                }
+                for (int cc = 0; cc < 3; cc++) { gPts[po + cc] += a[cc]; gNrm[po + cc] += a[3 + cc]; }
+                gSd[so] += a[6];
+                gSd[so + 1] += a[7];
+                continue;
+            }
+            double g = gradIn[o];
+            for (int c = 0; c < 3; c++) { gPts[po + c] += g * w.d[c]; gNrm[po + c] += g * w.d[3 + c]; }
+            gSd[so] += g * w.d[6];
+            gSd[so + 1] += g * w.d[7];
         }
     }
+    free(coef);
+    free(have);
 }
 
 /* sampling.covariance (samplingCuda.cu:323-361), FP64. out (bn,N,2,2). */
@@ -438,6 +531,42 @@ int o_filter_backward(int bn, int N, int W, int H, int k, double f, const int32_
     return 0;
 }
 
+/* The exact derivative of the composite w.r.t. each weight (no gates), without
+ * division: in depth order d out_i / d w_r = T_r for i = r and
+ * -w_i prod_{t < i, t != r} (1 - w_t) for i > r (alphacompositing.cuh:38-68 is the
+ * forward; the reference's adjoint, restated in o_filter_backward, drops terms). */
+int o_filter_backward_exact(int bn, int N, int W, int H, int k, double f, const int32_t* idx,
+                            const double* w, const double* gin, const float* pts, const float* nrm,
+                            int pb, double* gout)
+{
+    if (k > 1024) return -1;
+    float ff = (float)f;
+    #pragma omp parallel for schedule(static)
+    for (long pix = 0; pix < (long)bn * W * H; pix++) {
+        int b = (int)(pix / ((long)W * H));
+        long r = pix % ((long)W * H);
+        int i = (int)(r / H), j = (int)(r % H);
+        int order[1024];
+        long base = (long)k * pix;
+        long poff = pb > 1 ? 3L * N * b : 0;
+        o_sorted_order(i, j, W, H, ff, k, idx + base, pts + poff, nrm + poff, order);
+        double front = 1.0;                       /* T_a */
+        for (int a = 0; a < k; a++) {
+            int kd = order[a];
+            double g = AT(gin[base + kd] * front);
+            double skip = front;                  /* prod over t < c, t != a */
+            for (int c = a + 1; c < k; c++) {
+                int id = order[c];
+                g += AT(-(gin[base + id] * w[base + id] * skip));
+                skip *= (1.0 - w[base + id]);
+            }
+            gout[base + kd] = g;
+            front *= (1.0 - w[base + kd]);
+        }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Composited Horvitz-Thompson estimate (no reference counterpart: the        */
 /* Poisson-rate form of splatting -> filtering -> interpolation, splattingCuda.cu:18-66, */
@@ -484,7 +613,7 @@ int o_htc(int bn, int N, int W, int H, int k, double f, double smoothing, const 
                 double p[3] = { pts[po], pts[po + 1], pts[po + 2] };
                 double q[3] = { nrm[po], nrm[po + 1], nrm[po + 2] };
                 const double* sdv = sd + 2 * ((sdBatch > 1 ? b * N : 0) + id);
-                wd[n] = ewa_weight_dual_k(i, j, W, H, f, p, q, sdv, smoothing, NULL, &kap[n]);
+                wd[n] = ewa_weight_dual_k(i, j, W, H, f, p, q, sdv, smoothing, NULL, &kap[n], NULL);
                 const double w = wd[n].v < 1.0 ? wd[n].v : 1.0;
                 al[n] = w * ipi[base + s];
                 dep[n] = o_depth_f32(i, j, (float)W, (float)H, ff, pts + po, nrm + po);
@@ -1242,7 +1371,7 @@ int o_num_launches_export(int k) { return o_num_launches(k); }
 
 /* sequentialPoissonKernel for one pixel, all launches.  out: k indices. */
 static int o_sample_pixel(const OTree* T, int i, int j, int H, int k, uint32_t seed_lo, uint32_t seed_hi,
-                          uint32_t view, int32_t* out, OCapMap* cm)
+                          uint32_t view, int32_t* out, OCapMap* cm, int* ties)
 {
     float keys[4096];
     for (int s = 0; s < k; s++) { keys[s] = INFINITY; out[s] = -1; }
@@ -1291,6 +1420,8 @@ static int o_sample_pixel(const OTree* T, int i, int j, int H, int k, uint32_t s
                 if (key < keys[s]) {
                     int ti = out[s]; out[s] = id; id = ti;
                     float tk = keys[s]; keys[s] = key; key = tk;
+                } else if (key == keys[s] && id != -1 && key < INFINITY) {
+                    *ties = 1;        /* the carried entry passes an equal key */
                 }
                 if (id == -1) break;
             }
@@ -1317,12 +1448,17 @@ int o_sample_view(int N, int W, int H, int k, double f, double smoothing, double
 
 /* As o_sample_view, but only pixel rows i0 <= i < i1 are sampled (others get -1);
  * *t_tree receives the tree-build wall time (used by the CPU baseline). */
+/* pixels of the last sample_view(_rows) call whose sorted list met two equal keys */
+static long g_tie_pixels;
+long o_tie_pixels(void) { return g_tie_pixels; }
+
 int o_sample_view_rows(int N, int W, int H, int k, double f, double smoothing, double prec,
                        const float* pts, const float* nrm, const float* sd,
                        uint32_t seed_lo, uint32_t seed_hi, uint32_t view, int32_t* out,
                        float* h_out, int* nc_out, int i0, int i1, double* t_tree)
 {
     if (k > 4096) return -1;
+    g_tie_pixels = 0;
     OTree T;
     memset(&T, 0, sizeof(T));
     long nn = 2L * (N > 0 ? N : 1);
@@ -1351,7 +1487,12 @@ int o_sample_view_rows(int N, int W, int H, int k, double f, double smoothing, d
             int i = (int)(pix / H), j = (int)(pix % H);
             int32_t* o = out + (long)k * pix;
             if (N == 0 || T.nc == 0 || i < i0 || i >= i1) { for (int s = 0; s < k; s++) o[s] = -1; continue; }
-            int r = o_sample_pixel(&T, i, j, H, k, seed_lo, seed_hi, view, o, cm);
+            int ties = 0;
+            int r = o_sample_pixel(&T, i, j, H, k, seed_lo, seed_hi, view, o, cm, &ties);
+            if (ties) {
+                #pragma omp atomic
+                g_tie_pixels++;
+            }
             if (r < 0) {
                 #pragma omp critical
                 rc = r;

@@ -4,15 +4,23 @@ half: splattingCuda.cu:84,214; transformCuda.cu:53,135; interpolationCuda.cu:68,
 shadingCuda.cu:63,153,225,315; zSortingCuda.cu:100,220):
 
   * float64: every op and adjoint against the oracle's FP64 restatement,
-    elementwise within 1e-10 relative plus 256 float64 roundoffs of each
+    elementwise within 1e-10 relative plus 1024 float64 roundoffs of each
     output's magnitude (its terms' sum of |term| times their evaluation
-    condition);
-  * float16 filtering: equal to the float32 op on the same (half-rounded)
-    inputs up to the rounding of the half output;
+    condition; the floor matters only far in the Gaussian tails, e.g. a weight
+    of 1e-259 on an edge-on surfel, where it is 2e-9 relative);
+  * float16 filtering: the float32 op on the same (half-rounded) inputs, within
+    half a half-precision ulp of its float32 result plus one float32 rounding
+    (the half kernel rounds its last product straight to half, once, where the
+    float32 op rounds to float32 first; either is a correct rounding);
   * FP64 central differences on the GPU: the hand-written adjoint of the
     reference's op chain splatting -> filtering -> interpolation (fixed sampled
-    indices, 256 splats at 64x64) against central differences (step 1e-4) of
-    the chain's float64 forward, 32 random parameters of each kind.
+    indices, 256 splats at 64x64) against central differences of the chain's
+    float64 forward at steps 1e-4 and 2.5e-5, 32 random parameters of each kind;
+    a parameter whose two differences disagree (a depth-order change of two
+    splats within the step makes the chain discontinuous there: about one point
+    coordinate in five on this overlapping 256-splat sphere) is replaced by
+    another draw, at most 48 times per kind.
+
 """
 import numpy as np
 import pytest
@@ -32,7 +40,17 @@ def t(x, dev, dt=torch.float64):
 
 
 def close64(got, ref, what, mag=None):
-    return assert_close(got.cpu().numpy(), ref, RTOL, what, mag=mag, ops=256, eps=EPS64, atol=1e-300)
+    return assert_close(got.cpu().numpy(), ref, RTOL, what, mag=mag, ops=1024, eps=EPS64, atol=1e-300)
+
+
+def within_half_rounding(h16, f32):
+    """h16 (float16) is a rounding to half of a value within one float32 rounding of f32."""
+    a = h16.double().cpu().numpy()
+    b = f32.double().cpu().numpy()
+    r16 = f32.half().cpu().numpy()
+    ulp = np.maximum(np.spacing(np.abs(r16)), np.spacing(np.abs(h16.cpu().numpy()))).astype(np.float64)
+    bound = 0.5 * ulp + np.abs(b) * 2.0 ** -24
+    return bool(np.all(np.abs(a - b) <= bound)), float(np.mean(a != r16.astype(np.float64)))
 
 
 @pytest.fixture(scope="module")
@@ -86,10 +104,12 @@ def test_filtering_f64_and_f16(dev, k):
     o16 = filtering.forward(t(idx, dev), wh, W, H, f, ph, nh)
     o32 = filtering.forward(t(idx, dev), wh.float(), W, H, f, ph.float(), nh.float())
     assert o16.dtype == torch.float16
-    assert torch.equal(o16, o32.half())
+    ok, frac = within_half_rounding(o16, o32)
+    assert ok and frac < 0.01, frac
     b16 = filtering.backward(t(idx, dev), wh, gh, W, H, f, ph, nh)
     b32 = filtering.backward(t(idx, dev), wh.float(), gh.float(), W, H, f, ph.float(), nh.float())
-    assert torch.equal(b16, b32.half())
+    ok, frac = within_half_rounding(b16, b32)
+    assert ok and frac < 0.01, frac
 
 
 @pytest.mark.parametrize("affine", [True, False])
@@ -157,8 +177,10 @@ def test_mixed_dtypes_rejected(dev, case):
 def test_gpu_fp64_central_differences_of_the_op_chain(dev):
     """L = sum g . interpolation(filtering(splatting(p, n, sd)), albedo) on fixed
     indices (the reference's sampler draws them), all in float64 on the GPU;
-    the raw adjoints (splatting.backward of filtering.backward of
-    interpolation.backward) against (L(x + h) - L(x - h)) / 2h, h = 1e-4."""
+    the raw adjoints (splatting.backward of the exact filtering.backward -- the
+    reference's drops terms at |w| or |1 - w| <= 1e-4 -- of
+    interpolation.backward) against (L(x + h) - L(x - h)) / 2h at h = 2.5e-5, where
+    the difference at h = 1e-4 agrees with it (the chain is smooth over the step)."""
     from cudaExtensions import filtering, interpolation, sampling, splatting
     sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)
     W, H, f = sc.width, sc.height, sc.focal
@@ -177,21 +199,38 @@ def test_gpu_fp64_central_differences_of_the_op_chain(dev):
     w = splatting.forward(W, H, f, idx, P, Nn, S, 0.0)
     dw = filtering.forward(idx, w, W, H, f, P, Nn)
     gDW, _ = interpolation.backward(g, idx, dw, A)
-    gW = filtering.backward(idx, w, gDW, W, H, f, P, Nn)
+    gW = filtering.backward(idx, w, gDW, W, H, f, P, Nn, exact=True)
     gP, gN, gS = splatting.backward(W, H, f, gW, idx, P, Nn, S, 0.0)
     h = 1e-4
     checked = 0
     for X, G, name in ((P, gP, "points"), (Nn, gN, "normals"), (S, gS, "std-devs")):
         flat = G.reshape(-1)
         cand = torch.nonzero(flat.abs() > 1e-3 * flat.abs().max()).reshape(-1).cpu().numpy()
-        for q in rng.choice(cand, size=32, replace=False):
+        scale = float(flat.abs().max())
+
+        def central(q, step):
             Xp, Xm = X.clone(), X.clone()
-            Xp.view(-1)[q] += h
-            Xm.view(-1)[q] -= h
+            Xp.view(-1)[q] += step
+            Xm.view(-1)[q] -= step
             args_p = [Xp if Y is X else Y for Y in (P, Nn, S)]
             args_m = [Xm if Y is X else Y for Y in (P, Nn, S)]
-            fd = (loss(*args_p) - loss(*args_m)) / (2 * h)
+            return (loss(*args_p) - loss(*args_m)) / (2 * step)
+
+        draws = rng.permutation(cand)
+        redrawn = 0
+        done = 0
+        for q in draws:
+            fd, fd4 = central(q, h), central(q, h / 4)
+            if abs(fd - fd4) > 1e-4 * abs(fd4) + 1e-6 * scale:
+                redrawn += 1                  # not smooth over [x - h, x + h]
+                assert redrawn <= 48, (name, "too many depth-order changes", int(q), fd, fd4)
+                continue
             an = float(flat[q])
-            assert abs(fd - an) <= 1e-4 * abs(an) + 1e-6 * float(flat.abs().max()), (name, int(q), fd, an)
-            checked += 1
+            assert abs(fd4 - an) <= 1e-4 * abs(an) + 1e-6 * scale, (name, int(q), fd4, an)
+            done += 1
+            if done == 32:
+                break
+        print("central differences of %s: 32 checked, %d redrawn (a depth-order change within the step)"
+              % (name, redrawn))
+        checked += done
     assert checked == 96

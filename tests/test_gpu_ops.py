@@ -107,6 +107,19 @@ def test_filter_forward_backward(dev, k):
     rg = O.filter_backward(idx, w, g, sc.width, sc.height, sc.focal, p[None], n[None])
     mg = magnitude(O.filter_backward, idx, w, g, sc.width, sc.height, sc.focal, p[None], n[None])
     assert_close(gw.cpu().numpy(), rg, RTOL, "filter backward", mag=mg)
+    # the exact adjoint, on weights that put slots inside the reference's gates
+    # (|w| <= 1e-4, |1 - w| <= 1e-4) where the two differ
+    w2 = w.copy()
+    gated = rng.uniform(size=idx.shape)
+    w2[gated < 0.1] = 5e-5
+    w2[gated > 0.9] = np.float32(1.0 - 5e-5)
+    w2[idx < 0] = 0.0
+    ge = filtering.backward(t(idx, dev), t(w2, dev), t(g, dev), sc.width, sc.height, sc.focal, t(p[None], dev),
+                            t(n[None], dev), exact=True)
+    re = O.filter_backward(idx, w2, g, sc.width, sc.height, sc.focal, p[None], n[None], exact=True)
+    me = magnitude(O.filter_backward, idx, w2, g, sc.width, sc.height, sc.focal, p[None], n[None], exact=True)
+    assert_close(ge.cpu().numpy(), re, RTOL, "exact filter backward", mag=me)
+    assert np.abs(re - O.filter_backward(idx, w2, g, sc.width, sc.height, sc.focal, p[None], n[None])).max() > 1e-2
 
 
 @pytest.mark.parametrize("affine", [True, False])

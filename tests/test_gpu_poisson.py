@@ -60,6 +60,17 @@ def test_poisson_tile_scale_batch_and_edges(dev):
     gpu_poisson(dev, W, H, f, p[None], nb[None], sd[None], 2, 50.0, seed=1)
     with pytest.raises(RuntimeError, match="more than numSamples"):
         sampling.check_errors()
+    # a non-finite or negative tile multiplier draws nothing in its tile and is
+    # reported (a NaN rate would otherwise make every pi 1); other tiles as before
+    bad = ts[:1].copy()
+    bad[0, 0, 0] = np.nan
+    bad[0, 1, 0] = -1.0
+    ids, ipi = gpu_poisson(dev, W, H, f, P[:1], NB[:1], sd[None], 48, 5.0, seed=3, ts=bad)
+    with pytest.raises(RuntimeError, match="multiplier"):
+        sampling.check_errors()
+    assert np.all(ids[0, :32, :16] == -1)
+    rid, ripi = O.poisson(W, H, f, P[:1], NB[:1], sd[None], 48, 5.0, seed=3, tile_scale=ts[:1])
+    assert np.array_equal(ids[0, 32:], rid[0, 32:]) and np.array_equal(ids[0, :, 16:], rid[0, :, 16:])
 
 
 def test_poisson_gradients_unbiased(dev):

@@ -8,7 +8,8 @@ perturbed by N(0, 0.01^2), in both gradient modes:
   * "reference": the EWA gradients of points and std-devs clipped row-wise to
     norm <= 1 with NaN -> 0, and no gradient of the normals
     (extensions/__init__.py:203-233);
-  * "raw": the exact adjoints, the normals' included.
+  * "raw": the exact adjoints, the normals' included, and the exact compositing
+    adjoint (the reference's drops terms where |w| or |1 - w| <= 1e-4).
 
 The sampled indices must equal the oracle's sampler on the same camera-space
 inputs bit for bit; the image and the world-space gradients of positions,
@@ -100,8 +101,9 @@ def test_config0_frame_matches_oracle_chain(dev, mode):
     m_gimg = 2.0 * (m_img + np.abs(tgt))
     g_dw, g_A = O.interp_backward(g_img, idx, dw, A64)
     m_gdw, m_gA = magnitude(O.interp_backward, m_gimg, idx, m_dw, A64)
-    g_ewa = O.filter_backward(idx, ewa, g_dw, W, H, f, camP[None], camN[None])
-    m_gewa = magnitude(O.filter_backward, idx, m_ewa, m_gdw, W, H, f, camP[None], camN[None])
+    exact = mode == "raw"
+    g_ewa = O.filter_backward(idx, ewa, g_dw, W, H, f, camP[None], camN[None], exact=exact)
+    m_gewa = magnitude(O.filter_backward, idx, m_ewa, m_gdw, W, H, f, camP[None], camN[None], exact=exact)
     g_cP, g_cN, g_S = O.splat_backward(W, H, f, g_ewa, idx, P64, N64, S64)
     m_cP, m_cN, m_S = magnitude(O.splat_backward, W, H, f, m_gewa, idx, P64, N64, S64)
     if mode == "reference":
@@ -110,13 +112,18 @@ def test_config0_frame_matches_oracle_chain(dev, mode):
         m_cP, m_S = m_cP / sP, m_S / sS
     _, g_P = O.transform_backward(True, g_cP, Mv, sc.positions[None])
     _, m_P = magnitude(O.transform_backward, True, m_cP, Mv, sc.positions[None])
-    assert_close(leaves[0].grad.cpu().numpy(), g_P, RTOL, mode + ": grad positions", mag=m_P)
-    assert_close(leaves[2].grad.cpu().numpy(), g_S, RTOL, mode + ": grad std-devs", mag=m_S)
-    assert_close(leaves[3].grad.cpu().numpy(), g_A, RTOL, mode + ": grad albedo", mag=m_gA)
+    # elements twelve orders of magnitude below a tensor's largest carry only terms
+    # of weights below e^-87, which float32 evaluation flushes to zero
+    floor = lambda ref: 1e-12 * float(np.abs(ref).max())
+    assert_close(leaves[0].grad.cpu().numpy(), g_P, RTOL, mode + ": grad positions", mag=m_P, atol=floor(g_P))
+    assert_close(leaves[2].grad.cpu().numpy(), g_S, RTOL, mode + ": grad std-devs", mag=m_S, atol=floor(g_S))
+    assert_close(leaves[3].grad.cpu().numpy(), g_A, RTOL, mode + ": grad albedo", mag=m_gA, atol=floor(g_A))
     if mode == "reference":
-        assert leaves[1].grad is None           # EWASplatting returns no normal gradient
+        # EWASplatting returns no normal gradient (a zero tensor where another op's
+        # backward still reaches the normals)
+        assert leaves[1].grad is None or not bool(leaves[1].grad.any())
     else:
         _, g_N = O.transform_backward(False, g_cN, Mv, sc.normals[None])
         _, m_N = magnitude(O.transform_backward, False, m_cN, Mv, sc.normals[None])
-        assert_close(leaves[1].grad.cpu().numpy(), g_N, RTOL, mode + ": grad normals", mag=m_N)
+        assert_close(leaves[1].grad.cpu().numpy(), g_N, RTOL, mode + ": grad normals", mag=m_N, atol=floor(g_N))
     assert np.abs(g_P).max() > 0 and np.abs(g_S).max() > 0

@@ -35,6 +35,33 @@ def test_sample_bit_exact(dev, cfg, sd_scale, k):
         mism.shape[0], got.shape[0] * got.shape[1], mism[0], got[tuple(mism[0])], ref[tuple(mism[0])])
 
 
+def test_sample_equal_keys(dev):
+    """Equal keys, where the reference's insertion (a strict < that carries each
+    displaced entry down the list, sequentialPoissonRadixTree.cuh:15-41) orders
+    and drops entries unlike a sort by (key, draw number): 256 copies of a splat
+    at every pixel centre of a 48x48 image, so the leaves there weigh exactly 1
+    and their keys are the 24-bit uniforms themselves; k = 256.  The oracle's
+    lists meet equal keys at about 30 pixels; the GPU's indices must equal it
+    bit for bit everywhere."""
+    from cudaExtensions import sampling
+    W = H = 48
+    f = 64.0
+    ii, jj = np.meshgrid(np.arange(W), np.arange(H), indexing="ij")
+    x = (ii - W / 2).ravel().astype(np.float32)
+    y = (H / 2 - jj).ravel().astype(np.float32)
+    P = np.repeat(np.stack([x, y, np.full_like(x, -f)], 1), 256, 0).astype(np.float32)
+    N = np.tile(np.float32([0.0, 0.0, 1.0]), (P.shape[0], 1))
+    S = np.full((P.shape[0], 2), 1.0, np.float32)
+    ref = O.sample_view(W, H, f, P, N, S, 256, 0.0, 0.01, seed=4)
+    assert O.tie_pixels() >= 10
+    got = sampling.sample(W, H, f, t(P[None], dev), t(N[None], dev), t(S[None], dev), t(S[None], dev), 256, 0.0,
+                          0.01, seed=4)
+    sampling.check_errors()
+    got = got[0].cpu().numpy()
+    mism = np.argwhere((got != ref).any(-1))
+    assert mism.shape[0] == 0, "%d / %d pixels differ, first %s" % (mism.shape[0], W * H, mism[0])
+
+
 @pytest.mark.parametrize("precision,smoothing", [(0.001, 0.0), (0.05, 0.25)])
 def test_sample_precision_smoothing(dev, precision, smoothing):
     sc, p, n, nb, sd = scene_inputs(0, sd_scale=2.0)

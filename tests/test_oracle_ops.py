@@ -152,6 +152,38 @@ def test_compositing_backward_matches_central_differences():
     assert np.abs(g - fd).max() <= 1e-6 * max(np.abs(fd).max(), 1.0)
 
 
+def test_exact_compositing_backward_at_gated_weights():
+    """The exact adjoint (no dropped terms) against central differences where the
+    reference's gates act: weights at 1e-5 (|w| <= 1e-4 drops the slot's own term)
+    and within 1e-5 of 1 (|1 - w| <= 1e-4 drops its occlusion term), among ordinary
+    ones.  Where no gate acts the two adjoints agree; where one does they differ."""
+    sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)
+    W, H, f = 12, 10, sc.focal * 12 / sc.width
+    rng = np.random.default_rng(4)
+    k = 7
+    idx = rng.integers(0, p.shape[0], size=(1, W, H, k)).astype(np.int32)
+    w = rng.uniform(0.05, 0.95, size=idx.shape)
+    w[..., 2] = 1e-5
+    w[..., 4] = 1.0 - 1e-5
+    gin = rng.normal(size=idx.shape)
+    pts, nrm = p[None], nb[None]
+    ge = O.filter_backward(idx, w, gin, W, H, f, pts, nrm, exact=True)
+    gr = O.filter_backward(idx, w, gin, W, H, f, pts, nrm)
+    eps = 1e-7
+    fd = np.zeros_like(w)
+    for s in range(k):
+        wp = w.copy(); wp[..., s] += eps
+        wm = w.copy(); wm[..., s] -= eps
+        fd[..., s] = ((gin * O.filter_forward(idx, wp, W, H, f, pts, nrm)).sum(-1) -
+                      (gin * O.filter_forward(idx, wm, W, H, f, pts, nrm)).sum(-1)) / (2 * eps)
+    assert np.abs(ge - fd).max() <= 1e-6 * max(np.abs(fd).max(), 1.0)
+    assert np.abs(ge[..., 2] - gr[..., 2]).max() > 1e-3 and np.abs(ge[..., 4] - gr[..., 4]).max() > 1e-3
+    # with every weight away from 0 and 1 the two adjoints are the same derivative
+    w2 = rng.uniform(0.05, 0.95, size=idx.shape)
+    assert np.allclose(O.filter_backward(idx, w2, gin, W, H, f, pts, nrm, exact=True),
+                       O.filter_backward(idx, w2, gin, W, H, f, pts, nrm), rtol=1e-12, atol=1e-14)
+
+
 # ------------------------------------------------------------------ transform
 @pytest.mark.parametrize("affine", [True, False])
 def test_transform_vs_autograd(affine):

@@ -644,7 +644,7 @@ __device__ __forceinline__ C ray_depth(int i, int j, C Wf, C Hf, C ff, const C* 
 //   rank_s = #{d_j < d_s} + #{j < s : d_j == d_s}.
 template <bool BACKWARD, class S>
 __global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, int H, int k,
-                                                     typename CompOf<S>::type ff,
+                                                     typename CompOf<S>::type ff, typename CompOf<S>::type gate,
                                                      const int32_t* __restrict__ idx, const S* __restrict__ wts,
                                                      const S* __restrict__ gin,
                                                      const S* __restrict__ pts, const S* __restrict__ nrm, int pb,
@@ -720,17 +720,18 @@ __global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, in
     int src = 62 - rank;
     C after = __shfl(A, src < 0 ? 0 : src, 64);
     if (rank == 63) after = C(0);
-    const C eps = C(1e-4);
+    // the reference's gates (|w| > 1e-4, |1 - w| > 1e-4); gate < 0 keeps every term
+    // (the exact derivative: nothing here is divided, so no term needs dropping)
     C g = C(0);
-    if (fabs(w) > eps) g += gi * rem;
-    if (fabs(C(1) - w) > eps) g -= rem * after;
+    if (fabs(w) > gate) g += gi * rem;
+    if (fabs(C(1) - w) > gate) g -= rem * after;
     if (act) st_c(out, base + s, g);
 }
 
 // General k (64 < k <= 1024): one workgroup per pixel, ranks from LDS, serial scans.
 template <bool BACKWARD, class S>
 __global__ __launch_bounds__(1024) void k_filter_block(long npix, int N, int W, int H, int k,
-                                                       typename CompOf<S>::type ff,
+                                                       typename CompOf<S>::type ff, typename CompOf<S>::type gate,
                                                        const int32_t* __restrict__ idx, const S* __restrict__ wts,
                                                        const S* __restrict__ gin,
                                                        const S* __restrict__ pts, const S* __restrict__ nrm, int pb,
@@ -785,10 +786,9 @@ __global__ __launch_bounds__(1024) void k_filter_block(long npix, int N, int W, 
     }
     __syncthreads();
     if (!act) return;
-    const C eps = C(1e-4);
     C g = C(0);
-    if (fabs(w) > eps) g += gi * rem;
-    if (fabs(C(1) - w) > eps) g -= rem * sd_[rank];
+    if (fabs(w) > gate) g += gi * rem;
+    if (fabs(C(1) - w) > gate) g -= rem * sd_[rank];
     st_c(out, base + s, g);
 }
 
@@ -1345,17 +1345,18 @@ int splat_backward_t(int bn, int N, int W, int H, int k, double f, double smooth
 
 template <bool BWD, class S>
 int filter_t(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const S* wts, const S* gin,
-             const S* pts, const S* nrm, int pb, S* out, hipStream_t st)
+             const S* pts, const S* nrm, int pb, S* out, hipStream_t st, bool exact = false)
 {
     using C = typename CompOf<S>::type;
     if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || k > 1024 || (pb != 1 && pb != bn)) return UDE_ERR_ARG;
     long npix = (long)bn * W * H;
+    const C gate = exact ? C(-1) : C(1e-4);
     if (k <= 64)
-        k_filter_wave<BWD, S><<<grid1(npix, 4), 256, 0, st>>>(npix, N, W, H, k, (C)f, idx, wts, gin, pts, nrm, pb,
-                                                               out);
+        k_filter_wave<BWD, S><<<grid1(npix, 4), 256, 0, st>>>(npix, N, W, H, k, (C)f, gate, idx, wts, gin, pts, nrm,
+                                                               pb, out);
     else
-        k_filter_block<BWD, S><<<(unsigned)npix, ((k + 63) / 64) * 64, 0, st>>>(npix, N, W, H, k, (C)f, idx, wts,
-                                                                                 gin, pts, nrm, pb, out);
+        k_filter_block<BWD, S><<<(unsigned)npix, ((k + 63) / 64) * 64, 0, st>>>(npix, N, W, H, k, (C)f, gate, idx,
+                                                                                 wts, gin, pts, nrm, pb, out);
     return launch_status();
 }
 
@@ -1523,6 +1524,20 @@ int ude_filter_backward_f64(int bn, int N, int W, int H, int k, double f, const 
                             hipStream_t st)
 {
     return filter_t<true, double>(bn, N, W, H, k, f, idx, wts, gin, pts, nrm, pb, out, st);
+}
+
+int ude_filter_backward_exact(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const float* wts,
+                              const float* gin, const float* pts, const float* nrm, int pb, float* out,
+                              hipStream_t st)
+{
+    return filter_t<true, float>(bn, N, W, H, k, f, idx, wts, gin, pts, nrm, pb, out, st, true);
+}
+
+int ude_filter_backward_exact_f64(int bn, int N, int W, int H, int k, double f, const int32_t* idx,
+                                  const double* wts, const double* gin, const double* pts, const double* nrm, int pb,
+                                  double* out, hipStream_t st)
+{
+    return filter_t<true, double>(bn, N, W, H, k, f, idx, wts, gin, pts, nrm, pb, out, st, true);
 }
 
 int ude_filter_backward_f16(int bn, int N, int W, int H, int k, double f, const int32_t* idx, const void* wts,

@@ -109,6 +109,7 @@ struct TreeWs {
     int* momList;
     float* trie;
     int2* reach;          // per launch and pixel lane: leaf range the capacity descent reached
+    int* tieQueue;                    // [0]: count, then (sampler lane, final key bits) per lane that met equal keys
     float2* leafXY;       // sorted position -> screen position of the leaf
     float* coef;
     long coefCap;
@@ -163,6 +164,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     long lanes = (long)((W + 7) / 8) * ((H + 7) / 8) * 64;
     w.trie = fixedOrder ? nullptr : (float*)take(16L * trie_entries(k) * lanes);
     w.reach = fixedOrder ? nullptr : (int2*)take(8L * 16 * lanes);
+    w.tieQueue = fixedOrder ? nullptr : (int*)take(4 * (2 * lanes + 1));
     w.leafXY = (float2*)take(8 * n);
     w.coef = fixedOrder ? nullptr : (float*)take(4 * coefCap);
     w.coefCap = fixedOrder ? 0 : coefCap;
@@ -1459,6 +1461,24 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
     for (int la = drawn; la < L; la++) rch[(long)la * 64] = make_int2(0, 0);
 }
 
+// The key word of one candidate: (u / E(leaf) bits, draw number (launch << 16 |
+// ordinal)); false where the key is not finite (E = 0: never inserted).  Keys are
+// >= 0, so their bit patterns order like the floats.
+__device__ __forceinline__ bool draw_word(float qx, float qy, float2 xy, float is2h, uint32_t x, int la, int ord,
+                                          unsigned long long& ent)
+{
+    float tx = (qx - xy.x) * is2h;
+    float ty = (qy - xy.y) * is2h;
+    float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
+    float wgt = 0.0f;
+    wgt = wgt + (1.0f * ex) * ey;
+    float p = fmaxf(0.0f, wgt);
+    float key = u01(x) / p;
+    if (!(key < __builtin_huge_valf())) return false;
+    ent = ((unsigned long long)__float_as_uint(key) << 32) | (uint32_t)((la << 16) | ord);
+    return true;
+}
+
 // Sequential Poisson keys over the leaves each launch reached (the walk's
 // capacity descents do not depend on the keys, so they run first, in
 // k_sample_walk, at the occupancy their long dependent loads need): in a Karras
@@ -1466,14 +1486,28 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
 // count), in the left-to-right order nextLeaf visits
 // (sequentialPoissonDynamicTree.cuh:16-53), and every leaf carries the single
 // Hermite coefficient 1, so E(leaf) = e^{-tx^2} e^{-ty^2}.  Key u / E with u the
-// Philox word of (leaf ordinal, launch, pixel, view); the k smallest (key, draw
-// number) pairs in order are the sample, kept in a per-lane bounded max-heap in
-// LDS and heap-sorted at the end.
+// Philox word of (leaf ordinal, launch, pixel, view).  The reference keeps the
+// keys in a sorted list of k slots, inserting with a strict < and carrying each
+// displaced entry down (insertSample, sequentialPoissonRadixTree.cuh:15-41).
+// With distinct keys that list is the k smallest (key, draw number) words in
+// order, and they are kept here in a per-lane bounded max-heap in LDS,
+// heap-sorted at the end.  Equal keys are where the two differ: a carried entry
+// passes over entries equal to it, so an insertion ahead of a run of equal keys
+// moves the run's first entry to the run's end, and a full list drops the front
+// of its last run.  That matters exactly when two equal keys are in the heap at
+// once: the sorted heap holds two equal neighbours, or an entry it dropped ties
+// with the final maximum -- dropped keys never increase, so it is enough that
+// the last one dropped equals the final maximum (a run dropped whole leaves the
+// sample as the heap has it).  Such a lane (rare: none in 16,384 pixels of the
+// 1M-splat frame) queues itself, and k_sample_replay redraws it through the
+// reference's insertion.  (A replay path inside this kernel, never
+// taken, cost 45 % of its time in code generation.)
 __global__ __launch_bounds__(64) void k_sample_draw(int W, int H, int k, int L, uint32_t seedLo, uint32_t seedHi,
                                                     uint32_t view, const ViewState* __restrict__ vs,
                                                     const float2* __restrict__ leafXY,
                                                     const int32_t* __restrict__ leafId,
-                                                    const int2* __restrict__ reach, int32_t* __restrict__ out)
+                                                    const int2* __restrict__ reach, int32_t* __restrict__ out,
+                                                    int* __restrict__ tieQueue)
 {
     extern __shared__ unsigned long long list[];
     const int lane = threadIdx.x;
@@ -1491,6 +1525,7 @@ __global__ __launch_bounds__(64) void k_sample_draw(int W, int H, int k, int L, 
     const uint32_t pixel = (uint32_t)(j + H * i);
     const int2* rch = reach + (long)blockIdx.x * L * 64 + lane;
     int firstPos[16];                                   // per launch: first leaf position drawn
+    unsigned long long lastDrop = empty;               // the last word the heap dropped
     for (int la = 0; la < L; la++) {
         const int2 fc = rch[(long)la * 64];
         const int first = fc.x, cnt = fc.y;
@@ -1508,34 +1543,179 @@ __global__ __launch_bounds__(64) void k_sample_draw(int W, int H, int k, int L, 
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 if (kk0 + q >= cnt) break;
-                float tx = (qx - xy[q].x) * is2h;
-                float ty = (qy - xy[q].y) * is2h;
-                float ex = pexpf(-(tx * tx)), ey = pexpf(-(ty * ty));
-                float wgt = 0.0f;
-                wgt = wgt + (1.0f * ex) * ey;
-                float p = fmaxf(0.0f, wgt);
-                uint32_t x = (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w;
-                float key = u01(x) / p;
-                if (!(key < __builtin_huge_valf())) continue;
-                // (key, draw number) in one ordered word: keys are >= 0, so
-                // their bit patterns order like the floats
-                unsigned long long ent = ((unsigned long long)__float_as_uint(key) << 32) |
-                                         (uint32_t)((la << 16) | (kk0 + q));
-                if (ent < list[lane]) heap_replace_top(list, lane, k, ent);
+                unsigned long long ent;
+                if (!draw_word(qx, qy, xy[q], is2h, (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w,
+                               la, kk0 + q, ent))
+                    continue;
+                const unsigned long long top = list[lane];
+                if (ent < top) {
+                    lastDrop = top;
+                    heap_replace_top(list, lane, k, ent);
+                }
             }
         }
     }
-    // heap -> ascending order (heapsort), then draw numbers -> splat ids
+    // heap -> ascending order (heapsort)
     for (int end = k - 1; end > 0; end--) {
         unsigned long long x = list[end * 64 + lane];
         list[end * 64 + lane] = list[lane];
         heap_replace_top(list, lane, end, x);
     }
+    bool tie = lastDrop != empty && key_of(lastDrop) == key_of(list[(k - 1) * 64 + lane]);
+    for (int s = 1; s < k && !tie; s++) {
+        const unsigned long long b = list[s * 64 + lane];
+        tie = b != empty && key_of(b) == key_of(list[(s - 1) * 64 + lane]);
+    }
+    if (tie) {
+        const int e = atomicAdd(tieQueue, 1);
+        tieQueue[1 + 2 * e] = (int)blockIdx.x * 64 + lane;
+        tieQueue[2 + 2 * e] = (int)(uint32_t)(list[(k - 1) * 64 + lane] >> 32);   // the final maximum key
+    }
+    // draw numbers -> splat ids
     int32_t* o = out + ((long)j + (long)H * i) * k;
     for (int s = 0; s < k; s++) {
         uint32_t dn = (uint32_t)(list[s * 64 + lane] & 0xffffffffull);
         o[s] = dn == 0xffffffffu ? -1 : leafId[firstPos[dn >> 16] + (int)(dn & 0xffffu)];
     }
+}
+
+// The pixels k_sample_draw queued, drawn again through the reference's insertion,
+// one wavefront per pixel: list position s lives in row s / 64 of lane s % 64
+// (k <= 256: at most 4 rows).  Inserting word c with the strict < of the reference
+// puts c at s0, the count of keys <= c's; behind it a displaced entry is carried
+// on and passes over entries of its own key, so each run of equal keys behind s0
+// keeps its entries in place except its first, which moves to the position after
+// the run (and the first entry of the last run leaves a full list; a carried
+// empty slot ends the insertion, and the empty slots form the last run).  So
+// position s > s0 keeps its entry inside a run and otherwise takes the first
+// entry of the run ending at s - 1; s0 takes c.  Candidates are keyed 64 at a
+// time, one per lane, and inserted in the reference's order; one whose key is
+// above the final list's largest (queued by k_sample_draw: the heap ends with the
+// same largest key) is skipped, as it only ever joins runs the list drops whole.
+// A persistent grid drains the queue.
+constexpr int kReplayRows = 4;
+__global__ __launch_bounds__(64) void k_sample_replay(int W, int H, int k, int L, unsigned drawBlocks, uint32_t seedLo,
+                                                      uint32_t seedHi, uint32_t view, const ViewState* __restrict__ vs,
+                                                      const float2* __restrict__ leafXY,
+                                                      const int32_t* __restrict__ leafId,
+                                                      const int2* __restrict__ reach, const int* __restrict__ tieQueue,
+                                                      int32_t* __restrict__ out)
+{
+    const int lane = threadIdx.x;
+    const int count = tieQueue[0];
+    const unsigned long long empty = ((unsigned long long)0x7f800000u << 32) | 0xffffffffull;
+    const int R = (k + 63) >> 6;
+    for (int e = blockIdx.x; e < count; e += gridDim.x) {
+        const int sl = tieQueue[1 + 2 * e];             // sampler lane: draw block * 64 + lane
+        const float kmax = __uint_as_float((uint32_t)tieQueue[2 + 2 * e]);
+        const int blk = sl >> 6, pl = sl & 63;
+        const int tilesY = (H + 7) >> 3;
+        const uint32_t tile = xcd_remap((uint32_t)blk, drawBlocks);
+        const int ti = tile / tilesY, tj = tile - (tile / tilesY) * tilesY;
+        const int i = ti * 8 + (pl >> 3), j = tj * 8 + (pl & 7);
+        const float is2h = vs->is2h;
+        const float qx = (float)i, qy = (float)j;
+        const uint32_t pixel = (uint32_t)(j + H * i);
+        const int2* rch = reach + (long)blk * L * 64 + pl;
+        unsigned long long v[kReplayRows];
+#pragma unroll
+        for (int r = 0; r < kReplayRows; r++) v[r] = empty;
+        for (int la = 0; la < L; la++) {
+            const int2 fc = rch[(long)la * 64];
+            const int first = fc.x, cnt = fc.y;
+            for (int c0 = 0; c0 < cnt; c0 += 64) {
+                // this lane's candidate of the chunk
+                const int kk = c0 + lane;
+                unsigned long long mine = empty;
+                bool okc = false;
+                if (kk < cnt) {
+                    const u32x4 rnd = philox4x32_10(u32x4{ (uint32_t)(kk >> 2), (uint32_t)la, pixel, view }, seedLo,
+                                                    seedHi);
+                    const int q = kk & 3;
+                    okc = draw_word(qx, qy, leafXY[first + kk], is2h,
+                                    (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w, la, kk, mine) &&
+                          !(key_of(mine) > kmax);
+                }
+                const unsigned long long okm = __ballot(okc);
+                const int nc = cnt - c0 < 64 ? cnt - c0 : 64;
+                for (int t = 0; t < nc; t++) {
+                    if (!((okm >> t) & 1ull)) continue;       // key not finite (never inserted) or above kmax
+                    const unsigned long long c = __shfl(mine, t, 64);
+                    const float kc = key_of(c);
+                    int s0 = 0;
+#pragma unroll
+                    for (int r = 0; r < kReplayRows; r++)
+                        if (r < R) s0 += __popcll(__ballot(r * 64 + lane < k && !(kc < key_of(v[r]))));
+                    if (s0 >= k) continue;                                // not below any key: not inserted
+                    // run starts at or after s0 (s0 itself is one), prefix maximum over positions
+                    // every shuffle runs on all 64 lanes; lane 0 then takes row r - 1's
+                    // lane 63 (or nothing in row 0) as its predecessor
+                    int start[kReplayRows];
+                    int carry = -1;
+#pragma unroll
+                    for (int r = 0; r < kReplayRows; r++) {
+                        if (r >= R) break;                                // wave-uniform
+                        const int pos = r * 64 + lane;
+                        const unsigned long long up = __shfl_up(v[r], 1, 64);
+                        const unsigned long long wrap = __shfl(v[r > 0 ? r - 1 : 0], 63, 64);
+                        const unsigned long long prev = lane > 0 ? up : r > 0 ? wrap : empty;
+                        const bool st = pos >= s0 && pos < k && (pos == s0 || key_of(v[r]) != key_of(prev));
+                        int m = st ? pos : -1;
+#pragma unroll
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const int u = __shfl_up(m, o, 64);
+                            if (lane >= o && u > m) m = u;
+                        }
+                        if (m < carry) m = carry;
+                        start[r] = m;
+                        carry = __shfl(m, 63, 64);
+                    }
+                    // the first entry of the run ending at pos - 1: start of pos - 1, then its entry
+                    unsigned long long nv[kReplayRows];
+#pragma unroll
+                    for (int r = 0; r < kReplayRows; r++) {
+                        if (r >= R) break;
+                        const int pos = r * 64 + lane;
+                        const int spUp = __shfl_up(start[r], 1, 64);
+                        const int spWrap = __shfl(start[r > 0 ? r - 1 : 0], 63, 64);
+                        const int sp = lane > 0 ? spUp : r > 0 ? spWrap : -1;
+                        unsigned long long from = empty;
+#pragma unroll
+                        for (int rr = 0; rr < kReplayRows; rr++) {
+                            if (rr >= R) break;
+                            const unsigned long long g = __shfl(v[rr], sp & 63, 64);
+                            if (sp >= 0 && (sp >> 6) == rr) from = g;
+                        }
+                        const unsigned long long up = __shfl_up(v[r], 1, 64);
+                        const unsigned long long wrap = __shfl(v[r > 0 ? r - 1 : 0], 63, 64);
+                        const unsigned long long prev = lane > 0 ? up : r > 0 ? wrap : empty;
+                        // inside a run (the empty slots, which end a list that is not full,
+                        // are one run: a carried empty slot ends the insertion there)
+                        const bool inside = pos > s0 && key_of(v[r]) == key_of(prev);
+                        nv[r] = pos >= k ? empty : pos < s0 ? v[r] : pos == s0 ? c : inside ? v[r] : from;
+                    }
+#pragma unroll
+                    for (int r = 0; r < kReplayRows; r++)
+                        if (r < R) v[r] = nv[r];
+                }
+            }
+        }
+        int32_t* o = out + ((long)j + (long)H * i) * k;
+#pragma unroll
+        for (int r = 0; r < kReplayRows; r++) {
+            const int pos = r * 64 + lane;
+            if (pos < k) {
+                const uint32_t dn = (uint32_t)(v[r] & 0xffffffffull);
+                o[pos] = dn == 0xffffffffu ? -1 : leafId[rch[(long)(dn >> 16) * 64].x + (int)(dn & 0xffffu)];
+            }
+        }
+    }
+}
+
+__global__ void k_fill_i32(int* p, long n, int v)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
 }
 
 __global__ void k_fill_f32(float* p, long n, float v)
@@ -2090,6 +2270,15 @@ __global__ __launch_bounds__(256) void k_walk_records(const ViewState* __restric
 // (step, 0x80000000, pixel, view), one block per geometric-skip step of the
 // pixel (word x the skip, word y the acceptance test).  A skipped or thinned
 // subtree is left through its escape pointer.
+// The first kPoissonStage samples of each lane are staged in LDS (slot-major,
+// [slot][lane], so the walk's stores hit 64 distinct banks) and written out at
+// the end together with the empty slots, one tile row (2^lth contiguous pixels
+// x kcap slots) per coalesced pass of the wavefront; only samples past the
+// staged ones are stored directly.  A lane's scattered 4-byte stores would
+// otherwise leave partly written lines in L2 for the length of the walk.
+constexpr int kPoissonStage = 16;
+constexpr size_t kPoissonStageBytes = (size_t)kPoissonStage * 64 * 8;    // per wavefront
+
 __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg, int W, int H, int kcap, float rate,
                                                                    const float* __restrict__ tscale, int tsH,
                                                                    uint32_t seedLo, uint32_t seedHi,
@@ -2103,9 +2292,12 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                                                                    int32_t* __restrict__ outIds,
                                                                    float* __restrict__ outIpi, int* err)
 {
+    extern __shared__ int32_t stage[];
     const int lane = threadIdx.x & 63;
     int i, j;
     if (!tile_lane(cfg, W, H, lane, i, j)) return;           // wave-uniform
+    int32_t* sid = stage + (threadIdx.x >> 6) * (2 * kPoissonStage * 64);
+    float* sip = (float*)(sid + kPoissonStage * 64);
     const bool valid = i < W && j < H;
     const uint32_t view = view0 + blockIdx.y;
     if (seedDev) {
@@ -2135,6 +2327,9 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         const Band B0 = vs->band[0], B1 = vs->band[1];
         float lam = rate;
         if (tscale) lam = lam * tscale[(long)(i >> 4) * tsH + (j >> 4)];
+        // a negative or non-finite multiplier would make every candidate's pi 1
+        // (fminf drops a NaN): draw nothing there and say so
+        if (!(lam >= 0.0f && lam <= 3.0e38f)) { atomicOr(err, 16); lam = 0.0f; }
         const float t0 = 0.5f / fmaxf(D0, B0.c);
         const float t1 = 0.5f / fmaxf(dens1[pix], B1.c);
         const float k0 = t0 * B0.c, k1 = t1 * B1.c;
@@ -2175,8 +2370,13 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                         float pi = fminf(1.0f, lam * pl);
                         float a = u01(blk.y);
                         if (a * U <= pi) {            // P = pi / U, exactly 1 when pi = U
-                            if (ns < kcap) { oid[ns] = leafId[pos]; oip[ns] = 1.0f / pi; ns++; }
-                            else over = true;
+                            if (ns < kcap) {
+                                const int32_t id = leafId[pos];
+                                const float ip = 1.0f / pi;
+                                if (ns < kPoissonStage) { sid[ns * 64 + lane] = id; sip[ns * 64 + lane] = ip; }
+                                else { oid[ns] = id; oip[ns] = ip; }
+                                ns++;
+                            } else over = true;
                         }
                     }
                 }
@@ -2185,8 +2385,9 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
             if (nd < 0) break;
         }
     }
-    // empty slots: a row of the tile (2^lth pixels) is contiguous in memory and is
-    // padded by the whole wavefront, so the stores coalesce
+    // staged samples and empty slots: a row of the tile (2^lth pixels) is contiguous
+    // in memory and is written by the whole wavefront, so the stores coalesce
+    __builtin_amdgcn_wave_barrier();
     const int th = 1 << cfg.lth;
     for (int r = 0; r < (64 >> cfg.lth); r++) {
         const int l0 = r << cfg.lth;
@@ -2200,7 +2401,11 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
             const int e = e0 + lane;
             const int c = e < total ? e / kcap : 0;
             const int nsc = __shfl(ns, l0 + c, 64);
-            if (e < total && e - c * kcap >= nsc) { rid[e] = -1; rip[e] = 0.0f; }
+            const int q = e - c * kcap;                   // slot of pixel l0 + c
+            if (e < total) {
+                if (q >= nsc) { rid[e] = -1; rip[e] = 0.0f; }
+                else if (q < kPoissonStage) { rid[e] = sid[q * 64 + l0 + c]; rip[e] = sip[q * 64 + l0 + c]; }
+            }
         }
     }
     if (over) atomicOr(err, 4);
@@ -2447,7 +2652,11 @@ void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32
     if (timed) (void)hipEventRecord(e0, st);
     k_sample_walk<<<tiles, 64, 0, st>>>(W, H, k, L, w.vs, w.geo, w.lnk, w.aux, w.coef, w.trie, trie_entries(k),
                                         w.reach, err);
-    k_sample_draw<<<tiles, 64, lds, st>>>(W, H, k, L, seedLo, seedHi, view, w.vs, w.leafXY, w.idsA, w.reach, out);
+    k_fill_i32<<<1, 64, 0, st>>>(w.tieQueue, 1, 0);
+    k_sample_draw<<<tiles, 64, lds, st>>>(W, H, k, L, seedLo, seedHi, view, w.vs, w.leafXY, w.idsA, w.reach, out,
+                                          w.tieQueue);
+    k_sample_replay<<<64, 64, 0, st>>>(W, H, k, L, tiles, seedLo, seedHi, view, w.vs, w.leafXY, w.idsA, w.reach,
+                                       w.tieQueue, out);
     if (timed) (void)hipEventRecord(e1, st);
 }
 
@@ -2631,7 +2840,7 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         hipEvent_t e0, e1;
         bool timed = timing_pair(2, &e0, &e1);
         if (timed) (void)hipEventRecord(e0, st);
-        k_poisson<<<tile_grid(pc, W, H, span), 64 * pc.wpb, 0, st>>>(
+        k_poisson<<<tile_grid(pc, W, H, span), 64 * pc.wpb, kPoissonStageBytes * pc.wpb, st>>>(
             pc, W, H, kcap, (float)rate, ts, tsH, sLo, sHi, seedDev, (uint32_t)g, w.vs, per, w.pnode, w.leafXY,
             w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag);
         if (timed) (void)hipEventRecord(e1, st);

@@ -65,6 +65,8 @@ _SIGS = {
     "ude_splat_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
     "ude_filter_forward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
     "ude_filter_backward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
+    "ude_filter_backward_exact": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
+    "ude_filter_backward_exact_f64": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
     "ude_transform_forward": (c_int, [c_int] * 4 + [c_ptr] * 4),
     "ude_transform_forward_facing": (c_int, [c_int] * 3 + [c_ptr] * 5),
     "ude_transform_backward": (c_int, [c_int] * 4 + [c_ptr] * 6),

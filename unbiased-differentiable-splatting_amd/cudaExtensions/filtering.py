@@ -37,7 +37,11 @@ def forward(indices, weights, width, height, focallength, points, normals):
     return out
 
 
-def backward(indices, weights, gradIn, width, height, focallength, points, normals):
+def backward(indices, weights, gradIn, width, height, focallength, points, normals, exact=False):
+    """Gradient w.r.t. the weights.  By default the reference's adjoint, which drops
+    a slot's own term where |w| <= 1e-4 and its occlusion term where |1 - w| <= 1e-4
+    (alphacompositingGrad.cuh:7-26); exact=True keeps every term (the exact
+    derivative; float32 and float64)."""
     check_shapes(indices, weights, width, height, points, normals)
     _lib.expect(gradIn, "gradIn", *indices.shape)
     dev = _lib.require_gpu(indices, weights, gradIn, points, normals)
@@ -47,8 +51,10 @@ def backward(indices, weights, gradIn, width, height, focallength, points, norma
     bn, W, H, k = indices.shape
     if k > 1024:
         raise ValueError("filtering supports at most 1024 samples per pixel")
+    if exact and dt == torch.float16:
+        raise TypeError("filtering.backward: exact=True takes float32 or float64 tensors")
     out = torch.empty(weights.shape, dtype=dt, device=dev)
-    rc = _lib.fn("ude_filter_backward", dt)(bn, points.shape[1], int(width), int(height), k, float(focallength),
+    rc = _lib.fn("ude_filter_backward_exact" if exact else "ude_filter_backward", dt)(bn, points.shape[1], int(width), int(height), k, float(focallength),
                                             _lib.ptr(indices), _lib.ptr(weights), _lib.ptr(gradIn), _lib.ptr(points),
                                             _lib.ptr(normals), points.shape[0], _lib.ptr(out), _lib.stream(dev))
     _lib.check(rc, "filtering.backward")

@@ -46,6 +46,9 @@ def _error_flag(errors, dev):
 def raise_errors(e):
     """Raise for the error bits ``e`` of a sampling call (0: nothing)."""
     e = int(e)
+    if e & 16:
+        raise RuntimeError("sampling.samplePoisson: a tile's rate multiplier was negative or not finite "
+                           "(nothing drawn there)")
     if e & 8:
         raise RuntimeError("sampling.samplePoisson: a pixel's tree walk exceeded its node bound")
     if e & 4:
@@ -147,7 +150,8 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
     one launch over all views (bn times the workspace of one view), else view
     by view.  ``errors``: optional int32 (1,) device tensor the call ORs its
     error bits into (bit 4: a pixel had more than numSamples inclusions, the
-    extra ones dropped; bit 8: a walk exceeded its node bound); it is never
+    extra ones dropped; bit 8: a walk exceeded its node bound; bit 16: a tile's
+    rate times its multiplier was negative or not finite, nothing drawn there); it is never
     cleared here, so one flag can collect a whole training run and be checked at
     a point that synchronises anyway (raise_errors(errors.item()))."""
     global last_error

@@ -10,7 +10,9 @@ SphericalHarmonicShading :524-581).
     clipping to norm <= 1 (conditionalNormalization), NaN -> 0, and no gradient
     for the normals (extensions/__init__.py:203-233);
   * ``"raw"`` returns the exact adjoint, including the normals' gradient, with
-    no clipping (needed for unbiasedness and finite-difference checks).
+    no clipping, and composites with the exact compositing adjoint (the
+    reference's drops terms where |w| or |1 - w| <= 1e-4) -- needed for
+    unbiasedness and finite-difference checks.
 """
 import torch as th
 
@@ -94,7 +96,7 @@ class DepthFiltering(th.autograd.Function):
     def backward(ctx, gradIn):
         indices, weights, points, normals = ctx.saved_tensors
         gW = filtering.backward(indices, weights, gradIn.contiguous(), ctx.width, ctx.height, ctx.focallength,
-                                points, normals)
+                                points, normals, exact=GRAD_MODE["ewa"] == "raw")
         return None, gW, None, None, None, None, None
 
 

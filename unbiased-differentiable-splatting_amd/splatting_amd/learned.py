@@ -94,6 +94,7 @@ class RateNet:
         sum(scale * counts) is unchanged, and kept within [lo, hi]."""
         shape = counts.shape
         y = self.forward(feats.reshape(-1, N_IN)).reshape(shape).double()
+        y = torch.nan_to_num(y, nan=0.0, posinf=20.0, neginf=-20.0)     # a non-finite feature: plain rate
         c = counts.double()
         tot = c.sum().clamp(min=1e-30)
         y = y - (y * c).sum() / tot
