@@ -31,6 +31,10 @@ typedef struct ihipStream_t* ude_stream_t;
 #define UDE_ERR_ARG (-1)
 #define UDE_ERR_LAUNCH (-2)
 #define UDE_ERR_WORKSPACE (-3)
+#define UDE_ERR_SCRATCH (-4)     /* the allocation-free entry point's sizes exceed its static scratch */
+
+/* Size of the static device scratch of ude_splat_backward_static (64 MiB per device). */
+#define UDE_STATIC_SCRATCH_BYTES (64u << 20)
 
 /* ---- sampling ---------------------------------------------------------- */
 
@@ -183,6 +187,21 @@ int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoo
                        const float* gradIn, const int32_t* idx, const float* pts, const float* nrm,
                        const float* sd, int sdBatch, float* gPts, float* gNrm, float* gSd,
                        void* ws, size_t wsBytes, ude_stream_t stream);
+
+/* No counterpart in the reference: ude_splat_backward for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and outputs as
+ * ude_splat_backward without the workspace: the kernels run on a static device
+ * array of UDE_STATIC_SCRATCH_BYTES per device (its address taken once per device,
+ * on the first call with that device current).  Needs
+ * ude_splat_workspace_bytes(bn, N) <= UDE_STATIC_SCRATCH_BYTES (bn * N <= about
+ * 1M splat-views), else returns UDE_ERR_SCRATCH.  Allocates nothing.  Every call on
+ * one device shares that device's array: calls that overlap on one device (other
+ * streams, other host threads) race; calls ordered on one stream do not. */
+int ude_splat_backward_static(int bn, int N, int W, int H, int k, double f, double smoothing,
+                              const float* gradIn, const int32_t* idx, const float* pts, const float* nrm,
+                              const float* sd, int sdBatch, float* gPts, float* gNrm, float* gSd,
+                              ude_stream_t stream);
 
 /* ---- filtering (z-sort + alpha compositing) ----------------------------- */
 

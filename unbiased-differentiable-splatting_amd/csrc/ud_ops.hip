@@ -14,7 +14,37 @@
 
 #include <hip/hip_fp16.h>
 
+#include <mutex>
+
 using namespace ud;
+
+// Scratch of the allocation-free splatting.backward (ude_splat_backward_static):
+// per (view, splat) the 32 B screen-Gaussian record and the 32 B moment record,
+// UDE_STATIC_SCRATCH_BYTES in all, held by every device the library runs on until
+// the process ends.  Host code never passes this name to a kernel (on the host it
+// is a stand-in, not a device address): it uses the per-device address below.
+static __device__ __align__(256) unsigned char scratch[UDE_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices];           // device address of `scratch` on each device, once known
+std::mutex g_scratch_mu;
+
+// The device address of `scratch` on the current device, taken once per device
+// (hipGetSymbolAddress with that device current); nullptr on failure.
+void* scratch_on_current_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch[dev] = ptr;
+    }
+    return g_scratch[dev];
+}
+} // namespace
 
 namespace {
 
@@ -1483,6 +1513,22 @@ int ude_splat_backward_f64(int bn, int N, int W, int H, int k, double f, double 
 {
     return splat_backward_t<double>(bn, N, W, H, k, f, smoothing, gradIn, idx, pts, nrm, sd, sdBatch, gPts, gNrm,
                                     gSd, ws, wsBytes, st);
+}
+
+int ude_splat_backward_static(int bn, int N, int W, int H, int k, double f, double smoothing,
+                              const float* gradIn, const int32_t* idx, const float* pts, const float* nrm,
+                              const float* sd, int sdBatch, float* gPts, float* gNrm, float* gSd, hipStream_t st)
+{
+    if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
+    if (N == 0) return UDE_OK;
+    const size_t need = ude_splat_workspace_bytes(bn, N);
+    if (need > (size_t)UDE_STATIC_SCRATCH_BYTES) return UDE_ERR_SCRATCH;
+    void* ws = scratch_on_current_device();
+    if (!ws) return UDE_ERR_LAUNCH;
+    // the kernels of ude_splat_backward on the static scratch; a fill kernel clears the
+    // moments, so the previous call's contents never leak into this one
+    return splat_backward_t<float>(bn, N, W, H, k, f, smoothing, gradIn, idx, pts, nrm, sd, sdBatch, gPts, gNrm,
+                                   gSd, ws, (size_t)UDE_STATIC_SCRATCH_BYTES, st);
 }
 
 int ude_covariance(int bn, int N, int W, int H, double f, double smoothing, const float* pts,

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(_PKG, "libudsplat.so")
 _lib = None
 
 UDE_OK = 0
-_ERRORS = {-1: "invalid argument", -2: "kernel launch failed", -3: "workspace too small"}
+_ERRORS = {-1: "invalid argument", -2: "kernel launch failed", -3: "workspace too small",
+           -4: "sizes exceed the static scratch of the allocation-free entry point"}
 
 c_int = ctypes.c_int
 c_dbl = ctypes.c_double
@@ -63,6 +64,7 @@ _SIGS = {
     "ude_splat_workspace_bytes": (c_size, [c_int] * 2),
     "ude_splat_forward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 4 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "ude_splat_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
+    "ude_splat_backward_static": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4),
     "ude_filter_forward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
     "ude_filter_backward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
     "ude_filter_backward_exact": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
