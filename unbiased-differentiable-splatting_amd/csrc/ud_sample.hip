@@ -40,6 +40,7 @@ namespace {
 
 constexpr int kMaxDegree = 16;
 constexpr int kMaxDepth = 64;        // descent trace bound (reference STACKSIZE)
+constexpr int kTrieFields = 5;       // 4-byte fields per descent-trie entry (TrieRef)
 constexpr int kCullTile = 1024;      // elements per workgroup in the cull pass
 constexpr int kSortTile = 2048;      // elements per workgroup in a sort pass
 constexpr int kFixedDegree = 6;      // default expansion order of the fixed-order densities (hermitefmm.cuh:11)
@@ -79,6 +80,17 @@ struct NodeLnk {
 struct NodeAux {
     int off, first, last, pad;   // pad: a leaf's splat id
 };
+// per internal node: its two children as the sampler's walk reads them (one
+// record per descent; a node's own fields arrive with its parent's record)
+struct SampChild {
+    float lox, loy, hix, hiy;    // box
+    float cx, cy, r2;            // expansion centre, squared truncation radius
+    int deg;                     // Hermite degree, -1 when the walk goes below
+    int off, count, id, pad;     // coefficient offset, leaves under it, node index
+};
+struct SampRec {
+    SampChild l, r;
+};
 // per node: what the per-pixel Poisson walk reads of it in one load
 struct PNode {
     float lox, loy, hix, hiy;    // box
@@ -108,6 +120,7 @@ struct TreeWs {
     int* seeds;
     int* momList;
     float* trie;
+    SampRec* srec;        // per internal node: children records of the sampler's walk
     int2* reach;          // per launch and pixel lane: leaf range the capacity descent reached
     int* tieQueue;                    // [0]: count, then (sampler lane, final key bits) per lane that met equal keys
     float2* leafXY;       // sorted position -> screen position of the leaf
@@ -162,7 +175,8 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     w.seeds = (int*)take(4 * n);
     w.momList = (int*)take(4 * n);
     long lanes = (long)((W + 7) / 8) * ((H + 7) / 8) * 64;
-    w.trie = fixedOrder ? nullptr : (float*)take(16L * trie_entries(k) * lanes);
+    w.trie = fixedOrder ? nullptr : (float*)take(4L * kTrieFields * trie_entries(k) * lanes);
+    w.srec = fixedOrder ? nullptr : (SampRec*)take(sizeof(SampRec) * n);
     w.reach = fixedOrder ? nullptr : (int2*)take(8L * 16 * lanes);
     w.tieQueue = fixedOrder ? nullptr : (int*)take(4 * (2 * lanes + 1));
     w.leafXY = (float2*)take(8 * n);
@@ -1199,6 +1213,34 @@ __device__ __noinline__ float tree_weight(const TreeView& T, int entry, float qx
 
 __device__ __forceinline__ float key_of(unsigned long long v) { return __uint_as_float((uint32_t)(v >> 32)); }
 
+// Children records of the sampler's walk, one per internal node (after the tree's
+// degrees and coefficient offsets are final).
+__device__ __forceinline__ SampChild samp_child(int c, const NodeGeo* __restrict__ geo,
+                                                const NodeLnk* __restrict__ lnk, const NodeAux* __restrict__ aux)
+{
+    const NodeGeo g = geo[c];
+    SampChild r;
+    r.lox = g.lox; r.loy = g.loy; r.hix = g.hix; r.hiy = g.hiy;
+    r.cx = g.cx; r.cy = g.cy; r.r2 = g.r2; r.deg = g.deg;
+    r.off = aux[c].off; r.count = lnk[c].count; r.id = c; r.pad = 0;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_samp_records(const ViewState* __restrict__ vs,
+                                                      const NodeGeo* __restrict__ geo,
+                                                      const NodeLnk* __restrict__ lnk,
+                                                      const NodeAux* __restrict__ aux, SampRec* __restrict__ rec)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = vs->nc;
+    if (nc <= 1 || p >= nc - 1) return;
+    const NodeLnk lk = lnk[p];
+    SampRec r;
+    r.l = samp_child(lk.left, geo, lnk, aux);
+    r.r = samp_child(lk.right, geo, lnk, aux);
+    rec[p] = r;
+}
+
 // Sampler: one lane per pixel, 8x8 pixel tile per wavefront.
 //
 // Every FGT weight the descents need is a sequential sum over a subset of the
@@ -1223,11 +1265,12 @@ __device__ __forceinline__ float key_of(unsigned long long v) { return __uint_as
 // entry index issues one coalesced access.
 // per-depth arrays of the current DFS path (private memory; every index is
 // wave-uniform)
-// Accumulators of the current DFS path, one per depth and lane: the first
-// kDepthLds depths in LDS ([depth][lane], indices are wave-uniform), deeper ones
-// (rare: a Karras tree is rarely more than ~25 levels deep above its k-source
-// nodes) in private memory.
-constexpr int kDepthLds = 28;
+// Accumulators of the current DFS path, one per depth and lane, kept only for
+// the depths a descent compares (the leading nodes with more than k sources and
+// the first one below them): the first kDepthLds depths in LDS ([depth][lane],
+// indices are wave-uniform), deeper ones (rare: about 20 levels cover k = 40 at
+// 1M splats) in private memory.
+constexpr int kDepthLds = 20;
 struct DepthF {
     float* lds;                     // this wavefront's [kDepthLds][64] slice, offset by the lane
     float a[kMaxDepth - kDepthLds];
@@ -1244,14 +1287,30 @@ struct DepthI {
     __device__ void set(int i, int v) { a[i] = v; }
 };
 
+// An entry holds the weights of its node's two children and their capacities, so
+// a descent step reads everything it compares in one round of loads:
+// field 0 the left child's capacity, 1 / 2 the left / right weight, 3 the child
+// entries, 4 the right child's capacity.  Fields are interleaved by lane
+// ([entry][field][lane]): lanes that walk the same nodes create their entries in
+// step, so their accesses coalesce.
 struct TrieRef {
     float UD_GLOBAL* base;
     long stride;                    // lanes in the grid
-    __device__ float get(int e, int f) const { return base[(4L * e + f) * stride]; }
-    __device__ void set(int e, int f, float v) const { base[(4L * e + f) * stride] = v; }
+    __device__ float get(int e, int f) const { return base[((long)kTrieFields * e + f) * stride]; }
+    __device__ void set(int e, int f, float v) const { base[((long)kTrieFields * e + f) * stride] = v; }
     __device__ int kids(int e) const { return __float_as_int(get(e, 3)); }
     __device__ void setKids(int e, int v) const { set(e, 3, __int_as_float(v)); }
-    __device__ void init(int e, float wUnset) const { set(e, 0, 1.0f); set(e, 1, wUnset); set(e, 2, wUnset); setKids(e, 0); }
+    // one half of field 3 (left child entry + 1 in the low half, right in the high
+    // half), stored alone so that no load of the other half is needed
+    __device__ void setKidHalf(int e, bool right, int v) const
+    {
+        unsigned short UD_GLOBAL* h = (unsigned short UD_GLOBAL*)(base + ((long)kTrieFields * e + 3) * stride);
+        h[right ? 1 : 0] = (unsigned short)v;
+    }
+    __device__ void init(int e, float wUnset) const
+    {
+        set(e, 0, 1.0f); set(e, 1, wUnset); set(e, 2, wUnset); setKids(e, 0); set(e, 4, 1.0f);
+    }
 };
 
 // Replace the top of a per-lane max-heap of n words (LDS, slot-major
@@ -1276,7 +1335,8 @@ __device__ __forceinline__ void heap_replace_top(unsigned long long* list, int l
 
 __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, const ViewState* __restrict__ vs,
                                                     const NodeGeo* geo, const NodeLnk* lnk, const NodeAux* aux,
-                                                    const float* coef, float* trieWs, int tcap,
+                                                    const float* coef, const SampRec* __restrict__ srec,
+                                                    float* trieWs, int tcap,
                                                     int2* __restrict__ reach, int* __restrict__ errOut)
 {
     TreeView T;
@@ -1314,35 +1374,53 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
         // ---- one depth-first pass over the union of the 64 lanes' FGT walks ----
         // The wavefront visits nodes in the reference's left-to-right order; a lane
         // is active inside a node exactly when its own walk enters that node, so
-        // each lane sees its own frontier in its own order.  Node records and
-        // Hermite coefficients are then read at wave-uniform addresses.
-        __shared__ int s_right[kMaxDepth], s_count[kMaxDepth];
+        // each lane sees its own frontier in its own order.  A node's fields come
+        // with its parent's children record (SampRec): a descent loads one record,
+        // a truncated node or a right sibling popped from the stack loads nothing,
+        // an expanded node only its coefficients.  Trie entries are numbered for
+        // the wavefront (a big node any lane walks gets one; a lane that does not
+        // walk it finds unset weights there, as in an entry it would create), so
+        // the path's entries are wave-uniform and every trie store coalesces.
+        __shared__ SampChild s_rinfo[kMaxDepth];           // right siblings of the path
         __shared__ unsigned long long s_walk[kMaxDepth];
+        __shared__ int s_pent[kMaxDepth];                  // trie entry of each path node, -1 none
         __shared__ float s_coef[(kMaxDegree + 1) * (kMaxDegree + 1)];
-        // per-depth accumulators and trie entries of the current path
         __shared__ float s_pacc[kDepthLds * 64];
         DepthF pacc;
         pacc.lds = s_pacc + lane;
-        DepthI pent;
         int big = 0;                 // leading path nodes with more than k sources
         uint64_t rightBits = 0;      // bit d: node at depth d is its parent's right child
-        int d = 0, nd = 0;
+        int d = 0;
         bool act = valid;
         bool overflow = false;
+        SampChild cur;               // the node at depth d (wave-uniform)
+        {
+            const NodeGeo g0 = ld_geo(T, 0);
+            cur.lox = g0.lox; cur.loy = g0.loy; cur.hix = g0.hix; cur.hiy = g0.hiy;
+            cur.cx = g0.cx; cur.cy = g0.cy; cur.r2 = g0.r2; cur.deg = g0.deg;
+            cur.off = T.aux[0].off; cur.count = T.lnk[0].count; cur.id = 0; cur.pad = 0;
+        }
+        // a node's weight into its parent's entry (the root's: the total)
         auto report = [&](int dd, float w) {
             if (dd == 0) { total = w; return; }
-            int pe = pent.get(dd - 1);
+            const int pe = s_pent[dd - 1];
             if (pe >= 0) tr.set(pe, ((rightBits >> dd) & 1ull) ? 2 : 1, w);
         };
+        auto uni = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
         while (true) {
-            nd = __builtin_amdgcn_readfirstlane(nd);
-            NodeGeo g = ld_geo(T, nd);
-            NodeLnk lk = ld_lnk(T, nd);        // issued together with the geometry
-            const int coefOff = T.aux[nd].off;   // and the coefficient offset
+            // cur is wave-uniform (the root's record, a children record, or the LDS
+            // stack); say so, so that it lives in scalar registers
+            cur.id = __builtin_amdgcn_readfirstlane(cur.id);
+            cur.deg = __builtin_amdgcn_readfirstlane(cur.deg);
+            cur.off = __builtin_amdgcn_readfirstlane(cur.off);
+            cur.count = __builtin_amdgcn_readfirstlane(cur.count);
+            NodeGeo g;
+            g.lox = uni(cur.lox); g.loy = uni(cur.loy); g.hix = uni(cur.hix); g.hiy = uni(cur.hiy);
+            g.cx = uni(cur.cx); g.cy = uni(cur.cy); g.r2 = uni(cur.r2); g.deg = cur.deg;
             float wv = 0.0f;
             bool walk = false;
             if (g.deg >= 0) {
-                wv = eval_node_staged(T, g, coefOff, qx, qy, act, s_coef, lane);
+                wv = eval_node_staged(T, g, cur.off, qx, qy, act, s_coef, lane);
                 if (act) {
                     // ancestors whose weight a descent can compare: depth 0 and every
                     // node whose parent has more than k sources (a prefix of the path,
@@ -1356,27 +1434,23 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
             unsigned long long wm = __ballot(walk);
             if (wm != 0ull) {
                 if (d >= kMaxDepth - 1) { overflow = true; break; }
-                s_right[d] = lk.right;
-                s_count[d] = lk.count;
-                s_walk[d] = wm;
+                const SampRec R = srec[cur.id];           // the one load of a descent
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) { s_rinfo[d] = R.r; s_walk[d] = wm; }
                 if (act && !walk) report(d, 0.0f);       // truncated for this lane
-                if (walk) {
-                    pacc.set(d, 0.0f);
-                    int e = -1;
-                    if (d == 0) e = 0;
-                    else if (lk.count > k && pent.get(d - 1) >= 0 && nt < DFSMAX) {
-                        e = nt++;
-                        tr.init(e, unset);
-                        int pe = pent.get(d - 1);
-                        int pk = tr.kids(pe);
-                        if ((rightBits >> d) & 1ull) tr.setKids(pe, (pk & 0xffff) | ((e + 1) << 16));
-                        else tr.setKids(pe, (pk & ~0xffff) | (e + 1));
-                    }
-                    pent.set(d, e);
+                int e = -1;
+                if (d == 0) e = 0;
+                else if (cur.count > k && s_pent[d - 1] >= 0 && nt < DFSMAX) {
+                    e = nt++;
+                    tr.init(e, unset);
+                    tr.setKidHalf(s_pent[d - 1], (rightBits >> d) & 1ull, e + 1);
                 }
-                if (big == d && __builtin_amdgcn_readfirstlane(lk.count) > k) big = d + 1;
+                if (lane == 0) s_pent[d] = e;
+                __builtin_amdgcn_wave_barrier();
+                if (walk && d <= big) pacc.set(d, 0.0f);
+                if (big == d && cur.count > k) big = d + 1;
                 d++;
-                nd = lk.left;
+                cur = R.l;
                 rightBits &= ~(1ull << d);
                 act = walk;
                 continue;
@@ -1388,14 +1462,16 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
                 if (d == 0) { done = true; break; }
                 if (!((rightBits >> d) & 1ull)) {
                     rightBits |= (1ull << d);
-                    nd = s_right[d - 1];
+                    __builtin_amdgcn_wave_barrier();
+                    cur = s_rinfo[d - 1];
                     act = (s_walk[d - 1] >> lane) & 1ull;
                     break;
                 }
                 d--;
                 if (big > d) big = d;
                 act = (s_walk[d] >> lane) & 1ull;
-                if (act) report(d, pacc.get(d));
+                // its accumulator is kept exactly when its parent has an entry
+                if (act && (d == 0 || s_pent[d - 1] >= 0)) report(d, pacc.get(d));
             }
             if (done) break;
         }
@@ -1405,12 +1481,18 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
         total = fmaxf(0.0f, total);
         if (!bad && total >= prec) {
             int pe[kMaxDepth];
-            float pp[kMaxDepth];
+            float pp[kMaxDepth], pc[kMaxDepth];
             for (int la = 0; la < L && !bad; la++) {
                 int e = 0, nd = 0, depth = 0;
-                while (T.lnk[nd].count > k) {
-                    NodeLnk lk = ld_lnk(T, nd);
+                uint64_t sideBits = 0;         // bit d: the step to depth d went right
+                pe[0] = 0;
+                NodeLnk lk = ld_lnk(T, nd);
+                while (lk.count > k) {
+                    // one round of loads: the node's link (issued at the end of the last
+                    // step) and the entry's weights, children and capacities
                     float wl = tr.get(e, 1), wr = tr.get(e, 2);
+                    const int kids = tr.kids(e);
+                    const float cl = tr.get(e, 0), cr = tr.get(e, 4);
                     if (wl != wl) {
                         wl = tree_weight(T, lk.left, qx, qy);
                         wr = tree_weight(T, lk.right, qx, qy);
@@ -1420,10 +1502,7 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
                     wl = fmaxf(0.0f, wl); wr = fmaxf(0.0f, wr);
                     float sum = wl + wr;
                     wl = wl / sum; wr = wr / sum;
-                    int kids = tr.kids(e);
                     int el = (kids & 0xffff) - 1, er = (kids >> 16) - 1;
-                    float cl = el >= 0 ? tr.get(el, 0) : 1.0f;
-                    float cr = er >= 0 ? tr.get(er, 0) : 1.0f;
                     float sl = wl * cl, sr = wr * cr;
                     bool goRight = sl < sr;
                     int ce = goRight ? er : el;
@@ -1432,25 +1511,31 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
                         if (nt >= TMAX) { bad = true; break; }
                         ce = nt++;
                         tr.init(ce, unset);
-                        tr.setKids(e, goRight ? ((kids & 0xffff) | ((ce + 1) << 16)) : ((kids & ~0xffff) | (ce + 1)));
+                        tr.setKidHalf(e, goRight, ce + 1);
                     }
                     depth++;
                     if (depth >= kMaxDepth) { bad = true; break; }
                     pe[depth] = ce;
                     pp[depth] = goRight ? wr : wl;
+                    pc[depth] = goRight ? cr : cl;     // its capacity, unchanged until the update below
+                    if (goRight) sideBits |= 1ull << depth;
                     e = ce;
                     nd = cnode;
+                    lk = ld_lnk(T, nd);
                 }
                 if (bad) break;
+                // capacities: an entry's lives in its parent's entry (field 4 if it is
+                // the right child, else 0); the path's were read on the way down
                 float prob = 1.0f;
                 for (int dd = depth; dd >= 1; dd--) {
-                    int ce = pe[dd];
-                    tr.set(ce, 0, fmaxf(0.0f, tr.get(ce, 0) - prob));
+                    const int par = pe[dd - 1];
+                    const int fld = ((sideBits >> dd) & 1ull) ? 4 : 0;
+                    tr.set(par, fld, fmaxf(0.0f, pc[dd] - prob));
                     prob = prob * pp[dd];
                 }
                 // the reached subtree's leaves are the contiguous sorted range
                 // [first, first + count); k_sample_draw keys them
-                const int cnt = T.lnk[nd].count;
+                const int cnt = lk.count;
                 const int first = (nd >= L0) ? nd - L0 : T.aux[nd].first;
                 rch[(long)la * 64] = make_int2(first, cnt);
                 drawn = la + 1;
@@ -2612,7 +2697,7 @@ int num_launches(int k)
 }
 
 // trie entries per lane: room for the walk plus 32 new entries per descent
-int trie_entries(int k) { return 192 + 32 * num_launches(k); }
+int trie_entries(int k) { return 384 + 32 * num_launches(k); }
 
 // Optional timing of one sampling kernel (0 the walk + draw pair of
 // sampling.sample, 1 k_fmm_eval, 2 k_poisson): while enabled, its launches are
@@ -2641,17 +2726,18 @@ bool timing_pair(int kernel, hipEvent_t* a, hipEvent_t* b)
     return true;
 }
 
-void launch_sample(int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32_t view, const TreeWs& w,
+void launch_sample(int N, int W, int H, int k, uint32_t seedLo, uint32_t seedHi, uint32_t view, const TreeWs& w,
                    int32_t* out, int* err, hipStream_t st)
 {
+    k_samp_records<<<grid1(N, 256), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.srec);
     unsigned tiles = (unsigned)(((W + 7) / 8) * ((H + 7) / 8));
     size_t lds = (size_t)k * 64 * sizeof(unsigned long long);
     const int L = num_launches(k);
     hipEvent_t e0, e1;
     bool timed = timing_pair(0, &e0, &e1);
     if (timed) (void)hipEventRecord(e0, st);
-    k_sample_walk<<<tiles, 64, 0, st>>>(W, H, k, L, w.vs, w.geo, w.lnk, w.aux, w.coef, w.trie, trie_entries(k),
-                                        w.reach, err);
+    k_sample_walk<<<tiles, 64, 0, st>>>(W, H, k, L, w.vs, w.geo, w.lnk, w.aux, w.coef, w.srec, w.trie,
+                                        trie_entries(k), w.reach, err);
     k_fill_i32<<<1, 64, 0, st>>>(w.tieQueue, 1, 0);
     k_sample_draw<<<tiles, 64, lds, st>>>(W, H, k, L, seedLo, seedHi, view, w.vs, w.leafXY, w.idsA, w.reach, out,
                                           w.tieQueue);
@@ -2746,7 +2832,7 @@ int ude_sample(int bn, int N, int W, int H, int k, double f, double smoothing, d
         if (rc == UDE_OK) rc = build_moments(N, k, w, st);
         if (rc != UDE_OK) return rc;
         int32_t* ob = out + perView * b;
-        launch_sample(W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
+        launch_sample(N, W, H, k, sLo, sHi, (uint32_t)b, w, ob, errFlag, st);
         k_copy_err<<<1, 64, 0, st>>>(w.vs, errFlag);
     }
     hipError_t e = hipGetLastError();
