@@ -1260,11 +1260,6 @@ __global__ __launch_bounds__(256) void k_samp_records(const ViewState* __restric
 // k smallest (key, draw number) words, kept here in a per-lane max-heap in LDS
 // (slot-major) and heap-sorted once at the end: O(log k) LDS steps per
 // accepted key instead of a lane-divergent O(k) shift.
-// Trie of descent states, one slot array per lane in device memory, fields
-// interleaved by lane ([entry][field][lane]) so a wavefront touching the same
-// entry index issues one coalesced access.
-// per-depth arrays of the current DFS path (private memory; every index is
-// wave-uniform)
 // Accumulators of the current DFS path, one per depth and lane, kept only for
 // the depths a descent compares (the leading nodes with more than k sources and
 // the first one below them): the first kDepthLds depths in LDS ([depth][lane],
@@ -1281,30 +1276,27 @@ struct DepthF {
         else a[i - kDepthLds] = v;
     }
 };
-struct DepthI {
-    int a[kMaxDepth];
-    __device__ int get(int i) const { return a[i]; }
-    __device__ void set(int i, int v) { a[i] = v; }
-};
 
 // An entry holds the weights of its node's two children and their capacities, so
 // a descent step reads everything it compares in one round of loads:
 // field 0 the left child's capacity, 1 / 2 the left / right weight, 3 the child
-// entries, 4 the right child's capacity.  Fields are interleaved by lane
-// ([entry][field][lane]): lanes that walk the same nodes create their entries in
-// step, so their accesses coalesce.
+// entries, 4 the right child's capacity.  Each wavefront owns one contiguous
+// block of the workspace ([tile][entry][field][lane]): lanes that walk the same
+// nodes create their entries in step, so their accesses coalesce, and a
+// wavefront's whole trie (under 1 MB) sits in one or two pages, so its loads do
+// not miss the address-translation caches (with the fields of all tiles
+// interleaved, consecutive fields of one entry lay 1 MB apart).
 struct TrieRef {
-    float UD_GLOBAL* base;
-    long stride;                    // lanes in the grid
-    __device__ float get(int e, int f) const { return base[((long)kTrieFields * e + f) * stride]; }
-    __device__ void set(int e, int f, float v) const { base[((long)kTrieFields * e + f) * stride] = v; }
+    float UD_GLOBAL* base;          // this lane's slot in its tile's block
+    __device__ float get(int e, int f) const { return base[(kTrieFields * e + f) * 64]; }
+    __device__ void set(int e, int f, float v) const { base[(kTrieFields * e + f) * 64] = v; }
     __device__ int kids(int e) const { return __float_as_int(get(e, 3)); }
     __device__ void setKids(int e, int v) const { set(e, 3, __int_as_float(v)); }
     // one half of field 3 (left child entry + 1 in the low half, right in the high
     // half), stored alone so that no load of the other half is needed
     __device__ void setKidHalf(int e, bool right, int v) const
     {
-        unsigned short UD_GLOBAL* h = (unsigned short UD_GLOBAL*)(base + ((long)kTrieFields * e + 3) * stride);
+        unsigned short UD_GLOBAL* h = (unsigned short UD_GLOBAL*)(base + (kTrieFields * e + 3) * 64);
         h[right ? 1 : 0] = (unsigned short)v;
     }
     __device__ void init(int e, float wUnset) const
@@ -1350,8 +1342,7 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
     const int DFSMAX = tcap - L * 32;                 // entries the walk may create
     const int lane = threadIdx.x;
     TrieRef tr;
-    tr.stride = (long)gridDim.x * 64;
-    tr.base = (float UD_GLOBAL*)trieWs + (long)blockIdx.x * 64 + lane;
+    tr.base = (float UD_GLOBAL*)trieWs + (long)blockIdx.x * kTrieFields * tcap * 64 + lane;
     const int tilesY = (H + 7) >> 3;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int ti = tile / tilesY, tj = tile - (tile / tilesY) * tilesY;
