@@ -43,7 +43,12 @@ def lib():
         _lib.o_sample_view.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3
         _lib.o_fmm_weights_view.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 4
         _lib.o_poisson_view.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 4 + [ctypes.c_void_p] * 4 +
-                                       [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3)
+                                       [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_double])
+        _lib.o_foot_view.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 +
+                                    [ctypes.c_void_p] * 5)
+        _lib.o_foot_view.restype = ctypes.c_int
+        _lib.o_foot_terms.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3
+        _lib.o_foot_bounds.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3
         _lib.o_set_fixed_degree.restype = ctypes.c_int
         _lib.o_set_fixed_degree.argtypes = [ctypes.c_int]
         _lib.o_poisson_wide_h.restype = ctypes.c_float
@@ -474,10 +479,16 @@ def fmm_weights(W, H, f, pts, nrm, sd, smoothing=0.0, precision=0.01):
     return np.stack(outs)
 
 
-def poisson_view(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.01, seed=0, view=0, tile_scale=None):
+PROPOSALS = {"bands": 0, "footprint": 1}
+
+
+def poisson_view(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.01, seed=0, view=0, tile_scale=None,
+                 proposal="bands", footprint_scale=0.0):
     """Poisson-rate sampling of one view: (ids (W,H,kcap) int32, 1/pi (W,H,kcap) float32,
     densities (D0, D1) (2,W,H) float32, overflow flag).  tile_scale: optional
-    (ceil(W/16), ceil(H/16)) rate multipliers."""
+    (ceil(W/16), ceil(H/16)) rate multipliers.  proposal "footprint": band 0 plus
+    each splat's own footprint, normalised by footprint_scale (> 0) or by the
+    view's mean footprint mass over h^1.5 (D1 is then 0)."""
     pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float32)
     ts = None
     if tile_scale is not None:
@@ -490,19 +501,57 @@ def poisson_view(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.0
                               ctypes.c_double(f), ctypes.c_double(smoothing), ctypes.c_double(precision),
                               ctypes.c_double(rate), _p(ts) if ts is not None else None, _p(pts), _p(nrm), _p(sd),
                               ctypes.c_uint32(seed & 0xFFFFFFFF), ctypes.c_uint32((seed >> 32) & 0xFFFFFFFF),
-                              ctypes.c_uint32(view), _p(ids), _p(ipi), _p(dens))
+                              ctypes.c_uint32(view), _p(ids), _p(ipi), _p(dens),
+                              ctypes.c_int(PROPOSALS[proposal]), ctypes.c_double(footprint_scale))
     if rc < 0:
         raise RuntimeError("oracle poisson_view failed: %d" % rc)
     return ids, ipi, dens, rc
 
 
-def poisson(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.01, seed=0, tile_scale=None):
+def foot_view(W, H, f, pts, nrm, sd, smoothing=0.0, precision=0.01):
+    """The footprint proposal's records of one view's sorted leaves and its node
+    bounds: (leaf (nc, 8) float32: x, y, t00, t01, t11, amp, m, 0; node (2 nc - 1, 2)
+    float32: largest amp, smallest m; sorted splat ids (nc,) int32; parent links
+    (2 nc - 1,) int32; total footprint mass)."""
+    pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float32)
+    N = pts.shape[0]
+    leaf = np.zeros((max(N, 1), 8), np.float32)
+    node = np.zeros((max(2 * N, 1), 2), np.float32)
+    ids = np.zeros(max(N, 1), np.int32)
+    par = np.zeros(max(2 * N, 1), np.int32)
+    mass = np.zeros(1, np.float64)
+    nc = lib().o_foot_view(ctypes.c_int(N), ctypes.c_int(W), ctypes.c_int(H), ctypes.c_double(f),
+                           ctypes.c_double(smoothing), ctypes.c_double(precision), _p(pts), _p(nrm), _p(sd),
+                           _p(leaf), _p(node), _p(ids), _p(par), _p(mass))
+    if nc < 0:
+        raise RuntimeError("oracle foot_view failed: %d" % nc)
+    return leaf[:nc], node[:max(2 * nc - 1, 0)], ids[:nc], par[:max(2 * nc - 1, 0)], float(mass[0])
+
+
+def foot_terms(leaf, q):
+    """Footprint terms min(1, w(q), envelope(q)) of leaf records (n, 8) at points q (n, 2)."""
+    leaf = _c(leaf, np.float32); q = _c(q, np.float32)
+    out = np.zeros(leaf.shape[0], np.float32)
+    lib().o_foot_terms(ctypes.c_int(leaf.shape[0]), _p(leaf), _p(q), _p(out))
+    return out
+
+
+def foot_bounds(node, d2):
+    """Node bounds min(1, amax exp(-d2 mmin) (1 + 2^-20)) of node records (n, 2) at squared box distances (n,)."""
+    node = _c(node, np.float32); d2 = _c(d2, np.float32)
+    out = np.zeros(node.shape[0], np.float32)
+    lib().o_foot_bounds(ctypes.c_int(node.shape[0]), _p(node), _p(d2), _p(out))
+    return out
+
+
+def poisson(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.01, seed=0, tile_scale=None,
+            proposal="bands", footprint_scale=0.0):
     """Poisson-rate sampling for a batch: (ids (bn,W,H,kcap), 1/pi (bn,W,H,kcap))."""
     ids, ipi = [], []
     for b in range(pts.shape[0]):
         sdb = sd[b] if sd.shape[0] > 1 else sd[0]
         a, p_, _, _ = poisson_view(W, H, f, pts[b], nrm[b], sdb, kcap, rate, smoothing, precision, seed, b,
-                                   None if tile_scale is None else tile_scale[b])
+                                   None if tile_scale is None else tile_scale[b], proposal, footprint_scale)
         ids.append(a); ipi.append(p_)
     return np.stack(ids), np.stack(ipi)
 

@@ -934,7 +934,7 @@ static float o_dot3f(const float* a, const float* b)
 /* screen-space covariance V (row-major) of one splat in FP32, the operation
  * order of samplingCuda.cu:106-150 */
 static void o_cov_f32(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd,
-                      float smoothing, float V[4])
+                      float smoothing, float V[4], float* detA_out)
 {
     float sx, sy;
     o_screen_f32(Wf, Hf, ff, p, &sx, &sy);
@@ -956,6 +956,7 @@ static void o_cov_f32(float Wf, float Hf, float ff, const float* p, const float*
     u1[2] = u0[0] * n[1] - u0[1] * n[0];
     float A00 = nt0, A01 = o_dot3f(t1, u0), A10 = 0.0f, A11 = o_dot3f(t1, u1);
     float detA = A00 * A11 - A01 * A10;
+    if (detA_out) *detA_out = detA;
     float idA = 1.0f / detA;
     float J00 = A11 * idA, J01 = -A01 * idA, J10 = -A10 * idA, J11 = A00 * idA;
     float d0 = sd[0] * sd[0], d1 = sd[1] * sd[1];
@@ -973,7 +974,7 @@ static void o_cov_f32(float Wf, float Hf, float ff, const float* p, const float*
 float o_iso_hraw(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
 {
     float V[4];
-    o_cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V);
+    o_cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V, NULL);
     float V00 = V[0], V01 = V[1], V10 = V[2], V11 = V[3];
     float dV = V00 * V11 - V01 * V10;
     float idV = 1.0f / dV;
@@ -993,7 +994,7 @@ float o_iso_hraw(float Wf, float Hf, float ff, const float* p, const float* n, c
 float o_prop_h(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing)
 {
     float V[4];
-    o_cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V);
+    o_cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V, NULL);
     float tr = V[0] + V[3];
     float det = V[0] * V[3] - V[1] * V[2];
     float disc = 0.25f * (tr * tr) - det;
@@ -1563,12 +1564,16 @@ typedef struct {
     int nband;
     int p;
     int ok;
+    float* fleaf;     /* footprint proposal, per sorted leaf: x, y, t00, t01, t11, amp, m, 0 */
+    float* nfoot;     /* per node: largest amp, smallest m over its leaves */
+    double fmass;     /* sum of the leaves' footprint masses, in leaf order */
 } OFixed;
 
 static void o_fixed_free(OFixed* F)
 {
     for (int b = 0; b < 2; b++) { free(F->band[b].c6); free(F->band[b].acc6); }
     free(F->T.morton); free(F->T.ids); free(F->ib); free(F->fb); free(F->T.coef);
+    free(F->fleaf); free(F->nfoot);
 }
 
 /* query radius of a bandwidth, samplingCuda.cu:251-252 */
@@ -1816,8 +1821,105 @@ static float o_mix(const OMix* M, float qx, float qy, float x, float y)
     return m;
 }
 
+/* Footprint proposal (this build's definition, no reference counterpart): a
+ * splat's EWA weight w(q) = amp exp(-(u^2 + v^2) / 2), (u, v) = T (q - s),
+ * T^T T = V^-1 upper triangular, amp = 1 / (2 pi sqrt(det V) |det A|), V and A
+ * from the FP32 covariance above; the isotropic envelope amp exp(-|q - s|^2 m),
+ * m = 1 / (2 lambda_max(V)); the mass (integral of min(w, 1)) 2 pi sqrt(det V)
+ * (amp if amp <= 1, else 1 + ln amp).  Degenerate geometry: all zero. */
+static void o_foot_f32(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd, float smoothing,
+                       float* t00, float* t01, float* t11, float* amp, float* m, float* mass)
+{
+    float V[4], detA;
+    o_cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V, &detA);
+    float tr = V[0] + V[3];
+    float det = V[0] * V[3] - V[1] * V[2];
+    float disc = 0.25f * (tr * tr) - det;
+    float lmax = 0.5f * tr + sqrtf(fmaxf(disc, 0.0f));
+    *t00 = sqrtf(V[3] / det);
+    *t01 = -V[1] / sqrtf(V[3] * det);
+    *t11 = 1.0f / sqrtf(V[3]);
+    const float area = (2.0f * 3.14159265358979f) * sqrtf(det);
+    *amp = 1.0f / (area * fabsf(detA));
+    *m = 0.5f / lmax;
+    *mass = area * (*amp <= 1.0f ? *amp : 1.0f + ologf(*amp));
+    const int ok = det > 0.0f && lmax > 0.0f && V[3] > 0.0f && isfinite(*t00) && isfinite(*t01) && isfinite(*t11) &&
+                   isfinite(*amp) && isfinite(*m) && isfinite(*mass);
+    if (!ok) { *t00 = *t01 = *t11 = 0.0f; *amp = 0.0f; *m = 0.0f; *mass = 0.0f; }
+}
+
+/* footprint term of one leaf at q: min(1, w(q), envelope(q)) */
+static float o_foot_leaf(const float* L, float qx, float qy, float d2)
+{
+    const float dx = qx - L[0], dy = qy - L[1];
+    float u = L[2] * dx;
+    u = u + L[3] * dy;
+    const float v = L[4] * dy;
+    float e = u * u;
+    e = e + v * v;
+    const float w = L[5] * oexpf(-(0.5f * e));
+    const float env = L[5] * oexpf(-(d2 * L[6]));
+    return fminf(1.0f, fminf(w, env));
+}
+
+/* node bound: amax exp(-d2 mmin), inflated by 2^-20 */
+static float o_foot_bound(const float* nf, float d2)
+{
+    const float b = nf[0] * oexpf(-(d2 * nf[1]));
+    return fminf(1.0f, b * 1.00000095367431640625f);
+}
+
+static float o_foot_mix(float k0, float a0, float kf, float d2, float f)
+{
+    const float e0 = oexpf(-(d2 * a0));
+    float m = k0 * e0;
+    m = m + kf * f;
+    return m;
+}
+
+static float o_dist2(float qx, float qy, float x, float y)
+{
+    const float dx = qx - x, dy = qy - y;
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    return d2;
+}
+
+/* footprint records of the sorted leaves and the node bounds (max / min over
+ * each node's leaves: order-free, so any traversal gives the device's values) */
+static void o_foot_build(OFixed* F, int W, int H, double f, double smoothing, const float* pts, const float* nrm,
+                         const float* sd)
+{
+    const OTree* T = &F->T;
+    const int nc = T->nc;
+    const int L0 = nc - 1;
+    F->fleaf = (float*)calloc((size_t)8 * (nc > 0 ? nc : 1), sizeof(float));
+    F->nfoot = (float*)calloc((size_t)2 * (2 * nc > 1 ? 2 * nc : 1), sizeof(float));
+    F->fmass = 0.0;
+    for (int nd = 0; nd < L0; nd++) { F->nfoot[2 * nd] = 0.0f; F->nfoot[2 * nd + 1] = INFINITY; }
+    for (int pos = 0; pos < nc; pos++) {
+        const int id = T->ids[pos];
+        float* L = F->fleaf + 8 * pos;
+        const int lf = L0 + pos;
+        L[0] = T->cen[2 * lf]; L[1] = T->cen[2 * lf + 1];
+        float mass;
+        o_foot_f32((float)W, (float)H, (float)f, pts + 3L * id, nrm + 3L * id, sd + 2L * id, (float)smoothing,
+                   &L[2], &L[3], &L[4], &L[5], &L[6], &mass);
+        L[7] = 0.0f;
+        F->fmass += (double)mass;
+        F->nfoot[2 * lf] = L[5];
+        F->nfoot[2 * lf + 1] = L[6];
+        for (int nd = lf; nd != 0;) {
+            nd = T->parent[nd];
+            F->nfoot[2 * nd] = fmaxf(F->nfoot[2 * nd], L[5]);
+            F->nfoot[2 * nd + 1] = fminf(F->nfoot[2 * nd + 1], L[6]);
+        }
+    }
+}
+
 static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float lambda,
-                           uint32_t seed_lo, uint32_t seed_hi, uint32_t view, int32_t* ids, float* ipi)
+                           uint32_t seed_lo, uint32_t seed_hi, uint32_t view, int32_t* ids, float* ipi,
+                           int fmode, float fscale)
 {
     const OTree* T = &F->T;
     for (int s = 0; s < kcap; s++) { ids[s] = -1; ipi[s] = 0.0f; }
@@ -1825,14 +1927,25 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
     float qx = (float)i, qy = (float)j;
     float D0 = o_fixed_eval(F, 0, qx, qy);
     if (!(D0 >= T->prec)) return 0;
-    float D1 = o_fixed_eval(F, 1, qx, qy);
+    float D1 = fmode ? 0.0f : o_fixed_eval(F, 1, qx, qy);
     /* normalisers floored at one splat's peak c_b: where the pixel sits in a gap,
      * D_b (truncated at the query radius) misses the nearby splats it normalises */
     const float t0 = 0.5f / fmaxf(D0, F->band[0].c);
-    const float t1 = 0.5f / fmaxf(D1, F->band[1].c);
+    const float t1 = fmode ? 0.0f : 0.5f / fmaxf(D1, F->band[1].c);
     OMix M;
-    M.k0 = t0 * F->band[0].c; M.k1 = t1 * F->band[1].c;
-    M.a0 = F->band[0].is2h * F->band[0].is2h; M.a1 = F->band[1].is2h * F->band[1].is2h;
+    M.k0 = t0 * F->band[0].c; M.k1 = fmode ? 0.0f : t1 * F->band[1].c;
+    M.a0 = F->band[0].is2h * F->band[0].is2h; M.a1 = fmode ? 0.0f : F->band[1].is2h * F->band[1].is2h;
+    /* footprint half: f_i / (kw max(D0, c0)); kw the caller's scale or the mean
+     * footprint mass over h^1.5 */
+    float kf = 0.0f;
+    if (fmode) {
+        float kw = fscale;
+        if (!(kw > 0.0f)) {
+            const double hh = (double)F->band[0].h;
+            kw = (float)((F->fmass / (double)T->nc) / (hh * sqrt(hh)));
+        }
+        if (kw > 0.0f && kw < 3.0e38f) kf = t0 / kw;
+    }
     const int L0 = T->nc - 1;
     const uint32_t pixel = (uint32_t)(j + H * i);
     uint32_t step = 0;
@@ -1842,7 +1955,9 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
         const float* b = T->box + 4 * nd;
         float cx = fminf(fmaxf(qx, b[0]), b[2]);
         float cy = fminf(fmaxf(qy, b[1]), b[3]);
-        float pB = o_mix(&M, qx, qy, cx, cy);            /* proposal bound over the node */
+        const float d2b = o_dist2(qx, qy, cx, cy);
+        float pB = fmode ? o_foot_mix(M.k0, M.a0, kf, d2b, o_foot_bound(F->nfoot + 2 * nd, d2b))
+                         : o_mix(&M, qx, qy, cx, cy);    /* proposal bound over the node */
         float U = fminf(1.0f, lambda * pB);
         int descend = 0;
         if (pB >= O_PMIN && U > 0.0f) {
@@ -1862,7 +1977,14 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
                     if (!(r < (float)(last - pos))) break;
                     pos += (int)r + 1;
                     int lf = L0 + pos;
-                    float pl = o_mix(&M, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]);
+                    float pl;
+                    if (fmode) {
+                        const float* Lr = F->fleaf + 8 * pos;
+                        const float d2 = o_dist2(qx, qy, Lr[0], Lr[1]);
+                        pl = o_foot_mix(M.k0, M.a0, kf, d2, o_foot_leaf(Lr, qx, qy, d2));
+                    } else {
+                        pl = o_mix(&M, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]);
+                    }
                     if (!(pl >= O_PMIN)) continue;       /* outside the support */
                     float pi = fminf(1.0f, lambda * pl);
                     float a = uu[1];
@@ -1890,12 +2012,14 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
  * dens (optional, 2 W H): D0 then D1 per pixel */
 int o_poisson_view(int N, int W, int H, int kcap, double f, double smoothing, double prec, double lambda,
                    const float* tscale, const float* pts, const float* nrm, const float* sd, uint32_t seed_lo,
-                   uint32_t seed_hi, uint32_t view, int32_t* ids, float* ipi, float* dens)
+                   uint32_t seed_hi, uint32_t view, int32_t* ids, float* ipi, float* dens, int fmode, double fscale)
 {
     const int tsH = (H + 15) / 16;
     OFixed F;
-    int rc = (N > 0) ? o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, 2, &F) : 0;
+    int rc = (N > 0) ? o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, fmode ? 1 : 2, &F) : 0;
     if (N <= 0) memset(&F, 0, sizeof(F));
+    if (rc == 0 && fmode && F.ok) o_foot_build(&F, W, H, f, smoothing, pts, nrm, sd);
+    const float fs = fscale > 0.0 ? (float)fscale : 0.0f;
     int over = 0;
     if (rc == 0) {
         #pragma omp parallel for schedule(dynamic, 16) reduction(|:over)
@@ -1904,15 +2028,55 @@ int o_poisson_view(int N, int W, int H, int kcap, double f, double smoothing, do
             float lam = (float)lambda;
             if (tscale) lam = lam * tscale[(long)(i >> 4) * tsH + (j >> 4)];
             over |= o_poisson_pixel(&F, i, j, H, kcap, lam, seed_lo, seed_hi, view,
-                                    ids + (long)kcap * pix, ipi + (long)kcap * pix);
+                                    ids + (long)kcap * pix, ipi + (long)kcap * pix, fmode, fs);
             if (dens) {
                 dens[pix] = F.ok ? o_fixed_eval(&F, 0, (float)i, (float)j) : 0.0f;
-                dens[(long)W * H + pix] = F.ok ? o_fixed_eval(&F, 1, (float)i, (float)j) : 0.0f;
+                dens[(long)W * H + pix] = (F.ok && !fmode) ? o_fixed_eval(&F, 1, (float)i, (float)j) : 0.0f;
             }
         }
     }
     if (N > 0) o_fixed_free(&F);
     return rc < 0 ? rc : over;
+}
+
+/* footprint proposal records of one view (for tests): leaf (nc, 8), node
+ * (2 nc - 1, 2), sorted splat ids (nc), parent links (2 nc - 1), total mass;
+ * returns nc or < 0 */
+int o_foot_view(int N, int W, int H, double f, double smoothing, double prec, const float* pts, const float* nrm,
+                const float* sd, float* leaf, float* node, int32_t* ids, int32_t* parent, double* mass)
+{
+    OFixed F;
+    if (N <= 0) return 0;
+    int rc = o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, 1, &F);
+    if (rc < 0) { o_fixed_free(&F); return rc; }
+    const int nc = F.T.nc;
+    *mass = 0.0;
+    if (F.ok && nc > 0) {
+        o_foot_build(&F, W, H, f, smoothing, pts, nrm, sd);
+        memcpy(leaf, F.fleaf, sizeof(float) * 8 * (size_t)nc);
+        memcpy(node, F.nfoot, sizeof(float) * 2 * (size_t)(2 * nc - 1));
+        memcpy(ids, F.T.ids, sizeof(int32_t) * (size_t)nc);
+        memcpy(parent, F.T.parent, sizeof(int32_t) * (size_t)(2 * nc - 1));
+        *mass = F.fmass;
+    }
+    o_fixed_free(&F);
+    return F.ok ? nc : 0;
+}
+
+/* footprint terms f(q) of n (leaf record, point) pairs and node bounds of n
+ * (node record, squared box distance) pairs (for tests) */
+void o_foot_terms(int n, const float* leaf, const float* q, float* out)
+{
+    for (int i = 0; i < n; i++) {
+        const float* L = leaf + 8L * i;
+        const float d2 = o_dist2(q[2 * i], q[2 * i + 1], L[0], L[1]);
+        out[i] = o_foot_leaf(L, q[2 * i], q[2 * i + 1], d2);
+    }
+}
+
+void o_foot_bounds(int n, const float* node, const float* d2, float* out)
+{
+    for (int i = 0; i < n; i++) out[i] = o_foot_bound(node + 2L * i, d2[i]);
 }
 
 /* wide proposal bandwidth of one view (for tests) */

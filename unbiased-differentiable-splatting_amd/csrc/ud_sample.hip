@@ -98,6 +98,13 @@ struct PNode {
     int esc;                     // next node in preorder after the subtree, -1 after the last
     int first, last;             // sorted leaf range
 };
+// per sorted leaf: the footprint proposal's record (foot_f32)
+struct FootLeaf {
+    float x, y;                  // screen position (the tree's)
+    float t00, t01, t11, amp;    // whitening and peak of the EWA weight
+    float m, pad;                // envelope exponent factor
+};
+
 // per internal node: what the fixed-order density walk needs of its two
 // children, so that one wave-uniform load serves a descent
 struct FmmRec {
@@ -132,6 +139,10 @@ struct TreeWs {
     float* fixCoef[2];    // per band: 49 coefficients per accepted node
     float* hraw;          // per splat id: proposal bandwidth sqrt(lambda_max(V))
     PNode* pnode;         // per node: Poisson walk record
+    FootLeaf* footLeaf;   // per sorted leaf: footprint proposal record
+    float2* nodeFoot;     // per node: (largest amp, smallest m) of its leaves
+    int* footCnt;         // per internal node: fan-in arrival count
+    double* footSum;      // sum of the visible splats' footprint masses
     FmmRec* frec;         // per internal node: children for the density walk
     float* dens[2];       // per pixel and band: density D_b(q)
 };
@@ -184,6 +195,10 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     w.coefCap = fixedOrder ? 0 : coefCap;
     w.hraw = nullptr;
     w.pnode = nullptr;
+    w.footLeaf = nullptr;
+    w.nodeFoot = nullptr;
+    w.footCnt = nullptr;
+    w.footSum = nullptr;
     w.frec = nullptr;
     for (int b = 0; b < 2; b++) { w.fixOff[b] = w.fixList[b] = nullptr; w.fixCoef[b] = w.dens[b] = nullptr; }
     const int nb = mode == kModePoisson ? 2 : mode == kModeFixed ? 1 : 0;
@@ -196,6 +211,10 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     if (mode == kModePoisson) {
         w.hraw = (float*)take(4 * n);
         w.pnode = (PNode*)take(sizeof(PNode) * nn);
+        w.footLeaf = (FootLeaf*)take(sizeof(FootLeaf) * n);
+        w.nodeFoot = (float2*)take(sizeof(float2) * nn);
+        w.footCnt = (int*)take(4 * n);
+        w.footSum = (double*)take(8);
         w.dens[0] = (float*)take(4L * W * H);
         w.dens[1] = (float*)take(4L * W * H);
     }
@@ -207,7 +226,7 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
 // Screen-space covariance V (row-major) of one splat in FP32, the operation
 // order of samplingCuda.cu:106-150.
 __device__ void cov_f32(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd,
-                        float smoothing, float V[4])
+                        float smoothing, float V[4], float* detAOut = nullptr)
 {
     float sx, sy;
     screen_f32(Wf, Hf, ff, p, sx, sy);
@@ -225,6 +244,7 @@ __device__ void cov_f32(float Wf, float Hf, float ff, const float* p, const floa
     float u1[3] = { u0[1] * n[2] - u0[2] * n[1], u0[2] * n[0] - u0[0] * n[2], u0[0] * n[1] - u0[1] * n[0] };
     float A00 = nt0, A01 = dot3(t1, u0), A10 = 0.0f, A11 = dot3(t1, u1);
     float detA = A00 * A11 - A01 * A10;
+    if (detAOut) *detAOut = detA;
     float idA = 1.0f / detA;
     float J00 = A11 * idA, J01 = -A01 * idA, J10 = -A10 * idA, J11 = A00 * idA;
     float d0 = sd[0] * sd[0], d1 = sd[1] * sd[1];
@@ -266,6 +286,59 @@ __device__ float prop_h(float Wf, float Hf, float ff, const float* p, const floa
     float disc = 0.25f * (tr * tr) - det;
     float lmax = 0.5f * tr + sqrtf(fmaxf(disc, 0.0f));
     return sqrtf(lmax);
+}
+
+// Footprint proposal of Poisson-rate sampling (no reference counterpart): the
+// splat's own EWA weight w(q) = amp exp(-(u^2 + v^2) / 2), (u, v) = T (q - s),
+// T^T T = V^-1 (upper triangular), amp = 1 / (2 pi sqrt(det V) |det A|), with V
+// and A the FP32 screen covariance and Jacobian of cov_f32, and its isotropic
+// envelope amp exp(-|q - s|^2 m), m = 1 / (2 lambda_max(V)) >= the envelope's
+// exponent along any direction.  Degenerate geometry gives amp = 0 (the splat is
+// then drawn through the band-0 half of the mixture alone).
+__device__ void foot_f32(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd,
+                         float smoothing, float& t00, float& t01, float& t11, float& amp, float& m, float& mass)
+{
+    float V[4], detA;
+    cov_f32(Wf, Hf, ff, p, n, sd, smoothing, V, &detA);
+    float tr = V[0] + V[3];
+    float det = V[0] * V[3] - V[1] * V[2];
+    float disc = 0.25f * (tr * tr) - det;
+    float lmax = 0.5f * tr + sqrtf(fmaxf(disc, 0.0f));
+    t00 = sqrtf(V[3] / det);
+    t01 = -V[1] / sqrtf(V[3] * det);
+    t11 = 1.0f / sqrtf(V[3]);
+    const float area = (2.0f * 3.14159265358979f) * sqrtf(det);     // 2 pi sqrt(det V)
+    amp = 1.0f / (area * fabsf(detA));
+    m = 0.5f / lmax;
+    // integral of min(w, 1) over the plane
+    mass = area * (amp <= 1.0f ? amp : 1.0f + plogf(amp));
+    const bool ok = det > 0.0f && lmax > 0.0f && V[3] > 0.0f && isfinite(t00) && isfinite(t01) && isfinite(t11) &&
+                    isfinite(amp) && isfinite(m) && isfinite(mass);
+    if (!ok) { t00 = t01 = t11 = 0.0f; amp = 0.0f; m = 0.0f; mass = 0.0f; }
+}
+
+// the footprint term of one leaf at q: min(1, w(q), envelope(q)); the envelope
+// cap makes it a value the node bound (the same expression over the node's box
+// with the node's largest amp and smallest m) dominates in floating point
+__device__ __forceinline__ float foot_leaf(const FootLeaf& L, float qx, float qy, float d2)
+{
+    const float dx = qx - L.x, dy = qy - L.y;
+    float u = L.t00 * dx;
+    u = u + L.t01 * dy;
+    const float v = L.t11 * dy;
+    float e = u * u;
+    e = e + v * v;
+    const float w = L.amp * pexpf(-(0.5f * e));
+    const float env = L.amp * pexpf(-(d2 * L.m));
+    return fminf(1.0f, fminf(w, env));
+}
+
+// node bound of the footprint term: amax exp(-d2 mmin), inflated by 2^-20 so a
+// one-ulp step of the exponential between two arguments cannot put a leaf above it
+__device__ __forceinline__ float foot_bound(float2 nf, float d2)
+{
+    const float b = nf.x * pexpf(-(d2 * nf.y));
+    return fminf(1.0f, b * 1.00000095367431640625f);
 }
 
 // isotropic bandwidth capped at one pixel (samplingCuda.cu:160)
@@ -2287,6 +2360,15 @@ __device__ __forceinline__ float mix_d2(float k0, float a0, float k1, float a1, 
     return m;
 }
 
+// band-0 half plus footprint half: k0 e^{-a0 d2} + kf f (same order as mix_d2)
+__device__ __forceinline__ float foot_mix(float k0, float a0, float kf, float d2, float f)
+{
+    const float e0 = pexpf(-(d2 * a0));
+    float m = k0 * e0;
+    m = m + kf * f;
+    return m;
+}
+
 __device__ __forceinline__ float dist2(float qx, float qy, float x, float y)
 {
     const float dx = qx - x, dy = qy - y;
@@ -2336,6 +2418,85 @@ __global__ __launch_bounds__(256) void k_walk_records(const ViewState* __restric
     if (nd < L0) frec[nd] = fmm_record(nd, L0, geo, lnk, fixOff0, fixOff1);
 }
 
+__global__ __launch_bounds__(256) void k_foot_clear(const ViewState* __restrict__ vs, int* footCnt, double* footSum,
+                                                    VStride vsd)
+{
+    vs = vw(vs, vsd.ws);
+    footCnt = vw(footCnt, vsd.ws);
+    footSum = vw(footSum, vsd.ws);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < vs->nc - 1) footCnt[t] = 0;
+    if (t == 0) *footSum = 0.0;
+}
+
+// Footprint records of the sorted leaves and, bottom up, every internal node's
+// (largest amp, smallest m) over its leaves: the second thread to reach a node
+// merges its two children (release fence before the arrival count, acquire
+// after, as in the tree's fan-in).  footCnt and footSum start at zero.  The
+// visible splats' footprint masses (integrals of min(w, 1)) are summed for the
+// normaliser of the footprint half.
+__global__ __launch_bounds__(256) void k_foot_nodes(float Wf, float Hf, float ff, float smoothing,
+                                                    const float* __restrict__ pts, const float* __restrict__ nrm,
+                                                    const float* __restrict__ sd, const ViewState* __restrict__ vs,
+                                                    const int32_t* __restrict__ ids,
+                                                    const float2* __restrict__ leafXY,
+                                                    const NodeLnk* __restrict__ lnk, FootLeaf* __restrict__ footLeaf,
+                                                    float2* nodeFoot, int* footCnt, double* footSum, VStride vsd)
+{
+    pts = vw(pts, vsd.pts);
+    nrm = vw(nrm, vsd.pts);
+    sd = vw(sd, vsd.sd);
+    vs = vw(vs, vsd.ws);
+    ids = vw(ids, vsd.ws);
+    leafXY = vw(leafXY, vsd.ws);
+    lnk = vw(lnk, vsd.ws);
+    footLeaf = vw(footLeaf, vsd.ws);
+    nodeFoot = vw(nodeFoot, vsd.ws);
+    footCnt = vw(footCnt, vsd.ws);
+    footSum = vw(footSum, vsd.ws);
+    __shared__ double sm[4];
+    const int n = vs->nc;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double mass = 0.0;
+    if (t < n) {
+        const int id = ids[t];
+        FootLeaf L;
+        const float2 xy = leafXY[t];
+        L.x = xy.x; L.y = xy.y;
+        float ms;
+        foot_f32(Wf, Hf, ff, pts + 3L * id, nrm + 3L * id, sd + 2L * id, smoothing, L.t00, L.t01, L.t11, L.amp, L.m,
+                 ms);
+        L.pad = 0.0f;
+        footLeaf[t] = L;
+        mass = (double)ms;
+        const int L0 = n - 1;
+        int nd = L0 + t;
+        float2 v = make_float2(L.amp, L.m);
+        nodeFoot[nd] = v;
+        while (true) {
+            const int p = lnk[nd].parent;
+            if (p < 0) break;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            const int old = atomicAdd(&footCnt[p], 1);
+            if (old == 0) break;                          // the sibling's thread merges
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const NodeLnk pl = lnk[p];
+            const int sib = pl.left == nd ? pl.right : pl.left;
+            const float2 o = nodeFoot[sib];
+            v = make_float2(fmaxf(v.x, o.x), fminf(v.y, o.y));
+            nodeFoot[p] = v;
+            nd = p;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) mass += __shfl_xor(mass, o, 64);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mass;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+        if (b != 0.0) atomicAdd(footSum, b);
+    }
+}
+
 // One lane per pixel (one pixel tile per wavefront, TileCfg).  The walk
 // visits nodes left to right; a node's bound U (the mixture at the point of its
 // box closest to q) is >= pi_i of each of its leaves.  U = 0: skip the
@@ -2366,7 +2527,10 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                                                                    const float* __restrict__ dens0,
                                                                    const float* __restrict__ dens1, size_t densStride,
                                                                    int32_t* __restrict__ outIds,
-                                                                   float* __restrict__ outIpi, int* err)
+                                                                   float* __restrict__ outIpi, int* err, int fmode,
+                                                                   float fscale, const FootLeaf* __restrict__ footLeaf,
+                                                                   const float2* __restrict__ nodeFoot,
+                                                                   const double* __restrict__ footSum)
 {
     extern __shared__ int32_t stage[];
     const int lane = threadIdx.x & 63;
@@ -2387,6 +2551,11 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
     leafId = vw(leafId, vstride);
     dens0 = vw(dens0, densStride);
     dens1 = vw(dens1, densStride);
+    if (fmode) {
+        footLeaf = vw(footLeaf, vstride);
+        nodeFoot = vw(nodeFoot, vstride);
+        footSum = vw(footSum, vstride);
+    }
     const long perView = (long)W * H;
     outIds += (long)blockIdx.y * perView * kcap;
     outIpi += (long)blockIdx.y * perView * kcap;
@@ -2407,9 +2576,21 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         // (fminf drops a NaN): draw nothing there and say so
         if (!(lam >= 0.0f && lam <= 3.0e38f)) { atomicOr(err, 16); lam = 0.0f; }
         const float t0 = 0.5f / fmaxf(D0, B0.c);
-        const float t1 = 0.5f / fmaxf(dens1[pix], B1.c);
-        const float k0 = t0 * B0.c, k1 = t1 * B1.c;
-        const float a0 = B0.is2h * B0.is2h, a1 = B1.is2h * B1.is2h;
+        const float t1 = fmode ? 0.0f : 0.5f / fmaxf(dens1[pix], B1.c);
+        const float k0 = t0 * B0.c, k1 = fmode ? 0.0f : t1 * B1.c;
+        const float a0 = B0.is2h * B0.is2h, a1 = fmode ? 0.0f : B1.is2h * B1.is2h;
+        // footprint half: f_i(q) / (kw max(D0, c0)), kw = the visible splats' mean
+        // footprint mass over h^1.5 (so that both halves expect about lam / 2
+        // samples), or the caller's scale
+        float kf = 0.0f;
+        if (fmode) {
+            float kw = fscale;
+            if (!(kw > 0.0f)) {
+                const double hh = (double)B0.h;
+                kw = (float)((*footSum / (double)n) / (hh * sqrt(hh)));
+            }
+            if (kw > 0.0f && kw < 3.0e38f) kf = t0 / kw;
+        }
         const float qx = (float)i, qy = (float)j;
         const uint32_t pixel = (uint32_t)pix;
         uint32_t step = 0;                            // geometric-skip steps of this pixel
@@ -2418,9 +2599,12 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         while (true) {
             if (++visits > 2 * n) { atomicOr(err, 8); break; }
             const PNode pn = pnode[nd];
+            const float2 nf = fmode ? nodeFoot[nd] : make_float2(0.0f, 0.0f);
             float cx = fminf(fmaxf(qx, pn.lox), pn.hix);
             float cy = fminf(fmaxf(qy, pn.loy), pn.hiy);
-            float pB = mix_d2(k0, a0, k1, a1, dist2(qx, qy, cx, cy));   // proposal bound over the node
+            const float d2b = dist2(qx, qy, cx, cy);
+            // proposal bound over the node
+            float pB = fmode ? foot_mix(k0, a0, kf, d2b, foot_bound(nf, d2b)) : mix_d2(k0, a0, k1, a1, d2b);
             float U = fminf(1.0f, lam * pB);
             const int last = pn.last;
             int next = pn.esc;                        // past the subtree
@@ -2440,8 +2624,15 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                         float r = plogf(u) * il1;         // failures before the next candidate
                         if (!(r < (float)(last - pos))) break;
                         pos += (int)r + 1;
-                        const float2 lp = leafXY[pos];
-                        float pl = mix_d2(k0, a0, k1, a1, dist2(qx, qy, lp.x, lp.y));
+                        float pl;
+                        if (fmode) {
+                            const FootLeaf L = footLeaf[pos];
+                            const float d2 = dist2(qx, qy, L.x, L.y);
+                            pl = foot_mix(k0, a0, kf, d2, foot_leaf(L, qx, qy, d2));
+                        } else {
+                            const float2 lp = leafXY[pos];
+                            pl = mix_d2(k0, a0, k1, a1, dist2(qx, qy, lp.x, lp.y));
+                        }
                         if (!(pl >= kProposalPMin)) continue;     // outside the support
                         float pi = fminf(1.0f, lam * pl);
                         float a = u01(blk.y);
@@ -2737,6 +2928,11 @@ void launch_sample(int N, int W, int H, int k, uint32_t seedLo, uint32_t seedHi,
     if (timed) (void)hipEventRecord(e1, st);
 }
 
+// proposal of ude_sample_poisson: 0 the two density bands, 1 band 0 plus the
+// footprint half; scale 0 = normaliser from the view, > 0 the caller's
+int g_pois_mode = 0;
+double g_pois_scale = 0.0;
+
 // launch shapes of k_fmm_eval (0) and k_poisson (1); set by ude_set_tile_config
 TileCfg g_tile[2] = {{8, 3, 4, 0}, {8, 3, 4, 0}};
 
@@ -2893,6 +3089,8 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
     const uint32_t sLo = (uint32_t)seed, sHi = (uint32_t)(seed >> 32);
     const TileCfg pc = g_tile[1];
     const int p = g_fixed_p;
+    const int fmode = g_pois_mode;
+    const float fscale = (float)g_pois_scale;
     for (int g = 0; g < bn; g += span) {
         // every view of the group in one launch per build stage (view = blockIdx.y)
         TreeWs w;
@@ -2903,27 +3101,59 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         const float* sb = sd + (sdBatch > 1 ? 2L * N * g : 0);
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st, span, vsd);
         if (rc != UDE_OK) return rc;
-        k_wide_hist<<<dim3(grid1(N, 256) < 256u ? grid1(N, 256) : 256u, span), 256, 0, st>>>(w.vs, w.idsA, w.hraw,
-                                                                                          vsd);
-        k_wide_band<<<dim3(1, span), 256, 0, st>>>(w.vs, vsd);
-        fixed_expansions(N, 2, p, w, st, span, vsd);
+        // the footprint proposal needs no wide band: one density band, the
+        // footprint records and their node bounds instead
+        const int nbands = fmode ? 1 : 2;
+        if (!fmode) {
+            k_wide_hist<<<dim3(grid1(N, 256) < 256u ? grid1(N, 256) : 256u, span), 256, 0, st>>>(w.vs, w.idsA,
+                                                                                              w.hraw, vsd);
+            k_wide_band<<<dim3(1, span), 256, 0, st>>>(w.vs, vsd);
+        }
+        fixed_expansions(N, nbands, p, w, st, span, vsd);
         k_walk_records<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.fixOff[0],
-                                                                       w.fixOff[1], w.pnode, w.frec, vsd);
+                                                                       fmode ? nullptr : w.fixOff[1], w.pnode,
+                                                                       w.frec, vsd);
+        if (fmode) {
+            k_foot_clear<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.footCnt, w.footSum, vsd);
+            k_foot_nodes<<<dim3(grid1(N, 256), span), 256, 0, st>>>((float)W, (float)H, (float)f, (float)smoothing,
+                                                                    pb, nb, sb, w.vs, w.idsA, w.leafXY, w.lnk,
+                                                                    w.footLeaf, w.nodeFoot, w.footCnt, w.footSum,
+                                                                    vsd);
+        }
         float* d0 = densOut ? densOut + perView * 2 * g : w.dens[0];
         float* d1 = densOut ? d0 + perView : w.dens[1];
         const size_t dstride = densOut ? sizeof(float) * 2 * perView : per;
-        fmm_eval(W, H, span, 2, p, w, per, d0, dstride, (size_t)((char*)d1 - (char*)d0), st);
+        fmm_eval(W, H, span, nbands, p, w, per, d0, dstride, (size_t)((char*)d1 - (char*)d0), st);
+        if (fmode && densOut)
+            for (int v = 0; v < span; v++)
+                k_fill_f32<<<grid1(perView, 256), 256, 0, st>>>(d1 + perView * 2 * v, perView, 0.0f);
         const float* ts = tileScale ? tileScale + (long)tsW * tsH * g : nullptr;
         hipEvent_t e0, e1;
         bool timed = timing_pair(2, &e0, &e1);
         if (timed) (void)hipEventRecord(e0, st);
         k_poisson<<<tile_grid(pc, W, H, span), 64 * pc.wpb, kPoissonStageBytes * pc.wpb, st>>>(
             pc, W, H, kcap, (float)rate, ts, tsH, sLo, sHi, seedDev, (uint32_t)g, w.vs, per, w.pnode, w.leafXY,
-            w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag);
+            w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag, fmode,
+            fscale, w.footLeaf, w.nodeFoot, w.footSum);
         if (timed) (void)hipEventRecord(e1, st);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? UDE_OK : UDE_ERR_LAUNCH;
+}
+
+int ude_set_poisson_proposal(int mode, double footprintScale)
+{
+    if (mode != 0 && mode != 1) return UDE_ERR_ARG;
+    if (!(footprintScale <= 3.0e38)) return UDE_ERR_ARG;          // NaN or too large for float
+    g_pois_mode = mode;
+    g_pois_scale = footprintScale > 0.0 ? footprintScale : 0.0;
+    return UDE_OK;
+}
+
+int ude_get_poisson_proposal(double* footprintScale)
+{
+    if (footprintScale) *footprintScale = g_pois_scale;
+    return g_pois_mode;
 }
 
 int ude_set_fixed_degree(int p)

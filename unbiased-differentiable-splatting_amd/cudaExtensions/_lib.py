@@ -38,6 +38,8 @@ _SIGS = {
     "ude_mlp_forward": (c_int, [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
     "ude_mlp_loss_grad": (c_int, [c_int] + [c_ptr] * 7 + [c_size, c_ptr]),
     "ude_adam_step": (c_int, [c_int] + [c_ptr] * 4 + [ctypes.c_float] * 4 + [c_int, c_ptr]),
+    "ude_set_poisson_proposal": (c_int, [c_int, c_dbl]),
+    "ude_get_poisson_proposal": (c_int, [c_ptr]),
     "ude_set_fixed_degree": (c_int, [c_int]),
     "ude_get_fixed_degree": (c_int, []),
     "ude_set_tile_config": (c_int, [c_int] * 5),

@@ -125,9 +125,12 @@ def approximateFMMWeights(width, height, focallength, points, normals, colors, s
     return out
 
 
+PROPOSALS = {"bands": 0, "footprint": 1}
+
+
 def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, numSamples, smoothing,
                   precision=0.001, seed=None, rate=None, tileScale=None, densities=None, seedTensor=None,
-                  batched=True, errors=None):
+                  batched=True, errors=None, proposal="bands", footprintScale=None):
     """Poisson-rate sampling with inclusion probabilities (no reference counterpart).
 
     Every visible splat i is included at pixel q independently with the
@@ -153,7 +156,14 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
     extra ones dropped; bit 8: a walk exceeded its node bound; bit 16: a tile's
     rate times its multiplier was negative or not finite, nothing drawn there); it is never
     cleared here, so one flag can collect a whole training run and be checked at
-    a point that synchronises anyway (raise_errors(errors.item()))."""
+    a point that synchronises anyway (raise_errors(errors.item())).
+    ``proposal``: "bands" (the mixture above) or "footprint": band 0 plus each
+    splat's own footprint, pi_i = min(1, r (g0_i / (2 N0) + f_i / (2 kw N0))) with
+    f_i = min(1, w_i) capped by its isotropic envelope, w_i the splat's EWA weight
+    (FP32 screen covariance), N0 = max(D0, c0); kw = ``footprintScale`` or, when it
+    is None, the visible splats' mean footprint mass over h^1.5 (each half then
+    expects about r / 2 samples).  The footprint proposal evaluates one density
+    band (D1 in ``densities`` is 0)."""
     global last_error
     _check_splats(points, normals, stdDevs)
     dev = _lib.require_gpu(points, normals, stdDevs)
@@ -184,6 +194,12 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
         if densities.dtype != torch.float32 or tuple(densities.shape) != (bn, 2, W, H) or \
                 not densities.is_contiguous():
             raise ValueError("densities must be a contiguous float32 tensor of shape (bn, 2, W, H)")
+    if proposal not in PROPOSALS:
+        raise ValueError("proposal must be 'bands' or 'footprint'")
+    fs = 0.0 if footprintScale is None else float(footprintScale)
+    if proposal == "footprint" and footprintScale is not None and not (0.0 < fs < 3.0e38):
+        raise ValueError("footprintScale must be a positive finite number")
+    _lib.check(L.ude_set_poisson_proposal(PROPOSALS[proposal], fs), "sampling.samplePoisson")
     rc = L.ude_sample_poisson(bn, N, W, H, k, float(focallength), float(smoothing), float(precision), rate,
                               _lib.ptr(ts), _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs),
                               stdDevs.shape[0], s & 0xFFFFFFFFFFFFFFFF, _lib.ptr(seedTensor), _lib.ptr(ids),
