@@ -261,9 +261,12 @@ def timed_steps(fn, steps, kernel=None):
 
 
 def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
-    """The plain-rate step and, when the trained rate network reaches the plain
-    gradient variance with fewer samples, the learned-rate step at that rate.
-    Returns (chosen step, its name, the comparison record)."""
+    """The plain-rate step (band proposal at RATE) and the alternatives that reach
+    its gradient variance: the footprint proposal at the rate its measured kappa
+    gives, and, when the trained rate network needs fewer samples than the plain
+    rates, the learned-rate step.  Each alternative with kappa < 1 is checked at
+    its rate and timed against the plain step; the fastest is chosen.  Returns
+    (chosen step, its name, the comparison record)."""
     import torch
     from cudaExtensions import sampling
     from splatting_amd import learned as L
@@ -275,37 +278,100 @@ def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
 
     rec = {"estimator": "Horvitz-Thompson (sum of min(w, 1) / pi times albedo, no occlusion), L2 loss, raw adjoints",
            "plain_rate": RATE, "kcap": KCAP}
+    steps = {"plain": plain_step}
+    notes = []
+    # footprint proposal (band 0 plus each splat's own EWA footprint)
+    cmpf = L.compare_proposals(fr, list(range(30_000, 30_000 + seeds)))
+    rec["footprint"] = cmpf
+    if cmpf["kappa"] < 1.0:
+        # the rate of equal variance: bracketed from the kappa estimate (the
+        # variance falls faster than 1 / rate over this range: heavy tails at low
+        # rates), then bisected in log rate; each value is the median of 8 seed
+        # groups of 8
+        kf = cmpf["kcap_footprint"]
+        target_v = cmpf["band_grad_variance"]
+        tried = []
+
+        def var_at(r, it):
+            fr.proposal = "footprint"
+            try:
+                v, _ = L.gradient_variance(fr, list(range(60_000 + 1000 * it, 60_000 + 1000 * it + seeds)), rate=r,
+                                           kcap=kf, groups=8)
+            finally:
+                fr.proposal = "bands"
+            tried.append([r, v])
+            return v
+
+        r0 = cmpf["footprint_rate_equal_samples"] * cmpf["kappa"]
+        lo = hi = None
+        r, it = r0, 0
+        while (lo is None or hi is None) and it < 6:
+            if var_at(r, it) > target_v:
+                lo = r
+                r = r * 2.0 if hi is None else (r * hi) ** 0.5
+            else:
+                hi = r
+                r = r * 0.5 if lo is None else (lo * r) ** 0.5
+            it += 1
+        if lo is not None and hi is not None:
+            for _ in range(2):
+                mid = (lo * hi) ** 0.5
+                if var_at(mid, it) > target_v:
+                    lo = mid
+                else:
+                    hi = mid
+                it += 1
+        cmpf["rate_search"] = tried
+        if hi is None:
+            notes.append("the footprint proposal did not reach the band variance at the rates tried")
+        else:
+            rf = hi                                       # the lowest rate measured at or below the target
+            cmpf.update({"rate_equal_variance": rf, "grad_variance_at_rate": [v for r_, v in tried if r_ == rf][-1]})
+
+            def footprint_step():
+                counter[0] += 1
+                fr.proposal = "footprint"
+                try:
+                    fr.step(7000 + counter[0], rate=rf, kcap=kf)
+                finally:
+                    fr.proposal = "bands"
+
+            steps["footprint"] = footprint_step
+    else:
+        notes.append("footprint needs %.2fx the samples" % cmpf["kappa"])
+    # learned per-tile rates
     if not use_learned or not os.path.exists(L.DEFAULT_WEIGHTS):
-        rec["chosen"] = "plain (%s)" % ("learned rates not compared" if not use_learned else "no trained rate network")
-        return plain_step, "plain", rec
-    net = L.RateNet.load(L.DEFAULT_WEIGHTS, dev)
-    cmp = L.compare_rates(fr, net, list(range(20_000, 20_000 + seeds)))
-    ts = cmp.pop("tile_scale")
-    rec.update(cmp)
-    kappa = cmp["kappa"]
-    if kappa >= 1.0:
-        rec["chosen"] = "plain (fallback: the learned rates need %.2fx the samples for the plain variance)" % kappa
-        return plain_step, "plain", rec
-    kcl = cmp["kcap_learned"]
+        notes.append("learned rates not compared" if not use_learned else "no trained rate network")
+    else:
+        net = L.RateNet.load(L.DEFAULT_WEIGHTS, dev)
+        cmp = L.compare_rates(fr, net, list(range(20_000, 20_000 + seeds)))
+        ts = cmp.pop("tile_scale")
+        rec.update(cmp)
+        kappa = cmp["kappa"]
+        if kappa >= 1.0:
+            notes.append("the learned rates need %.2fx the samples" % kappa)
+        else:
+            kcl = cmp["kcap_learned"]
 
-    def learned_step():
-        counter[0] += 1
-        _, _, _, g = fr.step(5000 + counter[0], tile_scale=ts, rate=RATE * kappa, kcap=kcl)
-        P, Nc, nb = fr.camera()
-        f2 = sampling.tileFeatures(fr.W, fr.H, fr.f, P, nb, fr.S, 0.0, fr.prec, fr.dens, g)
-        ts.copy_(net.tile_scale(f2[0], L.tile_counts(fr.last_ids)[0])[None])
+            def learned_step():
+                counter[0] += 1
+                _, _, _, g = fr.step(5000 + counter[0], tile_scale=ts, rate=RATE * kappa, kcap=kcl)
+                P, Nc, nb = fr.camera()
+                f2 = sampling.tileFeatures(fr.W, fr.H, fr.f, P, nb, fr.S, 0.0, fr.prec, fr.dens, g)
+                ts.copy_(net.tile_scale(f2[0], L.tile_counts(fr.last_ids)[0])[None])
 
-    v_check, _ = L.gradient_variance(fr, list(range(40_000, 40_000 + seeds)), tile_scale=ts, rate=RATE * kappa,
-                                     kcap=kcl, groups=4)
-    plain_ms, _, _ = timed_steps(plain_step, probe_steps)
-    learned_ms, _, _ = timed_steps(learned_step, probe_steps)
-    rec.update({"learned_rate": RATE * kappa, "learned_grad_variance_at_rate": v_check,
-                "probe_ms": {"plain": plain_ms, "learned": learned_ms}})
-    if learned_ms < plain_ms:
-        rec["chosen"] = "learned"
-        return learned_step, "learned", rec
-    rec["chosen"] = "plain (fallback: the learned step is not faster at equal variance)"
-    return plain_step, "plain", rec
+            v_check, _ = L.gradient_variance(fr, list(range(40_000, 40_000 + seeds)), tile_scale=ts,
+                                             rate=RATE * kappa, kcap=kcl, groups=4)
+            rec.update({"learned_rate": RATE * kappa, "learned_grad_variance_at_rate": v_check})
+            steps["learned"] = learned_step
+    print("bench: footprint kappa %.3f %s, samples per pixel %.2f (bands %.2f)" % (
+        cmpf["kappa"], cmpf["kappa_ci95"], cmpf["samples_per_pixel_footprint"], cmpf["samples_per_pixel_bands"]),
+        file=sys.stderr, flush=True)
+    probe = {name: timed_steps(st, probe_steps)[0] for name, st in steps.items()}
+    rec["probe_ms"] = probe
+    chosen = min(probe, key=probe.get)
+    rec["chosen"] = chosen + ("" if not notes else " (%s)" % "; ".join(notes))
+    return steps[chosen], chosen, rec
 
 
 def reference_path(dev, args, sc_n):
@@ -328,8 +394,11 @@ def reference_path(dev, args, sc_n):
 
     for _ in range(args.warmup):
         step()
+    # the frame rate from plain steps; the per-op events and the sampler's kernel
+    # events from separate instrumented steps (they add their own launches)
+    ms, _, _ = timed_steps(step, args.steps)
     timer.active = True
-    ms, ks_ms, ks_n = timed_steps(step, args.steps, "sample")
+    _, ks_ms, ks_n = timed_steps(step, args.steps, "sample")
     timer.active = False
     ops = timer.summary(args.steps)
     Np, Pp, k = sc.n, sc.width * sc.height, 40
@@ -410,12 +479,13 @@ def poisson_extra(dev, args, sc):
     from splatting_amd import learned as L
     target = L.make_target(sc, dev, RATE, KCAP)
     fr = L.PoissonFrame(sc, dev, rate=RATE, kcap=KCAP, target=target)
-    print("bench: plain vs learned rates", file=sys.stderr, flush=True)
+    print("bench: proposals and rates at equal gradient variance", file=sys.stderr, flush=True)
     step, chosen, rates = poisson_paths(dev, fr, use_learned=not args.no_rates)
     for _ in range(args.warmup):
         step()
     print("bench: Poisson-rate frames (%s rates)" % chosen, file=sys.stderr, flush=True)
-    ms, fe_ms, fe_n = timed_steps(step, args.steps, "fmm_eval")
+    ms, _, _ = timed_steps(step, args.steps)
+    _, fe_ms, fe_n = timed_steps(step, max(args.steps // 2, 1), "fmm_eval")
     _, po_ms, po_n = timed_steps(step, max(args.steps // 2, 1), "poisson")
     # Algorithmic bytes per launch, n = splats in view, P = pixels:
     #   k_fmm_eval (both bands): per band the 80 B children record of each of the n - 1
@@ -440,7 +510,8 @@ def poisson_extra(dev, args, sc):
         k["traffic"] = measured_traffic(TRAFFIC_SUMMARY[name])
     return {"workload": "the same scene, Poisson-rate sampling at rate %g (%.2f samples per pixel, 1/pi weights), EWA "
                         "weights, albedo interpolation without occlusion (Horvitz-Thompson, unbiased), L2, fwd+bwd; "
-                        "%s rates at the plain rates' gradient variance" % (RATE, spp, chosen),
+                        "the fastest of the band proposal at rate %g, the footprint proposal and the learned rates "
+                        "at the band proposal's gradient variance: %s" % (RATE, spp, RATE, chosen),
             "fps": 1000.0 / ms, "ms_per_step": ms, "samples_per_pixel": spp, "splats_in_view": n_vis,
             "compositing": False, "rates": rates, "kernels": kern}
 
@@ -449,7 +520,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--splats", type=int, default=None, help="override the splat count (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rates", action="store_true", help="time the plain rates without the learned comparison")

@@ -96,15 +96,17 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
                        float* densOut, int* errFlag, void* ws, size_t wsBytes, ude_stream_t stream);
 
 /* Proposal of ude_sample_poisson (no counterpart in the reference).  mode 0:
- * the two density bands above.  mode 1: band 0 plus each splat's own
- * footprint, p_i = g0_i / (2 N0) + f_i / (2 kw N0), f_i(q) = min(1, w_i(q),
- * its isotropic envelope) with w_i the splat's EWA weight (FP32 screen
- * covariance and Jacobian, amp exp(-(u^2 + v^2) / 2)), N0 = max(D0, c0), and
+ * the two density bands above.  mode 1: the wide band plus each splat's own
+ * footprint, p_i = g1_i / (2 N1) + f_i / (2 kw N1), f_i(q) = min(1, w_i(q),
+ * E(|q - s_i|)) with w_i the splat's EWA weight (FP32 screen covariance and
+ * Jacobian, amp exp(-(u^2 + v^2) / 2)), E(d) = exp(-max(0, d - 2)^2 / 2h_1^2) a
+ * cap common to all splats (a node's bound is E at its box distance),
+ * N1 = max(D1, c1), and
  * kw = footprintScale, or, when footprintScale <= 0, the visible splats' mean
- * footprint mass (integral of min(w, 1)) over h^1.5, which gives each half about
- * r / 2 expected samples.  Mode 1 evaluates one density band (densOut's D1 is
- * 0).  Applies to later calls in the process (one setting, not per thread).
- * ude_get_poisson_proposal returns the mode and writes the scale. */
+ * footprint mass (integral of min(w, 1)) over h_1^1.5, which gives each half
+ * about r / 2 expected samples.  Applies to later calls in the process (one
+ * setting, not per thread).  ude_get_poisson_proposal returns the mode and
+ * writes the scale. */
 int ude_set_poisson_proposal(int mode, double footprintScale);
 int ude_get_poisson_proposal(double* footprintScale);
 

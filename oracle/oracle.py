@@ -45,10 +45,10 @@ def lib():
         _lib.o_poisson_view.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_double] * 4 + [ctypes.c_void_p] * 4 +
                                        [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_double])
         _lib.o_foot_view.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 3 +
-                                    [ctypes.c_void_p] * 5)
+                                    [ctypes.c_void_p] * 4)
         _lib.o_foot_view.restype = ctypes.c_int
-        _lib.o_foot_terms.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3
-        _lib.o_foot_bounds.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3
+        _lib.o_foot_terms.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_float] + [ctypes.c_void_p] * 2
+        _lib.o_foot_bounds.argtypes = [ctypes.c_int, ctypes.c_float] + [ctypes.c_void_p] * 2
         _lib.o_set_fixed_degree.restype = ctypes.c_int
         _lib.o_set_fixed_degree.argtypes = [ctypes.c_int]
         _lib.o_poisson_wide_h.restype = ctypes.c_float
@@ -486,9 +486,9 @@ def poisson_view(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.0
                  proposal="bands", footprint_scale=0.0):
     """Poisson-rate sampling of one view: (ids (W,H,kcap) int32, 1/pi (W,H,kcap) float32,
     densities (D0, D1) (2,W,H) float32, overflow flag).  tile_scale: optional
-    (ceil(W/16), ceil(H/16)) rate multipliers.  proposal "footprint": band 0 plus
-    each splat's own footprint, normalised by footprint_scale (> 0) or by the
-    view's mean footprint mass over h^1.5 (D1 is then 0)."""
+    (ceil(W/16), ceil(H/16)) rate multipliers.  proposal "footprint": the wide
+    band plus each splat's own footprint, normalised by footprint_scale (> 0) or
+    by the view's mean footprint mass over h_1^1.5."""
     pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float32)
     ts = None
     if tile_scale is not None:
@@ -509,38 +509,36 @@ def poisson_view(W, H, f, pts, nrm, sd, kcap, rate, smoothing=0.0, precision=0.0
 
 
 def foot_view(W, H, f, pts, nrm, sd, smoothing=0.0, precision=0.01):
-    """The footprint proposal's records of one view's sorted leaves and its node
-    bounds: (leaf (nc, 8) float32: x, y, t00, t01, t11, amp, m, 0; node (2 nc - 1, 2)
-    float32: largest amp, smallest m; sorted splat ids (nc,) int32; parent links
-    (2 nc - 1,) int32; total footprint mass)."""
+    """The footprint proposal's records of one view's sorted leaves: (leaf (nc, 8)
+    float32: x, y, t00, t01, t11, amp, 0, 0; sorted splat ids (nc,) int32; parent
+    links (2 nc - 1,) int32; total footprint mass; the cap's decay a1)."""
     pts = _c(pts, np.float32); nrm = _c(nrm, np.float32); sd = _c(sd, np.float32)
     N = pts.shape[0]
     leaf = np.zeros((max(N, 1), 8), np.float32)
-    node = np.zeros((max(2 * N, 1), 2), np.float32)
     ids = np.zeros(max(N, 1), np.int32)
     par = np.zeros(max(2 * N, 1), np.int32)
-    mass = np.zeros(1, np.float64)
+    mass = np.zeros(2, np.float64)
     nc = lib().o_foot_view(ctypes.c_int(N), ctypes.c_int(W), ctypes.c_int(H), ctypes.c_double(f),
                            ctypes.c_double(smoothing), ctypes.c_double(precision), _p(pts), _p(nrm), _p(sd),
-                           _p(leaf), _p(node), _p(ids), _p(par), _p(mass))
+                           _p(leaf), _p(ids), _p(par), _p(mass))
     if nc < 0:
         raise RuntimeError("oracle foot_view failed: %d" % nc)
-    return leaf[:nc], node[:max(2 * nc - 1, 0)], ids[:nc], par[:max(2 * nc - 1, 0)], float(mass[0])
+    return leaf[:nc], ids[:nc], par[:max(2 * nc - 1, 0)], float(mass[0]), np.float32(mass[1])
 
 
-def foot_terms(leaf, q):
-    """Footprint terms min(1, w(q), envelope(q)) of leaf records (n, 8) at points q (n, 2)."""
+def foot_terms(leaf, a1, q):
+    """Footprint terms min(1, w(q), E(|q - s|)) of leaf records (n, 8) at points q (n, 2)."""
     leaf = _c(leaf, np.float32); q = _c(q, np.float32)
     out = np.zeros(leaf.shape[0], np.float32)
-    lib().o_foot_terms(ctypes.c_int(leaf.shape[0]), _p(leaf), _p(q), _p(out))
+    lib().o_foot_terms(ctypes.c_int(leaf.shape[0]), _p(leaf), ctypes.c_float(a1), _p(q), _p(out))
     return out
 
 
-def foot_bounds(node, d2):
-    """Node bounds min(1, amax exp(-d2 mmin) (1 + 2^-20)) of node records (n, 2) at squared box distances (n,)."""
-    node = _c(node, np.float32); d2 = _c(d2, np.float32)
-    out = np.zeros(node.shape[0], np.float32)
-    lib().o_foot_bounds(ctypes.c_int(node.shape[0]), _p(node), _p(d2), _p(out))
+def foot_bounds(a1, d2):
+    """Node bounds of the footprint term at squared box distances d2 (n,)."""
+    d2 = _c(d2, np.float32)
+    out = np.zeros(d2.shape[0], np.float32)
+    lib().o_foot_bounds(ctypes.c_int(d2.shape[0]), ctypes.c_float(a1), _p(d2), _p(out))
     return out
 
 

@@ -1564,8 +1564,8 @@ typedef struct {
     int nband;
     int p;
     int ok;
-    float* fleaf;     /* footprint proposal, per sorted leaf: x, y, t00, t01, t11, amp, m, 0 */
-    float* nfoot;     /* per node: largest amp, smallest m over its leaves */
+    float* fleaf;     /* footprint proposal, per sorted leaf: x, y, t00, t01, t11, amp, 0, 0 */
+    float fa1;        /* the wide band's decay 1 / 2h_1^2, the cap's decay */
     double fmass;     /* sum of the leaves' footprint masses, in leaf order */
 } OFixed;
 
@@ -1573,7 +1573,7 @@ static void o_fixed_free(OFixed* F)
 {
     for (int b = 0; b < 2; b++) { free(F->band[b].c6); free(F->band[b].acc6); }
     free(F->T.morton); free(F->T.ids); free(F->ib); free(F->fb); free(F->T.coef);
-    free(F->fleaf); free(F->nfoot);
+    free(F->fleaf);
 }
 
 /* query radius of a bandwidth, samplingCuda.cu:251-252 */
@@ -1848,8 +1848,17 @@ static void o_foot_f32(float Wf, float Hf, float ff, const float* p, const float
     if (!ok) { *t00 = *t01 = *t11 = 0.0f; *amp = 0.0f; *m = 0.0f; *mass = 0.0f; }
 }
 
-/* footprint term of one leaf at q: min(1, w(q), envelope(q)) */
-static float o_foot_leaf(const float* L, float qx, float qy, float d2)
+/* the footprint cap E(d) = exp(-max(0, d - 2)^2 a1): one envelope for every
+ * splat, so a node's bound is E at its box distance */
+#define O_FOOT_RCAP 2.0f
+static float o_foot_env(float a1, float d)
+{
+    const float e = fmaxf(0.0f, d - O_FOOT_RCAP);
+    return oexpf(-((e * e) * a1));
+}
+
+/* footprint term of one leaf at q: min(1, w(q), E(|q - s|)) */
+static float o_foot_leaf(const float* L, float a1, float qx, float qy, float d2)
 {
     const float dx = qx - L[0], dy = qy - L[1];
     float u = L[2] * dx;
@@ -1858,15 +1867,13 @@ static float o_foot_leaf(const float* L, float qx, float qy, float d2)
     float e = u * u;
     e = e + v * v;
     const float w = L[5] * oexpf(-(0.5f * e));
-    const float env = L[5] * oexpf(-(d2 * L[6]));
-    return fminf(1.0f, fminf(w, env));
+    return fminf(1.0f, fminf(w, o_foot_env(a1, sqrtf(d2))));
 }
 
-/* node bound: amax exp(-d2 mmin), inflated by 2^-20 */
-static float o_foot_bound(const float* nf, float d2)
+/* node bound: E at the squared box distance, inflated by 2^-20 */
+static float o_foot_bound(float a1, float d2)
 {
-    const float b = nf[0] * oexpf(-(d2 * nf[1]));
-    return fminf(1.0f, b * 1.00000095367431640625f);
+    return fminf(1.0f, o_foot_env(a1, sqrtf(d2)) * 1.00000095367431640625f);
 }
 
 static float o_foot_mix(float k0, float a0, float kf, float d2, float f)
@@ -1885,35 +1892,26 @@ static float o_dist2(float qx, float qy, float x, float y)
     return d2;
 }
 
-/* footprint records of the sorted leaves and the node bounds (max / min over
- * each node's leaves: order-free, so any traversal gives the device's values) */
+/* footprint records of the sorted leaves and the sum of their masses (in leaf order) */
 static void o_foot_build(OFixed* F, int W, int H, double f, double smoothing, const float* pts, const float* nrm,
                          const float* sd)
 {
     const OTree* T = &F->T;
     const int nc = T->nc;
     const int L0 = nc - 1;
+    F->fa1 = F->band[1].is2h * F->band[1].is2h;
     F->fleaf = (float*)calloc((size_t)8 * (nc > 0 ? nc : 1), sizeof(float));
-    F->nfoot = (float*)calloc((size_t)2 * (2 * nc > 1 ? 2 * nc : 1), sizeof(float));
     F->fmass = 0.0;
-    for (int nd = 0; nd < L0; nd++) { F->nfoot[2 * nd] = 0.0f; F->nfoot[2 * nd + 1] = INFINITY; }
     for (int pos = 0; pos < nc; pos++) {
         const int id = T->ids[pos];
         float* L = F->fleaf + 8 * pos;
         const int lf = L0 + pos;
         L[0] = T->cen[2 * lf]; L[1] = T->cen[2 * lf + 1];
-        float mass;
+        float m, mass;
         o_foot_f32((float)W, (float)H, (float)f, pts + 3L * id, nrm + 3L * id, sd + 2L * id, (float)smoothing,
-                   &L[2], &L[3], &L[4], &L[5], &L[6], &mass);
-        L[7] = 0.0f;
+                   &L[2], &L[3], &L[4], &L[5], &m, &mass);
+        L[6] = L[7] = 0.0f;
         F->fmass += (double)mass;
-        F->nfoot[2 * lf] = L[5];
-        F->nfoot[2 * lf + 1] = L[6];
-        for (int nd = lf; nd != 0;) {
-            nd = T->parent[nd];
-            F->nfoot[2 * nd] = fmaxf(F->nfoot[2 * nd], L[5]);
-            F->nfoot[2 * nd + 1] = fminf(F->nfoot[2 * nd + 1], L[6]);
-        }
     }
 }
 
@@ -1927,24 +1925,25 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
     float qx = (float)i, qy = (float)j;
     float D0 = o_fixed_eval(F, 0, qx, qy);
     if (!(D0 >= T->prec)) return 0;
-    float D1 = fmode ? 0.0f : o_fixed_eval(F, 1, qx, qy);
+    float D1 = o_fixed_eval(F, 1, qx, qy);
     /* normalisers floored at one splat's peak c_b: where the pixel sits in a gap,
      * D_b (truncated at the query radius) misses the nearby splats it normalises */
     const float t0 = 0.5f / fmaxf(D0, F->band[0].c);
-    const float t1 = fmode ? 0.0f : 0.5f / fmaxf(D1, F->band[1].c);
+    const float t1 = 0.5f / fmaxf(D1, F->band[1].c);
     OMix M;
-    M.k0 = t0 * F->band[0].c; M.k1 = fmode ? 0.0f : t1 * F->band[1].c;
-    M.a0 = F->band[0].is2h * F->band[0].is2h; M.a1 = fmode ? 0.0f : F->band[1].is2h * F->band[1].is2h;
-    /* footprint half: f_i / (kw max(D0, c0)); kw the caller's scale or the mean
-     * footprint mass over h^1.5 */
+    M.k0 = t0 * F->band[0].c; M.k1 = t1 * F->band[1].c;
+    M.a0 = F->band[0].is2h * F->band[0].is2h; M.a1 = F->band[1].is2h * F->band[1].is2h;
+    /* footprint mode: the wide band's half, and in place of band 0 the footprint
+     * half f_i / (kw max(D1, c1)); kw the caller's scale or the mean footprint
+     * mass over h_1^1.5 */
     float kf = 0.0f;
     if (fmode) {
         float kw = fscale;
         if (!(kw > 0.0f)) {
-            const double hh = (double)F->band[0].h;
+            const double hh = (double)F->band[1].h;
             kw = (float)((F->fmass / (double)T->nc) / (hh * sqrt(hh)));
         }
-        if (kw > 0.0f && kw < 3.0e38f) kf = t0 / kw;
+        if (kw > 0.0f && kw < 3.0e38f) kf = t1 / kw;
     }
     const int L0 = T->nc - 1;
     const uint32_t pixel = (uint32_t)(j + H * i);
@@ -1956,7 +1955,7 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
         float cx = fminf(fmaxf(qx, b[0]), b[2]);
         float cy = fminf(fmaxf(qy, b[1]), b[3]);
         const float d2b = o_dist2(qx, qy, cx, cy);
-        float pB = fmode ? o_foot_mix(M.k0, M.a0, kf, d2b, o_foot_bound(F->nfoot + 2 * nd, d2b))
+        float pB = fmode ? o_foot_mix(M.k1, M.a1, kf, d2b, o_foot_bound(F->fa1, d2b))
                          : o_mix(&M, qx, qy, cx, cy);    /* proposal bound over the node */
         float U = fminf(1.0f, lambda * pB);
         int descend = 0;
@@ -1981,7 +1980,7 @@ static int o_poisson_pixel(const OFixed* F, int i, int j, int H, int kcap, float
                     if (fmode) {
                         const float* Lr = F->fleaf + 8 * pos;
                         const float d2 = o_dist2(qx, qy, Lr[0], Lr[1]);
-                        pl = o_foot_mix(M.k0, M.a0, kf, d2, o_foot_leaf(Lr, qx, qy, d2));
+                        pl = o_foot_mix(M.k1, M.a1, kf, d2, o_foot_leaf(Lr, F->fa1, qx, qy, d2));
                     } else {
                         pl = o_mix(&M, qx, qy, T->cen[2 * lf], T->cen[2 * lf + 1]);
                     }
@@ -2016,7 +2015,7 @@ int o_poisson_view(int N, int W, int H, int kcap, double f, double smoothing, do
 {
     const int tsH = (H + 15) / 16;
     OFixed F;
-    int rc = (N > 0) ? o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, fmode ? 1 : 2, &F) : 0;
+    int rc = (N > 0) ? o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, 2, &F) : 0;
     if (N <= 0) memset(&F, 0, sizeof(F));
     if (rc == 0 && fmode && F.ok) o_foot_build(&F, W, H, f, smoothing, pts, nrm, sd);
     const float fs = fscale > 0.0 ? (float)fscale : 0.0f;
@@ -2031,7 +2030,7 @@ int o_poisson_view(int N, int W, int H, int kcap, double f, double smoothing, do
                                     ids + (long)kcap * pix, ipi + (long)kcap * pix, fmode, fs);
             if (dens) {
                 dens[pix] = F.ok ? o_fixed_eval(&F, 0, (float)i, (float)j) : 0.0f;
-                dens[(long)W * H + pix] = (F.ok && !fmode) ? o_fixed_eval(&F, 1, (float)i, (float)j) : 0.0f;
+                dens[(long)W * H + pix] = F.ok ? o_fixed_eval(&F, 1, (float)i, (float)j) : 0.0f;
             }
         }
     }
@@ -2039,44 +2038,45 @@ int o_poisson_view(int N, int W, int H, int kcap, double f, double smoothing, do
     return rc < 0 ? rc : over;
 }
 
-/* footprint proposal records of one view (for tests): leaf (nc, 8), node
- * (2 nc - 1, 2), sorted splat ids (nc), parent links (2 nc - 1), total mass;
- * returns nc or < 0 */
+/* footprint proposal records of one view (for tests): leaf (nc, 8), sorted
+ * splat ids (nc), parent links (2 nc - 1), total mass (mass[0]) and the cap's
+ * decay a1 (mass[1]); returns nc or < 0 */
 int o_foot_view(int N, int W, int H, double f, double smoothing, double prec, const float* pts, const float* nrm,
-                const float* sd, float* leaf, float* node, int32_t* ids, int32_t* parent, double* mass)
+                const float* sd, float* leaf, int32_t* ids, int32_t* parent, double* mass)
 {
     OFixed F;
+    mass[0] = 0.0;
+    mass[1] = 0.0;
     if (N <= 0) return 0;
-    int rc = o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, 1, &F);
+    int rc = o_fixed_build(N, W, H, f, smoothing, prec, pts, nrm, sd, 2, &F);
     if (rc < 0) { o_fixed_free(&F); return rc; }
     const int nc = F.T.nc;
-    *mass = 0.0;
     if (F.ok && nc > 0) {
         o_foot_build(&F, W, H, f, smoothing, pts, nrm, sd);
         memcpy(leaf, F.fleaf, sizeof(float) * 8 * (size_t)nc);
-        memcpy(node, F.nfoot, sizeof(float) * 2 * (size_t)(2 * nc - 1));
         memcpy(ids, F.T.ids, sizeof(int32_t) * (size_t)nc);
         memcpy(parent, F.T.parent, sizeof(int32_t) * (size_t)(2 * nc - 1));
-        *mass = F.fmass;
+        mass[0] = F.fmass;
+        mass[1] = (double)F.fa1;
     }
     o_fixed_free(&F);
     return F.ok ? nc : 0;
 }
 
-/* footprint terms f(q) of n (leaf record, point) pairs and node bounds of n
- * (node record, squared box distance) pairs (for tests) */
-void o_foot_terms(int n, const float* leaf, const float* q, float* out)
+/* footprint terms f(q) of n (leaf record, point) pairs, and the node bound at
+ * n squared box distances (for tests) */
+void o_foot_terms(int n, const float* leaf, float a1, const float* q, float* out)
 {
     for (int i = 0; i < n; i++) {
         const float* L = leaf + 8L * i;
         const float d2 = o_dist2(q[2 * i], q[2 * i + 1], L[0], L[1]);
-        out[i] = o_foot_leaf(L, q[2 * i], q[2 * i + 1], d2);
+        out[i] = o_foot_leaf(L, a1, q[2 * i], q[2 * i + 1], d2);
     }
 }
 
-void o_foot_bounds(int n, const float* node, const float* d2, float* out)
+void o_foot_bounds(int n, float a1, const float* d2, float* out)
 {
-    for (int i = 0; i < n; i++) out[i] = o_foot_bound(node + 2L * i, d2[i]);
+    for (int i = 0; i < n; i++) out[i] = o_foot_bound(a1, d2[i]);
 }
 
 /* wide proposal bandwidth of one view (for tests) */

@@ -34,8 +34,8 @@ def boxes_and_chains(leaf, par):
 @pytest.mark.parametrize("cfg,sds", [(0, 2.0), (2, 8.0)])
 def test_node_bounds_dominate_leaf_terms(cfg, sds):
     sc, p, n, nb, sd = scene_inputs(cfg, sd_scale=sds)
-    leaf, node, ids, par, mass = O.foot_view(sc.width, sc.height, sc.focal, p, nb, sd)
-    assert leaf.shape[0] > 100 and (leaf[:, 5] > 0).mean() > 0.9
+    leaf, ids, par, mass, a1 = O.foot_view(sc.width, sc.height, sc.focal, p, nb, sd)
+    assert leaf.shape[0] > 100 and (leaf[:, 5] > 0).mean() > 0.9 and a1 > 0
     lo, hi, chains = boxes_and_chains(leaf, par)
     rng = np.random.default_rng(cfg)
     pos = rng.integers(0, leaf.shape[0], 4000)
@@ -44,7 +44,7 @@ def test_node_bounds_dominate_leaf_terms(cfg, sds):
     far = rng.uniform(0, [sc.width, sc.height], size=(pos.size, 2))
     q = np.rint(np.concatenate([near, far])).astype(np.float32)
     pos = np.concatenate([pos, pos])
-    fvals = O.foot_terms(leaf[pos], q)
+    fvals = O.foot_terms(leaf[pos], a1, q)
     assert (fvals > 1e-3).sum() > 500
     rows, d2s, fs = [], [], []
     for k, ps in enumerate(pos):
@@ -53,7 +53,7 @@ def test_node_bounds_dominate_leaf_terms(cfg, sds):
             d = (q[k] - c).astype(np.float32)
             d2 = np.float32(d[0] * d[0]) + np.float32(d[1] * d[1])
             rows.append(a); d2s.append(d2); fs.append(fvals[k])
-    b = O.foot_bounds(node[np.array(rows)], np.array(d2s, np.float32))
+    b = O.foot_bounds(a1, np.array(d2s, np.float32))
     fs = np.array(fs, np.float32)
     assert np.all(b >= fs), "a node bound below a leaf term: %d of %d" % ((b < fs).sum(), b.size)
 
@@ -62,7 +62,7 @@ def test_node_bounds_dominate_leaf_terms(cfg, sds):
 def test_footprint_term_is_the_ewa_weight(cfg, sds):
     sc, p, n, nb, sd = scene_inputs(cfg, sd_scale=sds)
     W, H, f = sc.width, sc.height, sc.focal
-    leaf, node, ids, par, mass = O.foot_view(W, H, f, p, nb, sd)
+    leaf, ids, par, mass, a1 = O.foot_view(W, H, f, p, nb, sd)
     rng = np.random.default_rng(7 + cfg)
     idx = np.full((1, W, H, 1), -1, np.int32)
     qs, ls = [], []
@@ -75,10 +75,14 @@ def test_footprint_term_is_the_ewa_weight(cfg, sds):
     qs = np.array(qs, np.float32); ls = np.array(ls)
     wref = O.splat_forward(W, H, f, idx, p[None].astype(np.float64), nb[None].astype(np.float64),
                            sd[None].astype(np.float64))[0, qs[:, 0].astype(int), qs[:, 1].astype(int), 0]
-    fv = O.foot_terms(leaf[ls], qs)
-    ok = (leaf[ls, 5] > 0) & (wref > 1e-4)
-    assert ok.sum() > 150
-    rel = np.abs(fv[ok] - wref[ok]) / wref[ok]
+    fv = O.foot_terms(leaf[ls], a1, qs)
+    # the cap E(d) = exp(-max(0, d - 2)^2 a1), in FP64 from its definition
+    d = np.hypot(qs[:, 0] - leaf[ls, 0].astype(np.float64), qs[:, 1] - leaf[ls, 1].astype(np.float64))
+    cap = np.exp(-np.maximum(0.0, d - 2.0) ** 2 * float(a1))
+    ref = np.minimum(np.minimum(wref, 1.0), cap)
+    ok = (leaf[ls, 5] > 0) & (ref > 1e-4)
+    assert ok.sum() > 150 and (cap < wref).sum() > 10
+    rel = np.abs(fv[ok] - ref[ok]) / ref[ok]
     # FP32 covariance against the FP64 chain: a few hundred ulp for ordinary splats,
     # more for surfels seen nearly edge-on; the median pins the formula
     assert np.median(rel) < 2e-5 and np.mean(rel < 1e-3) > 0.98, (np.median(rel), np.max(rel))
@@ -86,10 +90,10 @@ def test_footprint_term_is_the_ewa_weight(cfg, sds):
 
 def test_footprint_mass_closed_form():
     sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)
-    leaf, node, ids, par, mass = O.foot_view(sc.width, sc.height, sc.focal, p, nb, sd)
+    leaf, ids, par, mass, a1 = O.foot_view(sc.width, sc.height, sc.focal, p, nb, sd)
     L = leaf.astype(np.float64)
     good = L[:, 5] > 0
-    area = 2.0 * np.pi / (L[:, 2] * L[:, 4])              # 2 pi sqrt(det V), det V^-1 = (t00 t11)^2
+    area = 2.0 * np.pi / np.where(good, L[:, 2] * L[:, 4], 1.0)   # 2 pi sqrt(det V), det V^-1 = (t00 t11)^2
     closed = np.where(L[:, 5] <= 1.0, area * L[:, 5], area * (1.0 + np.log(np.maximum(L[:, 5], 1e-30))))
     closed = np.where(good, closed, 0.0)
     # the oracle's total (FP32 per leaf, summed in FP64) against the closed form in FP64

@@ -102,8 +102,16 @@ struct PNode {
 struct FootLeaf {
     float x, y;                  // screen position (the tree's)
     float t00, t01, t11, amp;    // whitening and peak of the EWA weight
-    float m, pad;                // envelope exponent factor
+    float pad[2];
 };
+// the footprint term is capped by one envelope for every splat, E(d) =
+// exp(-max(0, d - kFootRcap)^2 a1) with a1 = 1 / 2h_1^2, the wide band's decay:
+// min(1, w) equals w where w is below E, which holds beyond kFootRcap pixels
+// for every splat whose largest axis is within h_1 and whose core min(1, w) = 1
+// reaches at most kFootRcap; for the others (edge-on needles, the widest 0.1 %)
+// the cap trims the footprint term and the wide band's half still covers them.
+// A node's bound is then E at its box distance, with no per-node data.
+constexpr float kFootRcap = 2.0f;
 
 // per internal node: what the fixed-order density walk needs of its two
 // children, so that one wave-uniform load serves a descent
@@ -140,8 +148,6 @@ struct TreeWs {
     float* hraw;          // per splat id: proposal bandwidth sqrt(lambda_max(V))
     PNode* pnode;         // per node: Poisson walk record
     FootLeaf* footLeaf;   // per sorted leaf: footprint proposal record
-    float2* nodeFoot;     // per node: (largest amp, smallest m) of its leaves
-    int* footCnt;         // per internal node: fan-in arrival count
     double* footSum;      // sum of the visible splats' footprint masses
     FmmRec* frec;         // per internal node: children for the density walk
     float* dens[2];       // per pixel and band: density D_b(q)
@@ -196,8 +202,6 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
     w.hraw = nullptr;
     w.pnode = nullptr;
     w.footLeaf = nullptr;
-    w.nodeFoot = nullptr;
-    w.footCnt = nullptr;
     w.footSum = nullptr;
     w.frec = nullptr;
     for (int b = 0; b < 2; b++) { w.fixOff[b] = w.fixList[b] = nullptr; w.fixCoef[b] = w.dens[b] = nullptr; }
@@ -212,8 +216,6 @@ size_t carve(void* ws, int N, int W, int H, int k, int mode, TreeWs* t)
         w.hraw = (float*)take(4 * n);
         w.pnode = (PNode*)take(sizeof(PNode) * nn);
         w.footLeaf = (FootLeaf*)take(sizeof(FootLeaf) * n);
-        w.nodeFoot = (float2*)take(sizeof(float2) * nn);
-        w.footCnt = (int*)take(4 * n);
         w.footSum = (double*)take(8);
         w.dens[0] = (float*)take(4L * W * H);
         w.dens[1] = (float*)take(4L * W * H);
@@ -294,7 +296,7 @@ __device__ float prop_h(float Wf, float Hf, float ff, const float* p, const floa
 // and A the FP32 screen covariance and Jacobian of cov_f32, and its isotropic
 // envelope amp exp(-|q - s|^2 m), m = 1 / (2 lambda_max(V)) >= the envelope's
 // exponent along any direction.  Degenerate geometry gives amp = 0 (the splat is
-// then drawn through the band-0 half of the mixture alone).
+// then drawn through the wide-band half of the mixture alone).
 __device__ void foot_f32(float Wf, float Hf, float ff, const float* p, const float* n, const float* sd,
                          float smoothing, float& t00, float& t01, float& t11, float& amp, float& m, float& mass)
 {
@@ -317,10 +319,15 @@ __device__ void foot_f32(float Wf, float Hf, float ff, const float* p, const flo
     if (!ok) { t00 = t01 = t11 = 0.0f; amp = 0.0f; m = 0.0f; mass = 0.0f; }
 }
 
-// the footprint term of one leaf at q: min(1, w(q), envelope(q)); the envelope
-// cap makes it a value the node bound (the same expression over the node's box
-// with the node's largest amp and smallest m) dominates in floating point
-__device__ __forceinline__ float foot_leaf(const FootLeaf& L, float qx, float qy, float d2)
+// the footprint cap E(d) at distance d (see kFootRcap)
+__device__ __forceinline__ float foot_env(float a1, float d)
+{
+    const float e = fmaxf(0.0f, d - kFootRcap);
+    return pexpf(-((e * e) * a1));
+}
+
+// the footprint term of one leaf at q: min(1, w(q), E(|q - s|))
+__device__ __forceinline__ float foot_leaf(const FootLeaf& L, float a1, float qx, float qy, float d2)
 {
     const float dx = qx - L.x, dy = qy - L.y;
     float u = L.t00 * dx;
@@ -329,16 +336,14 @@ __device__ __forceinline__ float foot_leaf(const FootLeaf& L, float qx, float qy
     float e = u * u;
     e = e + v * v;
     const float w = L.amp * pexpf(-(0.5f * e));
-    const float env = L.amp * pexpf(-(d2 * L.m));
-    return fminf(1.0f, fminf(w, env));
+    return fminf(1.0f, fminf(w, foot_env(a1, sqrtf(d2))));
 }
 
-// node bound of the footprint term: amax exp(-d2 mmin), inflated by 2^-20 so a
-// one-ulp step of the exponential between two arguments cannot put a leaf above it
-__device__ __forceinline__ float foot_bound(float2 nf, float d2)
+// node bound of the footprint term at squared box distance d2: E, inflated by
+// 2^-20 so that a one-ulp step of the exponential cannot put a leaf above it
+__device__ __forceinline__ float foot_bound(float a1, float d2)
 {
-    const float b = nf.x * pexpf(-(d2 * nf.y));
-    return fminf(1.0f, b * 1.00000095367431640625f);
+    return fminf(1.0f, foot_env(a1, sqrtf(d2)) * 1.00000095367431640625f);
 }
 
 // isotropic bandwidth capped at one pixel (samplingCuda.cu:160)
@@ -2360,7 +2365,7 @@ __device__ __forceinline__ float mix_d2(float k0, float a0, float k1, float a1, 
     return m;
 }
 
-// band-0 half plus footprint half: k0 e^{-a0 d2} + kf f (same order as mix_d2)
+// wide-band half plus footprint half: k1 e^{-a1 d2} + kf f (same order as mix_d2)
 __device__ __forceinline__ float foot_mix(float k0, float a0, float kf, float d2, float f)
 {
     const float e0 = pexpf(-(d2 * a0));
@@ -2418,30 +2423,15 @@ __global__ __launch_bounds__(256) void k_walk_records(const ViewState* __restric
     if (nd < L0) frec[nd] = fmm_record(nd, L0, geo, lnk, fixOff0, fixOff1);
 }
 
-__global__ __launch_bounds__(256) void k_foot_clear(const ViewState* __restrict__ vs, int* footCnt, double* footSum,
-                                                    VStride vsd)
-{
-    vs = vw(vs, vsd.ws);
-    footCnt = vw(footCnt, vsd.ws);
-    footSum = vw(footSum, vsd.ws);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < vs->nc - 1) footCnt[t] = 0;
-    if (t == 0) *footSum = 0.0;
-}
-
-// Footprint records of the sorted leaves and, bottom up, every internal node's
-// (largest amp, smallest m) over its leaves: the second thread to reach a node
-// merges its two children (release fence before the arrival count, acquire
-// after, as in the tree's fan-in).  footCnt and footSum start at zero.  The
-// visible splats' footprint masses (integrals of min(w, 1)) are summed for the
-// normaliser of the footprint half.
-__global__ __launch_bounds__(256) void k_foot_nodes(float Wf, float Hf, float ff, float smoothing,
-                                                    const float* __restrict__ pts, const float* __restrict__ nrm,
-                                                    const float* __restrict__ sd, const ViewState* __restrict__ vs,
-                                                    const int32_t* __restrict__ ids,
-                                                    const float2* __restrict__ leafXY,
-                                                    const NodeLnk* __restrict__ lnk, FootLeaf* __restrict__ footLeaf,
-                                                    float2* nodeFoot, int* footCnt, double* footSum, VStride vsd)
+// Footprint records of the sorted leaves, and the sum of the visible splats'
+// footprint masses (integrals of min(w, 1)) for the normaliser of the footprint
+// half (footSum starts at zero).
+__global__ __launch_bounds__(256) void k_foot_leaves(float Wf, float Hf, float ff, float smoothing,
+                                                     const float* __restrict__ pts, const float* __restrict__ nrm,
+                                                     const float* __restrict__ sd, const ViewState* __restrict__ vs,
+                                                     const int32_t* __restrict__ ids,
+                                                     const float2* __restrict__ leafXY, FootLeaf* __restrict__ footLeaf,
+                                                     double* footSum, VStride vsd)
 {
     pts = vw(pts, vsd.pts);
     nrm = vw(nrm, vsd.pts);
@@ -2449,10 +2439,7 @@ __global__ __launch_bounds__(256) void k_foot_nodes(float Wf, float Hf, float ff
     vs = vw(vs, vsd.ws);
     ids = vw(ids, vsd.ws);
     leafXY = vw(leafXY, vsd.ws);
-    lnk = vw(lnk, vsd.ws);
     footLeaf = vw(footLeaf, vsd.ws);
-    nodeFoot = vw(nodeFoot, vsd.ws);
-    footCnt = vw(footCnt, vsd.ws);
     footSum = vw(footSum, vsd.ws);
     __shared__ double sm[4];
     const int n = vs->nc;
@@ -2463,30 +2450,12 @@ __global__ __launch_bounds__(256) void k_foot_nodes(float Wf, float Hf, float ff
         FootLeaf L;
         const float2 xy = leafXY[t];
         L.x = xy.x; L.y = xy.y;
-        float ms;
-        foot_f32(Wf, Hf, ff, pts + 3L * id, nrm + 3L * id, sd + 2L * id, smoothing, L.t00, L.t01, L.t11, L.amp, L.m,
+        float m, ms;
+        foot_f32(Wf, Hf, ff, pts + 3L * id, nrm + 3L * id, sd + 2L * id, smoothing, L.t00, L.t01, L.t11, L.amp, m,
                  ms);
-        L.pad = 0.0f;
+        L.pad[0] = L.pad[1] = 0.0f;
         footLeaf[t] = L;
         mass = (double)ms;
-        const int L0 = n - 1;
-        int nd = L0 + t;
-        float2 v = make_float2(L.amp, L.m);
-        nodeFoot[nd] = v;
-        while (true) {
-            const int p = lnk[nd].parent;
-            if (p < 0) break;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            const int old = atomicAdd(&footCnt[p], 1);
-            if (old == 0) break;                          // the sibling's thread merges
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const NodeLnk pl = lnk[p];
-            const int sib = pl.left == nd ? pl.right : pl.left;
-            const float2 o = nodeFoot[sib];
-            v = make_float2(fmaxf(v.x, o.x), fminf(v.y, o.y));
-            nodeFoot[p] = v;
-            nd = p;
-        }
     }
     for (int o = 32; o > 0; o >>= 1) mass += __shfl_xor(mass, o, 64);
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mass;
@@ -2495,6 +2464,12 @@ __global__ __launch_bounds__(256) void k_foot_nodes(float Wf, float Hf, float ff
         const double b = (sm[0] + sm[1]) + (sm[2] + sm[3]);
         if (b != 0.0) atomicAdd(footSum, b);
     }
+}
+
+__global__ void k_zero_f64(double* p, size_t stride, int views)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < views) *(double*)((char*)p + stride * v) = 0.0;
 }
 
 // One lane per pixel (one pixel tile per wavefront, TileCfg).  The walk
@@ -2529,7 +2504,6 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                                                                    int32_t* __restrict__ outIds,
                                                                    float* __restrict__ outIpi, int* err, int fmode,
                                                                    float fscale, const FootLeaf* __restrict__ footLeaf,
-                                                                   const float2* __restrict__ nodeFoot,
                                                                    const double* __restrict__ footSum)
 {
     extern __shared__ int32_t stage[];
@@ -2553,7 +2527,6 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
     dens1 = vw(dens1, densStride);
     if (fmode) {
         footLeaf = vw(footLeaf, vstride);
-        nodeFoot = vw(nodeFoot, vstride);
         footSum = vw(footSum, vstride);
     }
     const long perView = (long)W * H;
@@ -2576,20 +2549,21 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         // (fminf drops a NaN): draw nothing there and say so
         if (!(lam >= 0.0f && lam <= 3.0e38f)) { atomicOr(err, 16); lam = 0.0f; }
         const float t0 = 0.5f / fmaxf(D0, B0.c);
-        const float t1 = fmode ? 0.0f : 0.5f / fmaxf(dens1[pix], B1.c);
-        const float k0 = t0 * B0.c, k1 = fmode ? 0.0f : t1 * B1.c;
-        const float a0 = B0.is2h * B0.is2h, a1 = fmode ? 0.0f : B1.is2h * B1.is2h;
-        // footprint half: f_i(q) / (kw max(D0, c0)), kw = the visible splats' mean
-        // footprint mass over h^1.5 (so that both halves expect about lam / 2
-        // samples), or the caller's scale
+        const float t1 = 0.5f / fmaxf(dens1[pix], B1.c);
+        const float k0 = t0 * B0.c, k1 = t1 * B1.c;
+        const float a0 = B0.is2h * B0.is2h, a1 = B1.is2h * B1.is2h;
+        // footprint mode: the wide band's half as above, and in place of band 0
+        // f_i(q) / (kw max(D1, c1)), kw = the visible splats' mean footprint mass
+        // over h_1^1.5 (so that both halves expect about lam / 2 samples), or the
+        // caller's scale
         float kf = 0.0f;
         if (fmode) {
             float kw = fscale;
             if (!(kw > 0.0f)) {
-                const double hh = (double)B0.h;
+                const double hh = (double)B1.h;
                 kw = (float)((*footSum / (double)n) / (hh * sqrt(hh)));
             }
-            if (kw > 0.0f && kw < 3.0e38f) kf = t0 / kw;
+            if (kw > 0.0f && kw < 3.0e38f) kf = t1 / kw;
         }
         const float qx = (float)i, qy = (float)j;
         const uint32_t pixel = (uint32_t)pix;
@@ -2599,12 +2573,11 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
         while (true) {
             if (++visits > 2 * n) { atomicOr(err, 8); break; }
             const PNode pn = pnode[nd];
-            const float2 nf = fmode ? nodeFoot[nd] : make_float2(0.0f, 0.0f);
             float cx = fminf(fmaxf(qx, pn.lox), pn.hix);
             float cy = fminf(fmaxf(qy, pn.loy), pn.hiy);
             const float d2b = dist2(qx, qy, cx, cy);
             // proposal bound over the node
-            float pB = fmode ? foot_mix(k0, a0, kf, d2b, foot_bound(nf, d2b)) : mix_d2(k0, a0, k1, a1, d2b);
+            float pB = fmode ? foot_mix(k1, a1, kf, d2b, foot_bound(a1, d2b)) : mix_d2(k0, a0, k1, a1, d2b);
             float U = fminf(1.0f, lam * pB);
             const int last = pn.last;
             int next = pn.esc;                        // past the subtree
@@ -2628,7 +2601,7 @@ __global__ __launch_bounds__(64 * kMaxWavesPerBlock) void k_poisson(TileCfg cfg,
                         if (fmode) {
                             const FootLeaf L = footLeaf[pos];
                             const float d2 = dist2(qx, qy, L.x, L.y);
-                            pl = foot_mix(k0, a0, kf, d2, foot_leaf(L, qx, qy, d2));
+                            pl = foot_mix(k1, a1, kf, d2, foot_leaf(L, a1, qx, qy, d2));
                         } else {
                             const float2 lp = leafXY[pos];
                             pl = mix_d2(k0, a0, k1, a1, dist2(qx, qy, lp.x, lp.y));
@@ -2928,7 +2901,7 @@ void launch_sample(int N, int W, int H, int k, uint32_t seedLo, uint32_t seedHi,
     if (timed) (void)hipEventRecord(e1, st);
 }
 
-// proposal of ude_sample_poisson: 0 the two density bands, 1 band 0 plus the
+// proposal of ude_sample_poisson: 0 the two density bands, 1 the wide band plus the
 // footprint half; scale 0 = normaliser from the view, > 0 the caller's
 int g_pois_mode = 0;
 double g_pois_scale = 0.0;
@@ -3101,32 +3074,23 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         const float* sb = sd + (sdBatch > 1 ? 2L * N * g : 0);
         int rc = build_structure(N, W, H, f, smoothing, prec, pb, nb, sb, w, st, span, vsd);
         if (rc != UDE_OK) return rc;
-        // the footprint proposal needs no wide band: one density band, the
-        // footprint records and their node bounds instead
-        const int nbands = fmode ? 1 : 2;
-        if (!fmode) {
-            k_wide_hist<<<dim3(grid1(N, 256) < 256u ? grid1(N, 256) : 256u, span), 256, 0, st>>>(w.vs, w.idsA,
-                                                                                              w.hraw, vsd);
-            k_wide_band<<<dim3(1, span), 256, 0, st>>>(w.vs, vsd);
-        }
-        fixed_expansions(N, nbands, p, w, st, span, vsd);
+        k_wide_hist<<<dim3(grid1(N, 256) < 256u ? grid1(N, 256) : 256u, span), 256, 0, st>>>(w.vs, w.idsA, w.hraw,
+                                                                                          vsd);
+        k_wide_band<<<dim3(1, span), 256, 0, st>>>(w.vs, vsd);
+        fixed_expansions(N, 2, p, w, st, span, vsd);
         k_walk_records<<<dim3(grid1(2L * N, 256), span), 256, 0, st>>>(w.vs, w.geo, w.lnk, w.aux, w.fixOff[0],
-                                                                       fmode ? nullptr : w.fixOff[1], w.pnode,
-                                                                       w.frec, vsd);
+                                                                       w.fixOff[1], w.pnode, w.frec, vsd);
+        // footprint proposal: the leaves' footprint records and their node bounds
         if (fmode) {
-            k_foot_clear<<<dim3(grid1(N, 256), span), 256, 0, st>>>(w.vs, w.footCnt, w.footSum, vsd);
-            k_foot_nodes<<<dim3(grid1(N, 256), span), 256, 0, st>>>((float)W, (float)H, (float)f, (float)smoothing,
-                                                                    pb, nb, sb, w.vs, w.idsA, w.leafXY, w.lnk,
-                                                                    w.footLeaf, w.nodeFoot, w.footCnt, w.footSum,
-                                                                    vsd);
+            k_zero_f64<<<(span + 63) / 64, 64, 0, st>>>(w.footSum, per, span);
+            k_foot_leaves<<<dim3(grid1(N, 256), span), 256, 0, st>>>((float)W, (float)H, (float)f, (float)smoothing,
+                                                                     pb, nb, sb, w.vs, w.idsA, w.leafXY, w.footLeaf,
+                                                                     w.footSum, vsd);
         }
         float* d0 = densOut ? densOut + perView * 2 * g : w.dens[0];
         float* d1 = densOut ? d0 + perView : w.dens[1];
         const size_t dstride = densOut ? sizeof(float) * 2 * perView : per;
-        fmm_eval(W, H, span, nbands, p, w, per, d0, dstride, (size_t)((char*)d1 - (char*)d0), st);
-        if (fmode && densOut)
-            for (int v = 0; v < span; v++)
-                k_fill_f32<<<grid1(perView, 256), 256, 0, st>>>(d1 + perView * 2 * v, perView, 0.0f);
+        fmm_eval(W, H, span, 2, p, w, per, d0, dstride, (size_t)((char*)d1 - (char*)d0), st);
         const float* ts = tileScale ? tileScale + (long)tsW * tsH * g : nullptr;
         hipEvent_t e0, e1;
         bool timed = timing_pair(2, &e0, &e1);
@@ -3134,7 +3098,7 @@ int ude_sample_poisson(int bn, int N, int W, int H, int kcap, double f, double s
         k_poisson<<<tile_grid(pc, W, H, span), 64 * pc.wpb, kPoissonStageBytes * pc.wpb, st>>>(
             pc, W, H, kcap, (float)rate, ts, tsH, sLo, sHi, seedDev, (uint32_t)g, w.vs, per, w.pnode, w.leafXY,
             w.idsA, d0, d1, dstride, outIds + perView * kcap * g, outIpi + perView * kcap * g, errFlag, fmode,
-            fscale, w.footLeaf, w.nodeFoot, w.footSum);
+            fscale, w.footLeaf, w.footSum);
         if (timed) (void)hipEventRecord(e1, st);
     }
     hipError_t e = hipGetLastError();

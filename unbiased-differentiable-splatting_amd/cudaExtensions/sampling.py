@@ -157,13 +157,13 @@ def samplePoisson(width, height, focallength, points, normals, colors, stdDevs, 
     rate times its multiplier was negative or not finite, nothing drawn there); it is never
     cleared here, so one flag can collect a whole training run and be checked at
     a point that synchronises anyway (raise_errors(errors.item())).
-    ``proposal``: "bands" (the mixture above) or "footprint": band 0 plus each
-    splat's own footprint, pi_i = min(1, r (g0_i / (2 N0) + f_i / (2 kw N0))) with
-    f_i = min(1, w_i) capped by its isotropic envelope, w_i the splat's EWA weight
-    (FP32 screen covariance), N0 = max(D0, c0); kw = ``footprintScale`` or, when it
-    is None, the visible splats' mean footprint mass over h^1.5 (each half then
-    expects about r / 2 samples).  The footprint proposal evaluates one density
-    band (D1 in ``densities`` is 0)."""
+    ``proposal``: "bands" (the mixture above) or "footprint": the wide band plus
+    each splat's own footprint, pi_i = min(1, r (g1_i / (2 N1) + f_i / (2 kw N1)))
+    with f_i = min(1, w_i, exp(-max(0, d - 2)^2 / 2h_1^2)) at distance d from the
+    splat, w_i its EWA weight (FP32 screen covariance), N1 = max(D1, c1);
+    kw = ``footprintScale``
+    or, when it is None, the visible splats' mean footprint mass over h_1^1.5
+    (each half then expects about r / 2 samples)."""
     global last_error
     _check_splats(points, normals, stdDevs)
     dev = _lib.require_gpu(points, normals, stdDevs)

@@ -126,7 +126,8 @@ class PoissonFrame:
     Gradients are with respect to world positions / normals, std-devs and albedo,
     summed over the views."""
 
-    def __init__(self, scene, device, rate=10.0, kcap=40, precision=0.01, target=None, composite=False):
+    def __init__(self, scene, device, rate=10.0, kcap=40, precision=0.01, target=None, composite=False,
+                 proposal="bands", footprint_scale=None):
         from cudaExtensions import transform
         self.sc = scene
         self.dev = device
@@ -140,6 +141,8 @@ class PoissonFrame:
         self.A = t(scene.albedo)[None].contiguous()
         self.rate, self.kcap, self.prec = rate, kcap, precision
         self.composite = bool(composite)
+        self.proposal = proposal                 # samplePoisson's proposal: "bands" or "footprint"
+        self.footprint_scale = footprint_scale
         self.target = target
         self.W, self.H, self.f = scene.width, scene.height, scene.focal
         self.dens = torch.zeros((self.V, 2, self.W, self.H), dtype=torch.float32, device=device)
@@ -168,7 +171,8 @@ class PoissonFrame:
         P, Nc, nb = self.camera()
         ids, ipi = sampling.samplePoisson(self.W, self.H, self.f, P, nb, None, self.S, kcap or self.kcap, 0.0,
                                           self.prec, seed=seed, rate=rate or self.rate, tileScale=tile_scale,
-                                          densities=self.dens, seedTensor=seed_tensor, errors=self.errors)
+                                          densities=self.dens, seedTensor=seed_tensor, errors=self.errors,
+                                          proposal=self.proposal, footprintScale=self.footprint_scale)
         if self._ws is None or self._ws_shape != tuple(ids.shape):
             self._ws = estimator.workspace(ids, self.S, composite=self.composite)
             self._ws_shape = tuple(ids.shape)
@@ -316,6 +320,50 @@ def kappa_interval(plain_groups, learned_groups, resamples=4000, seed=0):
     r = [np.median(q[rng.integers(0, q.size, q.size)]) / np.median(p[rng.integers(0, p.size, p.size)])
          for _ in range(resamples)]
     return [float(np.percentile(r, 2.5)), float(np.percentile(r, 97.5))]
+
+
+def samples_per_pixel(frame, seed, rate=None, kcap=None):
+    frame.image(seed, rate=rate, kcap=kcap)
+    return float((frame.last_ids >= 0).float().sum(-1).mean().item())
+
+
+def compare_proposals(frame, seeds, groups=8, kcap=None):
+    """The band proposal against the footprint proposal on one frame at the same
+    expected sample count: the footprint rate is set so that its mean samples per
+    pixel equal the bands' at frame.rate (two secant steps on the count, which
+    grows with the rate), then gradient variances are measured over disjoint seed
+    sets as in compare_rates (median of seed groups); a second band measurement
+    gives the noise.  kappa = V_footprint / V_bands at equal samples, so the
+    footprint proposal reaches the bands' variance at about rate_equal * kappa.
+    The footprint draws use capacity ``kcap`` (default twice the frame's: its
+    samples concentrate on the pixels a splat covers).  The frame's proposal is
+    left as it was."""
+    S = len(seeds)
+    kf = kcap or 2 * frame.kcap
+    base = seeds[0]
+    keep = frame.proposal
+    try:
+        frame.proposal = "bands"
+        n_b = samples_per_pixel(frame, base - 1)
+        gb, gr, gf = [], [], []
+        v_b, _ = gradient_variance(frame, [base + s for s in range(S)], groups=groups, out_groups=gb)
+        v_rep, _ = gradient_variance(frame, [base + S + s for s in range(S)], groups=groups, out_groups=gr)
+        frame.proposal = "footprint"
+        r = frame.rate
+        n_f = samples_per_pixel(frame, base - 1, rate=r, kcap=kf)
+        for _ in range(2):
+            r = r * n_b / max(n_f, 1e-9)
+            n_f = samples_per_pixel(frame, base - 1, rate=r, kcap=kf)
+        v_f, _ = gradient_variance(frame, [base + 2 * S + s for s in range(S)], rate=r, kcap=kf, groups=groups,
+                                   out_groups=gf)
+    finally:
+        frame.proposal = keep
+    return {"band_grad_variance": v_b, "band_grad_variance_repeat": v_rep,
+            "measurement_noise": abs(v_rep / v_b - 1.0),
+            "footprint_rate_equal_samples": r, "footprint_grad_variance_equal_samples": v_f,
+            "kappa": v_f / v_b, "kappa_ci95": kappa_interval(gb + gr, gf),
+            "samples_per_pixel_bands": n_b, "samples_per_pixel_footprint": n_f, "kcap_footprint": kf,
+            "variance_seeds": S, "variance_seed_groups": groups}
 
 
 def rate_features(frame, net, seed):
