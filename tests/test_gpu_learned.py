@@ -111,7 +111,7 @@ def test_learned_rates_at_full_scale(dev, record_property):
     in 8 groups per measurement.  A gain is asserted
     only when the whole 95 % interval of kappa lies below 1; otherwise the test
     records the interval and is skipped, naming it (the benchmark then falls back
-    to the plain rates, which is the north star's fallback)."""
+    to the plain rates)."""
     from splatting_amd import learned as L, scenes
     sc = scenes.training_scene(101)
     target = L.make_target(sc, dev, 10.0, 40, seeds=range(10_000, 10_004))
