@@ -281,7 +281,7 @@ def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
     steps = {"plain": plain_step}
     notes = []
     # footprint proposal (band 0 plus each splat's own EWA footprint)
-    cmpf = L.compare_proposals(fr, list(range(30_000, 30_000 + seeds)))
+    cmpf = L.compare_proposals(fr, list(range(30_000, 30_000 + 2 * seeds)), groups=16)
     rec["footprint"] = cmpf
     if cmpf["kappa"] < 1.0:
         # the rate of equal variance: bracketed from the kappa estimate (the
