@@ -1078,6 +1078,65 @@ __global__ void k_interp_forward(long npix, int W, int H, int k, int ab, int N, 
     }
 }
 
+// interpolation.forward for float, C = 3, k <= 64: one wavefront per 64
+// consecutive pixels.  Their slot rows are one contiguous block of 64 k indices
+// and weights, read with coalesced 16-byte loads into LDS (rows padded to k + 1
+// words, so the lanes' row reads fall in distinct banks); each lane then sums its
+// pixel's slots in slot order, as the per-pixel kernel does (same result).
+constexpr int kInterpTileMaxK = 64;
+__global__ __launch_bounds__(64) void k_interp_forward_tile(long npix, int W, int H, int k, int ab, int N,
+                                                            const int32_t* __restrict__ idx,
+                                                            const float* __restrict__ wts,
+                                                            const float* __restrict__ attr, float* __restrict__ out)
+{
+    extern __shared__ int32_t sh[];
+    int32_t* sI = sh;                                  // [64][k + 1]
+    float* sW = (float*)(sh + 64 * (k + 1));
+    const int lane = threadIdx.x;
+    const long pix0 = (long)blockIdx.x * 64;
+    const int np = npix - pix0 < 64 ? (int)(npix - pix0) : 64;
+    const int kp = k + 1;
+    const int total = np * k;                          // contiguous words of this block's rows
+    const int32_t* gi = idx + pix0 * k;
+    const float* gw = wts + pix0 * k;
+    if ((k & 3) == 0) {
+        // 16-byte loads (k % 4 == 0 keeps every row, and the block, 16-byte aligned)
+        const int4* gi4 = (const int4*)gi;
+        const float4* gw4 = (const float4*)gw;
+        for (int e = lane; e < (total >> 2); e += 64) {
+            const int4 a = gi4[e];
+            const float4 b = gw4[e];
+            const int r = (4 * e) / k, c = 4 * e - r * k;
+            int32_t* di = sI + r * kp + c;
+            float* dw = sW + r * kp + c;
+            di[0] = a.x; di[1] = a.y; di[2] = a.z; di[3] = a.w;
+            dw[0] = b.x; dw[1] = b.y; dw[2] = b.z; dw[3] = b.w;
+        }
+    } else {
+        for (int e = lane; e < total; e += 64) {
+            const int r = e / k, c = e - r * k;
+            sI[r * kp + c] = gi[e];
+            sW[r * kp + c] = gw[e];
+        }
+    }
+    __syncthreads();
+    if (lane >= np) return;
+    const long pix = pix0 + lane;
+    const long b = pix / ((long)W * H);
+    const long abase = ab > 1 ? b * N : 0;
+    float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+    const int32_t* ri = sI + lane * kp;
+    const float* rw = sW + lane * kp;
+    for (int s = 0; s < k; s++) {
+        const int id = ri[s];
+        if (id < 0) continue;
+        const float w = rw[s];
+        const float* a = attr + 3 * (abase + id);
+        o0 += w * a[0]; o1 += w * a[1]; o2 += w * a[2];
+    }
+    out[3 * pix] = o0; out[3 * pix + 1] = o1; out[3 * pix + 2] = o2;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_interp_backward(long nslots, int W, int H, int k, int ab, int N, int C,
                                                          const T* __restrict__ g,
@@ -1419,6 +1478,14 @@ int interp_forward_t(int bn, int W, int H, int k, int ab, int N, int C, const in
     if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || C <= 0 || (ab != 1 && ab != bn)) return UDE_ERR_ARG;
     long npix = (long)bn * W * H;
     const int vec4 = (k % 4 == 0) && ((uintptr_t)idx % 16 == 0) && ((uintptr_t)wts % 16 == 0);
+    if constexpr (sizeof(T) == 4) {
+        if (C == 3 && k <= kInterpTileMaxK && ((k & 3) != 0 || vec4)) {
+            const size_t lds = sizeof(float) * 2 * 64 * (size_t)(k + 1);
+            k_interp_forward_tile<<<(unsigned)((npix + 63) / 64), 64, lds, st>>>(
+                npix, W, H, k, ab, N, idx, (const float*)wts, (const float*)attr, (float*)out);
+            return launch_status();
+        }
+    }
     k_interp_forward<T><<<grid1(npix, 256), 256, 0, st>>>(npix, W, H, k, ab, N, C, vec4, idx, wts, attr, out);
     return launch_status();
 }
