@@ -260,7 +260,7 @@ def timed_steps(fn, steps, kernel=None):
     return max(e0.elapsed_time(e1), wall * 1e3) / steps, k_ms, k_n
 
 
-def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
+def poisson_paths(dev, fr, seeds=64, probe_steps=10, compare=True):
     """The plain-rate step (band proposal at RATE) and the alternatives that reach
     its gradient variance: the footprint proposal at the rate its measured kappa
     gives, and, when the trained rate network needs fewer samples than the plain
@@ -280,7 +280,10 @@ def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
            "plain_rate": RATE, "kcap": KCAP}
     steps = {"plain": plain_step}
     notes = []
-    # footprint proposal (band 0 plus each splat's own EWA footprint)
+    if not compare:
+        rec["chosen"] = "plain (proposals and learned rates not compared)"
+        return plain_step, "plain", rec
+    # footprint proposal (the wide band plus each splat's own EWA footprint)
     cmpf = L.compare_proposals(fr, list(range(30_000, 30_000 + 2 * seeds)), groups=16)
     rec["footprint"] = cmpf
     if cmpf["kappa"] < 1.0:
@@ -340,8 +343,8 @@ def poisson_paths(dev, fr, seeds=64, probe_steps=10, use_learned=True):
     else:
         notes.append("footprint needs %.2fx the samples" % cmpf["kappa"])
     # learned per-tile rates
-    if not use_learned or not os.path.exists(L.DEFAULT_WEIGHTS):
-        notes.append("learned rates not compared" if not use_learned else "no trained rate network")
+    if not os.path.exists(L.DEFAULT_WEIGHTS):
+        notes.append("no trained rate network")
     else:
         net = L.RateNet.load(L.DEFAULT_WEIGHTS, dev)
         cmp = L.compare_rates(fr, net, list(range(20_000, 20_000 + seeds)))
@@ -480,7 +483,7 @@ def poisson_extra(dev, args, sc):
     target = L.make_target(sc, dev, RATE, KCAP)
     fr = L.PoissonFrame(sc, dev, rate=RATE, kcap=KCAP, target=target)
     print("bench: proposals and rates at equal gradient variance", file=sys.stderr, flush=True)
-    step, chosen, rates = poisson_paths(dev, fr, use_learned=not args.no_rates)
+    step, chosen, rates = poisson_paths(dev, fr, compare=not args.no_rates)
     for _ in range(args.warmup):
         step()
     print("bench: Poisson-rate frames (%s rates)" % chosen, file=sys.stderr, flush=True)
@@ -523,7 +526,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--splats", type=int, default=None, help="override the splat count (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-rates", action="store_true", help="time the plain rates without the learned comparison")
+    ap.add_argument("--no-rates", action="store_true",
+                    help="time the band proposal at the plain rate without the footprint / learned comparisons")
     ap.add_argument("--no-poisson", action="store_true", help="skip the Poisson-rate path")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-view (box room) workload")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -561,6 +565,8 @@ def main():
                    "compositing": True, "parallelism": "none (1 GPU)"},
         "roofline": roof,
         "ops_ms": ref.pop("ops_ms"),
+        "ops_ms_note": "HIP events around each op call in separate instrumented steps: an op's span includes any "
+                       "time the device waits for the host to enqueue the op's launches",
     }
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
