@@ -90,7 +90,7 @@ def test_covariance(dev, case):
     assert_close(c.cpu().numpy(), ref, RTOL, "covariance")
 
 
-@pytest.mark.parametrize("k", [12, 100])
+@pytest.mark.parametrize("k", [12, 40, 64, 100])
 def test_filter_forward_backward(dev, k):
     from cudaExtensions import filtering
     sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)[:5]
@@ -120,6 +120,32 @@ def test_filter_forward_backward(dev, k):
     me = magnitude(O.filter_backward, idx, w2, g, sc.width, sc.height, sc.focal, p[None], n[None], exact=True)
     assert_close(ge.cpu().numpy(), re, RTOL, "exact filter backward", mag=me)
     assert np.abs(re - O.filter_backward(idx, w2, g, sc.width, sc.height, sc.focal, p[None], n[None])).max() > 1e-2
+
+
+@pytest.mark.parametrize("k", [40, 64])
+def test_filter_equal_depths(dev, k):
+    """Slots drawn from three splats, so most pixels hold runs of equal depths with
+    different weights: the compositing order among equal depths is the slot order
+    (the reference's stable insertion sort), forward and backward."""
+    from cudaExtensions import filtering
+    sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)[:5]
+    rng = np.random.default_rng(77 + k)
+    idx = random_slots(rng, 1, sc.width, sc.height, k, 3, p_empty=0.3)
+    w = rng.uniform(0.0, 1.0, size=idx.shape).astype(np.float32)
+    w[idx < 0] = rng.uniform(0.0, 1.0, size=int((idx < 0).sum())).astype(np.float32)   # ties at depth 10000 too
+    g = rng.normal(size=idx.shape).astype(np.float32)
+    out = filtering.forward(t(idx, dev), t(w, dev), sc.width, sc.height, sc.focal, t(p[None], dev), t(n[None], dev))
+    ref = O.filter_forward(idx, w, sc.width, sc.height, sc.focal, p[None], n[None])
+    assert_close(out.cpu().numpy(), ref, RTOL, "filter forward, equal depths")
+    gw = filtering.backward(t(idx, dev), t(w, dev), t(g, dev), sc.width, sc.height, sc.focal, t(p[None], dev),
+                            t(n[None], dev))
+    rg = O.filter_backward(idx, w, g, sc.width, sc.height, sc.focal, p[None], n[None])
+    mg = magnitude(O.filter_backward, idx, w, g, sc.width, sc.height, sc.focal, p[None], n[None])
+    assert_close(gw.cpu().numpy(), rg, RTOL, "filter backward, equal depths", mag=mg)
+    # a wrong order among equal depths changes the forward output by far more than the tolerance
+    wrong = O.filter_forward(idx[..., ::-1].copy(), w[..., ::-1].copy(), sc.width, sc.height, sc.focal, p[None],
+                             n[None])[..., ::-1]
+    assert np.abs(wrong - ref).max() > 1e-2
 
 
 @pytest.mark.parametrize("affine", [True, False])

@@ -669,9 +669,73 @@ __device__ __forceinline__ C ray_depth(int i, int j, C Wf, C Hf, C ff, const C* 
     return (d > C(0)) ? d : C(10000);
 }
 
+// Cross-lane moves of the 64-lane wavefront through DPP (gfx9 controls:
+// row_shr:n = 0x110 + n inside 16-lane rows, row_bcast:15 = 0x142 and
+// row_bcast:31 = 0x143 across rows).  Lanes whose source is outside the row, or
+// whose row the row mask leaves out, return `ident`.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_from(float ident, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(ident), __float_as_int(v), CTRL, ROWMASK, 0xf,
+                                                      false));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_from(double ident, double v)
+{
+    const long long a = __double_as_longlong(ident), b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)a, (int)b, CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(a >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Inclusive product over lanes 0..s (Hillis-Steele: shifts 1, 2, 4, 8 inside
+// rows, then the row broadcasts).
+template <int CTRL, int RM, class C>
+__device__ __forceinline__ void prod_step(C& v) { v = v * dpp_from<CTRL, RM>(C(1), v); }
+template <class C>
+__device__ __forceinline__ C wave_prod_incl(C v)
+{
+    prod_step<0x111, 0xf>(v);
+    prod_step<0x112, 0xf>(v);
+    prod_step<0x114, 0xf>(v);
+    prod_step<0x118, 0xf>(v);
+    prod_step<0x142, 0xa>(v);
+    prod_step<0x143, 0xc>(v);
+    return v;
+}
+
+// Inclusive composition of affine maps x -> A + B x over lanes 0..s, the map of
+// the lower lane applied first: (A, B) o (tA, tB) = (A + B tA, B tB).
+template <int CTRL, int RM, class C>
+__device__ __forceinline__ void affine_step(C& A, C& B)
+{
+    const C tA = dpp_from<CTRL, RM>(C(0), A);
+    const C tB = dpp_from<CTRL, RM>(C(1), B);
+    A = A + B * tA;
+    B = B * tB;
+}
+template <class C>
+__device__ __forceinline__ void wave_affine_incl(C& A, C& B)
+{
+    affine_step<0x111, 0xf>(A, B);
+    affine_step<0x112, 0xf>(A, B);
+    affine_step<0x114, 0xf>(A, B);
+    affine_step<0x118, 0xf>(A, B);
+    affine_step<0x142, 0xa>(A, B);
+    affine_step<0x143, 0xc>(A, B);
+}
+
+__device__ __forceinline__ float quiet_nan(float) { return __int_as_float(0x7fc00000); }
+__device__ __forceinline__ double quiet_nan(double) { return __longlong_as_double(0x7ff8000000000000ll); }
+
 // One wavefront per pixel (k <= 64): lane s owns slot s.  The stable
 // insertion argsort of the reference becomes a rank per lane:
 //   rank_s = #{d_j < d_s} + #{j < s : d_j == d_s}.
+// The first term compares the lane's depth with every slot's, read from LDS by
+// broadcast (lanes past k hold NaN, which compares false); lanes of equal depth
+// get the same first term, so each ORs its bit into the mask at that index and
+// the second term is the count of mask bits below the lane.  Depths are never
+// NaN (ray_depth maps a non-positive or NaN quotient to 10000).
 template <bool BACKWARD, class S>
 __global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, int H, int k,
                                                      typename CompOf<S>::type ff, typename CompOf<S>::type gate,
@@ -682,7 +746,11 @@ __global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, in
 {
     using C = typename CompOf<S>::type;
     __shared__ C sbuf[4][2][64];
-    const int wv = threadIdx.x >> 6, s = threadIdx.x & 63;
+    __shared__ C skey[4][64];
+    __shared__ unsigned long long sgrp[4][64];
+    // the wavefront index is uniform: read it into a scalar so the pixel
+    // arithmetic below runs on the scalar unit
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), s = threadIdx.x & 63;
     long pix = (long)blockIdx.x * 4 + wv;
     if (pix >= npix) return;   // whole wave exits together
     long b = pix / ((long)W * H);
@@ -692,6 +760,7 @@ __global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, in
     const bool act = s < k;
     int id = act ? idx[base + s] : -1;
     C w = act ? ld_c(wts, base + s) : C(0);
+    C gi = (BACKWARD && act) ? ld_c(gin, base + s) : C(0);
     C d = C(10000);
     if (id >= 0) {
         long po = 3 * ((pb > 1 ? b * N : 0) + id);
@@ -699,27 +768,39 @@ __global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, in
         C n[3] = { ld_c(nrm, po), ld_c(nrm, po + 1), ld_c(nrm, po + 2) };
         d = ray_depth<C>(i, j, (C)W, (C)H, ff, p, n);
     }
-    int rank = 0;
-    for (int q = 0; q < k; q++) {
-        C dq = __shfl(d, q, 64);
-        rank += (dq < d) || (dq == d && q < s);
+    C* key = skey[wv];
+    key[s] = act ? d : quiet_nan(C(0));
+    sgrp[wv][s] = 0ull;
+    __builtin_amdgcn_wave_barrier();
+    int lt = 0;
+    const int k16 = (k + 15) & ~15;   // <= 64: the reads stay inside the row
+    for (int q0 = 0; q0 < k16; q0 += 16) {
+        C kq[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) kq[q] = key[q0 + q];
+#pragma unroll
+        for (int q = 0; q < 16; q++) lt += kq[q] < d;
     }
+    if (act) atomicOr(&sgrp[wv][lt], 1ull << s);
+    __builtin_amdgcn_wave_barrier();
+    const unsigned long long grp = act ? sgrp[wv][lt] : 0ull;
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(grp >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((unsigned)grp, lt));
     // exclusive prefix product of (1 - w) in rank order
     C* sa = sbuf[wv][0];
     C* sb = sbuf[wv][1];
     sb[s] = C(1);
+    if (BACKWARD) sa[s] = C(0);
     __builtin_amdgcn_wave_barrier();
-    if (act) sb[rank] = C(1) - w;
-    __builtin_amdgcn_wave_barrier();
-    C v = sb[s];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        C t = __shfl_up(v, o, 64);
-        if (s >= o) v *= t;
+    if (act) {
+        sb[rank] = C(1) - w;
+        if (BACKWARD) sa[rank] = gi * w;
     }
-    C excl = __shfl_up(v, 1, 64);
-    if (s == 0) excl = C(1);
-    C rem = __shfl(excl, rank, 64);     // transmittance in front of this slot
+    __builtin_amdgcn_wave_barrier();
+    const C incl = wave_prod_incl(sb[s]);
+    // transmittance in front of this slot: the inclusive product of the rank before it
+    C rem = __shfl(incl, rank > 0 ? rank - 1 : 0, 64);
+    if (rank == 0) rem = C(1);
     if (!BACKWARD) {
         if (act) st_c(out, base + s, rem * w);
         return;
@@ -731,21 +812,11 @@ __global__ __launch_bounds__(256) void k_filter_wave(long npix, int N, int W, in
     //   S_r = sum_{i>r} gin_i w_i prod_{r<t<i} (1 - w_t),
     // the suffix recurrence S_r = a_{r+1} + b_{r+1} S_{r+1} (a = gin w, b = 1 - w).
     // Nothing is divided, so a small w or 1 - w amplifies no rounding.
-    C gi = act ? ld_c(gin, base + s) : C(0);
-    __builtin_amdgcn_wave_barrier();
-    sa[s] = C(0);
-    __builtin_amdgcn_wave_barrier();
-    if (act) sa[rank] = gi * w;             // sb[rank] still holds 1 - w, identity elsewhere
-    __builtin_amdgcn_wave_barrier();
-    // lane l holds the affine map x -> A + B x of rank 63 - l; an inclusive scan
-    // composes ranks 63 .. 63 - l, outer (smaller rank) applied last
+    // Lane l holds the affine map x -> A + B x of rank 63 - l (identity past the
+    // last slot); the inclusive scan composes ranks 63 .. 63 - l, outer (smaller
+    // rank) applied last.
     C A = sa[63 - s], B = sb[63 - s];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        C tA = __shfl_up(A, o, 64);
-        C tB = __shfl_up(B, o, 64);
-        if (s >= o) { A = A + B * tA; B = B * tB; }
-    }
+    wave_affine_incl(A, B);
     // S_r = U_{r+1}, held by lane 63 - (r + 1)
     int src = 62 - rank;
     C after = __shfl(A, src < 0 ? 0 : src, 64);
