@@ -1280,14 +1280,16 @@ static float o_eval_node(const OTree* T, int nd, float qx, float qy)
     float ex = oexpf(-(tx * tx)), ey = oexpf(-(ty * ty));
     hx[0] = ex; hy[0] = ey;
     hx[1] = (2.0f * tx) * ex; hy[1] = (2.0f * ty) * ey;
-    for (int n = 2; n <= p; n++) {
-        hx[n] = (2.0f * tx) * hx[n - 1] - (2.0f * (float)(n - 1)) * hx[n - 2];
-        hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
+    for (int n = 2; n <= p; n++) {   /* hermitefmm.cuh:45, one fused multiply-add per step */
+        hx[n] = fmaf(2.0f * tx, hx[n - 1], -((2.0f * (float)(n - 1)) * hx[n - 2]));
+        hy[n] = fmaf(2.0f * ty, hy[n - 1], -((2.0f * (float)(n - 1)) * hy[n - 2]));
     }
     const float* c = T->coef + T->off[nd];
     float w = 0.0f;
     for (int a = 0; a <= p; a++)
-        for (int b = 0; b <= p; b++) w = w + (c[a * (p + 1) + b] * hx[a]) * hy[b];
+        /* one fused multiply-add per term: `weight += coeffs[index] * hermiteX[a] *
+           hermiteY[b]` as nvcc contracts it by default (-fmad=true) */
+        for (int b = 0; b <= p; b++) w = fmaf(c[a * (p + 1) + b] * hx[a], hy[b], w);
     return w;
 }
 

@@ -1162,25 +1162,31 @@ __device__ __forceinline__ float eval_node(const TreeView& T, const NodeGeo& g, 
         return w;
     }
     // Hermite functions h_n(t) = 2t h_{n-1} - 2(n-1) h_{n-2}, h_0 = e^{-t^2}
-    // (loops unrolled to the maximum degree and predicated, so hy stays in registers)
+    // (loops unrolled to the maximum degree and predicated, so hy stays in registers).
+    // Each term of the series is added with one fused multiply-add,
+    // w = fma(c[a][b] h_a, h_b, w): the reference's `weight += coeffs[index] *
+    // hermiteX[a] * hermiteY[b]` (hermitefmm.cuh:233) as nvcc's default
+    // contraction (-fmad=true) compiles it; each recurrence step is
+    // h_n = fma(2t, h_{n-1}, -(2(n-1) h_{n-2})) (hermitefmm.cuh:45), the form
+    // of the fixed-order densities below.
     float hy[kMaxDegree + 1];
     hy[0] = ey;
     hy[1] = (2.0f * ty) * ey;
 #pragma unroll
     for (int n = 2; n <= kMaxDegree; n++)
-        hy[n] = (n <= p) ? (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2] : 0.0f;
+        hy[n] = (n <= p) ? __builtin_fmaf(2.0f * ty, hy[n - 1], -((2.0f * (float)(n - 1)) * hy[n - 2])) : 0.0f;
     float w = 0.0f;
     float hprev = 0.0f, ha = ex;
     for (int a = 0; a <= p; a++) {
         if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
         else if (a >= 2) {
-            float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+            float hn = __builtin_fmaf(2.0f * tx, ha, -((2.0f * (float)(a - 1)) * hprev));
             hprev = ha; ha = hn;
         }
         const float UD_GLOBAL* row = c + a * (p + 1);
 #pragma unroll
         for (int b = 0; b <= kMaxDegree; b++)
-            if (b <= p) w = w + (row[b] * ha) * hy[b];
+            if (b <= p) w = __builtin_fmaf(row[b] * ha, hy[b], w);
     }
     return w;
 }
@@ -1198,17 +1204,17 @@ __device__ __forceinline__ float hermite_series(const float* sc, float tx, float
     hy[0] = ey;
     if (P >= 1) hy[P >= 1 ? 1 : 0] = (2.0f * ty) * ey;
 #pragma unroll
-    for (int n = 2; n <= P; n++) hy[n] = (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2];
+    for (int n = 2; n <= P; n++) hy[n] = __builtin_fmaf(2.0f * ty, hy[n - 1], -((2.0f * (float)(n - 1)) * hy[n - 2]));
     float w = 0.0f, hprev = 0.0f, ha = ex;
 #pragma unroll
     for (int a = 0; a <= P; a++) {
         if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
         else if (a >= 2) {
-            float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+            float hn = __builtin_fmaf(2.0f * tx, ha, -((2.0f * (float)(a - 1)) * hprev));
             hprev = ha; ha = hn;
         }
 #pragma unroll
-        for (int b = 0; b <= P; b++) w = w + (sc[a * (P + 1) + b] * ha) * hy[b];
+        for (int b = 0; b <= P; b++) w = __builtin_fmaf(sc[a * (P + 1) + b] * ha, hy[b], w);
     }
     return w;
 }
@@ -1240,18 +1246,18 @@ __device__ __forceinline__ float eval_node_staged(const TreeView& T, const NodeG
             hy[1] = (2.0f * ty) * ey;
 #pragma unroll
             for (int n = 2; n <= kMaxDegree; n++)
-                hy[n] = (n <= p) ? (2.0f * ty) * hy[n - 1] - (2.0f * (float)(n - 1)) * hy[n - 2] : 0.0f;
+                hy[n] = (n <= p) ? __builtin_fmaf(2.0f * ty, hy[n - 1], -((2.0f * (float)(n - 1)) * hy[n - 2])) : 0.0f;
             float hprev = 0.0f, ha = ex;
             for (int a = 0; a <= p; a++) {
                 if (a == 1) { hprev = ex; ha = (2.0f * tx) * ex; }
                 else if (a >= 2) {
-                    float hn = (2.0f * tx) * ha - (2.0f * (float)(a - 1)) * hprev;
+                    float hn = __builtin_fmaf(2.0f * tx, ha, -((2.0f * (float)(a - 1)) * hprev));
                     hprev = ha; ha = hn;
                 }
                 const float* row = sc + a * (p + 1);
 #pragma unroll
                 for (int b = 0; b <= kMaxDegree; b++)
-                    if (b <= p) w = w + (row[b] * ha) * hy[b];
+                    if (b <= p) w = __builtin_fmaf(row[b] * ha, hy[b], w);
             }
         }
         }
