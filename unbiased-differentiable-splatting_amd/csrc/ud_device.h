@@ -188,4 +188,19 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks)
     return (i < len) ? start + i : bid;
 }
 
+// Chunked round-robin remap for per-tile passes whose cost varies across the
+// image: runs of LC consecutive tiles are dealt to the 8 XCDs in turn (XCD x
+// takes runs x, x + 8, ...), so every XCD gets a sample of the whole image (its
+// dense and its empty parts) while a run's neighbouring tiles still share that
+// XCD's L2.  The last nblocks mod 8·LC blocks keep their own tile.  A permutation
+// of [0, nblocks).
+template <uint32_t LC>
+__device__ __forceinline__ uint32_t xcd_chunk_remap(uint32_t bid, uint32_t nblocks)
+{
+    const uint32_t full = nblocks / (8u * LC) * (8u * LC);
+    if (bid >= full) return bid;
+    const uint32_t x = bid % 8u, i = bid / 8u;
+    return (x + 8u * (i / LC)) * LC + i % LC;
+}
+
 } // namespace ud
