@@ -203,6 +203,24 @@ int ude_splat_backward(int bn, int N, int W, int H, int k, double f, double smoo
                        const float* sd, int sdBatch, float* gPts, float* gNrm, float* gSd,
                        void* ws, size_t wsBytes, ude_stream_t stream);
 
+/* splatting.backward followed by the post-processing of the reference's
+ * EWASplatting.backward -- replaces splatting/extensions/__init__.py:210-225
+ * (the cudaExtensions call, conditionalNormalization of the point and std-dev
+ * gradients, utils.py:147-150, and their NaN -> 0), fused into the splat pass:
+ * gPts and gSd rows become g / max(|g| + 1e-4, 1) with NaN entries set to 0;
+ * gNrm stays raw (the reference returns no normal gradient).  Per-view
+ * std-devs only (sdBatch = bn: a shared row is a sum over views, which the
+ * post-processing must follow).  gPts, gNrm (bn, N, 3) and gSd (bn, N, 2) fully
+ * written. */
+int ude_splat_backward_normalized(int bn, int N, int W, int H, int k, double f, double smoothing,
+                                  const float* gradIn, const int32_t* idx, const float* pts, const float* nrm,
+                                  const float* sd, float* gPts, float* gNrm, float* gSd, void* ws, size_t wsBytes,
+                                  ude_stream_t stream);
+int ude_splat_backward_normalized_f64(int bn, int N, int W, int H, int k, double f, double smoothing,
+                                      const double* gradIn, const int32_t* idx, const double* pts,
+                                      const double* nrm, const double* sd, double* gPts, double* gNrm,
+                                      double* gSd, void* ws, size_t wsBytes, ude_stream_t stream);
+
 /* No counterpart in the reference: ude_splat_backward for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments and outputs as

@@ -60,6 +60,39 @@ def test_splat_backward(dev, case, smoothing):
     assert_close(gs.cpu().numpy(), rs, RTOL, "grad stddevs", mag=ms)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_splat_backward_normalized(dev, dtype):
+    """splatting.backward(normalized=True) equals the raw backward followed by the
+    reference's torch post-processing (conditionalNormalization, NaN -> 0) of the
+    point and std-dev gradients; the normal gradient stays raw."""
+    import torch
+    from cudaExtensions import splatting
+    from splatting_amd.utils import conditionalNormalization
+    sc, p, n, nb, sd = scene_inputs(2, sd_scale=8.0)[:5]
+    rng = np.random.default_rng(21)
+    idx = near_slots(rng, p, sc.width, sc.height, sc.focal, 6)
+    g = (rng.normal(size=idx.shape) * 30.0).astype(dtype)
+    args = (sc.width, sc.height, sc.focal, t(g, dev), t(idx, dev), t(p[None].astype(dtype), dev),
+            t(n[None].astype(dtype), dev), t(sd[None].astype(dtype), dev), 0.0)
+    rp, rn, rs = splatting.backward(*args)
+    gp, gn, gs = splatting.backward(*args, normalized=True)
+    ep = torch.nan_to_num(conditionalNormalization(rp), nan=0.0, posinf=float("inf"), neginf=float("-inf"))
+    es = torch.nan_to_num(conditionalNormalization(rs), nan=0.0, posinf=float("inf"), neginf=float("-inf"))
+    # the two calls sum their slot moments with atomics in different orders
+    tol = 1e-5 if dtype == np.float32 else 1e-12
+    assert torch.allclose(gp, ep, rtol=tol, atol=tol * float(ep.abs().max()))
+    assert torch.allclose(gs, es, rtol=tol, atol=tol * float(es.abs().max()))
+    assert torch.allclose(gn, rn, rtol=tol, atol=tol * float(rn.abs().max()))
+    norms = rp.norm(dim=-1)
+    assert bool((norms > 1.5).any()) and bool(((norms > 0) & (norms < 0.5)).any())   # both branches taken
+    # two views sharing one std-dev row: the post-processing would have to follow the sum over views
+    p2, n2 = np.stack([p, p]).astype(dtype), np.stack([n, n]).astype(dtype)
+    idx2, g2 = np.concatenate([idx, idx]), np.concatenate([g, g])
+    with pytest.raises(ValueError):
+        splatting.backward(sc.width, sc.height, sc.focal, t(g2, dev), t(idx2, dev), t(p2, dev), t(n2, dev),
+                           t(sd[None].astype(dtype), dev), 0.0, normalized=True)
+
+
 def test_splat_batched_views_and_broadcast_sd(dev):
     """bn = 2 views with shared (1, N, 2) std-devs: gradients of the std-devs sum over views."""
     from cudaExtensions import splatting

@@ -260,7 +260,24 @@ __global__ __launch_bounds__(256) void k_splat_moments(long nslots, int N, int W
 // holds the six moments at [0, 6); records of kHtRec floats (the fused
 // estimator's) also hold the attribute gradient of the (view, splat) at
 // [8, 11), written to gAttr (ab == bn) or added into it (ab == 1 < bn).
-template <class T>
+// The reference's EWASplatting.backward post-processing of a gradient row
+// (splatting/extensions/__init__.py:219-225 with utils.py:147-150's
+// conditionalNormalization): g / max(|g| + 1e-4, 1), then NaN -> 0 (infinities
+// kept), in the element type.
+template <class T, int C>
+__device__ __forceinline__ void cond_normalize(T* g)
+{
+    T s = g[0] * g[0];
+    for (int c = 1; c < C; c++) s = s + g[c] * g[c];
+    T den = sqrt(s) + T(1e-4);
+    den = den < T(1) ? T(1) : den;
+    for (int c = 0; c < C; c++) {
+        const T v = g[c] / den;
+        g[c] = v != v ? T(0) : v;
+    }
+}
+
+template <class T, bool POST = false>
 __global__ __launch_bounds__(128) void k_splat_finalize(int bn, int N, int W, int H, double f, double smoothing,
                                                         const T* __restrict__ pts, const T* __restrict__ nrm,
                                                         const T* __restrict__ sd, int sdBatch,
@@ -339,14 +356,21 @@ __global__ __launch_bounds__(128) void k_splat_finalize(int bn, int N, int W, in
     double GA11 = -2.0 * F11 - m0 * J11;
     (void)F10;
     // dL/dp_c = sum GA : dA/dp_c + G_s . ds/dp_c ; dL/dn_c = sum GA : dA/dn_c
+    T gpt[3];
     for (int c = 0; c < 3; c++) {
         double gp = GA00 * A00.d[c] + GA01 * A01.d[c] + GA11 * A11.d[c] + Gsx * sx.d[c] + Gsy * sy.d[c];
         double gn = GA00 * A00.d[3 + c] + GA01 * A01.d[3 + c] + GA11 * A11.d[3 + c];
-        gPts[3 * t + c] = (T)gp;
+        gpt[c] = (T)gp;
         gNrm[3 * t + c] = (T)gn;
     }
-    if (sdBatch == bn) { gSd[so] = (T)gs0; gSd[so + 1] = (T)gs1; }     // one writer per std-dev
-    else { atomicAdd(gSd + so, (T)gs0); atomicAdd(gSd + so + 1, (T)gs1); }
+    T gsd[2] = { (T)gs0, (T)gs1 };
+    if (POST) {                      // sdBatch == bn: this thread owns the std-dev row
+        cond_normalize<T, 3>(gpt);
+        cond_normalize<T, 2>(gsd);
+    }
+    for (int c = 0; c < 3; c++) gPts[3 * t + c] = gpt[c];
+    if (sdBatch == bn) { gSd[so] = gsd[0]; gSd[so + 1] = gsd[1]; }     // one writer per std-dev
+    else { atomicAdd(gSd + so, gsd[0]); atomicAdd(gSd + so + 1, gsd[1]); }
 }
 
 // ---------------------------------------------------------------------------
@@ -1494,9 +1518,10 @@ int splat_forward_t(int bn, int N, int W, int H, int k, double f, double smoothi
 template <class T>
 int splat_backward_t(int bn, int N, int W, int H, int k, double f, double smoothing, const T* gradIn,
                      const int32_t* idx, const T* pts, const T* nrm, const T* sd, int sdBatch, T* gPts, T* gNrm,
-                     T* gSd, void* ws, size_t wsBytes, hipStream_t st)
+                     T* gSd, void* ws, size_t wsBytes, hipStream_t st, bool post = false)
 {
     if (bn <= 0 || W <= 0 || H <= 0 || k <= 0 || N < 0) return UDE_ERR_ARG;
+    if (post && sdBatch != bn) return UDE_ERR_ARG;   // a shared std-dev row is a sum over views
     if (N == 0) return UDE_OK;
     if (wsBytes < splat_ws_bytes<T>(bn, N)) return UDE_ERR_WORKSPACE;
     long nslots = (long)bn * W * H * k;
@@ -1506,8 +1531,14 @@ int splat_backward_t(int bn, int N, int W, int H, int k, double f, double smooth
                                                                  rec);
     k_fill<T><<<grid1((long)kMomRec * bn * N, 256), 256, 0, st>>>(mom, (long)kMomRec * bn * N, T(0));
     k_splat_moments<T><<<grid1(nslots, 256), 256, 0, st>>>(nslots, N, W, H, k, idx, gradIn, rec, mom);
-    k_splat_finalize<T><<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd, sdBatch,
-                                                                  rec, mom, kMomRec, 1, nullptr, gPts, gNrm, gSd);
+    if (post)
+        k_splat_finalize<T, true><<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd,
+                                                                            sdBatch, rec, mom, kMomRec, 1, nullptr,
+                                                                            gPts, gNrm, gSd);
+    else
+        k_splat_finalize<T><<<grid1((long)bn * N, 128), 128, 0, st>>>(bn, N, W, H, f, smoothing, pts, nrm, sd,
+                                                                      sdBatch, rec, mom, kMomRec, 1, nullptr, gPts,
+                                                                      gNrm, gSd);
     return launch_status();
 }
 
@@ -1659,6 +1690,24 @@ int ude_splat_backward_f64(int bn, int N, int W, int H, int k, double f, double 
 {
     return splat_backward_t<double>(bn, N, W, H, k, f, smoothing, gradIn, idx, pts, nrm, sd, sdBatch, gPts, gNrm,
                                     gSd, ws, wsBytes, st);
+}
+
+int ude_splat_backward_normalized(int bn, int N, int W, int H, int k, double f, double smoothing,
+                                  const float* gradIn, const int32_t* idx, const float* pts, const float* nrm,
+                                  const float* sd, float* gPts, float* gNrm, float* gSd, void* ws, size_t wsBytes,
+                                  hipStream_t st)
+{
+    return splat_backward_t<float>(bn, N, W, H, k, f, smoothing, gradIn, idx, pts, nrm, sd, bn, gPts, gNrm, gSd,
+                                   ws, wsBytes, st, true);
+}
+
+int ude_splat_backward_normalized_f64(int bn, int N, int W, int H, int k, double f, double smoothing,
+                                      const double* gradIn, const int32_t* idx, const double* pts,
+                                      const double* nrm, const double* sd, double* gPts, double* gNrm,
+                                      double* gSd, void* ws, size_t wsBytes, hipStream_t st)
+{
+    return splat_backward_t<double>(bn, N, W, H, k, f, smoothing, gradIn, idx, pts, nrm, sd, bn, gPts, gNrm, gSd,
+                                    ws, wsBytes, st, true);
 }
 
 int ude_splat_backward_static(int bn, int N, int W, int H, int k, double f, double smoothing,

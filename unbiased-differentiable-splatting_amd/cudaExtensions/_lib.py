@@ -51,6 +51,7 @@ _SIGS = {
     "ude_splat_forward_f64": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 4 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "ude_splat_backward_f64": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4 + [c_size,
                                                                                                           c_ptr]),
+    "ude_splat_backward_normalized_f64": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 9 + [c_size, c_ptr]),
     "ude_filter_forward_f64": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
     "ude_filter_backward_f64": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
     "ude_filter_forward_f16": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
@@ -67,6 +68,7 @@ _SIGS = {
     "ude_splat_forward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 4 + [c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "ude_splat_backward": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4 + [c_size, c_ptr]),
     "ude_splat_backward_static": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 5 + [c_int] + [c_ptr] * 4),
+    "ude_splat_backward_normalized": (c_int, [c_int] * 5 + [c_dbl] * 2 + [c_ptr] * 9 + [c_size, c_ptr]),
     "ude_filter_forward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 4 + [c_int, c_ptr, c_ptr]),
     "ude_filter_backward": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),
     "ude_filter_backward_exact": (c_int, [c_int] * 5 + [c_dbl] + [c_ptr] * 5 + [c_int, c_ptr, c_ptr]),

@@ -43,8 +43,11 @@ def forward(width, height, focallength, indices, points, normals, stdDevs, smoot
     return out
 
 
-def backward(width, height, focallength, gradIn, indices, points, normals, stdDevs, smoothing):
-    """[gradPoints, gradNormals, gradStdDevs] of sum(gradIn * weights)."""
+def backward(width, height, focallength, gradIn, indices, points, normals, stdDevs, smoothing, normalized=False):
+    """[gradPoints, gradNormals, gradStdDevs] of sum(gradIn * weights).  With
+    ``normalized`` (per-view std-devs only) the point and std-dev gradients come
+    back post-processed as the reference's EWASplatting.backward does
+    (conditionalNormalization, then NaN -> 0), in the same pass."""
     check_shapes(width, height, indices, points, normals, stdDevs)
     _lib.expect(gradIn, "gradIn", *indices.shape)
     dev = _lib.require_gpu(gradIn, indices, points, normals, stdDevs)
@@ -57,6 +60,15 @@ def backward(width, height, focallength, gradIn, indices, points, normals, stdDe
     gn = torch.empty(normals.shape, dtype=dt, device=dev)
     gs = (torch.empty if stdDevs.shape[0] == bn else torch.zeros)(stdDevs.shape, dtype=dt, device=dev)
     ws = _workspace(dt, bn, N, dev)
+    if normalized:
+        if stdDevs.shape[0] != bn:
+            raise ValueError("splatting.backward(normalized=True) needs per-view stdDevs (bn, N, 2)")
+        rc = _lib.fn("ude_splat_backward_normalized", dt)(
+            bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(gradIn), _lib.ptr(indices),
+            _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs), _lib.ptr(gp), _lib.ptr(gn), _lib.ptr(gs),
+            _lib.ptr(ws), ws.numel(), _lib.stream(dev))
+        _lib.check(rc, "splatting.backward")
+        return [gp, gn, gs]
     rc = _lib.fn("ude_splat_backward", dt)(bn, N, W, H, k, float(focallength), float(smoothing), _lib.ptr(gradIn),
                                            _lib.ptr(indices), _lib.ptr(points), _lib.ptr(normals), _lib.ptr(stdDevs),
                                            stdDevs.shape[0], _lib.ptr(gp), _lib.ptr(gn), _lib.ptr(gs), _lib.ptr(ws),

@@ -74,6 +74,11 @@ class EWASplatting(th.autograd.Function):
     @staticmethod
     def backward(ctx, gradWeights):
         indices, points, normals, stdDevs = ctx.saved_tensors
+        if GRAD_MODE["ewa"] != "raw" and stdDevs.shape[0] == indices.shape[0]:
+            # the reference's post-processing (extensions/__init__.py:219-225), fused
+            gp, _, gs = splatting.backward(ctx.width, ctx.height, ctx.focallength, gradWeights.contiguous(), indices,
+                                           points, normals, stdDevs, ctx.smoothing, normalized=True)
+            return None, None, None, None, gp, None, gs, None
         gp, gn, gs = splatting.backward(ctx.width, ctx.height, ctx.focallength, gradWeights.contiguous(), indices,
                                         points, normals, stdDevs, ctx.smoothing)
         if GRAD_MODE["ewa"] == "raw":
