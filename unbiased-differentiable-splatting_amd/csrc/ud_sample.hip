@@ -1692,33 +1692,46 @@ __global__ __launch_bounds__(64) void k_sample_draw(int W, int H, int k, int L, 
     const int2* rch = reach + (long)blockIdx.x * L * 64 + lane;
     int firstPos[16];                                   // per launch: first leaf position drawn
     unsigned long long lastDrop = empty;               // the last word the heap dropped
-    for (int la = 0; la < L; la++) {
-        const int2 fc = rch[(long)la * 64];
+    // The lane's groups of 4 leaves, launch after launch, in one loop: the
+    // wavefront runs as many iterations as its busiest lane has groups (a loop
+    // per launch would run the busiest lane of each launch).  Each lane keys
+    // the same leaves in the same order as a loop per launch.
+    int la = 0, kk0 = 0;
+    int2 fc = rch[0];
+    firstPos[0] = fc.x;
+    while (la < L && fc.y == 0) {
+        if (++la < L) { fc = rch[(long)la * 64]; firstPos[la] = fc.x; }
+    }
+    while (la < L) {
         const int first = fc.x, cnt = fc.y;
-        firstPos[la] = first;
-        for (int kk0 = 0; kk0 < cnt; kk0 += 4) {
-            u32x4 c4 = { (uint32_t)(kk0 >> 2), (uint32_t)la, pixel, view };
-            u32x4 rnd = philox4x32_10(c4, seedLo, seedHi);
-            float2 xy[4];
+        u32x4 c4 = { (uint32_t)(kk0 >> 2), (uint32_t)la, pixel, view };
+        u32x4 rnd = philox4x32_10(c4, seedLo, seedHi);
+        float2 xy[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                int pos = first + kk0 + q;
-                bool ok = kk0 + q < cnt;
-                xy[q] = ok ? leafXY[pos] : make_float2(0.0f, 0.0f);
-            }
+        for (int q = 0; q < 4; q++) {
+            int pos = first + kk0 + q;
+            bool ok = kk0 + q < cnt;
+            xy[q] = ok ? leafXY[pos] : make_float2(0.0f, 0.0f);
+        }
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if (kk0 + q >= cnt) break;
-                unsigned long long ent;
-                if (!draw_word(qx, qy, xy[q], is2h, (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w,
-                               la, kk0 + q, ent))
-                    continue;
-                const unsigned long long top = list[lane];
-                if (ent < top) {
-                    lastDrop = top;
-                    heap_replace_top(list, lane, k, ent);
-                }
+        for (int q = 0; q < 4; q++) {
+            if (kk0 + q >= cnt) break;
+            unsigned long long ent;
+            if (!draw_word(qx, qy, xy[q], is2h, (q == 0) ? rnd.x : (q == 1) ? rnd.y : (q == 2) ? rnd.z : rnd.w,
+                           la, kk0 + q, ent))
+                continue;
+            const unsigned long long top = list[lane];
+            if (ent < top) {
+                lastDrop = top;
+                heap_replace_top(list, lane, k, ent);
             }
+        }
+        kk0 += 4;
+        if (kk0 >= cnt) {
+            kk0 = 0;
+            do {
+                if (++la < L) { fc = rch[(long)la * 64]; firstPos[la] = fc.x; }
+            } while (la < L && fc.y == 0);
         }
     }
     // heap -> ascending order (heapsort)
