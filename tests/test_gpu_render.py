@@ -127,3 +127,32 @@ def test_config0_frame_matches_oracle_chain(dev, mode):
         _, m_N = magnitude(O.transform_backward, False, m_cN, Mv, sc.normals[None])
         assert_close(leaves[1].grad.cpu().numpy(), g_N, RTOL, mode + ": grad normals", mag=m_N, atol=floor(g_N))
     assert np.abs(g_P).max() > 0 and np.abs(g_S).max() > 0
+
+
+def test_fused_backface_equals_torch_where(dev):
+    """The renderer's camera-facing normals from the transform pass
+    (fuseBackface) give the same indices and image as the reference's
+    linearForward + torch.where path, and the same gradients up to the order
+    of their atomic sums."""
+    from splatting_amd import render, scenes
+    sc = scenes.config(0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)
+    M = t(sc.view)[None].contiguous()
+    zero = t(np.zeros((1, 1, 3)))
+    outs = []
+    for fused in (True, False):
+        opts = render.Options(numSamples=40, precision=0.01, shadingMode="None", device=dev)
+        r = render.Splatting(opts, None, sc.width, sc.height, sc.focal, 0.1, 10.0, 0.0)
+        r.seed, r.fuseBackface = 3, fused
+        leaves = [t(a)[None].requires_grad_(True) for a in (sc.positions, sc.normals, sc.stddevs, sc.albedo)]
+        image, indices, _ = r.forward(None, None, *leaves, zero, zero, zero, transformMatrix=M)
+        (image ** 2).sum().backward()
+        outs.append((image.detach(), indices, [a.grad for a in leaves]))
+    (i1, x1, g1), (i2, x2, g2) = outs
+    assert torch.equal(x1, x2)
+    assert torch.equal(i1, i2)
+    for a, b in zip(g1, g2):
+        if a is None or b is None:
+            assert a is None and b is None
+            continue
+        assert torch.allclose(a, b, rtol=1e-5, atol=1e-5 * float(b.abs().max()))

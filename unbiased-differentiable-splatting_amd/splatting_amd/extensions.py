@@ -63,6 +63,25 @@ class LinearTransform(th.autograd.Function):
         return gM, gV
 
 
+class LinearTransformFacing(th.autograd.Function):
+    """LinearTransform and the camera-facing copy of its result (negated where
+    its z component is positive, no gradient) in one pass: the reference's
+    linearForward followed by its backface torch.where (render.py:193-205)."""
+
+    @staticmethod
+    def forward(ctx, transformMatrix, vector):
+        ctx.save_for_backward(transformMatrix, vector)
+        out, facing = transform.linearForwardFacing(transformMatrix, vector)
+        ctx.mark_non_differentiable(facing)
+        return out, facing
+
+    @staticmethod
+    def backward(ctx, gradIn, gradFacing):
+        M, v = ctx.saved_tensors
+        gM, gV = transform.linearBackward(gradIn.contiguous(), M, v)
+        return gM, gV
+
+
 class EWASplatting(th.autograd.Function):
     @staticmethod
     def forward(ctx, width, height, focallength, indices, points, normals, stdDevs, smoothing):

@@ -1,8 +1,8 @@
 """Differentiable surface-splatting renderer (reference: splatting/render.py:193-376)."""
 import torch as th
 
-from .extensions import (AffineTransform, LinearTransform, FmmSampling, EWASplatting, DepthFiltering,
-                         LinearInterpolation, DiffuseShading)
+from .extensions import (AffineTransform, LinearTransform, LinearTransformFacing, FmmSampling, EWASplatting,
+                         DepthFiltering, LinearInterpolation, DiffuseShading)
 from .shading import Lambert, DeferredSphericalHaromic, ForwardSphericalHaromic
 from .utils import world2Camera, transformForward, normalize
 
@@ -62,6 +62,7 @@ class Splatting(th.nn.Module):
         self.forwardHarmonics = ForwardSphericalHaromic()
         self.seed = None
         self.keepIndicesOnHost = True
+        self.fuseBackface = True      # camera-facing normals from the transform pass where possible
 
     def forward(self, camPositions, camOrientations, pointPositions, pointNormals, pointStdDevs, pointDiffuse,
                 ambientLight, lampDirections, lampIntensities, alpha=None, transformMatrix=None):
@@ -72,8 +73,14 @@ class Splatting(th.nn.Module):
             transformMatrix = transformForward(transformMatrix, self.coordForward).squeeze(1)
         transformMatrix = transformMatrix.contiguous()
         camPoints = self.affineTransform(transformMatrix, pointPositions)
-        camNormals = self.linearTransform(transformMatrix, pointNormals)
-        pointNormals, backfaceNormals = self.backface(pointNormals, camNormals)
+        if (self.fuseBackface and pointNormals.dtype == th.float32
+                and self.shadingMode not in ("sphericalHarmonics", "forwardHarmonics")):
+            # only the spherical-harmonics modes read the flipped world normals:
+            # the camera-facing copy comes out of the transform pass itself
+            camNormals, backfaceNormals = LinearTransformFacing.apply(transformMatrix, pointNormals)
+        else:
+            camNormals = self.linearTransform(transformMatrix, pointNormals)
+            pointNormals, backfaceNormals = self.backface(pointNormals, camNormals)
         with th.no_grad():
             indices = self.sampling(self.width, self.height, self.focallength, camPoints.detach().contiguous(),
                                     backfaceNormals.detach().contiguous(), pointDiffuse, pointStdDevs.detach(),
