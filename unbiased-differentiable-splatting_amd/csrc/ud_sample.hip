@@ -1393,6 +1393,9 @@ struct TrieRef {
     {
         set(e, 0, 1.0f); set(e, 1, wUnset); set(e, 2, wUnset); setKids(e, 0); set(e, 4, 1.0f);
     }
+    // an entry of the union walk: both children's weights are stored for every
+    // lane when the walk leaves them, so only capacities and child links start here
+    __device__ void initCaps(int e) const { set(e, 0, 1.0f); setKids(e, 0); set(e, 4, 1.0f); }
 };
 
 // Replace the top of a per-lane max-heap of n words (LDS, slot-major
@@ -1482,10 +1485,13 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
             cur.off = T.aux[0].off; cur.count = T.lnk[0].count; cur.id = 0; cur.pad = 0;
         }
         // a node's weight into its parent's entry (the root's: the total)
-        auto report = [&](int dd, float w) {
-            if (dd == 0) { total = w; return; }
+        // a completed node's weight into its parent's entry, for every lane: the
+        // lanes that reached it store theirs, the others the unset marker (their
+        // descents evaluate it on demand); the root's weight is the total
+        auto report = [&](int dd, bool has, float w) {
+            if (dd == 0) { if (has) total = w; return; }
             const int pe = s_pent[dd - 1];
-            if (pe >= 0) tr.set(pe, ((rightBits >> dd) & 1ull) ? 2 : 1, w);
+            if (pe >= 0) tr.set(pe, ((rightBits >> dd) & 1ull) ? 2 : 1, has ? w : unset);
         };
         auto uni = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
         while (true) {
@@ -1518,12 +1524,11 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
                 const SampRec R = srec[cur.id];           // the one load of a descent
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0) { s_rinfo[d] = R.r; s_walk[d] = wm; }
-                if (act && !walk) report(d, 0.0f);       // truncated for this lane
                 int e = -1;
                 if (d == 0) e = 0;
                 else if (cur.count > k && s_pent[d - 1] >= 0 && nt < DFSMAX) {
                     e = nt++;
-                    tr.init(e, unset);
+                    tr.initCaps(e);
                     tr.setKidHalf(s_pent[d - 1], (rightBits >> d) & 1ull, e + 1);
                 }
                 if (lane == 0) s_pent[d] = e;
@@ -1536,8 +1541,9 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
                 act = walk;
                 continue;
             }
-            // the node is complete for every active lane; report and advance
-            if (act) report(d, wv);
+            // the node is complete for every active lane (wv: its expansion, or 0
+            // where it is truncated); report and advance
+            report(d, act, wv);
             bool done = false;
             while (true) {
                 if (d == 0) { done = true; break; }
@@ -1551,8 +1557,12 @@ __global__ __launch_bounds__(64) void k_sample_walk(int W, int H, int k, int L, 
                 d--;
                 if (big > d) big = d;
                 act = (s_walk[d] >> lane) & 1ull;
-                // its accumulator is kept exactly when its parent has an entry
-                if (act && (d == 0 || s_pent[d - 1] >= 0)) report(d, pacc.get(d));
+                // its accumulator is kept exactly when its parent has an entry; the
+                // lanes that reached it without walking it (truncated) report 0
+                if (d == 0 || s_pent[d - 1] >= 0) {
+                    const bool reached = d == 0 ? act : ((s_walk[d - 1] >> lane) & 1ull) != 0;
+                    report(d, reached, act ? pacc.get(d) : 0.0f);
+                }
             }
             if (done) break;
         }
